@@ -1,0 +1,40 @@
+"""CPU oracle for the metriko hot path: TEST INFRASTRUCTURE ONLY.
+
+This package is a plain FP64 restatement of komietty/metriko's numerical core
+(halfedge mesh, Knoeppel smoothest 4-RoSy field, seam/cut/comb, IGM setup,
+Poisson KKT, Gauss-Newton/LM passes and iterative rounding) written with
+numpy/scipy.  It is the checker for the HIP implementation in ``metriko_amd``:
+
+* only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+  ``bench.py`` may import it, and only to check or to time a CPU baseline;
+* the product path (``metriko_amd``, ``include/``) never imports, links or
+  executes anything from here.
+
+Every function cites the reference file:line it follows (paths relative to
+the reference's ``include/metriko/core``).  Where the reference delegates to a
+third-party library that is absent here (libigl v2.5.0, Eigen 3.4), the
+library's published algorithm is restated and named in the docstring.
+
+Parity status of each function (what pins it) is listed in ``PINNING`` below
+and checked by ``tests/test_oracle_*.py``.
+"""
+
+PINNING = {
+    # function                      : how it is pinned
+    "edge_topology": "internal consistency (twin/next/prev, EF orientation); "
+                     "libigl ids themselves: parity unpinned (libigl absent)",
+    "Hmesh.geometry": "Gauss-Bonnet, exterior angle sums, brute-force numpy recompute",
+    "connection_laplacian": "Hermitian, |offdiag|=1, planar closed form, dense eigh",
+    "solve_smallest_eig": "residual vs Rayleigh quotient, scipy eigsh(sigma=0), "
+                          "dense power iteration on tiny meshes",
+    "field_roots": "closed form",
+    "matching/singular": "Poincare-Hopf (sum = N*chi), rotation invariance",
+    "compute_seam": "cut-graph invariants (disk topology, singulars on seam); "
+                    "exact edge set vs libigl: parity unpinned",
+    "cut/comb/setup": "transition consistency, constraint rows vanish on x0",
+    "reduce_to_linearly_independent": "rank == numpy matrix_rank; row choice vs "
+                                      "Eigen SparseQR<COLAMD>: parity unpinned",
+    "fullpiv_kernel": "C @ K == 0, rank; brute force vs numpy on random ints",
+    "kkt": "scipy spsolve on the assembled KKT; dense lstsq on tiny meshes",
+    "lm/rounding": "integrality, seamlessness, literal dense normal-equation LM on tiny meshes",
+}
