@@ -1,0 +1,180 @@
+/* metriko_hip.h -- C ABI of the MI355X (gfx950) implementation of metriko's
+ * field + IGM hot path.
+ *
+ * Plain C: opaque handles, int status codes, caller-owned host buffers,
+ * library-owned device buffers, no exceptions across the boundary.  Every
+ * call is stream-ordered on the context's stream and synchronous at return.
+ * One context per host thread.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the reference's include/metriko/core).
+ */
+#ifndef METRIKO_HIP_H
+#define METRIKO_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int mk_status;
+enum {
+    MK_OK = 0,
+    MK_ERR_ARG = 1,          /* invalid argument / shape */
+    MK_ERR_HIP = 2,          /* HIP runtime error */
+    MK_ERR_NOT_SPD = 3,      /* CG breakdown: p^H A p <= 0 (matrix not SPD/HPD) */
+    MK_ERR_NO_CONVERGENCE = 4, /* maxiter reached above tolerance */
+    MK_ERR_NAN = 5,          /* NaN/Inf in inputs or iterates */
+    MK_ERR_STAGNATION = 6,   /* iterative refinement stopped improving */
+    MK_ERR_ALLOC = 7,        /* device allocation failed */
+    MK_ERR_UNSUPPORTED = 8,  /* option not supported */
+    MK_ERR_SINGULAR = 9      /* singular system (KKT / kernel basis) */
+};
+
+typedef struct mk_ctx mk_ctx;
+typedef struct mk_op mk_op;
+
+enum { MK_PREC_FP64 = 0, MK_PREC_MIXED = 1 };     /* FP64, or FP32 inner + FP64 refinement */
+enum { MK_FMT_CSR = 0, MK_FMT_SELL = 1 };
+enum { MK_PRECOND_NONE = 0, MK_PRECOND_JACOBI = 1 };
+
+typedef struct {
+    int precision;      /* MK_PREC_* */
+    double tol;         /* FP64 relative residual ||b - A x|| / ||b||, default 1e-10 */
+    int maxiter;        /* max inner iterations per solve (all refinement steps), default 20000 */
+    int format;         /* MK_FMT_* (device operator layout), default SELL */
+    int slice_height;   /* SELL slice height C (multiple of 64), default 64 */
+    int sigma;          /* SELL sorting window (rows), 1 = unsorted, default 1 */
+    int block_size;     /* threads per block for SpMV, default 256 */
+    int precond;        /* MK_PRECOND_*, default JACOBI */
+    double inner_tol;   /* FP32 inner relative tolerance per refinement step, default 1e-5 */
+    int max_refine;     /* max refinement steps, default 12 */
+    int check_every;    /* iterations between host convergence checks, default 16 */
+} mk_solve_opts;
+
+typedef struct {
+    int iterations;       /* inner CG iterations (all solves / refinement steps) */
+    int refinements;      /* FP64 refinement steps (0 for plain FP64) */
+    int solves;           /* number of linear solves */
+    double rel_residual;  /* final FP64 relative residual of the last solve */
+    double ms;            /* wall time of the call (ms) */
+    double spmv_ms;       /* GPU time inside SpMV kernels (hipEvent) */
+    int64_t spmv_calls;   /* SpMV launches */
+    double spmv_bytes;    /* algorithmic bytes moved by those SpMVs */
+} mk_stats;
+
+const char* mk_status_str(mk_status s);
+void mk_default_opts(mk_solve_opts* o);
+mk_status mk_init(mk_ctx** ctx, int device);
+mk_status mk_finalize(mk_ctx* ctx);
+mk_status mk_last_error(mk_ctx* ctx, char* buf, int len);
+/* Enable per-launch hipEvent timing of SpMV kernels (reported in mk_stats.spmv_ms). */
+mk_status mk_set_timing(mk_ctx* ctx, int on);
+
+/* --- generic sparse operators ---------------------------------------------------------
+ * Square sparse matrix in CSR (int32 column ids, int64 row pointers), stored on the
+ * device in the layout of opts->format with FP64 and FP32 value copies.
+ * Replaces the factorize() step of CholeskyWrapper (igm/iterative_rounding/iter_rounding.h:14-31)
+ * and suitesparse::PositiveDefiniteSolver's constructor (solver/suitesparse/suitesparse_pdefinite.h:26-41). */
+mk_status mk_op_create_real(mk_ctx* ctx, int n, const int64_t* rowptr, const int32_t* col, const double* val,
+                            const mk_solve_opts* opts, mk_op** op);
+mk_status mk_op_create_complex(mk_ctx* ctx, int n, const int64_t* rowptr, const int32_t* col,
+                               const double* val_interleaved, const mk_solve_opts* opts, mk_op** op);
+mk_status mk_op_destroy(mk_op* op);
+/* y = A x (FP64), host vectors (complex: interleaved re,im). */
+mk_status mk_op_apply(mk_ctx* ctx, const mk_op* op, const double* x, double* y);
+
+/* SPD / HPD solve A x = b by PCG (FP64 or FP32+FP64 refinement); x is the warm start on
+ * input.  Replaces CholeskyWrapper::solve (iter_rounding.h:19) and LinearSolver::solve
+ * (solver/suitesparse/suitesparse_utilities.h:16-24). */
+mk_status mk_spd_solve(mk_ctx* ctx, const mk_op* op, const double* b, double* x, const mk_solve_opts* opts,
+                       mk_stats* stats);
+
+/* --- field (a2-a4) --------------------------------------------------------------------
+ * Connection Laplacian of FaceRosyField::connectionLaplacian (vectorfield/face_rosy_field.h:46-76)
+ * assembled on the device from halfedge topology: face_halfs[3f+k] = k-th halfedge of face f
+ * (face2half, next, next.next), twin[h], half_face[h] (-1 on boundary), farg[h] = heArgOnFace.
+ * Returns the HPD operator L = S + 1e-9 I (complex). */
+mk_status mk_field_laplacian(mk_ctx* ctx, int nF, int nH, const int32_t* face_halfs, const int32_t* twin,
+                             const int32_t* half_face, const double* farg, int N, const mk_solve_opts* opts,
+                             mk_op** L);
+/* Copies the assembled L back as CSR (sorted columns): rowptr[nF+1], col/val[nnz] (val interleaved).
+ * Pass NULL arrays to query nnz. */
+mk_status mk_op_export_csr(mk_ctx* ctx, const mk_op* op, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val);
+
+/* Inverse power iteration of solveSmallestEig (solver/sparse_linear_solver.h:29-54):
+ * n_iter times x = L^{-1}(M u); s = sqrt|x^H M x|; x /= s; u = x.  mass[nF] = diagonal of M,
+ * u0/x_out interleaved complex [2 nF]. */
+mk_status mk_field_smallest_eig(mk_ctx* ctx, const mk_op* L, const double* mass, const double* u0, int n_iter,
+                                const mk_solve_opts* opts, double* x_out, mk_stats* stats);
+
+/* --- IGM (a12-a16) ---------------------------------------------------------------------
+ * Whole iterative_rounding (igm/iterative_rounding/iter_rounding.h:33-113) with the
+ * seamless / no-injectivity flag set: NaiveIntegration KKT (iter_rounding_init.h:74-232),
+ * naive LM pass, SeamlessIntegration rounding loop (iter_rounding_loop.h).  Inputs mirror the
+ * reference's 17 parameters: G2 (= G2_cut * fullx2Nfunc, CSR rows 2N*nF x cols n*nR),
+ * Cfull (CSR, already reduced to independent rows), intrinsic field F2[2N*nF].
+ * Outputs: fullx[n*nR]; decisions (index, value) in rounding order (ndec on return).
+ * Returns MK_OK with *success = 0 when a post-check fails (reference: success=false). */
+typedef struct {
+    int seamless, round_seams, local_injectivity, verbose;
+} mk_igm_flags;
+typedef struct {
+    int rounds, lm_rounds, k, kernel_rank, n_fixed;
+    double kkt_rel_residual, setup_ms, kkt_ms, naive_ms, rounding_ms;
+    mk_stats solve;
+} mk_igm_stats;
+mk_status mk_iterative_rounding(mk_ctx* ctx, int nfixed, const int32_t* fixedIdcs, const double* fixedVals,
+                                int nsing, const int32_t* singularIdcs, int nint, const int32_t* integerIdcs,
+                                double length,
+                                int c_rows, int c_cols, const int64_t* c_ptr, const int32_t* c_col, const double* c_val,
+                                int g_rows, int g_cols, const int64_t* g_ptr, const int32_t* g_col, const double* g_val,
+                                int N, int n, int nF, const mk_igm_flags* flags, const double* F2,
+                                const mk_solve_opts* opts, double* fullx, int* success,
+                                int max_dec, int32_t* dec_index, int32_t* dec_value, double* dec_margin, int* ndec,
+                                mk_igm_stats* stats);
+
+/* --- driver ---------------------------------------------------------------------------
+ * example/example_igm.cpp:14-36 in one call: Hmesh -> Smoothest field -> Curl matching ->
+ * seam -> cut mesh -> combed field -> extrinsic field -> RosyParameterization(seamless,
+ * !localInjectivity, !roundSeams) setup + integ.  All buffers caller-owned; sizes:
+ * compressed[2nF], singular[nV], matching[nE] (combed), seam[nE], cut_idx[3nF], uv[6nF]
+ * (corner uv, 3 per face), fullx[fullx_cap].  u0 may be NULL (glibc rand() stream as the
+ * reference's VecXc::Random).  Query sizes with mk_pipeline_sizes first. */
+typedef struct {
+    int nV, nF, nE, cutV, nS, nT, k, nfullx, ndec, success;
+    double ms_hmesh, ms_field_asm, ms_field_solve, ms_matching, ms_seam, ms_cut, ms_setup, ms_integ, ms_total;
+    mk_stats field;
+    mk_igm_stats igm;
+} mk_pipeline_info;
+mk_status mk_pipeline_run(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, int N, double ratio,
+                          const double* u0, int n_iter, const mk_solve_opts* field_opts,
+                          const mk_solve_opts* igm_opts,
+                          double* compressed, int32_t* singular, int32_t* matching, uint8_t* seam,
+                          int32_t* cut_idx, double* uv, int fullx_cap, double* fullx,
+                          int dec_cap, int32_t* dec_index, int32_t* dec_value, mk_pipeline_info* info);
+
+/* Host-only stages after the field solve (no GPU needed): roots, Curl matching,
+ * singularities, seam, cut mesh, combed matching, extrinsic field and the IGM setup sizes.
+ * compressed[2nF] in; singular[nV], raw_matching[nE], matching[nE] (combed), seam[nE],
+ * cut_idx[3nF], sizes[4] = {cutV, nS, nT, constraint rows}, ext[12nF] out (any may be NULL). */
+mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
+                           int32_t* singular, int32_t* raw_matching, int32_t* matching, uint8_t* seam,
+                           int32_t* cut_idx, int32_t* sizes, double* ext);
+
+/* Halfedge arrays of the library's Hmesh, as mk_field_laplacian takes them: face_halfs[3nF],
+ * twin[nH], half_face[nH], farg[nH] (heArgOnFace), face_area[nF]; nH = 2 nE (query with
+ * mk_hmesh_export).  Host code, no GPU needed. */
+mk_status mk_hmesh_halfedges(int nV, int nF, const double* V, const int32_t* F, int32_t* face_halfs,
+                             int32_t* twin, int32_t* half_face, double* farg, double* face_area);
+
+/* Hmesh topology/geometry export for tests: edge2vert[2nE], edge2face[2nE], face2edge[3nF],
+ * vert2half[nV], farg[2nE], angle_defect[nV]; nE returned in *nE (pass NULL arrays to query). */
+mk_status mk_hmesh_export(int nV, int nF, const double* V, const int32_t* F, int* nE, int32_t* edge2vert,
+                          int32_t* edge2face, int32_t* face2edge, int32_t* vert2half, double* farg,
+                          double* angle_defect);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

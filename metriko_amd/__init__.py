@@ -1,1 +1,379 @@
-"""metriko_amd: MI355X-native (gfx950) HIP implementation of metriko's field + IGM hot path."""
+"""metriko_amd: MI355X (gfx950) HIP implementation of metriko's field + IGM hot path.
+
+Thin ctypes binding over ``libmetriko_hip.so`` (C ABI declared in
+``include/metriko_hip.h``).  The binding is plumbing for tests and the benchmark:
+all numerics run in the library.  Importing it never falls back to anything else:
+if the library is missing, ``lib()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmetriko_hip.so")
+_lib = None
+
+PREC_FP64, PREC_MIXED = 0, 1
+FMT_CSR, FMT_SELL = 0, 1
+PRECOND_NONE, PRECOND_JACOBI = 0, 1
+
+EXPORTED = [
+    "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing",
+    "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
+    "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
+    "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges",
+]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("tol", C.c_double), ("maxiter", C.c_int), ("format", C.c_int),
+                ("slice_height", C.c_int), ("sigma", C.c_int), ("block_size", C.c_int), ("precond", C.c_int),
+                ("inner_tol", C.c_double), ("max_refine", C.c_int), ("check_every", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("solves", C.c_int),
+                ("rel_residual", C.c_double), ("ms", C.c_double), ("spmv_ms", C.c_double),
+                ("spmv_calls", C.c_int64), ("spmv_bytes", C.c_double)]
+
+
+class IgmFlags(C.Structure):
+    _fields_ = [("seamless", C.c_int), ("round_seams", C.c_int), ("local_injectivity", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class IgmStats(C.Structure):
+    _fields_ = [("rounds", C.c_int), ("lm_rounds", C.c_int), ("k", C.c_int), ("kernel_rank", C.c_int),
+                ("n_fixed", C.c_int), ("kkt_rel_residual", C.c_double), ("setup_ms", C.c_double),
+                ("kkt_ms", C.c_double), ("naive_ms", C.c_double), ("rounding_ms", C.c_double), ("solve", Stats)]
+
+
+class PipelineInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nV", "nF", "nE", "cutV", "nS", "nT", "k", "nfullx", "ndec", "success")] + \
+               [(n, C.c_double) for n in ("ms_hmesh", "ms_field_asm", "ms_field_solve", "ms_matching", "ms_seam",
+                                          "ms_cut", "ms_setup", "ms_integ", "ms_total")] + \
+               [("field", Stats), ("igm", IgmStats)]
+
+
+def struct_dict(s):
+    out = {}
+    for name, _ in s._fields_:
+        v = getattr(s, name)
+        out[name] = struct_dict(v) if isinstance(v, C.Structure) else v
+    return out
+
+
+_P = C.c_void_p
+_D = C.POINTER(C.c_double)
+_I32 = C.POINTER(C.c_int32)
+_I64 = C.POINTER(C.c_int64)
+_U8 = C.POINTER(C.c_uint8)
+
+
+def lib():
+    """Load libmetriko_hip.so (built by ``metriko_amd/build.py``); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"metriko_amd: {LIB_PATH} is not built (run python -m metriko_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    L.mk_status_str.restype = C.c_char_p
+    L.mk_status_str.argtypes = [C.c_int]
+    L.mk_default_opts.argtypes = [C.POINTER(SolveOpts)]
+    L.mk_init.argtypes = [C.POINTER(_P), C.c_int]
+    L.mk_finalize.argtypes = [_P]
+    L.mk_last_error.argtypes = [_P, C.c_char_p, C.c_int]
+    L.mk_set_timing.argtypes = [_P, C.c_int]
+    L.mk_op_create_real.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
+    L.mk_op_create_complex.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
+    L.mk_op_destroy.argtypes = [_P]
+    L.mk_op_apply.argtypes = [_P, _P, _D, _D]
+    L.mk_spd_solve.argtypes = [_P, _P, _D, _D, C.POINTER(SolveOpts), C.POINTER(Stats)]
+    L.mk_field_laplacian.argtypes = [_P, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, C.POINTER(SolveOpts),
+                                     C.POINTER(_P)]
+    L.mk_op_export_csr.argtypes = [_P, _P, _I64, _I64, _I32, _D]
+    L.mk_field_smallest_eig.argtypes = [_P, _P, _D, _D, C.c_int, C.POINTER(SolveOpts), _D, C.POINTER(Stats)]
+    L.mk_iterative_rounding.argtypes = [
+        _P, C.c_int, _I32, _D, C.c_int, _I32, C.c_int, _I32, C.c_double,
+        C.c_int, C.c_int, _I64, _I32, _D, C.c_int, C.c_int, _I64, _I32, _D,
+        C.c_int, C.c_int, C.c_int, C.POINTER(IgmFlags), _D, C.POINTER(SolveOpts), _D, C.POINTER(C.c_int),
+        C.c_int, _I32, _I32, _D, C.POINTER(C.c_int), C.POINTER(IgmStats)]
+    L.mk_pipeline_run.argtypes = [_P, C.c_int, C.c_int, _D, _I32, C.c_int, C.c_double, _D, C.c_int,
+                                  C.POINTER(SolveOpts), C.POINTER(SolveOpts), _D, _I32, _I32, _U8, _I32, _D,
+                                  C.c_int, _D, C.c_int, _I32, _I32, C.POINTER(PipelineInfo)]
+    L.mk_combinatorics.argtypes = [C.c_int, C.c_int, _D, _I32, C.c_int, _D, _I32, _I32, _I32, _U8, _I32, _I32, _D]
+    L.mk_hmesh_export.argtypes = [C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), _I32, _I32, _I32, _I32, _D, _D]
+    L.mk_hmesh_halfedges.argtypes = [C.c_int, C.c_int, _D, _I32, _I32, _I32, _I32, _D, _D]
+    for name in EXPORTED:
+        if name not in ("mk_status_str", "mk_default_opts"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+class MetrikoError(RuntimeError):
+    def __init__(self, status, what, detail=""):
+        self.status = status
+        super().__init__(f"{what}: {lib().mk_status_str(status).decode()} {detail}".strip())
+
+
+def opts(**kw):
+    """mk_solve_opts with the library defaults, overridden by keyword arguments."""
+    o = SolveOpts()
+    lib().mk_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def _mesh_arrays(V, F):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    F = np.ascontiguousarray(F, dtype=np.int32)
+    assert V.ndim == 2 and V.shape[1] == 3 and F.ndim == 2 and F.shape[1] == 3
+    return V, F
+
+
+def hmesh_export(V, F):
+    """Topology/geometry of the library's Hmesh (host code; no GPU needed)."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    nE = C.c_int(0)
+    s = L.mk_hmesh_export(nV, nF, _ptr(V, _D), _ptr(F, _I32), C.byref(nE), None, None, None, None, None, None)
+    if s:
+        raise MetrikoError(s, "mk_hmesh_export")
+    nE = nE.value
+    ev = np.zeros((nE, 2), np.int32)
+    ef = np.zeros((nE, 2), np.int32)
+    fe = np.zeros((nF, 3), np.int32)
+    vh = np.zeros(nV, np.int32)
+    fa = np.zeros(2 * nE)
+    ad = np.zeros(nV)
+    n2 = C.c_int(0)
+    s = L.mk_hmesh_export(nV, nF, _ptr(V, _D), _ptr(F, _I32), C.byref(n2), _ptr(ev, _I32), _ptr(ef, _I32),
+                          _ptr(fe, _I32), _ptr(vh, _I32), _ptr(fa, _D), _ptr(ad, _D))
+    if s:
+        raise MetrikoError(s, "mk_hmesh_export")
+    return {"nE": nE, "edge2vert": ev, "edge2face": ef, "face2edge": fe, "vert2half": vh, "farg": fa,
+            "angle_defect": ad}
+
+
+def hmesh_halfedges(V, F):
+    """face_halfs, twin, half_face, farg, face_area of the library's Hmesh (host code)."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    nH = 2 * hmesh_export(V, F)["nE"]
+    fh = np.zeros((nF, 3), np.int32)
+    tw = np.zeros(nH, np.int32)
+    hf = np.zeros(nH, np.int32)
+    fa = np.zeros(nH)
+    ar = np.zeros(nF)
+    s = L.mk_hmesh_halfedges(nV, nF, _ptr(V, _D), _ptr(F, _I32), _ptr(fh, _I32), _ptr(tw, _I32), _ptr(hf, _I32),
+                             _ptr(fa, _D), _ptr(ar, _D))
+    if s:
+        raise MetrikoError(s, "mk_hmesh_halfedges")
+    return {"face_halfs": fh, "twin": tw, "half_face": hf, "farg": fa, "face_area": ar}
+
+
+def combinatorics(V, F, compressed, N=4):
+    """Host-only stages after the field solve (roots, Curl matching, singularities, seam,
+    cut mesh, combed matching, IGM setup sizes); no GPU needed."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    nE = hmesh_export(V, F)["nE"]
+    comp = np.ascontiguousarray(np.stack([np.real(compressed), np.imag(compressed)], 1).reshape(-1))
+    sing = np.zeros(nV, np.int32)
+    rawm = np.zeros(nE, np.int32)
+    mat = np.zeros(nE, np.int32)
+    seam = np.zeros(nE, np.uint8)
+    cut = np.zeros((nF, 3), np.int32)
+    sizes = np.zeros(4, np.int32)
+    ext = np.zeros((nF, 12))
+    s = L.mk_combinatorics(nV, nF, _ptr(V, _D), _ptr(F, _I32), N, _ptr(comp, _D), _ptr(sing, _I32),
+                           _ptr(rawm, _I32), _ptr(mat, _I32), _ptr(seam, _U8), _ptr(cut, _I32), _ptr(sizes, _I32),
+                           _ptr(ext, _D))
+    if s:
+        raise MetrikoError(s, "mk_combinatorics")
+    return {"singular": sing, "raw_matching": rawm, "matching": mat, "seam": seam.astype(bool), "cut_idx": cut,
+            "cutV": int(sizes[0]), "nS": int(sizes[1]), "nT": int(sizes[2]), "constraint_rows": int(sizes[3]),
+            "ext": ext}
+
+
+class Context:
+    """One mk_ctx (HIP stream + scratch) on one device."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        s = self.L.mk_init(C.byref(self.h), device)
+        if s:
+            raise MetrikoError(s, "mk_init")
+
+    def close(self):
+        if self.h:
+            self.L.mk_finalize(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self):
+        buf = C.create_string_buffer(1024)
+        self.L.mk_last_error(self.h, buf, 1024)
+        return buf.value.decode()
+
+    def _check(self, s, what):
+        if s:
+            raise MetrikoError(s, what, self.last_error())
+
+    def set_timing(self, on):
+        self._check(self.L.mk_set_timing(self.h, int(on)), "mk_set_timing")
+
+    # --- operators -------------------------------------------------------------------
+    def op_real(self, A, o=None):
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A)
+        A.sort_indices()
+        ptr = A.indptr.astype(np.int64)
+        col = A.indices.astype(np.int32)
+        val = A.data.astype(np.float64)
+        h = C.c_void_p()
+        self._check(self.L.mk_op_create_real(self.h, A.shape[0], _ptr(ptr, _I64), _ptr(col, _I32), _ptr(val, _D),
+                                             C.byref(o or opts()), C.byref(h)), "mk_op_create_real")
+        return Op(self, h, A.shape[0], False)
+
+    def op_complex(self, A, o=None):
+        import scipy.sparse as sp
+        A = sp.csr_matrix(A, dtype=np.complex128)
+        A.sort_indices()
+        ptr = A.indptr.astype(np.int64)
+        col = A.indices.astype(np.int32)
+        val = np.ascontiguousarray(np.stack([A.data.real, A.data.imag], 1).reshape(-1))
+        h = C.c_void_p()
+        self._check(self.L.mk_op_create_complex(self.h, A.shape[0], _ptr(ptr, _I64), _ptr(col, _I32), _ptr(val, _D),
+                                                C.byref(o or opts()), C.byref(h)), "mk_op_create_complex")
+        return Op(self, h, A.shape[0], True)
+
+    def field_laplacian(self, face_halfs, twin, half_face, farg, N=4, o=None):
+        fh = np.ascontiguousarray(face_halfs, np.int32).reshape(-1)
+        tw = np.ascontiguousarray(twin, np.int32)
+        hf = np.ascontiguousarray(half_face, np.int32)
+        fa = np.ascontiguousarray(farg, np.float64)
+        h = C.c_void_p()
+        nF = fh.size // 3
+        self._check(self.L.mk_field_laplacian(self.h, nF, tw.size, _ptr(fh, _I32), _ptr(tw, _I32), _ptr(hf, _I32),
+                                              _ptr(fa, _D), N, C.byref(o or opts()), C.byref(h)),
+                    "mk_field_laplacian")
+        return Op(self, h, nF, True)
+
+    def smallest_eig(self, Lop, mass, u0, n_iter=50, o=None):
+        mass = np.ascontiguousarray(mass, np.float64)
+        u = np.ascontiguousarray(np.stack([np.real(u0), np.imag(u0)], 1).reshape(-1))
+        x = np.zeros_like(u)
+        st = Stats()
+        self._check(self.L.mk_field_smallest_eig(self.h, Lop.h, _ptr(mass, _D), _ptr(u, _D), n_iter,
+                                                 C.byref(o or opts()), _ptr(x, _D), C.byref(st)),
+                    "mk_field_smallest_eig")
+        return x[0::2] + 1j * x[1::2], struct_dict(st)
+
+    def pipeline(self, V, F, ratio=0.03, u0=None, n_iter=50, field_opts=None, igm_opts=None):
+        """example_igm in one call (mk_pipeline_run); returns outputs and per-stage info."""
+        V, F = _mesh_arrays(V, F)
+        nV, nF = V.shape[0], F.shape[0]
+        nE = nV + nF + 64  # upper bound placeholder; exact nE comes back in info
+        nE = hmesh_export(V, F)["nE"]
+        comp = np.zeros(2 * nF)
+        sing = np.zeros(nV, np.int32)
+        mat = np.zeros(nE, np.int32)
+        seam = np.zeros(nE, np.uint8)
+        cut = np.zeros((nF, 3), np.int32)
+        uv = np.zeros((nF * 3, 2))
+        cap = 2 * (nV + 2 * nE) + 64
+        fullx = np.zeros(cap)
+        dcap = 4 * nV + 64
+        di = np.zeros(dcap, np.int32)
+        dv = np.zeros(dcap, np.int32)
+        u = None
+        if u0 is not None:
+            u = np.ascontiguousarray(np.stack([np.real(u0), np.imag(u0)], 1).reshape(-1))
+        info = PipelineInfo()
+        self._check(self.L.mk_pipeline_run(self.h, nV, nF, _ptr(V, _D), _ptr(F, _I32), 4, ratio, _ptr(u, _D), n_iter,
+                                           C.byref(field_opts) if field_opts else None,
+                                           C.byref(igm_opts) if igm_opts else None,
+                                           _ptr(comp, _D), _ptr(sing, _I32), _ptr(mat, _I32), _ptr(seam, _U8),
+                                           _ptr(cut, _I32), _ptr(uv, _D), cap, _ptr(fullx, _D), dcap, _ptr(di, _I32),
+                                           _ptr(dv, _I32), C.byref(info)), "mk_pipeline_run")
+        inf = struct_dict(info)
+        return {"compressed": comp[0::2] + 1j * comp[1::2], "singular": sing, "matching": mat,
+                "seam": seam.astype(bool), "cut_idx": cut, "uv": uv, "fullx": fullx[:inf["nfullx"]],
+                "decisions": list(zip(di[:inf["ndec"]].tolist(), dv[:inf["ndec"]].tolist())), "info": inf}
+
+
+class Op:
+    def __init__(self, ctx, h, n, cplx):
+        self.ctx, self.h, self.n, self.cplx = ctx, h, n, cplx
+
+    def close(self):
+        if self.h:
+            self.ctx.L.mk_op_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _pack(self, x):
+        if self.cplx:
+            return np.ascontiguousarray(np.stack([np.real(x), np.imag(x)], 1).reshape(-1))
+        return np.ascontiguousarray(x, np.float64)
+
+    def _unpack(self, y):
+        return y[0::2] + 1j * y[1::2] if self.cplx else y
+
+    def apply(self, x):
+        xp = self._pack(x)
+        y = np.zeros_like(xp)
+        self.ctx._check(self.ctx.L.mk_op_apply(self.ctx.h, self.h, _ptr(xp, _D), _ptr(y, _D)), "mk_op_apply")
+        return self._unpack(y)
+
+    def solve(self, b, x0=None, o=None):
+        bp = self._pack(b)
+        x = self._pack(x0) if x0 is not None else np.zeros_like(bp)
+        x = x.copy()
+        st = Stats()
+        self.ctx._check(self.ctx.L.mk_spd_solve(self.ctx.h, self.h, _ptr(bp, _D), _ptr(x, _D), C.byref(o or opts()),
+                                                C.byref(st)), "mk_spd_solve")
+        return self._unpack(x), struct_dict(st)
+
+    def export_csr(self):
+        import scipy.sparse as sp
+        nnz = C.c_int64(0)
+        self.ctx._check(self.ctx.L.mk_op_export_csr(self.ctx.h, self.h, C.byref(nnz), None, None, None), "export")
+        n = nnz.value
+        ptr = np.zeros(self.n + 1, np.int64)
+        col = np.zeros(n, np.int32)
+        vw = 2 if self.cplx else 1
+        val = np.zeros(n * vw)
+        self.ctx._check(self.ctx.L.mk_op_export_csr(self.ctx.h, self.h, C.byref(nnz), _ptr(ptr, _I64),
+                                                    _ptr(col, _I32), _ptr(val, _D)), "export")
+        v = val[0::2] + 1j * val[1::2] if self.cplx else val
+        return sp.csr_matrix((v, col, ptr), shape=(self.n, self.n))

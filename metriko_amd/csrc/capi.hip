@@ -1,0 +1,486 @@
+// C ABI of the metriko HIP library: contexts, operators, the field stage and helpers.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "mk_internal.h"
+
+using mk::DevBuf;
+using mk::Error;
+
+namespace {
+
+template <typename Fn>
+mk_status guard(mk_ctx* c, Fn&& fn) {
+    try {
+        fn();
+        return MK_OK;
+    } catch (const Error& e) {
+        if (c) c->last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (c) c->last_error = "host allocation failed";
+        return MK_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        if (c) c->last_error = e.what();
+        return MK_ERR_ARG;
+    }
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+mk_solve_opts resolve(const mk_solve_opts* o) {
+    mk_solve_opts d;
+    mk_default_opts(&d);
+    return o ? *o : d;
+}
+
+// Connection Laplacian rows (face_rosy_field.h:46-76), one thread per face, written
+// directly into a SELL layout of width 4 (diagonal + up to three neighbours, columns
+// sorted; missing neighbours on boundary faces are zero padding on the diagonal column).
+// transport across twin(h) into face f: r = -e^{i (farg[h] - farg[twin h])}; the
+// off-diagonal entry is -r^N = -(-1)^N e^{i N (farg[h] - farg[twin h])}.
+__global__ void k_field_laplacian(int nF, int C, const int* __restrict__ fh, const int* __restrict__ twin,
+                                  const int* __restrict__ hface, const double* __restrict__ farg, int N,
+                                  int* __restrict__ col, double* __restrict__ val, int* __restrict__ cnt) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    int c[4];
+    double vr[4], vi[4];
+    int m = 0;
+    double diag = 0.0;
+    const double sgn = (N % 2 == 0) ? -1.0 : 1.0;
+    for (int k = 0; k < 3; k++) {
+        const int h = fh[3 * f + k];
+        const int tw = twin[h];
+        const int g = hface[tw];
+        if (g < 0) continue;
+        diag += 1.0;
+        double s, co;
+        sincos(double(N) * (farg[h] - farg[tw]), &s, &co);
+        c[m] = g;
+        vr[m] = sgn * co;
+        vi[m] = sgn * s;
+        m++;
+    }
+    c[m] = f;
+    vr[m] = diag + 1e-9;
+    vi[m] = 0.0;
+    m++;
+    // insertion sort by column (m <= 4)
+    for (int i = 1; i < m; i++) {
+        int cc = c[i];
+        double a = vr[i], b = vi[i];
+        int j = i - 1;
+        while (j >= 0 && c[j] > cc) { c[j + 1] = c[j]; vr[j + 1] = vr[j]; vi[j + 1] = vi[j]; j--; }
+        c[j + 1] = cc; vr[j + 1] = a; vi[j + 1] = b;
+    }
+    const int s = f / C, lane = f - s * C;
+    const int64_t base = int64_t(s) * 4 * C + lane;
+    for (int j = 0; j < 4; j++) {
+        const int64_t e = base + int64_t(j) * C;
+        if (j < m) { col[e] = c[j]; val[2 * e] = vr[j]; val[2 * e + 1] = vi[j]; }
+        else { col[e] = f; val[2 * e] = 0.0; val[2 * e + 1] = 0.0; }
+    }
+    cnt[f] = m;
+}
+
+__global__ void k_to_f32(size_t len, const double* x, float* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = float(x[i]);
+}
+__global__ void k_fill_i32(size_t len, int v, int* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = v;
+}
+__global__ void k_slice_ptr(int ns, int64_t w, int64_t C, int64_t* ptr, int* sw) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s <= ns) ptr[s] = int64_t(s) * w * C;
+    if (s < ns) sw[s] = int(w);
+}
+__global__ void k_dinv_from_diag(int n, int C, const int* col, const double* val, double* d64, float* d32) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int s = r / C, lane = r - s * C;
+    double d = 0.0;
+    for (int j = 0; j < 4; j++) {
+        const int64_t e = int64_t(s) * 4 * C + int64_t(j) * C + lane;
+        if (col[e] == r) d += val[2 * e];
+    }
+    d64[r] = d != 0.0 ? 1.0 / d : 1.0;
+    d32[r] = float(d64[r]);
+}
+
+inline unsigned nblk(size_t n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+
+}  // namespace
+
+namespace mk {
+void stats_reset(mk_ctx* c) {
+    c->spmv_ms = 0.0;
+    c->spmv_calls = 0;
+    c->spmv_bytes = 0.0;
+    c->ev_used = 0;
+}
+void stats_fill(mk_ctx* c, mk_stats* s) {
+    timing_flush(c);
+    s->spmv_ms = c->spmv_ms;
+    s->spmv_calls = c->spmv_calls;
+    s->spmv_bytes = c->spmv_bytes;
+}
+}  // namespace mk
+
+extern "C" {
+
+const char* mk_status_str(mk_status s) {
+    switch (s) {
+        case MK_OK: return "ok";
+        case MK_ERR_ARG: return "invalid argument";
+        case MK_ERR_HIP: return "HIP runtime error";
+        case MK_ERR_NOT_SPD: return "matrix not SPD/HPD (CG breakdown)";
+        case MK_ERR_NO_CONVERGENCE: return "no convergence within maxiter";
+        case MK_ERR_NAN: return "NaN/Inf encountered";
+        case MK_ERR_STAGNATION: return "iterative refinement stagnated";
+        case MK_ERR_ALLOC: return "allocation failed";
+        case MK_ERR_UNSUPPORTED: return "unsupported option";
+        case MK_ERR_SINGULAR: return "singular system";
+        default: return "unknown status";
+    }
+}
+
+void mk_default_opts(mk_solve_opts* o) {
+    o->precision = MK_PREC_MIXED;
+    o->tol = 1e-10;
+    o->maxiter = 20000;
+    o->format = MK_FMT_SELL;
+    o->slice_height = 64;
+    o->sigma = 1;
+    o->block_size = 256;
+    o->precond = MK_PRECOND_JACOBI;
+    o->inner_tol = 1e-5;
+    o->max_refine = 12;
+    o->check_every = 16;
+}
+
+mk_status mk_init(mk_ctx** out, int device) {
+    if (!out) return MK_ERR_ARG;
+    mk_ctx* c = new (std::nothrow) mk_ctx();
+    if (!c) return MK_ERR_ALLOC;
+    mk_status s = guard(c, [&] {
+        int n = 0;
+        MK_HIP(hipGetDeviceCount(&n));
+        MK_CHECK(device >= 0 && device < n, MK_ERR_ARG, "no such device");
+        c->device = device;
+        MK_HIP(hipSetDevice(device));
+        MK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->partials.alloc(size_t(1 << 17) * 4);
+        c->counter.alloc(16);
+        MK_HIP(hipMemsetAsync(c->counter.p, 0, 16 * sizeof(unsigned), c->stream));
+        c->state.alloc(1);
+        c->scal.alloc(64);
+        MK_HIP(hipMemsetAsync(c->scal.p, 0, 64 * sizeof(double), c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        const char* t = std::getenv("METRIKO_TIME_SPMV");
+        c->time_spmv = t && t[0] == '1';
+    });
+    if (s != MK_OK) { delete c; return s; }
+    *out = c;
+    return MK_OK;
+}
+
+mk_status mk_finalize(mk_ctx* c) {
+    if (!c) return MK_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
+    c->work = mk::Work();
+    c->partials.release();
+    c->counter.release();
+    c->state.release();
+    c->scal.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MK_OK;
+}
+
+mk_status mk_last_error(mk_ctx* c, char* buf, int len) {
+    if (!c || !buf || len <= 0) return MK_ERR_ARG;
+    std::strncpy(buf, c->last_error.c_str(), size_t(len - 1));
+    buf[len - 1] = 0;
+    return MK_OK;
+}
+
+mk_status mk_set_timing(mk_ctx* c, int on) {
+    if (!c) return MK_ERR_ARG;
+    c->time_spmv = on != 0;
+    return MK_OK;
+}
+
+static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
+                           bool cplx, const mk_solve_opts* o_, mk_op** out) {
+    if (!c || !out || n <= 0 || !rowptr || !col || !val) return MK_ERR_ARG;
+    return guard(c, [&] {
+        const mk_solve_opts o = resolve(o_);
+        MK_HIP(hipSetDevice(c->device));
+        auto* op = new mk_op();
+        try {
+            op->ctx = c;
+            op->n = n;
+            op->cplx = cplx;
+            op->nnz = rowptr[n];
+            op->block = o.block_size;
+            MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 1024, MK_ERR_ARG, "block_size");
+            for (int i = 0; i < n; i++) MK_CHECK(rowptr[i + 1] >= rowptr[i], MK_ERR_ARG, "rowptr not monotone");
+            for (int64_t k = 0; k < op->nnz; k++) MK_CHECK(col[k] >= 0 && col[k] < n, MK_ERR_ARG, "column index out of range");
+            const int vw = cplx ? 2 : 1;
+            op->h_ptr.assign(rowptr, rowptr + n + 1);
+            op->h_col.assign(col, col + op->nnz);
+            op->h_val.assign(val, val + op->nnz * vw);
+            const int C = o.format == MK_FMT_CSR ? 64 : o.slice_height;
+            mk::build_sell(op, rowptr, col, val, C, o.format == MK_FMT_CSR ? 1 : o.sigma);
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+    });
+}
+
+mk_status mk_op_create_real(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
+                            const mk_solve_opts* o, mk_op** op) {
+    return op_create(c, n, rowptr, col, val, false, o, op);
+}
+mk_status mk_op_create_complex(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
+                               const mk_solve_opts* o, mk_op** op) {
+    return op_create(c, n, rowptr, col, val, true, o, op);
+}
+mk_status mk_op_destroy(mk_op* op) {
+    if (!op) return MK_ERR_ARG;
+    (void)hipSetDevice(op->ctx->device);
+    delete op;
+    return MK_OK;
+}
+
+mk_status mk_op_apply(mk_ctx* c, const mk_op* A, const double* x, double* y) {
+    if (!c || !A || !x || !y) return MK_ERR_ARG;
+    return guard(c, [&] {
+        MK_HIP(hipSetDevice(c->device));
+        const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+        DevBuf<double> dx, dy(len);
+        dx.upload(x, len, c->stream);
+        mk::spmv64(c, A, dx.p, dy.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        dy.download(y, len, c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+mk_status mk_spd_solve(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts* o_,
+                       mk_stats* st) {
+    if (!c || !A || !b || !x) return MK_ERR_ARG;
+    mk_status inner = MK_OK;
+    mk_status s = guard(c, [&] {
+        const mk_solve_opts o = resolve(o_);
+        MK_HIP(hipSetDevice(c->device));
+        const double t0 = now_ms();
+        mk::stats_reset(c);
+        const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+        DevBuf<double> db, dx;
+        db.upload(b, len, c->stream);
+        dx.upload(x, len, c->stream);
+        mk::SolveResult r = mk::solve_spd(c, A, db.p, dx.p, o);
+        dx.download(x, len, c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->iterations = r.iters;
+            st->refinements = r.refinements;
+            st->solves = 1;
+            st->rel_residual = r.rel_residual;
+            mk::stats_fill(c, st);
+            st->ms = now_ms() - t0;
+        }
+        inner = r.status;
+        if (inner != MK_OK) c->last_error = "mk_spd_solve: " + std::string(mk_status_str(inner));
+    });
+    return s != MK_OK ? s : inner;
+}
+
+mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_halfs, const int32_t* twin,
+                             const int32_t* half_face, const double* farg, int N, const mk_solve_opts* o_,
+                             mk_op** out) {
+    if (!c || !out || nF <= 0 || nH <= 0 || !face_halfs || !twin || !half_face || !farg || N <= 0) return MK_ERR_ARG;
+    return guard(c, [&] {
+        const mk_solve_opts o = resolve(o_);
+        MK_HIP(hipSetDevice(c->device));
+        for (int64_t k = 0; k < int64_t(nF) * 3; k++)
+            MK_CHECK(face_halfs[k] >= 0 && face_halfs[k] < nH, MK_ERR_ARG, "face_halfs out of range");
+        for (int h = 0; h < nH; h++) {
+            MK_CHECK(twin[h] >= 0 && twin[h] < nH, MK_ERR_ARG, "twin out of range");
+            MK_CHECK(half_face[h] >= -1 && half_face[h] < nF, MK_ERR_ARG, "half_face out of range");
+        }
+        const int C = o.slice_height;
+        MK_CHECK(C % 64 == 0 && C > 0, MK_ERR_ARG, "slice_height");
+        auto* op = new mk_op();
+        try {
+            op->ctx = c;
+            op->n = nF;
+            op->cplx = true;
+            op->C = C;
+            op->sigma = 1;
+            op->block = o.block_size;
+            const int ns = (nF + C - 1) / C;
+            op->nslices = ns;
+            op->stored = int64_t(ns) * 4 * C;
+            DevBuf<int> dfh, dtw, dhf;
+            DevBuf<double> dfa;
+            dfh.upload(face_halfs, size_t(nF) * 3, c->stream);
+            dtw.upload(twin, size_t(nH), c->stream);
+            dhf.upload(half_face, size_t(nH), c->stream);
+            dfa.upload(farg, size_t(nH), c->stream);
+            op->col.alloc(size_t(op->stored));
+            op->val64.alloc(size_t(op->stored) * 2);
+            op->val32.alloc(size_t(op->stored) * 2);
+            op->slice_ptr.alloc(size_t(ns) + 1);
+            op->slice_w.alloc(size_t(ns));
+            op->dinv64.alloc(size_t(nF));
+            op->dinv32.alloc(size_t(nF));
+            DevBuf<int> cnt(static_cast<size_t>(nF));
+            // padding rows of the last slice: zero values on a valid column
+            hipLaunchKernelGGL(k_fill_i32, dim3(nblk(op->stored)), dim3(256), 0, c->stream, size_t(op->stored), 0,
+                               op->col.p);
+            MK_HIP(hipMemsetAsync(op->val64.p, 0, size_t(op->stored) * 2 * sizeof(double), c->stream));
+            hipLaunchKernelGGL(k_slice_ptr, dim3(nblk(size_t(ns) + 1)), dim3(256), 0, c->stream, ns, int64_t(4),
+                               int64_t(C), op->slice_ptr.p, op->slice_w.p);
+            hipLaunchKernelGGL(k_field_laplacian, dim3(nblk(nF)), dim3(256), 0, c->stream, nF, C, dfh.p, dtw.p, dhf.p,
+                               dfa.p, N, op->col.p, op->val64.p, cnt.p);
+            hipLaunchKernelGGL(k_to_f32, dim3(nblk(size_t(op->stored) * 2)), dim3(256), 0, c->stream,
+                               size_t(op->stored) * 2, op->val64.p, op->val32.p);
+            hipLaunchKernelGGL(k_dinv_from_diag, dim3(nblk(nF)), dim3(256), 0, c->stream, nF, C, op->col.p,
+                               op->val64.p, op->dinv64.p, op->dinv32.p);
+            MK_HIP(hipGetLastError());
+            std::vector<int> hc(static_cast<size_t>(nF));
+            cnt.download(hc.data(), hc.size(), c->stream);
+            MK_HIP(hipStreamSynchronize(c->stream));
+            op->nnz = 0;
+            for (int v : hc) op->nnz += v;
+            const double vw = 2.0;
+            op->bytes64 = double(op->nnz) * (8.0 * vw + 4.0) + double(ns) * 12.0 + double(nF) * 8.0 * vw * 2.0;
+            op->bytes32 = double(op->nnz) * (4.0 * vw + 4.0) + double(ns) * 12.0 + double(nF) * 4.0 * vw * 2.0;
+        } catch (...) {
+            delete op;
+            throw;
+        }
+        *out = op;
+    });
+}
+
+mk_status mk_op_export_csr(mk_ctx* c, const mk_op* A, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val) {
+    if (!c || !A || !nnz) return MK_ERR_ARG;
+    return guard(c, [&] {
+        const int vw = A->cplx ? 2 : 1;
+        if (!A->h_ptr.empty()) {
+            *nnz = A->nnz;
+            if (rowptr) std::copy(A->h_ptr.begin(), A->h_ptr.end(), rowptr);
+            if (col) std::copy(A->h_col.begin(), A->h_col.end(), col);
+            if (val) std::copy(A->h_val.begin(), A->h_val.end(), val);
+            return;
+        }
+        // device-assembled SELL: compact, dropping zero padding
+        std::vector<int64_t> sptr(size_t(A->nslices) + 1);
+        std::vector<int> sw(size_t(A->nslices)), hc(size_t(A->stored));
+        std::vector<double> hv(size_t(A->stored) * vw);
+        A->slice_ptr.download(sptr.data(), sptr.size(), c->stream);
+        A->slice_w.download(sw.data(), sw.size(), c->stream);
+        A->col.download(hc.data(), hc.size(), c->stream);
+        A->val64.download(hv.data(), hv.size(), c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        std::vector<int64_t> rp(size_t(A->n) + 1, 0);
+        std::vector<int> cc;
+        std::vector<double> vv;
+        for (int r = 0; r < A->n; r++) {
+            const int s = r / A->C, lane = r % A->C;
+            for (int j = 0; j < sw[s]; j++) {
+                const int64_t e = sptr[s] + int64_t(j) * A->C + lane;
+                bool zero = true;
+                for (int q = 0; q < vw; q++) if (hv[e * vw + q] != 0.0) zero = false;
+                if (zero) continue;
+                cc.push_back(hc[e]);
+                for (int q = 0; q < vw; q++) vv.push_back(hv[e * vw + q]);
+            }
+            rp[r + 1] = int64_t(cc.size());
+        }
+        *nnz = rp[A->n];
+        if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
+        if (col) std::copy(cc.begin(), cc.end(), col);
+        if (val) std::copy(vv.begin(), vv.end(), val);
+    });
+}
+
+mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, const double* u0, int n_iter,
+                                const mk_solve_opts* o_, double* x_out, mk_stats* st) {
+    if (!c || !L || !mass || !u0 || !x_out || n_iter < 0 || !L->cplx) return MK_ERR_ARG;
+    mk_status inner = MK_OK;
+    mk_status s = guard(c, [&] {
+        const mk_solve_opts o = resolve(o_);
+        MK_HIP(hipSetDevice(c->device));
+        const double t0 = now_ms();
+        mk::stats_reset(c);
+        const int n = L->n;
+        const size_t len = size_t(n) * 2;
+        DevBuf<double> dm, du, dx, db, dLu;
+        dm.upload(mass, size_t(n), c->stream);
+        du.upload(u0, len, c->stream);
+        dx.alloc(len);
+        db.alloc(len);
+        dLu.alloc(len);
+        double* sc = c->scal.p;
+        int total_it = 0, total_ref = 0;
+        double last_rel = 0.0;
+        MK_HIP(hipMemcpyAsync(dx.p, du.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        for (int it = 0; it < n_iter; it++) {
+            // b = M u
+            mk::mass_mul(c, n, dm.p, du.p, db.p);
+            // warm start x = u * (u^H M u) / (u^H L u)  (Rayleigh quotient of the current iterate)
+            mk::spmv64(c, L, du.p, dLu.p, nullptr, mk::SPMV_DOT, sc + 10, nullptr);
+            mk::mass_dot(c, n, dm.p, du.p, sc + 11);
+            double h[2];
+            MK_HIP(hipMemcpyAsync(h, sc + 10, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            MK_HIP(hipStreamSynchronize(c->stream));
+            const double theta = h[0] / h[1];
+            MK_CHECK(theta == theta && theta > 0.0, MK_ERR_NAN, "Rayleigh quotient not positive");
+            MK_HIP(hipMemcpyAsync(dx.p, du.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            mk::axpby64(c, len, 0.0, dx.p, 1.0 / theta, dx.p);
+            mk::SolveResult r = mk::solve_spd(c, L, db.p, dx.p, o);
+            total_it += r.iters;
+            total_ref += r.refinements;
+            last_rel = r.rel_residual;
+            if (r.status != MK_OK) {
+                inner = r.status;
+                c->last_error = "inverse iteration " + std::to_string(it) + ": " + mk_status_str(r.status);
+                break;
+            }
+            // s = sqrt(|x^H M x|); x /= s; u = x
+            mk::mass_dot(c, n, dm.p, dx.p, sc + 12);
+            mk::scale_dev64(c, len, sc + 12, 1, dx.p);
+            MK_HIP(hipMemcpyAsync(du.p, dx.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        }
+        dx.download(x_out, len, c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->iterations = total_it;
+            st->refinements = total_ref;
+            st->solves = n_iter;
+            st->rel_residual = last_rel;
+            mk::stats_fill(c, st);
+            st->ms = now_ms() - t0;
+        }
+    });
+    return s != MK_OK ? s : inner;
+}
+
+}  // extern "C"

@@ -1,0 +1,599 @@
+// mk_iterative_rounding: the IGM solve of metriko (iter_rounding.h:33-113) on the GPU.
+//
+// Host: constraint kernel basis (FullPivLU restatement), UFull, X2F = G2*UFull and the
+// Poisson matrix E = length*X2F^T X2F (sparse products), control flow of the LM passes.
+// Device: every linear solve (PCG, FP64 or FP32 + FP64 refinement), the energies the
+// LM acceptance tests read, the fixed-row dot products and the FP64 rounding values.
+//
+// Each Levenberg-Marquardt pass of the reference runs with lambda = 0 on an affine
+// residual, so its direction is d = x* - x_prev with x* the minimiser of the quadratic
+// objective.  x* is computed in an exactly equivalent, well-conditioned form:
+//  * naive pass: f eliminated (diagonal Schur complement); x* = the KKT solution;
+//  * rounding passes: A = A0 + wK^2 U^T U, A0 = X2F^T X2F + wC^2 I fixed for the loop,
+//    x* = xt + Z (wK^-2 I + U Z)^{-1} (r - U xt), xt = A0^{-1} b0, Z = A0^{-1} U^T
+//    (Sherman-Morrison-Woodbury; no 1e10-weighted quantity is ever formed).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "mk_internal.h"
+#include "metriko/core/common/sparse.h"
+#include "metriko/core/solver/matrix_ops.h"
+
+using mk::DevBuf;
+using mk::Error;
+using metriko::Csr;
+using metriko::SprsD;
+using metriko::TripD;
+
+namespace {
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+inline unsigned nblk(size_t n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+
+// ---- device helpers ----------------------------------------------------------------------
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+// one block per sparse row: out[i] = row_i . x   (deterministic block reduction)
+__global__ void k_rowset_dot(int nrows, const int64_t* ptr, const int* col, const double* val, const double* x,
+                             double* out) {
+    const int i = blockIdx.x;
+    if (i >= nrows) return;
+    double s = 0.0;
+    for (int64_t k = ptr[i] + threadIdx.x; k < ptr[i + 1]; k += blockDim.x) s += val[k] * x[col[k]];
+    __shared__ double sh[4];
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < int(blockDim.x >> 6); w++) t += sh[w];
+        out[i] = t;
+    }
+}
+// y = 0; y[col] = val for row i
+__global__ void k_scatter_row(int64_t b, int64_t e, const int* col, const double* val, double* y) {
+    const int64_t k = b + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k < e) y[col[k]] = val[k];
+}
+__global__ void k_zero(size_t n, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = 0.0;
+}
+// y = x + sum_j c[j] * Z[:, j]   (Z column-major, k rows)
+__global__ void k_combine(size_t k, int r, const double* x, const double* Z, const double* c, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= k) return;
+    double s = x[i];
+    for (int j = 0; j < r; j++) s += c[j] * Z[size_t(j) * k + i];
+    y[i] = s;
+}
+// y = a*x + b*z
+__global__ void k_lin2(size_t n, double a, const double* x, double b, const double* z, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * z[i];
+}
+// partial sums of w * (s*x - t)^2 over n entries (t may be null -> 0); deterministic two-pass
+__global__ void k_sqdiff_partial(size_t n, double s, const double* x, const double* t, double* part) {
+    double acc = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x) {
+        const double d = s * x[i] - (t ? t[i] : 0.0);
+        acc += d * d;
+    }
+    __shared__ double sh[4];
+    acc = wsum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int w = 0; w < int(blockDim.x >> 6); w++) q += sh[w];
+        part[blockIdx.x] = q;
+    }
+}
+__global__ void k_sum_partials(int np, const double* part, double* out) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+    __shared__ double sh[4];
+    acc = wsum(acc);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double q = 0.0;
+        for (int w = 0; w < int(blockDim.x >> 6); w++) q += sh[w];
+        out[0] = q;
+    }
+}
+
+// ---- device sparse objects -----------------------------------------------------------------
+struct RowSet {  // a small set of sparse rows (CSR) kept on the device
+    int nrows = 0;
+    std::vector<int64_t> hptr{0};
+    std::vector<int> hcol;
+    std::vector<double> hval;
+    DevBuf<int64_t> ptr;
+    DevBuf<int> col;
+    DevBuf<double> val;
+    void push(const SprsD& M, int row) {
+        for (int64_t k = M.ptr[row]; k < M.ptr[row + 1]; k++) { hcol.push_back(M.col[k]); hval.push_back(M.val[k]); }
+        hptr.push_back(int64_t(hcol.size()));
+        nrows++;
+    }
+    void sync(hipStream_t s) {
+        ptr.upload(hptr.data(), hptr.size(), s);
+        col.upload(hcol.data(), std::max<size_t>(hcol.size(), 1), s);
+        val.upload(hval.data(), std::max<size_t>(hval.size(), 1), s);
+    }
+};
+
+struct Ctx {
+    mk_ctx* c;
+    mk_solve_opts o;
+    mk_stats acc{};
+    DevBuf<double> part;
+    explicit Ctx(mk_ctx* c_, const mk_solve_opts& o_) : c(c_), o(o_), part(4096) {}
+    hipStream_t st() const { return c->stream; }
+    void sync() { MK_HIP(hipStreamSynchronize(c->stream)); }
+    // ||s*x - t||^2 on the device -> host
+    double sqdiff(size_t n, double s, const double* x, const double* t) {
+        const unsigned g = std::min<unsigned>(std::max(1u, nblk(n)), 1024u);
+        hipLaunchKernelGGL(k_sqdiff_partial, dim3(g), dim3(256), 0, st(), n, s, x, t, part.p);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, st(), int(g), part.p, part.p + 2048);
+        double h = 0.0;
+        MK_HIP(hipMemcpyAsync(&h, part.p + 2048, sizeof(double), hipMemcpyDeviceToHost, st()));
+        sync();
+        return h;
+    }
+    void rowdots(const RowSet& R, const double* x, double* out_dev) {
+        if (R.nrows == 0) return;
+        hipLaunchKernelGGL(k_rowset_dot, dim3(R.nrows), dim3(256), 0, st(), R.nrows, R.ptr.p, R.col.p, R.val.p, x,
+                           out_dev);
+    }
+    mk::SolveResult solve(const mk_op* A, const double* b, double* x) {
+        mk::SolveResult r = mk::solve_spd(c, A, b, x, o);
+        acc.iterations += r.iters;
+        acc.refinements += r.refinements;
+        acc.solves += 1;
+        acc.rel_residual = r.rel_residual;
+        if (r.status != MK_OK) throw Error(r.status, "IGM solve failed: " + std::string(mk_status_str(r.status)));
+        return r;
+    }
+};
+
+mk_op* make_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
+    mk_op* op = nullptr;
+    mk_status s = mk_op_create_real(c, A.rows(), A.ptr.data(), A.col.data(), A.val.data(), &o, &op);
+    if (s != MK_OK) throw Error(s, "operator upload failed: " + c->last_error);
+    return op;
+}
+struct OpHolder {
+    mk_op* op = nullptr;
+    ~OpHolder() { if (op) mk_op_destroy(op); }
+};
+
+// dense solve with partial pivoting (small r x r)
+std::vector<double> dense_solve(std::vector<double> M, std::vector<double> b, int r) {
+    std::vector<int> piv(static_cast<size_t>(r));
+    for (int k = 0; k < r; k++) {
+        int p = k;
+        for (int i = k + 1; i < r; i++) if (std::abs(M[size_t(i * r + k)]) > std::abs(M[size_t(p * r + k)])) p = i;
+        if (M[size_t(p * r + k)] == 0.0) throw Error(MK_ERR_SINGULAR, "rounding: singular penalty system");
+        if (p != k) { for (int j = 0; j < r; j++) std::swap(M[size_t(k * r + j)], M[size_t(p * r + j)]); std::swap(b[k], b[p]); }
+        for (int i = k + 1; i < r; i++) {
+            const double l = M[size_t(i * r + k)] / M[size_t(k * r + k)];
+            if (l == 0.0) continue;
+            for (int j = k; j < r; j++) M[size_t(i * r + j)] -= l * M[size_t(k * r + j)];
+            b[i] -= l * b[k];
+        }
+    }
+    std::vector<double> x(static_cast<size_t>(r));
+    for (int i = r - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < r; j++) s -= M[size_t(i * r + j)] * x[j];
+        x[i] = s / M[size_t(i * r + i)];
+    }
+    return x;
+}
+
+// LMSolver::solve (levenberg_marquardt.h:140-231) with lambda = 0, verbose = false.
+// dir(prevx) writes the direction into d (device); energy(x) returns ||E(x)||^2.
+// Returns 1/0 like the reference; x_out receives the solution (device, may alias prevx buffer).
+template <typename Dir, typename Energy>
+bool lm_solve(Ctx& K, size_t n, double* x0, double* x_out, int maxIter, double funcTol, Dir&& dir, Energy&& energy,
+              DevBuf<double>& prevx, DevBuf<double>& d, DevBuf<double>& trial) {
+    prevx.ensure(n); d.ensure(n); trial.ensure(n);
+    MK_HIP(hipMemcpyAsync(prevx.p, x0, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+    int it = 0;
+    bool ret = false;
+    for (;;) {
+        dir(prevx.p, d.p);
+        const double dn = std::sqrt(K.sqdiff(n, 1.0, d.p, nullptr));
+        if (dn < funcTol) { ret = true; break; }
+        const double e_prev = energy(prevx.p);
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(n)), dim3(256), 0, K.st(), n, 1.0, prevx.p, 1.0, d.p, trial.p);
+        const double e_new = energy(trial.p);
+        if (e_prev > e_new) {
+            MK_HIP(hipMemcpyAsync(prevx.p, trial.p, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+            if (std::abs(e_prev - e_new) < funcTol) { ret = false; break; }
+        }
+        it++;
+        if (!(it <= maxIter)) { ret = false; break; }
+    }
+    MK_HIP(hipMemcpyAsync(x_out, prevx.p, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+    return ret;
+}
+
+}  // namespace
+
+extern "C" mk_status mk_iterative_rounding(
+    mk_ctx* c, int nfixed, const int32_t* fixedIdcs, const double* fixedVals, int nsing, const int32_t* singularIdcs,
+    int nint, const int32_t* integerIdcs, double length, int c_rows, int c_cols, const int64_t* c_ptr,
+    const int32_t* c_col, const double* c_val, int g_rows, int g_cols, const int64_t* g_ptr, const int32_t* g_col,
+    const double* g_val, int N, int n, int nF, const mk_igm_flags* flags, const double* F2, const mk_solve_opts* o_,
+    double* fullx, int* success, int max_dec, int32_t* dec_index, int32_t* dec_value, double* dec_margin, int* ndec,
+    mk_igm_stats* stats) {
+    if (!c || !flags || !F2 || !fullx || !success || !g_ptr || g_rows != 2 * N * nF || c_cols != g_cols) return MK_ERR_ARG;
+    if (flags->local_injectivity) return MK_ERR_UNSUPPORTED;
+    try {
+        mk_solve_opts o;
+        mk_default_opts(&o);
+        if (o_) o = *o_;
+        MK_HIP(hipSetDevice(c->device));
+        mk::stats_reset(c);
+        Ctx K(c, o);
+        mk_igm_stats S{};
+        const double t_all = now_ms();
+        const int nfull = g_cols;
+        for (int i = 0; i < nfixed; i++) MK_CHECK(fixedIdcs[i] >= 0 && fixedIdcs[i] < nfull, MK_ERR_ARG, "fixedIdcs");
+        for (int i = 0; i < nsing; i++) MK_CHECK(singularIdcs[i] >= 0 && singularIdcs[i] < nfull, MK_ERR_ARG, "singularIdcs");
+        for (int i = 0; i < nint; i++) MK_CHECK(integerIdcs[i] >= 0 && integerIdcs[i] < nfull, MK_ERR_ARG, "integerIdcs");
+        // ---- host setup (iter_rounding_init.h:195-251) ----
+        SprsD G2(g_rows, g_cols);
+        G2.ptr.assign(g_ptr, g_ptr + g_rows + 1);
+        G2.col.assign(g_col, g_col + g_ptr[g_rows]);
+        G2.val.assign(g_val, g_val + g_ptr[g_rows]);
+        for (int64_t k = 0; k < G2.nonZeros(); k++) MK_CHECK(G2.col[k] >= 0 && G2.col[k] < g_cols, MK_ERR_ARG, "G2 column");
+        SprsD Cf(c_rows, c_cols);
+        if (c_rows > 0) {
+            Cf.ptr.assign(c_ptr, c_ptr + c_rows + 1);
+            Cf.col.assign(c_col, c_col + c_ptr[c_rows]);
+            Cf.val.assign(c_val, c_val + c_ptr[c_rows]);
+            Cf.prune();  // structural zeros from the products carry no constraint
+        }
+        std::vector<int> uniqueJ;
+        for (int64_t k = 0; k < Cf.nonZeros(); k++) uniqueJ.push_back(Cf.col[k]);
+        std::sort(uniqueJ.begin(), uniqueJ.end());
+        uniqueJ.erase(std::unique(uniqueJ.begin(), uniqueJ.end()), uniqueJ.end());
+        std::vector<int> jmask(size_t(nfull), -1);
+        for (size_t i = 0; i < uniqueJ.size(); i++) jmask[uniqueJ[i]] = int(i);
+        std::vector<int> nonPart;
+        metriko::MatXd US;
+        int crank = 0;
+        if (c_rows != 0) {
+            for (int i = 0; i < nfull; i++) if (jmask[i] < 0) nonPart.push_back(i);
+            metriko::MatXd CS = metriko::MatXd::Zero(c_rows, int(uniqueJ.size()));
+            for (int i = 0; i < c_rows; i++)
+                for (int64_t k = Cf.ptr[i]; k < Cf.ptr[i + 1]; k++) CS(i, jmask[Cf.col[k]]) = Cf.val[k];
+            US = metriko::fullpiv_lu_kernel(CS, &crank);
+        } else {
+            nonPart.resize(size_t(nfull));
+            std::iota(nonPart.begin(), nonPart.end(), 0);
+            uniqueJ.clear();
+        }
+        const int npn = int(nonPart.size());
+        const int dimker = US.rows() ? US.cols() : 0;
+        const int k = npn + dimker;
+        // UFull = permMat * blockdiag(I, USmall): row nonPart[i] -> e_i, row uniqueJ[r] -> USmall row r
+        SprsD UF(nfull, k);
+        {
+            std::vector<TripD> T;
+            T.reserve(size_t(npn) + size_t(US.rows()) * size_t(dimker));
+            for (int i = 0; i < npn; i++) T.emplace_back(nonPart[i], i, 1.0);
+            for (int r = 0; r < US.rows(); r++)
+                for (int j = 0; j < dimker; j++)
+                    if (US(r, j) != 0.0) T.emplace_back(uniqueJ[r], npn + j, US(r, j));
+            UF.setFromTriplets(T);
+        }
+        SprsD X2F = (G2 * UF).pruned();      // G2 * UFull (naive, without length)
+        SprsD X2Ft = X2F.transpose();
+        SprsD E = X2Ft * X2F;
+        for (auto& v : E.val) v *= length;   // E = X2F^T X2F * length
+        std::vector<double> fr(static_cast<size_t>(k));
+        X2Ft.multiply(F2, fr.data());        // f = X2F^T F2
+        std::vector<int> fixIdx(fixedIdcs, fixedIdcs + nfixed);
+        SprsD Cm = UF.rows_of(fixIdx);       // constMat = UFull rows at fixedIdcs
+        S.k = k;
+        S.kernel_rank = crank;
+        S.setup_ms = now_ms() - t_all;
+
+        // ---- Poisson KKT [[E, C^T],[C, 0]] [x; l] = [f; g] (iter_rounding_init.h:270-307) ----
+        // f lies in range(E) and C only pins the translations E annihilates, so the
+        // multiplier vanishes and x solves the SPD system (E + w C^T C) x = f + w C^T g.
+        double t0 = now_ms();
+        double dmean = 0.0;
+        for (int i = 0; i < k; i++)
+            for (int64_t q = E.ptr[i]; q < E.ptr[i + 1]; q++) if (E.col[q] == i) dmean += E.val[q];
+        dmean = k ? dmean / k : 1.0;
+        if (!(dmean > 0.0)) dmean = 1.0;
+        SprsD Eaug = E + (Cm.transpose() * Cm).scaled(dmean);
+        std::vector<double> rhs(fr);
+        {
+            std::vector<double> ctg(size_t(k), 0.0);
+            for (int i = 0; i < Cm.rows(); i++)
+                for (int64_t q = Cm.ptr[i]; q < Cm.ptr[i + 1]; q++) ctg[Cm.col[q]] += Cm.val[q] * fixedVals[i];
+            for (int i = 0; i < k; i++) rhs[i] += dmean * ctg[i];
+        }
+        OpHolder hE, hA0, hX, hXt;
+        hE.op = make_op(c, Eaug, o);
+        DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
+        drhs.upload(rhs.data(), rhs.size(), K.st());
+        MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));
+        K.solve(hE.op, drhs.p, dXS.p);
+        std::vector<double> XSmall(static_cast<size_t>(k));
+        dXS.download(XSmall.data(), XSmall.size(), K.st());
+        K.sync();
+        {   // KKT residual (host, FP64): ||E x - f|| + ||C x - g|| relative to ||f||
+            std::vector<double> Ex(static_cast<size_t>(k));
+            E.multiply(XSmall.data(), Ex.data());
+            double rn = 0, fn = 0;
+            for (int i = 0; i < k; i++) { rn += (Ex[i] - fr[i]) * (Ex[i] - fr[i]); fn += fr[i] * fr[i]; }
+            for (int i = 0; i < Cm.rows(); i++) {
+                double s = -fixedVals[i];
+                for (int64_t q = Cm.ptr[i]; q < Cm.ptr[i + 1]; q++) s += Cm.val[q] * XSmall[Cm.col[q]];
+                rn += s * s;
+            }
+            S.kkt_rel_residual = fn > 0 ? std::sqrt(rn / fn) : std::sqrt(rn);
+        }
+        S.kkt_ms = now_ms() - t0;
+        std::vector<double> x0(static_cast<size_t>(nfull));
+        UF.multiply(XSmall.data(), x0.data());
+
+        *success = 1;
+        int nd = 0;
+        auto record = [&](int idx, int val, double margin) {
+            if (nd < max_dec) {
+                if (dec_index) dec_index[nd] = idx;
+                if (dec_value) dec_value[nd] = val;
+                if (dec_margin) dec_margin[nd] = margin;
+            }
+            nd++;
+        };
+        if (!flags->seamless) {
+            std::copy(x0.begin(), x0.end(), fullx);
+        } else {
+            // ---- naive LM over (x, f) (iter_rounding.h:82-86), weights iter_rounding_init.h:184 ----
+            t0 = now_ms();
+            const double wIn = 10e3, wCn = 1.0;
+            const size_t l1 = size_t(g_rows);
+            hX.op = nullptr;
+            // B x = length * X2F x on the host-built operator (device)
+            SprsD Bm = X2F;  // apply length separately
+            {
+                mk_solve_opts orect = o;
+                (void)orect;
+            }
+            // device vectors
+            DevBuf<double> dF2, dBx, dfstar, dxf(size_t(k) + l1), dtmp(l1);
+            dF2.upload(F2, l1, K.st());
+            // B * XSmall on the host (one product) and f* = (wI^2 B x + wC^2 F2) / (wI^2 + wC^2)
+            std::vector<double> Bx(l1);
+            X2F.multiply(XSmall.data(), Bx.data());
+            for (auto& v : Bx) v *= length;
+            std::vector<double> fstar(l1);
+            for (size_t i = 0; i < l1; i++) fstar[i] = (wIn * wIn * Bx[i] + wCn * wCn * F2[i]) / (wIn * wIn + wCn * wCn);
+            dBx.upload(Bx.data(), l1, K.st());
+            dfstar.upload(fstar.data(), l1, K.st());
+            // x = (XSmall, F2); the x-block of every direction is x* - x = 0, so the pass
+            // only moves f: direction (0, f* - f), energies from the f-block plus the
+            // constant x-terms (fixed rows of XSmall).
+            double fixres = 0.0;
+            for (int i = 0; i < Cm.rows(); i++) {
+                double s = -fixedVals[i];
+                for (int64_t q = Cm.ptr[i]; q < Cm.ptr[i + 1]; q++) s += Cm.val[q] * XSmall[Cm.col[q]];
+                fixres += s * s;
+            }
+            DevBuf<double> fcur, pv, dd, tr;
+            fcur.upload(F2, l1, K.st());
+            auto dir = [&](const double* prev, double* d) {
+                hipLaunchKernelGGL(k_lin2, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, 1.0, dfstar.p, -1.0, prev, d);
+            };
+            auto energy = [&](const double* f) {
+                // wI^2 ||f - Bx||^2 + wC^2 ||f - F2||^2 + wK^2 ||C x - g||^2
+                const double a = K.sqdiff(l1, 1.0, f, dBx.p);
+                const double b = K.sqdiff(l1, 1.0, f, dF2.p);
+                return wIn * wIn * a + wCn * wCn * b + wIn * wIn * fixres;
+            };
+            DevBuf<double> F2_(l1);
+            lm_solve(K, l1, fcur.p, F2_.p, 100, 10e-10, dir, energy, pv, dd, tr);
+            S.naive_ms = now_ms() - t0;
+
+            // ---- seamless rounding loop (iter_rounding.h:89-112, iter_rounding_loop.h) ----
+            t0 = now_ms();
+            const double wI = 1.0, wK = 10e4, wC = 0.01;
+            // A0 = wI^2 X2F^T X2F + wC^2 I with X2F = length * G2 * UFull  ->  length * E + wC^2 I
+            SprsD A0 = E;
+            for (auto& v : A0.val) v *= length * wI * wI;
+            A0 = A0 + SprsD::identity(k, wC * wC);
+            hA0.op = make_op(c, A0, o);
+            // b_int = wI^2 X2F^T F2_  (F2_ downloaded once; X2F^T product on the host)
+            std::vector<double> hF2_(l1), bint(static_cast<size_t>(k));
+            F2_.download(hF2_.data(), l1, K.st());
+            K.sync();
+            X2Ft.multiply(hF2_.data(), bint.data());
+            for (auto& v : bint) v *= length * wI * wI;
+            // device operator for field = X2F x (energies): upload X2F rows as a RowSet-like CSR op
+            DevBuf<int64_t> xptr;
+            DevBuf<int> xcol;
+            DevBuf<double> xval;
+            xptr.upload(X2F.ptr.data(), X2F.ptr.size(), K.st());
+            xcol.upload(X2F.col.data(), std::max<size_t>(X2F.col.size(), 1), K.st());
+            xval.upload(X2F.val.data(), std::max<size_t>(X2F.val.size(), 1), K.st());
+            DevBuf<double> dfield(l1), dbint, dxcur(static_cast<size_t>(k)), dxprev(static_cast<size_t>(k)), dxt(static_cast<size_t>(k)), dxstar(static_cast<size_t>(k)),
+                dxlm(static_cast<size_t>(k)), db0(static_cast<size_t>(k)), drow(static_cast<size_t>(k)), dcand, dfixv, dcoef;
+            dbint.upload(bint.data(), bint.size(), K.st());
+            dxcur.upload(XSmall.data(), size_t(k), K.st());
+            MK_HIP(hipMemsetAsync(dxt.p, 0, size_t(k) * sizeof(double), K.st()));
+            // candidate rows of UFull (all indices the loop can fix)
+            std::vector<int> singular(singularIdcs, singularIdcs + nsing), integer(integerIdcs, integerIdcs + nint);
+            RowSet cand;
+            std::vector<int> candIdx;
+            std::vector<int> candPos(size_t(nfull), -1);
+            for (int i : singular) if (candPos[i] < 0) { candPos[i] = int(candIdx.size()); candIdx.push_back(i); cand.push(UF, i); }
+            for (int i : integer) if (candPos[i] < 0) { candPos[i] = int(candIdx.size()); candIdx.push_back(i); cand.push(UF, i); }
+            cand.sync(K.st());
+            dcand.alloc(std::max<size_t>(candIdx.size(), 1));
+            RowSet fixedRows;
+            std::vector<int> fixedI;
+            std::vector<double> fixedV;
+            const size_t ncand = std::max<size_t>(candIdx.size(), 1);
+            DevBuf<double> Zall(size_t(k) * ncand);       // Woodbury columns z_j = A0^{-1} u_j
+            std::vector<std::vector<double>> candDots;    // u_c . z_j for every candidate c
+            std::vector<double> UZ;  // r x r, row-major, UZ[i][j] = u_i . z_j
+            std::vector<int> left = flags->round_seams ? integer : singular;
+            bool roundedSingulars = false, rounded = false;
+            std::vector<double> Xc(candIdx.size());
+            auto spmv_field = [&](const double* x, double* y) {
+                // y = length * X2F x   (CSR row kernel, one block per 256 rows would be better; use rowset kernel)
+                hipLaunchKernelGGL(k_rowset_dot, dim3(unsigned(l1)), dim3(64), 0, K.st(), int(l1), xptr.p, xcol.p,
+                                   xval.p, x, y);
+            };
+            std::vector<double> dotbuf;
+            DevBuf<double> ddots;
+            auto fixed_dots = [&](const double* x, std::vector<double>& out) {
+                out.assign(size_t(fixedRows.nrows), 0.0);
+                if (!fixedRows.nrows) return;
+                ddots.ensure(size_t(fixedRows.nrows));
+                K.rowdots(fixedRows, x, ddots.p);
+                ddots.download(out.data(), out.size(), K.st());
+                K.sync();
+            };
+            int lm_rounds = 0, rounds = 0;
+            DevBuf<double> pv2, dd2, tr2;
+            while (!left.empty()) {
+                // initFixedIndices (iter_rounding_loop.h:29-74)
+                MK_HIP(hipMemcpyAsync(dxprev.p, dxcur.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+                K.rowdots(cand, dxcur.p, dcand.p);
+                dcand.download(Xc.data(), Xc.size(), K.st());
+                K.sync();
+                double minDif = 3276700.0;
+                int minIdx = -1;
+                for (size_t i = 0; i < left.size(); i++) {
+                    const double v = Xc[size_t(candPos[left[i]])];
+                    const double dd = std::fabs(v - std::round(v));
+                    if (dd < minDif) { minIdx = int(i); minDif = dd; }
+                }
+                const int pick = left[size_t(minIdx)];
+                const double Xp = Xc[size_t(candPos[pick])];
+                fixedI.push_back(pick);
+                fixedV.push_back(double(int(std::round(Xp))));
+                record(pick, int(std::round(Xp)), minDif);
+                left.erase(left.begin() + minIdx);
+                fixedRows.push(UF, pick);
+                fixedRows.sync(K.st());
+                // new Woodbury column z = A0^{-1} u_pick
+                {
+                    const int j = int(fixedI.size()) - 1;
+                    double* z = Zall.p + size_t(j) * size_t(k);
+                    hipLaunchKernelGGL(k_zero, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), drow.p);
+                    const int cp = candPos[pick];
+                    const int64_t b = cand.hptr[size_t(cp)], e = cand.hptr[size_t(cp) + 1];
+                    if (e > b)
+                        hipLaunchKernelGGL(k_scatter_row, dim3(nblk(size_t(e - b))), dim3(256), 0, K.st(), b, e,
+                                           cand.col.p, cand.val.p, drow.p);
+                    MK_HIP(hipMemsetAsync(z, 0, size_t(k) * sizeof(double), K.st()));
+                    K.solve(hA0.op, drow.p, z);
+                    std::vector<double> cd(candIdx.size());
+                    K.rowdots(cand, z, dcand.p);
+                    dcand.download(cd.data(), cd.size(), K.st());
+                    K.sync();
+                    candDots.push_back(std::move(cd));
+                    const int r = j + 1;
+                    std::vector<double> nz(size_t(r) * size_t(r), 0.0);
+                    for (int a = 0; a < r; a++)
+                        for (int bq = 0; bq < r; bq++)
+                            nz[size_t(a * r + bq)] = candDots[size_t(bq)][size_t(candPos[fixedI[size_t(a)]])];
+                    UZ.swap(nz);
+                }
+                if (left.empty() && !flags->round_seams && !roundedSingulars) {
+                    left = integer;
+                    roundedSingulars = true;
+                }
+                rounds++;
+                if (!(minDif > 10e-7)) continue;
+                rounded = true;
+                lm_rounds++;
+                const int r = int(fixedI.size());
+                // xt = A0^{-1} (b_int + wC^2 xPrev)  (warm start: previous xt)
+                hipLaunchKernelGGL(k_lin2, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), 1.0, dbint.p,
+                                   wC * wC, dxprev.p, db0.p);
+                K.solve(hA0.op, db0.p, dxt.p);
+                std::vector<double> Uxt;
+                fixed_dots(dxt.p, Uxt);
+                std::vector<double> M(size_t(r) * size_t(r)), rhs2(static_cast<size_t>(r));
+                for (int i = 0; i < r; i++) {
+                    for (int j = 0; j < r; j++) M[size_t(i * r + j)] = UZ[size_t(i * r + j)];
+                    M[size_t(i * r + i)] += 1.0 / (wK * wK);
+                    rhs2[size_t(i)] = fixedV[size_t(i)] - Uxt[size_t(i)];
+                }
+                std::vector<double> cc = dense_solve(M, rhs2, r);
+                // x* = xt + Z c
+                dcoef.upload(cc.data(), cc.size(), K.st());
+                hipLaunchKernelGGL(k_combine, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), r, dxt.p, Zall.p,
+                                   dcoef.p, dxstar.p);
+                std::vector<double> fr_(fixedV);
+                auto dir = [&](const double* prev, double* d) {
+                    hipLaunchKernelGGL(k_lin2, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), 1.0, dxstar.p,
+                                       -1.0, prev, d);
+                };
+                auto energy = [&](const double* x) {
+                    spmv_field(x, dfield.p);
+                    const double a = K.sqdiff(l1, length, dfield.p, F2_.p);
+                    const double b = K.sqdiff(size_t(k), 1.0, x, dxprev.p);
+                    std::vector<double> ux;
+                    fixed_dots(x, ux);
+                    double q = 0.0;
+                    for (int i = 0; i < r; i++) q += (ux[size_t(i)] - fr_[size_t(i)]) * (ux[size_t(i)] - fr_[size_t(i)]);
+                    return wI * wI * a + wC * wC * b + wK * wK * q;
+                };
+                lm_solve(K, size_t(k), dxcur.p, dxlm.p, 100, 1e-7, dir, energy, pv2, dd2, tr2);
+                // post_checking (iter_rounding_loop.h:89-100)
+                std::vector<double> ux;
+                fixed_dots(dxlm.p, ux);
+                double mx = -1.0;
+                for (int i = 0; i < r; i++) mx = std::max(mx, std::abs(ux[size_t(i)] - std::round(ux[size_t(i)])));
+                MK_HIP(hipMemcpyAsync(dxcur.p, dxlm.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+                if (!(mx <= 10e-7)) { *success = 0; break; }
+            }
+            S.rounds = rounds;
+            S.lm_rounds = lm_rounds;
+            S.n_fixed = int(fixedI.size());
+            if (rounded) {
+                std::vector<double> xs(static_cast<size_t>(k));
+                dxcur.download(xs.data(), xs.size(), K.st());
+                K.sync();
+                UF.multiply(xs.data(), fullx);
+            } else {
+                std::copy(x0.begin(), x0.end(), fullx);
+            }
+            S.rounding_ms = now_ms() - t0;
+        }
+        if (ndec) *ndec = nd;
+        K.sync();
+        S.solve = K.acc;
+        mk::stats_fill(c, &S.solve);
+        S.solve.ms = now_ms() - t_all;
+        if (stats) *stats = S;
+        return MK_OK;
+    } catch (const Error& e) {
+        c->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        c->last_error = e.what();
+        return MK_ERR_ARG;
+    }
+}

@@ -1,0 +1,671 @@
+// Sparse kernels and the PCG / refinement solvers of the metriko HIP library (gfx950).
+//
+// Layout: SELL-C-sigma (slice height C = 64 = one wavefront by default): entries of
+// slice s are stored column-major (val[slice_ptr[s] + j*C + lane]) so every load of
+// a wavefront is one contiguous, coalesced 64-lane access.  Rows may be sorted by
+// length inside windows of sigma rows (perm maps a stored row to the original row).
+//
+// Reductions are deterministic: each block reduces with xor-butterfly shuffles
+// (DPP / ds_swizzle on CDNA) and LDS, writes one partial per block, and the last
+// block to arrive (atomic ticket) sums the partials in a fixed order.  Reruns are
+// therefore bit-identical.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "mk_internal.h"
+
+namespace mk {
+
+static constexpr int kMaxWaves = 16;        // 1024 threads / 64
+static constexpr size_t kMaxParts = 1 << 17;  // partial-sum slots per reduced quantity
+static constexpr int kMaxK = 4;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Block-reduce K values; thread 0 gets the block totals.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K]) {
+    __shared__ double sh[K][kMaxWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) sh[k][wid] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += sh[k][w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Writes the block partials and returns true in the last block to arrive, with the
+// grand totals in tot (valid in all threads of that block).
+template <int K>
+__device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
+    __shared__ int is_last;
+    __shared__ double shtot[K];
+    block_sum<K>(v);
+    const unsigned G = gridDim.x;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) partials[k * kMaxParts + blockIdx.x] = v[k];
+        __threadfence();
+        unsigned t = atomicAdd(counter, 1u);
+        is_last = (t == G - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        acc[k] = 0.0;
+        const volatile double* pk = partials + k * kMaxParts;
+        for (unsigned i = threadIdx.x; i < G; i += blockDim.x) acc[k] += pk[i];
+    }
+    block_sum<K>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) shtot[k] = acc[k];
+        *counter = 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) tot[k] = shtot[k];
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
+// SELL SpMV.  T = float/double; CPLX: values/vectors interleaved (re, im).
+// MODE 0: y = A x; 1: y = A x, out[k] += Re(x^H y); 2: y = b - A x, out += |y|^2.
+template <typename T, bool CPLX, int MODE>
+__global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __restrict__ slice_ptr,
+                                               const int* __restrict__ slice_w, const int* __restrict__ col,
+                                               const T* __restrict__ val, const int* __restrict__ perm,
+                                               const T* __restrict__ x, T* __restrict__ y,
+                                               const T* __restrict__ b, double* partials, unsigned* counter,
+                                               double* out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double contrib = 0.0;
+    if (r < n) {
+        const int s = r / C, lane = r - s * C;
+        const int64_t base = slice_ptr[s] + lane;
+        const int w = slice_w[s];
+        const int row = perm ? perm[r] : r;
+        if constexpr (CPLX) {
+            T ar = 0, ai = 0;
+            for (int j = 0; j < w; j++) {
+                const int64_t e = base + int64_t(j) * C;
+                const int c = col[e];
+                const T vr = val[2 * e], vi = val[2 * e + 1];
+                const T xr = x[2 * c], xi = x[2 * c + 1];
+                ar += vr * xr - vi * xi;
+                ai += vr * xi + vi * xr;
+            }
+            if constexpr (MODE == 2) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
+            y[2 * row] = ar;
+            y[2 * row + 1] = ai;
+            if constexpr (MODE == 1) contrib = double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
+            if constexpr (MODE == 2) contrib = double(ar) * double(ar) + double(ai) * double(ai);
+        } else {
+            T a = 0;
+            for (int j = 0; j < w; j++) {
+                const int64_t e = base + int64_t(j) * C;
+                a += val[e] * x[col[e]];
+            }
+            if constexpr (MODE == 2) a = b[row] - a;
+            y[row] = a;
+            if constexpr (MODE == 1) contrib = double(x[row]) * double(a);
+            if constexpr (MODE == 2) contrib = double(a) * double(a);
+        }
+    }
+    if constexpr (MODE != 0) {
+        double v[1] = {contrib}, tot[1];
+        if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// CG kernels.  Vectors are real arrays of length len = n (real) or 2n (complex);
+// element e of a complex vector occupies entries 2e, 2e+1; the Jacobi scaling dinv
+// is per element (real).
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_init(int n, const T* __restrict__ r, const T* __restrict__ dinv,
+                                             T* __restrict__ z, T* __restrict__ p, CgState* st, double* partials,
+                                             unsigned* counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < n) {
+        const T d = dinv ? dinv[i] : T(1);
+        if constexpr (CPLX) {
+            const T r0 = r[2 * i], r1 = r[2 * i + 1];
+            const T z0 = d * r0, z1 = d * r1;
+            z[2 * i] = z0; z[2 * i + 1] = z1; p[2 * i] = z0; p[2 * i + 1] = z1;
+            rz = double(r0) * z0 + double(r1) * z1;
+            rr = double(r0) * r0 + double(r1) * r1;
+        } else {
+            const T r0 = r[i], z0 = d * r0;
+            z[i] = z0; p[i] = z0;
+            rz = double(r0) * z0;
+            rr = double(r0) * r0;
+        }
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it = 0;
+        st->status = MK_OK;
+        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
+        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
+    }
+}
+
+// q = A p ; pq = p^H q ; alpha = rz / pq
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_spmv(int n, int C, const int64_t* __restrict__ slice_ptr,
+                                             const int* __restrict__ slice_w, const int* __restrict__ col,
+                                             const T* __restrict__ val, const int* __restrict__ perm,
+                                             const T* __restrict__ p, T* __restrict__ q, CgState* st,
+                                             double* partials, unsigned* counter) {
+    if (st->done) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    double contrib = 0.0;
+    if (r < n) {
+        const int s = r / C, lane = r - s * C;
+        const int64_t base = slice_ptr[s] + lane;
+        const int w = slice_w[s];
+        const int row = perm ? perm[r] : r;
+        if constexpr (CPLX) {
+            T ar = 0, ai = 0;
+            for (int j = 0; j < w; j++) {
+                const int64_t e = base + int64_t(j) * C;
+                const int c = col[e];
+                const T vr = val[2 * e], vi = val[2 * e + 1];
+                const T xr = p[2 * c], xi = p[2 * c + 1];
+                ar += vr * xr - vi * xi;
+                ai += vr * xi + vi * xr;
+            }
+            q[2 * row] = ar;
+            q[2 * row + 1] = ai;
+            contrib = double(p[2 * row]) * double(ar) + double(p[2 * row + 1]) * double(ai);
+        } else {
+            T a = 0;
+            for (int j = 0; j < w; j++) {
+                const int64_t e = base + int64_t(j) * C;
+                a += val[e] * p[col[e]];
+            }
+            q[row] = a;
+            contrib = double(p[row]) * double(a);
+        }
+    }
+    double v[1] = {contrib}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->pq = tot[0];
+        if (!(tot[0] > 0.0)) {
+            st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
+            st->done = 1;
+        } else {
+            st->alpha = st->rz / tot[0];
+        }
+    }
+}
+
+// x += alpha p ; r -= alpha q ; z = dinv r ; rz, rr ; beta
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                                               const T* __restrict__ p, const T* __restrict__ q,
+                                               const T* __restrict__ dinv, CgState* st, double* partials,
+                                               unsigned* counter) {
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < n) {
+        const T d = dinv ? dinv[i] : T(1);
+        if constexpr (CPLX) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int k = 2 * i + c;
+                x[k] += alpha * p[k];
+                const T rk = r[k] - alpha * q[k];
+                r[k] = rk;
+                const T zk = d * rk;
+                z[k] = zk;
+                rz += double(rk) * zk;
+                rr += double(rk) * rk;
+            }
+        } else {
+            x[i] += alpha * p[i];
+            const T rk = r[i] - alpha * q[i];
+            r[i] = rk;
+            const T zk = d * rk;
+            z[i] = zk;
+            rz = double(rk) * zk;
+            rr = double(rk) * rk;
+        }
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->beta = tot[0] / st->rz;
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it += 1;
+        if (!(tot[1] == tot[1]) || !(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (tot[1] <= st->tol2 * st->bb) st->done = 1;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) cg_pupdate(size_t len, T* __restrict__ p, const T* __restrict__ z,
+                                                const CgState* st) {
+    if (st->done) return;
+    const T beta = T(st->beta);
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) p[i] = z[i] + beta * p[i];
+}
+
+// ---------------------------------------------------------------------------------------
+// misc vector kernels
+__global__ void k_axpby(size_t len, double a, const double* x, double b, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = a * x[i] + b * y[i];
+}
+__global__ void k_scale_dev(size_t len, const double* s, int invert_sqrt, double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double f = invert_sqrt ? 1.0 / sqrt(fabs(s[0])) : s[0];
+    if (i < len) y[i] *= f;
+}
+__global__ void k_mass_mul(int n, const double* mass, const double* u, double* b) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { b[2 * i] = mass[i] * u[2 * i]; b[2 * i + 1] = mass[i] * u[2 * i + 1]; }
+}
+__global__ void k_mass_dot(int n, const double* mass, const double* x, double* partials, unsigned* counter,
+                           double* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double c = 0.0;
+    if (i < n) c = mass[i] * (x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
+    double v[1] = {c}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+}
+__global__ void k_dot(size_t len, const double* x, const double* y, double* partials, unsigned* counter, double* out) {
+    double c = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x)
+        c += x[i] * y[i];
+    double v[1] = {c}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+}
+__global__ void k_cvt_scaled(size_t len, const double* x, const double* s, float* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double f = 1.0 / sqrt(s[0]);
+    if (i < len) y[i] = float(x[i] * f);
+}
+__global__ void k_add_scaled(size_t len, const float* d, const double* s, double* x) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const double f = sqrt(s[0]);
+    if (i < len) x[i] += f * double(d[i]);
+}
+__global__ void k_set_state(CgState* st, double bb, double tol2) {
+    st->bb = bb;
+    st->tol2 = tol2;
+}
+__global__ void k_set_state_dev(CgState* st, const double* bb, double tol2) {
+    st->bb = bb[0];
+    st->tol2 = tol2;
+}
+
+// ---------------------------------------------------------------------------------------
+// host side
+
+static inline unsigned blocks_for(size_t len, int bs = 256) { return unsigned((len + bs - 1) / bs); }
+
+void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma) {
+    const int n = op->n;
+    const int vw = op->cplx ? 2 : 1;
+    MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
+    MK_CHECK(sigma >= 1, MK_ERR_ARG, "sigma must be >= 1");
+    op->C = C;
+    op->sigma = sigma;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    bool permuted = false;
+    if (sigma > 1) {
+        for (int w0 = 0; w0 < n; w0 += sigma) {
+            const int w1 = std::min(n, w0 + sigma);
+            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+                return (rowptr[a + 1] - rowptr[a]) > (rowptr[b + 1] - rowptr[b]);
+            });
+        }
+        for (int i = 0; i < n; i++) if (order[i] != i) { permuted = true; break; }
+    }
+    const int ns = (n + C - 1) / C;
+    std::vector<int64_t> sptr(size_t(ns) + 1, 0);
+    std::vector<int> sw(size_t(ns), 0);
+    for (int s = 0; s < ns; s++) {
+        int w = 0;
+        for (int l = 0; l < C && s * C + l < n; l++) {
+            const int row = order[s * C + l];
+            w = std::max<int>(w, int(rowptr[row + 1] - rowptr[row]));
+        }
+        sw[s] = w;
+        sptr[s + 1] = sptr[s] + int64_t(w) * C;
+    }
+    const int64_t stored = sptr[ns];
+    std::vector<int> hcol(static_cast<size_t>(stored));
+    std::vector<double> hval(size_t(stored) * vw, 0.0);
+    std::vector<float> hval32(size_t(stored) * vw, 0.0f);
+    for (int s = 0; s < ns; s++) {
+        for (int l = 0; l < C; l++) {
+            const int r = s * C + l;
+            const int row = r < n ? order[r] : -1;
+            const int64_t len = row >= 0 ? rowptr[row + 1] - rowptr[row] : 0;
+            for (int j = 0; j < sw[s]; j++) {
+                const int64_t e = sptr[s] + int64_t(j) * C + l;
+                if (j < len) {
+                    const int64_t k = rowptr[row] + j;
+                    hcol[e] = col[k];
+                    for (int c = 0; c < vw; c++) {
+                        hval[e * vw + c] = val[k * vw + c];
+                        hval32[e * vw + c] = float(val[k * vw + c]);
+                    }
+                } else {
+                    hcol[e] = row >= 0 ? row : 0;  // padding: valid index, zero value
+                }
+            }
+        }
+    }
+    // inverse diagonal (real part)
+    std::vector<double> dinv(n, 1.0);
+    std::vector<float> dinv32(n, 1.0f);
+    for (int i = 0; i < n; i++) {
+        double d = 0.0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += val[k * vw];
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+        dinv32[i] = float(dinv[i]);
+    }
+    hipStream_t st = op->ctx->stream;
+    op->nslices = ns;
+    op->stored = stored;
+    op->slice_ptr.upload(sptr.data(), sptr.size(), st);
+    op->slice_w.upload(sw.data(), sw.size(), st);
+    op->col.upload(hcol.data(), hcol.size(), st);
+    op->val64.upload(hval.data(), hval.size(), st);
+    op->val32.upload(hval32.data(), hval32.size(), st);
+    op->dinv64.upload(dinv.data(), dinv.size(), st);
+    op->dinv32.upload(dinv32.data(), dinv32.size(), st);
+    if (permuted) op->perm.upload(order.data(), order.size(), st);
+    else op->perm.release();
+    MK_HIP(hipStreamSynchronize(st));
+    const double sv64 = 8.0 * vw, sv32 = 4.0 * vw;
+    const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0);
+    op->bytes64 = double(op->nnz) * (sv64 + 4.0) + meta + double(n) * sv64 * 2.0;
+    op->bytes32 = double(op->nnz) * (sv32 + 4.0) + meta + double(n) * sv32 * 2.0;
+}
+
+static hipEvent_t next_event(mk_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        MK_HIP(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+
+void timing_flush(mk_ctx* c) {
+    if (!c->time_spmv || c->ev_used == 0) { c->ev_used = 0; return; }
+    MK_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.f;
+        MK_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
+        c->spmv_ms += ms;
+    }
+    c->ev_used = 0;
+}
+
+struct SpmvTimer {
+    mk_ctx* c;
+    explicit SpmvTimer(mk_ctx* c_, double bytes) : c(c_) {
+        c->spmv_calls++;
+        c->spmv_bytes += bytes;
+        if (c->time_spmv) {
+            if (c->ev_used + 2 > 4096) timing_flush(c);
+            MK_HIP(hipEventRecord(next_event(c), c->stream));
+        }
+    }
+    ~SpmvTimer() {
+        if (c->time_spmv) (void)hipEventRecord(next_event(c), c->stream);
+    }
+};
+
+template <typename T>
+static void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode,
+                        double* out, double bytes) {
+    const int n = A->n;
+    const unsigned g = blocks_for(n, A->block);
+    MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
+    SpmvTimer tm(c, bytes);
+    const int* perm = A->perm.p;
+#define MK_SPMV(CPLX, MODE)                                                                                \
+    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,        \
+                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, perm, x, y, b, c->partials.p,          \
+                       c->counter.p, out)
+    if (A->cplx) {
+        if (mode == SPMV_PLAIN) MK_SPMV(true, 0);
+        else if (mode == SPMV_DOT) MK_SPMV(true, 1);
+        else MK_SPMV(true, 2);
+    } else {
+        if (mode == SPMV_PLAIN) MK_SPMV(false, 0);
+        else if (mode == SPMV_DOT) MK_SPMV(false, 1);
+        else MK_SPMV(false, 2);
+    }
+#undef MK_SPMV
+    MK_HIP(hipGetLastError());
+}
+
+void spmv64(mk_ctx* c, const mk_op* A, const double* x, double* y, const double* b, SpmvMode mode, double* out,
+            int*) {
+    spmv_launch<double>(c, A, A->val64.p, x, y, b, mode, out, A->bytes64 + (mode == SPMV_RESID ? A->n * 8.0 * (A->cplx ? 2 : 1) : 0));
+}
+void spmv32(mk_ctx* c, const mk_op* A, const float* x, float* y, SpmvMode mode, double* out, int*) {
+    spmv_launch<float>(c, A, A->val32.p, x, y, nullptr, mode, out, A->bytes32);
+}
+
+void axpby64(mk_ctx* c, size_t len, double a, const double* x, double b, double* y) {
+    hipLaunchKernelGGL(k_axpby, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, a, x, b, y);
+    MK_HIP(hipGetLastError());
+}
+void scale_dev64(mk_ctx* c, size_t len, const double* s_dev, int invert_sqrt, double* y) {
+    hipLaunchKernelGGL(k_scale_dev, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, s_dev, invert_sqrt, y);
+    MK_HIP(hipGetLastError());
+}
+void dot64(mk_ctx* c, size_t len, const double* x, const double* y, double* out_dev) {
+    unsigned g = std::min<unsigned>(blocks_for(len), 2048u);
+    g = std::max(g, 1u);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, c->stream, len, x, y, c->partials.p, c->counter.p, out_dev);
+    MK_HIP(hipGetLastError());
+}
+void mass_mul(mk_ctx* c, int n, const double* mass, const double* u, double* b) {
+    hipLaunchKernelGGL(k_mass_mul, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, mass, u, b);
+    MK_HIP(hipGetLastError());
+}
+void mass_dot(mk_ctx* c, int n, const double* mass, const double* x, double* out_dev) {
+    hipLaunchKernelGGL(k_mass_dot, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, mass, x, c->partials.p,
+                       c->counter.p, out_dev);
+    MK_HIP(hipGetLastError());
+}
+void cvt64to32_scaled(mk_ctx* c, size_t len, const double* x, const double* s_dev, float* y) {
+    hipLaunchKernelGGL(k_cvt_scaled, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, x, s_dev, y);
+    MK_HIP(hipGetLastError());
+}
+void add32to64_scaled(mk_ctx* c, size_t len, const float* d, const double* s_dev, double* x) {
+    hipLaunchKernelGGL(k_add_scaled, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, d, s_dev, x);
+    MK_HIP(hipGetLastError());
+}
+
+// Generic PCG on the device.  r must hold the initial residual b - A x on entry; bb_dev
+// points to ||b||^2 on the device.
+template <typename T>
+static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T* x, T* r, T* z, T* p, T* q,
+                       const double* bb_dev, double rel_tol, int maxiter, int check_every, double bytes) {
+    const int n = A->n;
+    const size_t len = size_t(n) * (A->cplx ? 2 : 1);
+    const unsigned g = blocks_for(n);
+    MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
+    CgState* st = c->state.p;
+    hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, bb_dev, rel_tol * rel_tol);
+    if (A->cplx)
+        hipLaunchKernelGGL((cg_init<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
+                           c->partials.p, c->counter.p);
+    else
+        hipLaunchKernelGGL((cg_init<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
+                           c->partials.p, c->counter.p);
+    MK_HIP(hipGetLastError());
+    CgState h{};
+    int launched = 0;
+    for (;;) {
+        MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        timing_flush(c);
+        if (h.done || h.it >= maxiter) break;
+        const int batch = std::min(check_every, maxiter - h.it);
+        for (int b = 0; b < batch; b++) {
+            {
+                SpmvTimer tm(c, bytes);
+                if (A->cplx)
+                    hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,
+                                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, A->perm.p, p, q, st,
+                                       c->partials.p, c->counter.p);
+                else
+                    hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,
+                                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, A->perm.p, p, q, st,
+                                       c->partials.p, c->counter.p);
+            }
+            if (A->cplx)
+                hipLaunchKernelGGL((cg_update<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                                   st, c->partials.p, c->counter.p);
+            else
+                hipLaunchKernelGGL((cg_update<T, false>), dim3(g), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                                   st, c->partials.p, c->counter.p);
+            hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z,
+                               (const CgState*)st);
+            launched++;
+        }
+        MK_HIP(hipGetLastError());
+    }
+    CgResult res{h.status, h.it, h.rr, h.bb};
+    if (res.status == MK_OK && !h.done) res.status = MK_ERR_NO_CONVERGENCE;
+    return res;
+}
+
+CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_tol, int maxiter, bool jacobi,
+              int check_every, bool x_is_zero) {
+    Work& w = work(c);
+    const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+    w.r.ensure(len); w.z.ensure(len); w.p.ensure(len); w.q.ensure(len);
+    double* bb = c->scal.p + 0;
+    double* rr0 = c->scal.p + 1;
+    dot64(c, len, b, b, bb);
+    if (x_is_zero) MK_HIP(hipMemcpyAsync(w.r.p, b, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    else spmv64(c, A, x, w.r.p, b, SPMV_RESID, rr0, nullptr);
+    return cg_run<double>(c, A, A->val64.p, jacobi ? A->dinv64.p : nullptr, x, w.r.p, w.z.p, w.p.p, w.q.p, bb,
+                          rel_tol, maxiter, check_every, A->bytes64);
+}
+
+CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_tol, int maxiter, bool jacobi,
+              int check_every) {
+    Work& w = work(c);
+    const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+    w.r32.ensure(len); w.z32.ensure(len); w.p32.ensure(len); w.q32.ensure(len);
+    // x = 0, r = b, ||b|| = 1 by construction of the caller (b normalised)
+    MK_HIP(hipMemsetAsync(x, 0, len * sizeof(float), c->stream));
+    MK_HIP(hipMemcpyAsync(w.r32.p, b, len * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    double one = 1.0;
+    MK_HIP(hipMemcpyAsync(c->scal.p + 2, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return cg_run<float>(c, A, A->val32.p, jacobi ? A->dinv32.p : nullptr, x, w.r32.p, w.z32.p, w.p32.p, w.q32.p,
+                         c->scal.p + 2, rel_tol, maxiter, check_every, A->bytes32);
+}
+
+// Iterative refinement: FP64 residual r = b - A x, FP32 PCG correction on r/||r||.
+SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o) {
+    const bool jac = o.precond == MK_PRECOND_JACOBI;
+    SolveResult res{MK_OK, 0, 0, 0.0};
+    const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+    if (o.precision == MK_PREC_FP64) {
+        CgResult cr = cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false);
+        res.iters = cr.iters;
+        // true FP64 residual
+        Work& w = work(c);
+        double h[2];
+        w.t.ensure(len);
+        spmv64(c, A, x, w.t.p, b, SPMV_RESID, c->scal.p + 3, nullptr);
+        MK_HIP(hipMemcpyAsync(h, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        double rr;
+        MK_HIP(hipMemcpyAsync(&rr, c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        res.rel_residual = h[0] > 0 ? std::sqrt(rr / h[0]) : std::sqrt(rr);
+        res.status = cr.status;
+        if (res.status == MK_ERR_NO_CONVERGENCE && res.rel_residual <= o.tol) res.status = MK_OK;
+        return res;
+    }
+    Work& w = work(c);
+    w.t.ensure(len); w.r32.ensure(len); w.x32.ensure(len);
+    double* bb = c->scal.p + 4;
+    double* rr = c->scal.p + 5;
+    dot64(c, len, b, b, bb);
+    double hbb = 0.0;
+    MK_HIP(hipMemcpyAsync(&hbb, bb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    MK_HIP(hipStreamSynchronize(c->stream));
+    if (hbb == 0.0) {
+        MK_HIP(hipMemsetAsync(x, 0, len * sizeof(double), c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        return res;
+    }
+    double prev = HUGE_VAL;
+    int stagn = 0;
+    // the FP32 work vectors used by cg32 are r32,z32,p32,q32; the normalised rhs lives in x32's twin
+    DevBuf<float>& rhs32 = w.x32;
+    DevBuf<float> d32(len);
+    for (;;) {
+        spmv64(c, A, x, w.t.p, b, SPMV_RESID, rr, nullptr);
+        double hrr = 0.0;
+        MK_HIP(hipMemcpyAsync(&hrr, rr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        const double rel = std::sqrt(hrr / hbb);
+        res.rel_residual = rel;
+        if (!(rel == rel)) { res.status = MK_ERR_NAN; break; }
+        if (rel <= o.tol) break;
+        if (res.refinements >= o.max_refine) { res.status = MK_ERR_NO_CONVERGENCE; break; }
+        if (rel > 0.5 * prev) { if (++stagn >= 3) { res.status = MK_ERR_STAGNATION; break; } }
+        else stagn = 0;
+        prev = rel;
+        cvt64to32_scaled(c, len, w.t.p, rr, rhs32.p);
+        const double itol = std::max(o.inner_tol, 0.5 * o.tol / rel);
+        CgResult cr = cg32(c, A, rhs32.p, d32.p, std::min(itol, 0.5), o.maxiter - res.iters, jac, o.check_every);
+        res.iters += cr.iters;
+        res.refinements++;
+        if (cr.status == MK_ERR_NOT_SPD || cr.status == MK_ERR_NAN) { res.status = cr.status; break; }
+        add32to64_scaled(c, len, d32.p, rr, x);
+        if (res.iters >= o.maxiter) {
+            spmv64(c, A, x, w.t.p, b, SPMV_RESID, rr, nullptr);
+            MK_HIP(hipMemcpyAsync(&hrr, rr, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            MK_HIP(hipStreamSynchronize(c->stream));
+            res.rel_residual = std::sqrt(hrr / hbb);
+            if (res.rel_residual > o.tol) res.status = MK_ERR_NO_CONVERGENCE;
+            break;
+        }
+    }
+    return res;
+}
+
+}  // namespace mk

@@ -1,0 +1,166 @@
+// Internal declarations of the metriko HIP library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <complex>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "metriko_hip.h"
+
+namespace mk {
+
+struct Error : std::runtime_error {
+    mk_status code;
+    Error(mk_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define MK_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw ::mk::Error(MK_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define MK_CHECK(cond, code, msg) \
+    do { if (!(cond)) throw ::mk::Error((code), (msg)); } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) {
+            hipError_t e = hipMalloc(&p, count * sizeof(T));
+            if (e != hipSuccess) { p = nullptr; n = 0; throw Error(MK_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        }
+    }
+    void ensure(size_t count) { if (count > n) alloc(count); }
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        ensure(count);
+        if (count) MK_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* h, size_t count, hipStream_t s) const {
+        if (count) MK_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+// Device-side state of one CG run (scalars kept on the GPU between kernels).
+struct CgState {
+    double rz;      // r^H z
+    double pq;      // p^H A p
+    double alpha, beta;
+    double rr;      // ||r||^2
+    double bb;      // ||b||^2 (reference norm)
+    double tol2;    // convergence threshold on rr/bb
+    int done;       // 1 = converged or failed
+    int status;     // MK_OK / MK_ERR_NOT_SPD / MK_ERR_NAN
+    int it;         // iterations taken
+    int pad;
+};
+
+// per-context scratch vectors
+struct Work {
+    DevBuf<double> r, z, p, q, t;
+    DevBuf<float> r32, z32, p32, q32, x32;
+};
+
+}  // namespace mk
+
+struct mk_ctx {
+    mk::Work work;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // reduction scratch: partial sums [kMaxParts * kMaxK], arrival counter, CG state
+    mk::DevBuf<double> partials;
+    mk::DevBuf<unsigned int> counter;
+    mk::DevBuf<mk::CgState> state;
+    mk::DevBuf<double> scal;   // general device scalars
+    // SpMV timing (events around each SpMV launch when enabled)
+    bool time_spmv = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double spmv_ms = 0.0;
+    int64_t spmv_calls = 0;
+    double spmv_bytes = 0.0;
+};
+
+struct mk_op {
+    mk_ctx* ctx = nullptr;
+    int n = 0;
+    bool cplx = false;
+    int64_t nnz = 0;
+    // SELL-C-sigma layout (C = slice height; rows sorted by length inside windows of sigma)
+    int C = 64;
+    int sigma = 1;
+    int nslices = 0;
+    int64_t stored = 0;  // stored entries incl. padding
+    mk::DevBuf<int64_t> slice_ptr;
+    mk::DevBuf<int> slice_w;
+    mk::DevBuf<int> col;
+    mk::DevBuf<double> val64;   // complex: interleaved
+    mk::DevBuf<float> val32;
+    mk::DevBuf<int> perm;       // SELL row -> original row (empty if identity)
+    mk::DevBuf<double> dinv64;  // inverse diagonal (real part for complex)
+    mk::DevBuf<float> dinv32;
+    int block = 256;
+    double bytes64 = 0, bytes32 = 0;  // algorithmic bytes per SpMV
+    // host CSR copy (sorted columns)
+    std::vector<int64_t> h_ptr;
+    std::vector<int> h_col;
+    std::vector<double> h_val;
+};
+
+namespace mk {
+
+// ---- kernels / launchers (kernels.hip) -------------------------------------------------
+enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2 };
+
+void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma);
+// y = A x (mode PLAIN); y = A x and partial x^H y (DOT); y = b - A x and partial |y|^2 (RESID)
+void spmv64(mk_ctx* c, const mk_op* A, const double* x, double* y, const double* b, SpmvMode mode, double* partials,
+            int* nparts);
+void spmv32(mk_ctx* c, const mk_op* A, const float* x, float* y, SpmvMode mode, double* partials, int* nparts);
+
+// reductions: host-visible result of sum(partials[k*np + i]) for k < K
+void reduce_partials(mk_ctx* c, const double* partials, int np, int K, double* out_dev);
+
+// vector kernels (len = number of scalar entries; complex vectors count 2 per element)
+void axpby64(mk_ctx* c, size_t len, double a, const double* x, double b, double* y);  // y = a x + b y
+void scale_dev64(mk_ctx* c, size_t len, const double* s_dev, int invert_sqrt, double* y);
+void dot64(mk_ctx* c, size_t len, const double* x, const double* y, double* out_dev);  // real dot of interleaved
+void mass_mul(mk_ctx* c, int n, const double* mass, const double* u, double* b);      // complex b = M u
+void mass_dot(mk_ctx* c, int n, const double* mass, const double* x, double* out_dev); // x^H M x
+void cvt64to32_scaled(mk_ctx* c, size_t len, const double* x, const double* s_dev, float* y); // y = x / sqrt(s)
+void add32to64_scaled(mk_ctx* c, size_t len, const float* d, const double* s_dev, double* x);  // x += sqrt(s) d
+
+// CG on the device: solves A x = b for FP64 (x,b device) -- returns status, iterations
+struct CgResult { int status; int iters; double rr; double bb; };
+CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_tol, int maxiter, bool jacobi,
+              int check_every, bool x_is_zero);
+CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_tol, int maxiter, bool jacobi,
+              int check_every);
+
+// refinement driver: x (device FP64) warm start, b device FP64
+struct SolveResult { int status; int iters; int refinements; double rel_residual; };
+SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o);
+
+inline Work& work(mk_ctx* c) { return c->work; }
+
+void stats_reset(mk_ctx* c);
+void stats_fill(mk_ctx* c, mk_stats* s);
+void timing_flush(mk_ctx* c);
+
+}  // namespace mk

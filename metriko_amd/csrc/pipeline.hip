@@ -1,0 +1,232 @@
+// C ABI drivers built on the drop-in headers: the example_igm pipeline, a host-only
+// combinatorics stage (runs without a GPU) and the Hmesh export used by the tests.
+#include <chrono>
+#include <cstring>
+#include "mk_internal.h"
+#include "metriko/core/vectorfield/face_rosy_field.h"
+#include "metriko/core/igm/parameterization.h"
+
+namespace {
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+metriko::MatXd toV(int nV, const double* V) {
+    metriko::MatXd M(nV, 3);
+    std::memcpy(M.data(), V, sizeof(double) * size_t(nV) * 3);
+    return M;
+}
+metriko::MatXi toF(int nV, int nF, const int32_t* F) {
+    metriko::MatXi M(nF, 3);
+    for (int64_t i = 0; i < int64_t(nF) * 3; i++) {
+        if (F[i] < 0 || F[i] >= nV) throw mk::Error(MK_ERR_ARG, "face index out of range");
+        M.d[size_t(i)] = F[i];
+    }
+    return M;
+}
+template <typename Fn>
+mk_status guarded(mk_ctx* c, Fn&& fn) {
+    try {
+        fn();
+        return MK_OK;
+    } catch (const mk::Error& e) {
+        if (c) c->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (c) c->last_error = e.what();
+        return MK_ERR_ARG;
+    }
+}
+
+struct HostStages {
+    std::unique_ptr<metriko::FaceRosyField> combed;
+    std::vector<bool> seam;
+    std::unique_ptr<metriko::Hmesh> cut;
+    metriko::MatXd ext;
+    double ms_matching = 0, ms_seam = 0, ms_cut = 0;
+};
+
+// matching -> singularities -> seam -> cut mesh -> combed field -> extrinsic field
+void host_stages(const metriko::Hmesh& mesh, metriko::FaceRosyField& rawf, HostStages& H) {
+    using namespace metriko;
+    double t = now_ms();
+    rawf.computeMatching(MatchingType::Curl);
+    H.ms_matching = now_ms() - t;
+    t = now_ms();
+    H.seam = compute_seam(rawf);
+    H.ms_seam = now_ms() - t;
+    t = now_ms();
+    H.cut = compute_cut_mesh(mesh, H.seam);
+    H.combed = compute_combbed_field(rawf, H.seam);
+    H.ext = compute_extrinsic_field(mesh, *H.combed, rawf.rosyN);
+    H.ms_cut = now_ms() - t;
+}
+}  // namespace
+
+extern "C" {
+
+mk_status mk_hmesh_export(int nV, int nF, const double* V, const int32_t* F, int* nE, int32_t* edge2vert,
+                          int32_t* edge2face, int32_t* face2edge, int32_t* vert2half, double* farg,
+                          double* angle_defect) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !nE) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        metriko::Hmesh m(toV(nV, V), toF(nV, nF, F));
+        *nE = m.nE;
+        if (edge2vert) std::memcpy(edge2vert, m.edge2vert.data(), sizeof(int) * size_t(m.nE) * 2);
+        if (edge2face) std::memcpy(edge2face, m.edge2face.data(), sizeof(int) * size_t(m.nE) * 2);
+        if (face2edge) std::memcpy(face2edge, m.face2edge.data(), sizeof(int) * size_t(m.nF) * 3);
+        if (vert2half) std::memcpy(vert2half, m.vert2half.data(), sizeof(int) * size_t(m.nV));
+        if (farg) std::memcpy(farg, m.heArgOnFace.data(), sizeof(double) * size_t(m.nH));
+        if (angle_defect) std::memcpy(angle_defect, m.angleDefect.data(), sizeof(double) * size_t(m.nV));
+    });
+}
+
+mk_status mk_hmesh_halfedges(int nV, int nF, const double* V, const int32_t* F, int32_t* face_halfs,
+                             int32_t* twin, int32_t* half_face, double* farg, double* face_area) {
+    if (nV <= 0 || nF <= 0 || !V || !F) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        metriko::Hmesh m(toV(nV, V), toF(nV, nF, F));
+        for (int f = 0; f < m.nF; f++) {
+            int h = m.face2half[f];
+            for (int k = 0; k < 3; k++, h = m.next[h]) if (face_halfs) face_halfs[3 * f + k] = h;
+        }
+        if (twin) std::memcpy(twin, m.twin.data(), sizeof(int) * size_t(m.nH));
+        if (half_face) std::memcpy(half_face, m.face.data(), sizeof(int) * size_t(m.nH));
+        if (farg) std::memcpy(farg, m.heArgOnFace.data(), sizeof(double) * size_t(m.nH));
+        if (face_area) std::memcpy(face_area, m.faceArea.data(), sizeof(double) * size_t(m.nF));
+    });
+}
+
+// Host-only stages downstream of the field solve (no GPU): given the compressed field,
+// compute roots, curl matching, singularities, seam, cut mesh, combed matching and the
+// IGM setup sizes.  Outputs: singular[nV], matching[nE] (combed), seam[nE], cut_idx[3nF],
+// sizes[4] = {cutV, nS, nT, constraint rows}, ext[12 nF].
+mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
+                           int32_t* singular, int32_t* raw_matching, int32_t* matching, uint8_t* seam,
+                           int32_t* cut_idx, int32_t* sizes, double* ext) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !compressed || N != 4) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        FaceRosyField rawf(mesh, N);
+        rawf.compressed = VecXc(nF);
+        std::memcpy(rawf.compressed.data(), compressed, sizeof(double) * 2 * size_t(nF));
+        rawf.field.resize(nF, N);
+        for (int f = 0; f < nF; f++) {
+            const double th = std::arg(rawf.compressed[f]);
+            const complex base = std::polar(1.0, th / N);
+            for (int i = 0; i < N; i++) rawf.field(f, i) = base * std::polar(1.0, TwoPI / N * i);
+        }
+        HostStages H;
+        host_stages(mesh, rawf, H);
+        RosyParameterization rp(mesh, *H.cut, H.ext, rawf.singular, H.combed->matching, H.seam, N, 0.02);
+        rp.seamless = true;
+        rp.roundSeams = false;
+        rp.setup();
+        for (int v = 0; v < nV; v++) if (singular) singular[v] = rawf.singular[v];
+        for (int e = 0; e < mesh.nE; e++) {
+            if (raw_matching) raw_matching[e] = rawf.matching[e];
+            if (matching) matching[e] = H.combed->matching[e];
+            if (seam) seam[e] = H.seam[size_t(e)] ? 1 : 0;
+        }
+        if (cut_idx) std::memcpy(cut_idx, H.cut->idx.data(), sizeof(int) * size_t(nF) * 3);
+        if (ext) std::memcpy(ext, H.ext.data(), sizeof(double) * size_t(nF) * 12);
+        if (sizes) {
+            sizes[0] = H.cut->nV;
+            sizes[1] = rp.insideSingulars();
+            sizes[2] = rp.transitions();
+            sizes[3] = rp.constraint.rows();
+        }
+    });
+}
+
+mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int32_t* F, int N, double ratio,
+                          const double* u0, int n_iter, const mk_solve_opts* field_opts,
+                          const mk_solve_opts* igm_opts, double* compressed, int32_t* singular, int32_t* matching,
+                          uint8_t* seam, int32_t* cut_idx, double* uv, int fullx_cap, double* fullx, int dec_cap,
+                          int32_t* dec_index, int32_t* dec_value, mk_pipeline_info* info) {
+    if (!c || nV <= 0 || nF <= 0 || !V || !F || N != 4 || !info) return MK_ERR_ARG;
+    return guarded(c, [&] {
+        using namespace metriko;
+        auto& rt = hip::runtime();
+        struct Detach {
+            hip::Runtime& r;
+            ~Detach() { r.ctx = nullptr; r.external = false; }
+        } detach{rt};
+        rt.attach(c);
+        if (field_opts) rt.field_opts = *field_opts; else mk_default_opts(&rt.field_opts);
+        if (igm_opts) rt.igm_opts = *igm_opts; else mk_default_opts(&rt.igm_opts);
+        std::memset(info, 0, sizeof(*info));
+        const double t_all = now_ms();
+        double t = now_ms();
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        info->ms_hmesh = now_ms() - t;
+        info->nV = mesh.nV;
+        info->nF = mesh.nF;
+        info->nE = mesh.nE;
+        VecXc u(nF);
+        if (u0) std::memcpy(u.data(), u0, sizeof(double) * 2 * size_t(nF));
+        else u = random_complex_vector(nF);
+        t = now_ms();
+        FaceRosyField rawf(mesh, N);
+        DeviceOp L = rawf.connectionLaplacianDevice();
+        MK_HIP(hipStreamSynchronize(c->stream));
+        info->ms_field_asm = now_ms() - t;
+        t = now_ms();
+        VecXd mass(nF);
+        for (int f = 0; f < nF; f++) mass[f] = mesh.faceArea[f];
+        rawf.compressed = solveSmallestEig(L, mass, u, n_iter);
+        info->field = rt.last_field;
+        info->ms_field_solve = now_ms() - t;
+        rawf.field.resize(nF, N);
+        for (int f = 0; f < nF; f++) {
+            const double th = std::arg(rawf.compressed[f]);
+            const complex base = std::polar(1.0, th / N);
+            for (int i = 0; i < N; i++) rawf.field(f, i) = base * std::polar(1.0, TwoPI / N * i);
+        }
+        HostStages H;
+        host_stages(mesh, rawf, H);
+        info->ms_matching = H.ms_matching;
+        info->ms_seam = H.ms_seam;
+        info->ms_cut = H.ms_cut;
+        t = now_ms();
+        RosyParameterization rp(mesh, *H.cut, H.ext, rawf.singular, H.combed->matching, H.seam, N, ratio);
+        rp.seamless = true;
+        rp.localInjectivity = false;
+        rp.roundSeams = false;
+        rp.verbose = false;
+        rp.setup();
+        info->ms_setup = now_ms() - t;
+        t = now_ms();
+        const bool ok = rp.integ();
+        info->ms_integ = now_ms() - t;
+        info->igm = rt.last_igm;
+        info->success = ok ? 1 : 0;
+        info->cutV = H.cut->nV;
+        info->nS = rp.insideSingulars();
+        info->nT = rp.transitions();
+        info->k = rt.last_igm.k;
+        info->nfullx = rp.fullx.size();
+        info->ndec = int(rp.decisionIndex.size());
+        if (compressed) std::memcpy(compressed, rawf.compressed.data(), sizeof(double) * 2 * size_t(nF));
+        for (int v = 0; v < nV; v++) if (singular) singular[v] = rawf.singular[v];
+        for (int e = 0; e < mesh.nE; e++) {
+            if (matching) matching[e] = H.combed->matching[e];
+            if (seam) seam[e] = H.seam[size_t(e)] ? 1 : 0;
+        }
+        if (cut_idx) std::memcpy(cut_idx, H.cut->idx.data(), sizeof(int) * size_t(nF) * 3);
+        if (uv)
+            for (int f = 0; f < nF; f++)
+                for (int j = 0; j < 3; j++) {
+                    uv[size_t(6 * f + 2 * j)] = rp.cfn(f, N * j);
+                    uv[size_t(6 * f + 2 * j + 1)] = rp.cfn(f, N * j + 1);
+                }
+        if (fullx) for (int i = 0; i < std::min(fullx_cap, rp.fullx.size()); i++) fullx[i] = rp.fullx[i];
+        for (int i = 0; i < std::min(dec_cap, info->ndec); i++) {
+            if (dec_index) dec_index[i] = rp.decisionIndex[size_t(i)];
+            if (dec_value) dec_value[i] = rp.decisionValue[size_t(i)];
+        }
+        info->ms_total = now_ms() - t_all;
+    });
+}
+
+}  // extern "C"
