@@ -36,7 +36,7 @@ typedef struct mk_op mk_op;
 
 enum { MK_PREC_FP64 = 0, MK_PREC_MIXED = 1 };     /* FP64, or FP32 inner + FP64 refinement */
 enum { MK_FMT_CSR = 0, MK_FMT_SELL = 1 };
-enum { MK_PRECOND_NONE = 0, MK_PRECOND_JACOBI = 1 };
+enum { MK_PRECOND_NONE = 0, MK_PRECOND_JACOBI = 1, MK_PRECOND_AMG = 2 };  /* AMG: smoothed aggregation V-cycle */
 
 typedef struct {
     int precision;      /* MK_PREC_* */
@@ -61,6 +61,8 @@ typedef struct {
     double spmv_ms;       /* GPU time inside SpMV kernels (hipEvent) */
     int64_t spmv_calls;   /* SpMV launches */
     double spmv_bytes;    /* algorithmic bytes moved by those SpMVs */
+    double setup_ms;      /* preconditioner (multigrid) setup time (ms), not in ms */
+    int mg_levels;        /* multigrid levels (0 = no multigrid) */
 } mk_stats;
 
 const char* mk_status_str(mk_status s);

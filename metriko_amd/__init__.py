@@ -16,7 +16,7 @@ _lib = None
 
 PREC_FP64, PREC_MIXED = 0, 1
 FMT_CSR, FMT_SELL = 0, 1
-PRECOND_NONE, PRECOND_JACOBI = 0, 1
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_AMG = 0, 1, 2
 
 EXPORTED = [
     "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing",
@@ -35,7 +35,8 @@ class SolveOpts(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("solves", C.c_int),
                 ("rel_residual", C.c_double), ("ms", C.c_double), ("spmv_ms", C.c_double),
-                ("spmv_calls", C.c_int64), ("spmv_bytes", C.c_double)]
+                ("spmv_calls", C.c_int64), ("spmv_bytes", C.c_double), ("setup_ms", C.c_double),
+                ("mg_levels", C.c_int)]
 
 
 class IgmFlags(C.Structure):

@@ -119,6 +119,60 @@ inline unsigned nblk(size_t n, int bs = 256) { return unsigned((n + bs - 1) / bs
 }  // namespace
 
 namespace mk {
+// host CSR copy of an operator (from its host arrays, or compacted from the device SELL layout)
+void host_csr(mk_ctx* c, const mk_op* A, std::vector<int64_t>& rp, std::vector<int>& cc, std::vector<double>& vv) {
+    const int vw = A->cplx ? 2 : 1;
+    if (!A->h_ptr.empty()) {
+        rp = A->h_ptr;
+        cc = A->h_col;
+        vv = A->h_val;
+        return;
+    }
+    std::vector<int64_t> sptr(size_t(A->nslices) + 1);
+    std::vector<int> sw(size_t(A->nslices)), hc(size_t(A->stored));
+    std::vector<double> hv(size_t(A->stored) * vw);
+    A->slice_ptr.download(sptr.data(), sptr.size(), c->stream);
+    A->slice_w.download(sw.data(), sw.size(), c->stream);
+    A->col.download(hc.data(), hc.size(), c->stream);
+    A->val64.download(hv.data(), hv.size(), c->stream);
+    std::vector<int> perm;
+    if (A->perm.p) {
+        perm.resize(size_t(A->n));
+        A->perm.download(perm.data(), perm.size(), c->stream);
+    }
+    MK_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> inv(size_t(A->n));
+    for (int r = 0; r < A->n; r++) inv[perm.empty() ? r : perm[r]] = r;
+    rp.assign(size_t(A->n) + 1, 0);
+    cc.clear();
+    vv.clear();
+    for (int row = 0; row < A->n; row++) {
+        const int r = inv[row];
+        const int s = r / A->C, lane = r % A->C;
+        for (int j = 0; j < sw[s]; j++) {
+            const int64_t e = sptr[s] + int64_t(j) * A->C + lane;
+            bool zero = true;
+            for (int q = 0; q < vw; q++) if (hv[e * vw + q] != 0.0) zero = false;
+            if (zero) continue;
+            cc.push_back(hc[e]);
+            for (int q = 0; q < vw; q++) vv.push_back(hv[e * vw + q]);
+        }
+        rp[row + 1] = int64_t(cc.size());
+    }
+}
+
+void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
+    mk_op* A = const_cast<mk_op*>(A_);
+    if (A->mg || o.precond != MK_PRECOND_AMG) return;
+    MK_CHECK(A->n == A->m, MK_ERR_ARG, "multigrid needs a square operator");
+    std::vector<int64_t> rp;
+    std::vector<int> cc;
+    std::vector<double> vv;
+    host_csr(c, A, rp, cc, vv);
+    A->mg = A->cplx ? mg_build_complex(c, A->n, rp.data(), cc.data(), vv.data(), o)
+                    : mg_build_real(c, A->n, rp.data(), cc.data(), vv.data(), o);
+}
+
 void stats_reset(mk_ctx* c) {
     c->spmv_ms = 0.0;
     c->spmv_calls = 0;
@@ -159,7 +213,7 @@ void mk_default_opts(mk_solve_opts* o) {
     o->slice_height = 64;
     o->sigma = 1;
     o->block_size = 256;
-    o->precond = MK_PRECOND_JACOBI;
+    o->precond = MK_PRECOND_AMG;
     o->inner_tol = 1e-5;
     o->max_refine = 12;
     o->check_every = 16;
@@ -230,6 +284,7 @@ static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_
         try {
             op->ctx = c;
             op->n = n;
+            op->m = n;
             op->cplx = cplx;
             op->nnz = rowptr[n];
             op->block = o.block_size;
@@ -286,6 +341,8 @@ mk_status mk_spd_solve(mk_ctx* c, const mk_op* A, const double* b, double* x, co
         const mk_solve_opts o = resolve(o_);
         MK_HIP(hipSetDevice(c->device));
         const double t0 = now_ms();
+        mk::ensure_mg(c, A, o);
+        const double t_setup = now_ms() - t0;
         mk::stats_reset(c);
         const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
         DevBuf<double> db, dx;
@@ -302,6 +359,8 @@ mk_status mk_spd_solve(mk_ctx* c, const mk_op* A, const double* b, double* x, co
             st->rel_residual = r.rel_residual;
             mk::stats_fill(c, st);
             st->ms = now_ms() - t0;
+            st->setup_ms = t_setup;
+            st->mg_levels = A->mg ? mk::mg_levels(A->mg) : 0;
         }
         inner = r.status;
         if (inner != MK_OK) c->last_error = "mk_spd_solve: " + std::string(mk_status_str(inner));
@@ -328,6 +387,7 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
         try {
             op->ctx = c;
             op->n = nF;
+            op->m = nF;
             op->cplx = true;
             op->C = C;
             op->sigma = 1;
@@ -381,38 +441,10 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
 mk_status mk_op_export_csr(mk_ctx* c, const mk_op* A, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val) {
     if (!c || !A || !nnz) return MK_ERR_ARG;
     return guard(c, [&] {
-        const int vw = A->cplx ? 2 : 1;
-        if (!A->h_ptr.empty()) {
-            *nnz = A->nnz;
-            if (rowptr) std::copy(A->h_ptr.begin(), A->h_ptr.end(), rowptr);
-            if (col) std::copy(A->h_col.begin(), A->h_col.end(), col);
-            if (val) std::copy(A->h_val.begin(), A->h_val.end(), val);
-            return;
-        }
-        // device-assembled SELL: compact, dropping zero padding
-        std::vector<int64_t> sptr(size_t(A->nslices) + 1);
-        std::vector<int> sw(size_t(A->nslices)), hc(size_t(A->stored));
-        std::vector<double> hv(size_t(A->stored) * vw);
-        A->slice_ptr.download(sptr.data(), sptr.size(), c->stream);
-        A->slice_w.download(sw.data(), sw.size(), c->stream);
-        A->col.download(hc.data(), hc.size(), c->stream);
-        A->val64.download(hv.data(), hv.size(), c->stream);
-        MK_HIP(hipStreamSynchronize(c->stream));
-        std::vector<int64_t> rp(size_t(A->n) + 1, 0);
+        std::vector<int64_t> rp;
         std::vector<int> cc;
         std::vector<double> vv;
-        for (int r = 0; r < A->n; r++) {
-            const int s = r / A->C, lane = r % A->C;
-            for (int j = 0; j < sw[s]; j++) {
-                const int64_t e = sptr[s] + int64_t(j) * A->C + lane;
-                bool zero = true;
-                for (int q = 0; q < vw; q++) if (hv[e * vw + q] != 0.0) zero = false;
-                if (zero) continue;
-                cc.push_back(hc[e]);
-                for (int q = 0; q < vw; q++) vv.push_back(hv[e * vw + q]);
-            }
-            rp[r + 1] = int64_t(cc.size());
-        }
+        mk::host_csr(c, A, rp, cc, vv);
         *nnz = rp[A->n];
         if (rowptr) std::copy(rp.begin(), rp.end(), rowptr);
         if (col) std::copy(cc.begin(), cc.end(), col);
@@ -427,7 +459,10 @@ mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, c
     mk_status s = guard(c, [&] {
         const mk_solve_opts o = resolve(o_);
         MK_HIP(hipSetDevice(c->device));
-        const double t0 = now_ms();
+        double t0 = now_ms();
+        mk::ensure_mg(c, L, o);
+        const double t_setup = now_ms() - t0;
+        t0 = now_ms();
         mk::stats_reset(c);
         const int n = L->n;
         const size_t len = size_t(n) * 2;
@@ -478,6 +513,8 @@ mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, c
             st->rel_residual = last_rel;
             mk::stats_fill(c, st);
             st->ms = now_ms() - t0;
+            st->setup_ms = t_setup;
+            st->mg_levels = L->mg ? mk::mg_levels(L->mg) : 0;
         }
     });
     return s != MK_OK ? s : inner;

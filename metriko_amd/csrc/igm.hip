@@ -154,8 +154,8 @@ struct Ctx {
         hipLaunchKernelGGL(k_rowset_dot, dim3(R.nrows), dim3(256), 0, st(), R.nrows, R.ptr.p, R.col.p, R.val.p, x,
                            out_dev);
     }
-    mk::SolveResult solve(const mk_op* A, const double* b, double* x) {
-        mk::SolveResult r = mk::solve_spd(c, A, b, x, o);
+    mk::SolveResult solve(const mk_op* A, const double* b, double* x, mk::Mg* pre = nullptr) {
+        mk::SolveResult r = mk::solve_spd(c, A, b, x, o, pre);
         acc.iterations += r.iters;
         acc.refinements += r.refinements;
         acc.solves += 1;
@@ -329,11 +329,24 @@ extern "C" mk_status mk_iterative_rounding(
             for (int i = 0; i < k; i++) rhs[i] += dmean * ctg[i];
         }
         OpHolder hE, hA0, hX, hXt;
+        // the rounding operator A0 = wI^2 X2F^T X2F + wC^2 I, X2F = length * G2 * UFull, i.e.
+        // length * E + wC^2 I (wI = 1, wC = 0.01, iter_rounding_loop.h:109); its multigrid
+        // hierarchy preconditions both the KKT solve and every rounding solve.
+        const double wI = 1.0, wK = 10e4, wC = 0.01;
+        {
+            const double ts = now_ms();
+            SprsD A0 = E;
+            for (auto& v : A0.val) v *= length * wI * wI;
+            A0 = A0 + SprsD::identity(k, wC * wC);
+            hA0.op = make_op(c, A0, o);
+            mk::ensure_mg(c, hA0.op, o);
+            S.setup_ms += now_ms() - ts;
+        }
         hE.op = make_op(c, Eaug, o);
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
         drhs.upload(rhs.data(), rhs.size(), K.st());
         MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));
-        K.solve(hE.op, drhs.p, dXS.p);
+        K.solve(hE.op, drhs.p, dXS.p, hA0.op->mg);
         std::vector<double> XSmall(static_cast<size_t>(k));
         dXS.download(XSmall.data(), XSmall.size(), K.st());
         K.sync();
@@ -414,12 +427,6 @@ extern "C" mk_status mk_iterative_rounding(
 
             // ---- seamless rounding loop (iter_rounding.h:89-112, iter_rounding_loop.h) ----
             t0 = now_ms();
-            const double wI = 1.0, wK = 10e4, wC = 0.01;
-            // A0 = wI^2 X2F^T X2F + wC^2 I with X2F = length * G2 * UFull  ->  length * E + wC^2 I
-            SprsD A0 = E;
-            for (auto& v : A0.val) v *= length * wI * wI;
-            A0 = A0 + SprsD::identity(k, wC * wC);
-            hA0.op = make_op(c, A0, o);
             // b_int = wI^2 X2F^T F2_  (F2_ downloaded once; X2F^T product on the host)
             std::vector<double> hF2_(l1), bint(static_cast<size_t>(k));
             F2_.download(hF2_.data(), l1, K.st());

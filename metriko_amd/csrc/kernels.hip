@@ -88,14 +88,16 @@ __device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, 
 
 // ---------------------------------------------------------------------------------------
 // SELL SpMV.  T = float/double; CPLX: values/vectors interleaved (re, im).
-// MODE 0: y = A x; 1: y = A x, out[k] += Re(x^H y); 2: y = b - A x, out += |y|^2.
+// MODE 0: y = A x; 1: y = A x, out = Re(x^H y); 2: y = b - A x, out = |y|^2;
+// 3: y = b - A x; 4: y += A x.  skip (optional): device flag, nonzero = do nothing.
 template <typename T, bool CPLX, int MODE>
 __global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __restrict__ slice_ptr,
                                                const int* __restrict__ slice_w, const int* __restrict__ col,
                                                const T* __restrict__ val, const int* __restrict__ perm,
                                                const T* __restrict__ x, T* __restrict__ y,
                                                const T* __restrict__ b, double* partials, unsigned* counter,
-                                               double* out) {
+                                               double* out, const int* skip) {
+    if (skip && *skip) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     double contrib = 0.0;
     if (r < n) {
@@ -113,7 +115,8 @@ __global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __
                 ar += vr * xr - vi * xi;
                 ai += vr * xi + vi * xr;
             }
-            if constexpr (MODE == 2) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
+            if constexpr (MODE == 2 || MODE == 3) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
+            if constexpr (MODE == 4) { ar += y[2 * row]; ai += y[2 * row + 1]; }
             y[2 * row] = ar;
             y[2 * row + 1] = ai;
             if constexpr (MODE == 1) contrib = double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
@@ -124,13 +127,14 @@ __global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __
                 const int64_t e = base + int64_t(j) * C;
                 a += val[e] * x[col[e]];
             }
-            if constexpr (MODE == 2) a = b[row] - a;
+            if constexpr (MODE == 2 || MODE == 3) a = b[row] - a;
+            if constexpr (MODE == 4) a += y[row];
             y[row] = a;
             if constexpr (MODE == 1) contrib = double(x[row]) * double(a);
             if constexpr (MODE == 2) contrib = double(a) * double(a);
         }
     }
-    if constexpr (MODE != 0) {
+    if constexpr (MODE == 1 || MODE == 2) {
         double v[1] = {contrib}, tot[1];
         if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
     }
@@ -267,6 +271,82 @@ __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __
     }
 }
 
+// preconditioned variant (z from an external preconditioner): init, update, beta
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_init_z(int n, const T* __restrict__ r, const T* __restrict__ z,
+                                               T* __restrict__ p, CgState* st, double* partials, unsigned* counter) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < n) {
+        if constexpr (CPLX) {
+            const T r0 = r[2 * i], r1 = r[2 * i + 1], z0 = z[2 * i], z1 = z[2 * i + 1];
+            p[2 * i] = z0; p[2 * i + 1] = z1;
+            rz = double(r0) * z0 + double(r1) * z1;
+            rr = double(r0) * r0 + double(r1) * r1;
+        } else {
+            const T r0 = r[i], z0 = z[i];
+            p[i] = z0;
+            rz = double(r0) * z0;
+            rr = double(r0) * r0;
+        }
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it = 0;
+        st->status = MK_OK;
+        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
+        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (!st->done && !(tot[0] > 0.0)) { st->status = MK_ERR_NOT_SPD; st->done = 1; }
+    }
+}
+
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_update_noz(int n, T* __restrict__ x, T* __restrict__ r,
+                                                   const T* __restrict__ p, const T* __restrict__ q, CgState* st,
+                                                   double* partials, unsigned* counter) {
+    if (st->done) return;
+    const T alpha = T(st->alpha);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < n) {
+        const int w = CPLX ? 2 : 1;
+#pragma unroll
+        for (int c = 0; c < w; c++) {
+            const int k = w * i + c;
+            x[k] += alpha * p[k];
+            const T rk = r[k] - alpha * q[k];
+            r[k] = rk;
+            rr += double(rk) * rk;
+        }
+    }
+    double v[1] = {rr}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rr = tot[0];
+        st->it += 1;
+        if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (tot[0] <= st->tol2 * st->bb) st->done = 1;
+    }
+}
+
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) cg_beta(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st,
+                                             double* partials, unsigned* counter) {
+    if (st->done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0;
+    if (i < n) {
+        if constexpr (CPLX) rz = double(r[2 * i]) * z[2 * i] + double(r[2 * i + 1]) * z[2 * i + 1];
+        else rz = double(r[i]) * z[i];
+    }
+    double v[1] = {rz}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
+        else { st->beta = tot[0] / st->rz; st->rz = tot[0]; }
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) cg_pupdate(size_t len, T* __restrict__ p, const T* __restrict__ z,
                                                 const CgState* st) {
@@ -380,15 +460,15 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
                         hval32[e * vw + c] = float(val[k * vw + c]);
                     }
                 } else {
-                    hcol[e] = row >= 0 ? row : 0;  // padding: valid index, zero value
+                    hcol[e] = (row >= 0 && row < op->m) ? row : 0;  // padding: valid column, zero value
                 }
             }
         }
     }
-    // inverse diagonal (real part)
+    // inverse diagonal (real part); rectangular operators keep 1
     std::vector<double> dinv(n, 1.0);
     std::vector<float> dinv32(n, 1.0f);
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < n && op->m == n; i++) {
         double d = 0.0;
         for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += val[k * vw];
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
@@ -449,8 +529,8 @@ struct SpmvTimer {
 };
 
 template <typename T>
-static void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode,
-                        double* out, double bytes) {
+void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode, double* out,
+                 double bytes, const int* skip) {
     const int n = A->n;
     const unsigned g = blocks_for(n, A->block);
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
@@ -459,26 +539,33 @@ static void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* 
 #define MK_SPMV(CPLX, MODE)                                                                                \
     hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,        \
                        A->slice_ptr.p, A->slice_w.p, A->col.p, val, perm, x, y, b, c->partials.p,          \
-                       c->counter.p, out)
-    if (A->cplx) {
-        if (mode == SPMV_PLAIN) MK_SPMV(true, 0);
-        else if (mode == SPMV_DOT) MK_SPMV(true, 1);
-        else MK_SPMV(true, 2);
-    } else {
-        if (mode == SPMV_PLAIN) MK_SPMV(false, 0);
-        else if (mode == SPMV_DOT) MK_SPMV(false, 1);
-        else MK_SPMV(false, 2);
+                       c->counter.p, out, skip)
+#define MK_SPMV_ALL(CPLX)                            \
+    switch (mode) {                                  \
+        case SPMV_PLAIN: MK_SPMV(CPLX, 0); break;    \
+        case SPMV_DOT: MK_SPMV(CPLX, 1); break;      \
+        case SPMV_RESID: MK_SPMV(CPLX, 2); break;    \
+        case SPMV_RESID_NORED: MK_SPMV(CPLX, 3); break; \
+        case SPMV_ACCUM: MK_SPMV(CPLX, 4); break;    \
     }
+    if (A->cplx) { MK_SPMV_ALL(true) } else { MK_SPMV_ALL(false) }
+#undef MK_SPMV_ALL
 #undef MK_SPMV
     MK_HIP(hipGetLastError());
 }
+template void spmv_launch<double>(mk_ctx*, const mk_op*, const double*, const double*, double*, const double*,
+                                  SpmvMode, double*, double, const int*);
+template void spmv_launch<float>(mk_ctx*, const mk_op*, const float*, const float*, float*, const float*, SpmvMode,
+                                 double*, double, const int*);
 
 void spmv64(mk_ctx* c, const mk_op* A, const double* x, double* y, const double* b, SpmvMode mode, double* out,
             int*) {
-    spmv_launch<double>(c, A, A->val64.p, x, y, b, mode, out, A->bytes64 + (mode == SPMV_RESID ? A->n * 8.0 * (A->cplx ? 2 : 1) : 0));
+    const double extra = (mode == SPMV_RESID || mode == SPMV_RESID_NORED || mode == SPMV_ACCUM)
+                             ? A->n * 8.0 * (A->cplx ? 2 : 1) : 0.0;
+    spmv_launch<double>(c, A, A->val64.p, x, y, b, mode, out, A->bytes64 + extra, nullptr);
 }
 void spmv32(mk_ctx* c, const mk_op* A, const float* x, float* y, SpmvMode mode, double* out, int*) {
-    spmv_launch<float>(c, A, A->val32.p, x, y, nullptr, mode, out, A->bytes32);
+    spmv_launch<float>(c, A, A->val32.p, x, y, nullptr, mode, out, A->bytes32, nullptr);
 }
 
 void axpby64(mk_ctx* c, size_t len, double a, const double* x, double b, double* y) {
@@ -515,22 +602,81 @@ void add32to64_scaled(mk_ctx* c, size_t len, const float* d, const double* s_dev
 
 // Generic PCG on the device.  r must hold the initial residual b - A x on entry; bb_dev
 // points to ||b||^2 on the device.
+static inline void mg_apply(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { mg_apply64(c, g, b, x, skip); }
+static inline void mg_apply(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { mg_apply32(c, g, b, x, skip); }
+
+// one PCG iteration with an external (multigrid) preconditioner
+template <typename T>
+static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T* r, T* z, T* p, T* q, double bytes) {
+    const int n = A->n;
+    const size_t len = size_t(n) * (A->cplx ? 2 : 1);
+    const unsigned g = blocks_for(n);
+    CgState* st = c->state.p;
+    {
+        SpmvTimer tm(c, bytes);
+        if (A->cplx)
+            hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, n, A->C, A->slice_ptr.p,
+                               A->slice_w.p, A->col.p, val, A->perm.p, p, q, st, c->partials.p, c->counter.p);
+        else
+            hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, n, A->C, A->slice_ptr.p,
+                               A->slice_w.p, A->col.p, val, A->perm.p, p, q, st, c->partials.p, c->counter.p);
+    }
+    if (A->cplx)
+        hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, p, q, st,
+                           c->partials.p, c->counter.p);
+    else
+        hipLaunchKernelGGL((cg_update_noz<T, false>), dim3(g), dim3(256), 0, c->stream, n, x, r, p, q, st,
+                           c->partials.p, c->counter.p);
+    mg_apply(c, mg, r, z, &st->done);
+    if (A->cplx)
+        hipLaunchKernelGGL((cg_beta<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
+                           c->counter.p);
+    else
+        hipLaunchKernelGGL((cg_beta<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
+                           c->counter.p);
+    hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z, (const CgState*)st);
+}
+
 template <typename T>
 static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T* x, T* r, T* z, T* p, T* q,
-                       const double* bb_dev, double rel_tol, int maxiter, int check_every, double bytes) {
+                       const double* bb_dev, double rel_tol, int maxiter, int check_every, double bytes,
+                       Mg* mg = nullptr) {
     const int n = A->n;
     const size_t len = size_t(n) * (A->cplx ? 2 : 1);
     const unsigned g = blocks_for(n);
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
     CgState* st = c->state.p;
     hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, bb_dev, rel_tol * rel_tol);
-    if (A->cplx)
+    if (mg) {
+        mg_apply(c, mg, r, z, nullptr);
+        if (A->cplx)
+            hipLaunchKernelGGL((cg_init_z<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
+                               c->counter.p);
+        else
+            hipLaunchKernelGGL((cg_init_z<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
+                               c->counter.p);
+    } else if (A->cplx)
         hipLaunchKernelGGL((cg_init<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
                            c->partials.p, c->counter.p);
     else
         hipLaunchKernelGGL((cg_init<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
                            c->partials.p, c->counter.p);
     MK_HIP(hipGetLastError());
+    if (mg) {
+        CgState h{};
+        for (;;) {
+            MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+            MK_HIP(hipStreamSynchronize(c->stream));
+            timing_flush(c);
+            if (h.done || h.it >= maxiter) break;
+            const int batch = std::min(check_every, maxiter - h.it);
+            for (int b = 0; b < batch; b++) cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes);
+            MK_HIP(hipGetLastError());
+        }
+        CgResult res{h.status, h.it, h.rr, h.bb};
+        if (res.status == MK_OK && !h.done) res.status = MK_ERR_NO_CONVERGENCE;
+        return res;
+    }
     CgState h{};
     int launched = 0;
     for (;;) {
@@ -569,7 +715,7 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
 }
 
 CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_tol, int maxiter, bool jacobi,
-              int check_every, bool x_is_zero) {
+              int check_every, bool x_is_zero, Mg* mg) {
     Work& w = work(c);
     const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
     w.r.ensure(len); w.z.ensure(len); w.p.ensure(len); w.q.ensure(len);
@@ -579,11 +725,11 @@ CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_
     if (x_is_zero) MK_HIP(hipMemcpyAsync(w.r.p, b, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     else spmv64(c, A, x, w.r.p, b, SPMV_RESID, rr0, nullptr);
     return cg_run<double>(c, A, A->val64.p, jacobi ? A->dinv64.p : nullptr, x, w.r.p, w.z.p, w.p.p, w.q.p, bb,
-                          rel_tol, maxiter, check_every, A->bytes64);
+                          rel_tol, maxiter, check_every, A->bytes64, mg);
 }
 
 CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_tol, int maxiter, bool jacobi,
-              int check_every) {
+              int check_every, Mg* mg) {
     Work& w = work(c);
     const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
     w.r32.ensure(len); w.z32.ensure(len); w.p32.ensure(len); w.q32.ensure(len);
@@ -593,16 +739,18 @@ CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_to
     double one = 1.0;
     MK_HIP(hipMemcpyAsync(c->scal.p + 2, &one, sizeof(double), hipMemcpyHostToDevice, c->stream));
     return cg_run<float>(c, A, A->val32.p, jacobi ? A->dinv32.p : nullptr, x, w.r32.p, w.z32.p, w.p32.p, w.q32.p,
-                         c->scal.p + 2, rel_tol, maxiter, check_every, A->bytes32);
+                         c->scal.p + 2, rel_tol, maxiter, check_every, A->bytes32, mg);
 }
 
 // Iterative refinement: FP64 residual r = b - A x, FP32 PCG correction on r/||r||.
-SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o) {
+SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o, Mg* pre) {
     const bool jac = o.precond == MK_PRECOND_JACOBI;
+    Mg* mg = (o.precond == MK_PRECOND_AMG) ? (pre ? pre : A->mg) : nullptr;
+    MK_CHECK(o.precond != MK_PRECOND_AMG || mg, MK_ERR_ARG, "AMG preconditioner requested but not built");
     SolveResult res{MK_OK, 0, 0, 0.0};
     const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
     if (o.precision == MK_PREC_FP64) {
-        CgResult cr = cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false);
+        CgResult cr = cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false, mg);
         res.iters = cr.iters;
         // true FP64 residual
         Work& w = work(c);
@@ -635,7 +783,8 @@ SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, con
     int stagn = 0;
     // the FP32 work vectors used by cg32 are r32,z32,p32,q32; the normalised rhs lives in x32's twin
     DevBuf<float>& rhs32 = w.x32;
-    DevBuf<float> d32(len);
+    DevBuf<float>& d32 = w.d32;
+    d32.ensure(len);
     for (;;) {
         spmv64(c, A, x, w.t.p, b, SPMV_RESID, rr, nullptr);
         double hrr = 0.0;
@@ -651,7 +800,7 @@ SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, con
         prev = rel;
         cvt64to32_scaled(c, len, w.t.p, rr, rhs32.p);
         const double itol = std::max(o.inner_tol, 0.5 * o.tol / rel);
-        CgResult cr = cg32(c, A, rhs32.p, d32.p, std::min(itol, 0.5), o.maxiter - res.iters, jac, o.check_every);
+        CgResult cr = cg32(c, A, rhs32.p, d32.p, std::min(itol, 0.5), o.maxiter - res.iters, jac, o.check_every, mg);
         res.iters += cr.iters;
         res.refinements++;
         if (cr.status == MK_ERR_NOT_SPD || cr.status == MK_ERR_NAN) { res.status = cr.status; break; }

@@ -1,0 +1,504 @@
+// Smoothed-aggregation algebraic multigrid, used as the PCG preconditioner of the
+// field Laplacian (complex Hermitian) and of the IGM Poisson operators (real SPD).
+//
+// Setup (host, FP64): strength of connection |a_ij|^2 >= theta^2 |a_ii||a_jj|; greedy
+// aggregation (seed / attach / leftover passes); tentative prolongator from a near-null
+// vector built per aggregate by following the operator's own phases along a BFS tree
+// (B_j = -conj(a_ij)/|a_ij| B_i: parallel transport for the connection Laplacian,
+// constants for a plain Laplacian); Jacobi-smoothed prolongator
+// P = (I - 4/(3 rho) D^-1 A) P_tent; Galerkin coarse operator P^H A P; a dense inverse on
+// the coarsest level.
+// Solve (device, FP64 or FP32): symmetric V-cycle with Chebyshev(2) pre/post smoothing
+// in D^-1 A on [rho/30, 1.1 rho]; every operator is a SELL-C-sigma matrix and every
+// kernel honours the CG "done" flag so a converged solve costs nothing further.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <numeric>
+#include <random>
+#include "mk_internal.h"
+#include "metriko/core/common/sparse.h"
+
+namespace mk {
+
+template <typename T>
+void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode, double* out,
+                 double bytes, const int* skip);
+
+struct MgLevel {
+    mk_op* A = nullptr;
+    mk_op* P = nullptr;  // n_l x n_{l+1}
+    mk_op* R = nullptr;  // n_{l+1} x n_l
+    double lmax = 2.0;
+    int n = 0;
+    DevBuf<double> b64, x64, r64, d64;
+    DevBuf<float> b32, x32, r32, d32;
+};
+
+struct Mg {
+    bool cplx = false;
+    std::vector<MgLevel> lv;
+    int nc = 0;
+    DevBuf<double> inv64;  // dense coarsest inverse (row-major, complex interleaved)
+    DevBuf<float> inv32;
+    int deg = 2;
+    double setup_ms = 0.0;
+    int64_t nnz_total = 0;
+};
+
+void mg_free(Mg* g) {
+    if (!g) return;
+    for (auto& L : g->lv) {
+        delete L.A;
+        delete L.P;
+        delete L.R;
+    }
+    delete g;
+}
+
+namespace {
+
+using metriko::Csr;
+using cd = std::complex<double>;
+
+inline double mag2(double a) { return a * a; }
+inline double mag2(const cd& a) { return std::norm(a); }
+inline double mag(double a) { return std::abs(a); }
+inline double mag(const cd& a) { return std::abs(a); }
+inline double conjv(double a) { return a; }
+inline cd conjv(const cd& a) { return std::conj(a); }
+inline double re(double a) { return a; }
+inline double re(const cd& a) { return a.real(); }
+
+template <typename T>
+Csr<T> conj_transpose(const Csr<T>& A) {
+    Csr<T> t = A.transpose();
+    for (auto& v : t.val) v = conjv(v);
+    return t;
+}
+
+template <typename T>
+std::vector<double> diag_of(const Csr<T>& A) {
+    std::vector<double> d(size_t(A.nr), 0.0);
+    for (int i = 0; i < A.nr; i++)
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) if (A.col[k] == i) d[i] = re(A.val[k]);
+    return d;
+}
+
+// spectral radius of D^-1 A by power iteration (fixed seed)
+template <typename T>
+double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
+    const int n = A.nr;
+    std::mt19937_64 g(12345);
+    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    std::vector<T> x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
+    for (auto& v : x) {
+        if constexpr (std::is_same<T, double>::value) v = U(g);
+        else v = cd(U(g), U(g));
+    }
+    double lam = 1.0;
+    for (int it = 0; it < 15; it++) {
+        A.multiply(x.data(), y.data());
+        double nrm = 0.0;
+        for (int i = 0; i < n; i++) { y[i] = y[i] / (d[i] != 0.0 ? d[i] : 1.0); nrm += mag2(y[i]); }
+        nrm = std::sqrt(nrm);
+        double xn = 0.0;
+        for (auto& v : x) xn += mag2(v);
+        xn = std::sqrt(xn);
+        lam = nrm / (xn > 0 ? xn : 1.0);
+        if (nrm == 0.0) break;
+        for (int i = 0; i < n; i++) x[i] = y[i] / nrm;
+    }
+    return lam;
+}
+
+template <typename T>
+Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, int& nagg) {
+    const int n = A.nr;
+    std::vector<int64_t> sptr(size_t(n) + 1, 0);
+    std::vector<int> scol;
+    std::vector<T> sval;
+    for (int i = 0; i < n; i++) {
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
+            const int j = A.col[k];
+            if (j == i) continue;
+            if (mag2(A.val[k]) >= theta * theta * std::abs(d[i]) * std::abs(d[j])) {
+                scol.push_back(j);
+                sval.push_back(A.val[k]);
+            }
+        }
+        sptr[i + 1] = int64_t(scol.size());
+    }
+    std::vector<int> agg(size_t(n), -1), root;
+    nagg = 0;
+    for (int i = 0; i < n; i++) {  // seeds: nodes whose strong neighbourhood is untouched
+        if (agg[i] != -1) continue;
+        bool free_nb = true;
+        for (int64_t k = sptr[i]; k < sptr[i + 1]; k++) if (agg[scol[k]] != -1) { free_nb = false; break; }
+        if (!free_nb) continue;
+        agg[i] = nagg;
+        for (int64_t k = sptr[i]; k < sptr[i + 1]; k++) agg[scol[k]] = nagg;
+        root.push_back(i);
+        nagg++;
+    }
+    std::vector<int> agg1 = agg;
+    for (int i = 0; i < n; i++) {  // attach to the strongest seeded neighbour
+        if (agg[i] != -1) continue;
+        int best = -1;
+        double bv = -1.0;
+        for (int64_t k = sptr[i]; k < sptr[i + 1]; k++) {
+            const int j = scol[k];
+            if (agg1[j] >= 0 && mag(sval[k]) > bv) { bv = mag(sval[k]); best = agg1[j]; }
+        }
+        if (best >= 0) agg[i] = best;
+    }
+    for (int i = 0; i < n; i++) {  // leftovers
+        if (agg[i] != -1) continue;
+        agg[i] = nagg;
+        for (int64_t k = sptr[i]; k < sptr[i + 1]; k++) if (agg[scol[k]] == -1) agg[scol[k]] = nagg;
+        root.push_back(i);
+        nagg++;
+    }
+    // near-null vector by BFS inside each aggregate following the operator's phases
+    std::vector<T> B(size_t(n), T(0));
+    std::vector<uint8_t> seen(size_t(n), 0);
+    std::vector<int> q;
+    for (int a = 0; a < nagg; a++) {
+        const int r0 = root[a];
+        B[r0] = T(1);
+        seen[r0] = 1;
+        q.assign(1, r0);
+        for (size_t h = 0; h < q.size(); h++) {
+            const int u = q[h];
+            for (int64_t k = sptr[u]; k < sptr[u + 1]; k++) {
+                const int j = scol[k];
+                if (seen[j] || agg[j] != a) continue;
+                seen[j] = 1;
+                B[j] = -conjv(sval[k]) / mag(sval[k]) * B[u];
+                q.push_back(j);
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) if (!seen[i]) B[i] = T(1);
+    std::vector<double> nrm(size_t(nagg), 0.0);
+    for (int i = 0; i < n; i++) nrm[agg[i]] += mag2(B[i]);
+    Csr<T> P(n, nagg);
+    P.col.resize(size_t(n));
+    P.val.resize(size_t(n));
+    for (int i = 0; i < n; i++) {
+        P.col[i] = agg[i];
+        P.val[i] = B[i] / std::sqrt(nrm[agg[i]]);
+        P.ptr[i + 1] = i + 1;
+    }
+    return P;
+}
+
+template <typename T>
+void dense_hpd_inverse(const Csr<T>& A, std::vector<double>& out, bool cplx) {
+    const int n = A.nr;
+    std::vector<T> M(size_t(n) * n, T(0)), L(size_t(n) * n, T(0)), Inv(size_t(n) * n, T(0));
+    for (int i = 0; i < n; i++)
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) M[size_t(i) * n + A.col[k]] = A.val[k];
+    // Cholesky M = L L^H
+    for (int j = 0; j < n; j++) {
+        double s = re(M[size_t(j) * n + j]);
+        for (int k = 0; k < j; k++) s -= mag2(L[size_t(j) * n + k]);
+        if (!(s > 0.0)) s = std::max(1e-300, std::abs(re(M[size_t(j) * n + j])) * 1e-14 + 1e-300);
+        const double ljj = std::sqrt(s);
+        L[size_t(j) * n + j] = T(ljj);
+        for (int i = j + 1; i < n; i++) {
+            T t = M[size_t(i) * n + j];
+            for (int k = 0; k < j; k++) t -= L[size_t(i) * n + k] * conjv(L[size_t(j) * n + k]);
+            L[size_t(i) * n + j] = t / ljj;
+        }
+    }
+    // inverse column by column: L y = e_c, L^H x = y
+    std::vector<T> y(static_cast<size_t>(n)), x(static_cast<size_t>(n));
+    for (int c = 0; c < n; c++) {
+        for (int i = 0; i < n; i++) {
+            T t = (i == c) ? T(1) : T(0);
+            for (int k = 0; k < i; k++) t -= L[size_t(i) * n + k] * y[k];
+            y[i] = t / L[size_t(i) * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            T t = y[i];
+            for (int k = i + 1; k < n; k++) t -= conjv(L[size_t(k) * n + i]) * x[k];
+            x[i] = t / conjv(L[size_t(i) * n + i]);
+        }
+        for (int i = 0; i < n; i++) Inv[size_t(i) * n + c] = x[i];
+    }
+    const int vw = cplx ? 2 : 1;
+    out.assign(size_t(n) * n * vw, 0.0);
+    for (size_t i = 0; i < size_t(n) * n; i++) {
+        if constexpr (std::is_same<T, double>::value) out[i] = Inv[i];
+        else { out[2 * i] = Inv[i].real(); out[2 * i + 1] = Inv[i].imag(); }
+    }
+}
+
+template <typename T>
+mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o) {
+    auto* op = new mk_op();
+    try {
+        op->ctx = c;
+        op->n = A.nr;
+        op->m = A.nc;
+        op->cplx = !std::is_same<T, double>::value;
+        op->nnz = A.nonZeros();
+        op->block = o.block_size;
+        build_sell(op, A.ptr.data(), A.col.data(), reinterpret_cast<const double*>(A.val.data()), o.slice_height,
+                   o.format == MK_FMT_CSR ? 1 : o.sigma);
+    } catch (...) {
+        delete op;
+        throw;
+    }
+    return op;
+}
+
+template <typename T>
+Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta, int coarse_max) {
+    auto* g = new Mg();
+    try {
+        g->cplx = !std::is_same<T, double>::value;
+        Csr<T> A = A0;
+        const int maxLevels = 14;
+        for (int l = 0; l < maxLevels; l++) {
+            MgLevel L;
+            L.n = A.nr;
+            std::vector<double> d = diag_of(A);
+            L.lmax = 1.1 * rho_dinv_a(A, d);
+            L.A = upload(c, A, o);
+            g->nnz_total += A.nonZeros();
+            const bool last = A.nr <= coarse_max || l == maxLevels - 1;
+            if (last) {
+                std::vector<double> inv;
+                dense_hpd_inverse(A, inv, g->cplx);
+                g->nc = A.nr;
+                g->inv64.upload(inv.data(), inv.size(), c->stream);
+                std::vector<float> inv32(inv.begin(), inv.end());
+                g->inv32.upload(inv32.data(), inv32.size(), c->stream);
+                MK_HIP(hipStreamSynchronize(c->stream));
+                g->lv.push_back(std::move(L));
+                break;
+            }
+            int nagg = 0;
+            Csr<T> Pt = tentative(A, d, theta, nagg);
+            if (nagg >= A.nr * 0.9 || nagg < 1) {  // no coarsening possible: make this the coarsest level
+                if (A.nr > 4000) throw Error(MK_ERR_UNSUPPORTED, "multigrid: aggregation stalled on a large level");
+                std::vector<double> inv;
+                dense_hpd_inverse(A, inv, g->cplx);
+                g->nc = A.nr;
+                g->inv64.upload(inv.data(), inv.size(), c->stream);
+                std::vector<float> inv32(inv.begin(), inv.end());
+                g->inv32.upload(inv32.data(), inv32.size(), c->stream);
+                MK_HIP(hipStreamSynchronize(c->stream));
+                g->lv.push_back(std::move(L));
+                break;
+            }
+            const double rho = L.lmax / 1.1;
+            const double omega = 4.0 / (3.0 * rho);
+            Csr<T> AP = metriko::spgemm(A, Pt);
+            // P = Pt - omega D^-1 A Pt
+            std::vector<metriko::Triplet<T>> tr;
+            tr.reserve(size_t(AP.nonZeros() + Pt.nonZeros()));
+            for (int i = 0; i < A.nr; i++) {
+                for (int64_t k = Pt.ptr[i]; k < Pt.ptr[i + 1]; k++) tr.emplace_back(i, Pt.col[k], Pt.val[k]);
+                const double di = d[i] != 0.0 ? d[i] : 1.0;
+                for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++) tr.emplace_back(i, AP.col[k], -omega / di * AP.val[k]);
+            }
+            Csr<T> P(A.nr, nagg);
+            P.setFromTriplets(tr);
+            Csr<T> R = conj_transpose(P);
+            Csr<T> Ac = metriko::spgemm(R, metriko::spgemm(A, P));
+            // exact Hermitian symmetry of the coarse operator
+            Csr<T> AcH = conj_transpose(Ac);
+            for (size_t k = 0; k < Ac.val.size(); k++) Ac.val[k] *= 0.5;
+            for (auto& v : AcH.val) v *= 0.5;
+            Ac = Ac + AcH;
+            L.P = upload(c, P, o);
+            L.R = upload(c, R, o);
+            g->lv.push_back(std::move(L));
+            A = std::move(Ac);
+        }
+        for (auto& L : g->lv) {
+            const size_t len = size_t(L.n) * (g->cplx ? 2 : 1);
+            L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len);
+            L.b32.alloc(len); L.x32.alloc(len); L.r32.alloc(len); L.d32.alloc(len);
+        }
+    } catch (...) {
+        mg_free(g);
+        throw;
+    }
+    return g;
+}
+
+// ---- device kernels ------------------------------------------------------------------------
+// x = c0 * dinv .* r (zero start) or x += ...; d = same increment.  Vectors of n elements (x2 complex)
+template <typename T, bool CPLX>
+__global__ void k_cheb_first(int n, const T* __restrict__ dinv, const T* __restrict__ r, T* __restrict__ d,
+                             T* __restrict__ x, T c0, int zero_start, const int* skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T s = c0 * dinv[i];
+    if constexpr (CPLX) {
+        const T d0 = s * r[2 * i], d1 = s * r[2 * i + 1];
+        d[2 * i] = d0; d[2 * i + 1] = d1;
+        if (zero_start) { x[2 * i] = d0; x[2 * i + 1] = d1; }
+        else { x[2 * i] += d0; x[2 * i + 1] += d1; }
+    } else {
+        const T d0 = s * r[i];
+        d[i] = d0;
+        if (zero_start) x[i] = d0;
+        else x[i] += d0;
+    }
+}
+template <typename T, bool CPLX>
+__global__ void k_cheb_next(int n, const T* __restrict__ dinv, const T* __restrict__ r, T* __restrict__ d,
+                            T* __restrict__ x, T a, T bcoef, const int* skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T s = bcoef * dinv[i];
+    if constexpr (CPLX) {
+        const T d0 = a * d[2 * i] + s * r[2 * i], d1 = a * d[2 * i + 1] + s * r[2 * i + 1];
+        d[2 * i] = d0; d[2 * i + 1] = d1;
+        x[2 * i] += d0; x[2 * i + 1] += d1;
+    } else {
+        const T d0 = a * d[i] + s * r[i];
+        d[i] = d0;
+        x[i] += d0;
+    }
+}
+// x = Inv b (dense n x n, row-major), one thread per row
+template <typename T, bool CPLX>
+__global__ void k_dense_gemv(int n, const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x,
+                             const int* skip) {
+    if (skip && *skip) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (CPLX) {
+        T ar = 0, ai = 0;
+        for (int j = 0; j < n; j++) {
+            const T mr = inv[2 * (size_t(i) * n + j)], mi = inv[2 * (size_t(i) * n + j) + 1];
+            const T br = b[2 * j], bi = b[2 * j + 1];
+            ar += mr * br - mi * bi;
+            ai += mr * bi + mi * br;
+        }
+        x[2 * i] = ar;
+        x[2 * i + 1] = ai;
+    } else {
+        T a = 0;
+        for (int j = 0; j < n; j++) a += inv[size_t(i) * n + j] * b[j];
+        x[i] = a;
+    }
+}
+
+inline unsigned nb(size_t n) { return unsigned((n + 255) / 256); }
+
+template <typename T>
+struct Sel;
+template <>
+struct Sel<double> {
+    static const double* val(const mk_op* A) { return A->val64.p; }
+    static const double* dinv(const mk_op* A) { return A->dinv64.p; }
+    static double* b(MgLevel& L) { return L.b64.p; }
+    static double* x(MgLevel& L) { return L.x64.p; }
+    static double* r(MgLevel& L) { return L.r64.p; }
+    static double* d(MgLevel& L) { return L.d64.p; }
+    static const double* inv(const Mg* g) { return g->inv64.p; }
+    static double bytes(const mk_op* A) { return A->bytes64; }
+};
+template <>
+struct Sel<float> {
+    static const float* val(const mk_op* A) { return A->val32.p; }
+    static const float* dinv(const mk_op* A) { return A->dinv32.p; }
+    static float* b(MgLevel& L) { return L.b32.p; }
+    static float* x(MgLevel& L) { return L.x32.p; }
+    static float* r(MgLevel& L) { return L.r32.p; }
+    static float* d(MgLevel& L) { return L.d32.p; }
+    static const float* inv(const Mg* g) { return g->inv32.p; }
+    static double bytes(const mk_op* A) { return A->bytes32; }
+};
+
+template <typename T>
+void chebyshev(mk_ctx* c, const Mg* g, MgLevel& L, const T* b, T* x, bool zero_start, const int* skip) {
+    const double lmax = L.lmax, lmin = L.lmax / 30.0;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    T* r = Sel<T>::r(L);
+    T* d = Sel<T>::d(L);
+    const int n = L.n;
+    const T* src = b;
+    if (!zero_start) {
+        spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
+        src = r;
+    }
+    if (g->cplx)
+        hipLaunchKernelGGL((k_cheb_first<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), src,
+                           d, x, T(1.0 / theta), zero_start ? 1 : 0, skip);
+    else
+        hipLaunchKernelGGL((k_cheb_first<T, false>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), src,
+                           d, x, T(1.0 / theta), zero_start ? 1 : 0, skip);
+    for (int k = 1; k < g->deg; k++) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
+        const T a = T(rho1 * rho0), bc = T(2.0 * rho1 / delta);
+        if (g->cplx)
+            hipLaunchKernelGGL((k_cheb_next<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), r,
+                               d, x, a, bc, skip);
+        else
+            hipLaunchKernelGGL((k_cheb_next<T, false>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), r,
+                               d, x, a, bc, skip);
+        rho0 = rho1;
+    }
+}
+
+template <typename T>
+void vcycle(mk_ctx* c, Mg* g, int l, const T* b, T* x, const int* skip) {
+    MgLevel& L = g->lv[size_t(l)];
+    if (l + 1 == int(g->lv.size())) {
+        if (g->cplx)
+            hipLaunchKernelGGL((k_dense_gemv<T, true>), dim3(nb(g->nc)), dim3(256), 0, c->stream, g->nc,
+                               Sel<T>::inv(g), b, x, skip);
+        else
+            hipLaunchKernelGGL((k_dense_gemv<T, false>), dim3(nb(g->nc)), dim3(256), 0, c->stream, g->nc,
+                               Sel<T>::inv(g), b, x, skip);
+        return;
+    }
+    chebyshev<T>(c, g, L, b, x, true, skip);
+    T* r = Sel<T>::r(L);
+    spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
+    MgLevel& Cn = g->lv[size_t(l) + 1];
+    spmv_launch<T>(c, L.R, Sel<T>::val(L.R), r, Sel<T>::b(Cn), nullptr, SPMV_PLAIN, nullptr, Sel<T>::bytes(L.R), skip);
+    vcycle<T>(c, g, l + 1, Sel<T>::b(Cn), Sel<T>::x(Cn), skip);
+    spmv_launch<T>(c, L.P, Sel<T>::val(L.P), Sel<T>::x(Cn), x, nullptr, SPMV_ACCUM, nullptr, Sel<T>::bytes(L.P), skip);
+    chebyshev<T>(c, g, L, b, x, false, skip);
+}
+
+}  // namespace
+
+Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o) {
+    Csr<double> A(n, n);
+    A.ptr.assign(ptr, ptr + n + 1);
+    A.col.assign(col, col + ptr[n]);
+    A.val.assign(val, val + ptr[n]);
+    return build_impl<double>(c, A, o, 0.08, 500);
+}
+Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val,
+                     const mk_solve_opts& o) {
+    Csr<cd> A(n, n);
+    A.ptr.assign(ptr, ptr + n + 1);
+    A.col.assign(col, col + ptr[n]);
+    A.val.resize(size_t(ptr[n]));
+    for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
+    return build_impl<cd>(c, A, o, 0.08, 400);
+}
+void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
+void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { vcycle<float>(c, g, 0, b, x, skip); }
+int mg_levels(const Mg* g) { return int(g->lv.size()); }
+double mg_setup_ms(const Mg* g) { return g->setup_ms; }
+void mg_set_setup_ms(Mg* g, double ms) { g->setup_ms = ms; }
+int64_t mg_nnz(const Mg* g) { return g->nnz_total; }
+int mg_level_rows(const Mg* g, int l) { return g->lv[size_t(l)].n; }
+
+}  // namespace mk

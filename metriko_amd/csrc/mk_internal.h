@@ -10,6 +10,8 @@
 
 namespace mk {
 
+struct Mg;
+
 struct Error : std::runtime_error {
     mk_status code;
     Error(mk_status c, const std::string& m) : std::runtime_error(m), code(c) {}
@@ -73,8 +75,19 @@ struct CgState {
 // per-context scratch vectors
 struct Work {
     DevBuf<double> r, z, p, q, t;
-    DevBuf<float> r32, z32, p32, q32, x32;
+    DevBuf<float> r32, z32, p32, q32, x32, d32;
 };
+
+// multigrid (mg.hip)
+Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
+Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
+void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip);
+void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
+int mg_levels(const Mg* g);
+// builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
+void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
+int64_t mg_nnz(const Mg* g);
+int mg_level_rows(const Mg* g, int l);
 
 }  // namespace mk
 
@@ -97,9 +110,14 @@ struct mk_ctx {
     double spmv_bytes = 0.0;
 };
 
+namespace mk { struct Mg; void mg_free(Mg*); }
+
 struct mk_op {
     mk_ctx* ctx = nullptr;
-    int n = 0;
+    int n = 0;          // rows
+    int m = 0;          // columns (== n for square operators)
+    mk::Mg* mg = nullptr;  // multigrid hierarchy used as preconditioner (owned)
+    ~mk_op() { if (mg) mk::mg_free(mg); }
     bool cplx = false;
     int64_t nnz = 0;
     // SELL-C-sigma layout (C = slice height; rows sorted by length inside windows of sigma)
@@ -126,7 +144,9 @@ struct mk_op {
 namespace mk {
 
 // ---- kernels / launchers (kernels.hip) -------------------------------------------------
-enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2 };
+// PLAIN: y = A x; DOT: y = A x and x^H y; RESID: y = b - A x and |y|^2;
+// RESID_NORED: y = b - A x; ACCUM: y += A x
+enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2, SPMV_RESID_NORED = 3, SPMV_ACCUM = 4 };
 
 void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma);
 // y = A x (mode PLAIN); y = A x and partial x^H y (DOT); y = b - A x and partial |y|^2 (RESID)
@@ -149,13 +169,14 @@ void add32to64_scaled(mk_ctx* c, size_t len, const float* d, const double* s_dev
 // CG on the device: solves A x = b for FP64 (x,b device) -- returns status, iterations
 struct CgResult { int status; int iters; double rr; double bb; };
 CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_tol, int maxiter, bool jacobi,
-              int check_every, bool x_is_zero);
+              int check_every, bool x_is_zero, Mg* mg = nullptr);
 CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_tol, int maxiter, bool jacobi,
-              int check_every);
+              int check_every, Mg* mg = nullptr);
 
-// refinement driver: x (device FP64) warm start, b device FP64
+// refinement driver: x (device FP64) warm start, b device FP64; pre overrides A's own multigrid
 struct SolveResult { int status; int iters; int refinements; double rel_residual; };
-SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o);
+SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o,
+                      Mg* pre = nullptr);
 
 inline Work& work(mk_ctx* c) { return c->work; }
 
