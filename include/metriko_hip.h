@@ -72,6 +72,10 @@ mk_status mk_finalize(mk_ctx* ctx);
 mk_status mk_last_error(mk_ctx* ctx, char* buf, int len);
 /* Enable per-launch hipEvent timing of SpMV kernels (reported in mk_stats.spmv_ms). */
 mk_status mk_set_timing(mk_ctx* ctx, int on);
+/* JSON array of per-kernel SpMV timings recorded since the last reset: one object per
+ * "<operator>:<kernel>:<fp32|fp64>" key with calls, ms (hipEvent), algorithmic bytes,
+ * bytes per call and rows.  reset != 0 clears the table afterwards. */
+mk_status mk_timing_report(mk_ctx* ctx, int reset, char* buf, int len);
 
 /* --- generic sparse operators ---------------------------------------------------------
  * Square sparse matrix in CSR (int32 column ids, int64 row pointers), stored on the
@@ -123,6 +127,7 @@ typedef struct {
 } mk_igm_flags;
 typedef struct {
     int rounds, lm_rounds, k, kernel_rank, n_fixed;
+    int null_vectors;  /* near-null vectors given to the A0 multigrid (0 = one-vector rule) */
     double kkt_rel_residual, setup_ms, kkt_ms, naive_ms, rounding_ms;
     mk_stats solve;
 } mk_igm_stats;
@@ -163,6 +168,14 @@ mk_status mk_pipeline_run(mk_ctx* ctx, int nV, int nF, const double* V, const in
 mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
                            int32_t* singular, int32_t* raw_matching, int32_t* matching, uint8_t* seam,
                            int32_t* cut_idx, int32_t* sizes, double* ext);
+
+/* Host-only multigrid PCG (no GPU; test and tuning aid): builds the same smoothed-aggregation
+ * hierarchy the device path uses for the CSR matrix (complex when cplx != 0, values
+ * interleaved) and solves A x = b with a host V-cycle.  Returns iterations, levels and the
+ * final relative residual. */
+mk_status mk_mg_host_pcg(int n, const int64_t* rowptr, const int32_t* col, const double* val, int cplx,
+                         const double* nullvec, int nnull, const double* b, double* x, double tol, int maxit,
+                         int* iters, int* levels, double* relres);
 
 /* Halfedge arrays of the library's Hmesh, as mk_field_laplacian takes them: face_halfs[3nF],
  * twin[nH], half_face[nH], farg[nH] (heArgOnFace), face_area[nF]; nH = 2 nE (query with

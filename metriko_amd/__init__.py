@@ -19,10 +19,10 @@ FMT_CSR, FMT_SELL = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_AMG = 0, 1, 2
 
 EXPORTED = [
-    "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing",
+    "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing", "mk_timing_report",
     "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
-    "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges",
+    "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
 ]
 
 
@@ -46,7 +46,8 @@ class IgmFlags(C.Structure):
 
 class IgmStats(C.Structure):
     _fields_ = [("rounds", C.c_int), ("lm_rounds", C.c_int), ("k", C.c_int), ("kernel_rank", C.c_int),
-                ("n_fixed", C.c_int), ("kkt_rel_residual", C.c_double), ("setup_ms", C.c_double),
+                ("n_fixed", C.c_int), ("null_vectors", C.c_int), ("kkt_rel_residual", C.c_double),
+                ("setup_ms", C.c_double),
                 ("kkt_ms", C.c_double), ("naive_ms", C.c_double), ("rounding_ms", C.c_double), ("solve", Stats)]
 
 
@@ -87,6 +88,7 @@ def lib():
     L.mk_finalize.argtypes = [_P]
     L.mk_last_error.argtypes = [_P, C.c_char_p, C.c_int]
     L.mk_set_timing.argtypes = [_P, C.c_int]
+    L.mk_timing_report.argtypes = [_P, C.c_int, C.c_char_p, C.c_int]
     L.mk_op_create_real.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
     L.mk_op_create_complex.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
     L.mk_op_destroy.argtypes = [_P]
@@ -107,6 +109,8 @@ def lib():
     L.mk_combinatorics.argtypes = [C.c_int, C.c_int, _D, _I32, C.c_int, _D, _I32, _I32, _I32, _U8, _I32, _I32, _D]
     L.mk_hmesh_export.argtypes = [C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), _I32, _I32, _I32, _I32, _D, _D]
     L.mk_hmesh_halfedges.argtypes = [C.c_int, C.c_int, _D, _I32, _I32, _I32, _I32, _D, _D]
+    L.mk_mg_host_pcg.argtypes = [C.c_int, _I64, _I32, _D, C.c_int, _D, C.c_int, _D, _D, C.c_double, C.c_int,
+                                 C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     for name in EXPORTED:
         if name not in ("mk_status_str", "mk_default_opts"):
             getattr(L, name).restype = C.c_int
@@ -183,6 +187,39 @@ def hmesh_halfedges(V, F):
     return {"face_halfs": fh, "twin": tw, "half_face": hf, "farg": fa, "face_area": ar}
 
 
+def mg_host_pcg(A, b, tol=1e-10, maxit=500, nullvec=None):
+    """Host-only multigrid-preconditioned CG on a CSR matrix (same hierarchy as the GPU path).
+    `nullvec` (n x k, real operators only) are near-null vectors for the aggregation.
+    Returns (x, iterations, levels, relative residual)."""
+    import scipy.sparse as sp
+    L = lib()
+    cplx = np.iscomplexobj(A) or np.iscomplexobj(b)
+    A = sp.csr_matrix(A, dtype=np.complex128 if cplx else np.float64)
+    A.sort_indices()
+    ptr = A.indptr.astype(np.int64)
+    col = A.indices.astype(np.int32)
+    if cplx:
+        val = np.ascontiguousarray(np.stack([A.data.real, A.data.imag], 1).reshape(-1))
+        bb = np.ascontiguousarray(np.stack([np.real(b), np.imag(b)], 1).reshape(-1))
+    else:
+        val = A.data.astype(np.float64)
+        bb = np.ascontiguousarray(b, np.float64)
+    x = np.zeros_like(bb)
+    nnull, nv = 0, None
+    if nullvec is not None:
+        nv = np.ascontiguousarray(np.asarray(nullvec, np.float64).reshape(A.shape[0], -1).T)
+        nnull = nv.shape[0]
+    it, lv, rr = C.c_int(0), C.c_int(0), C.c_double(0)
+    s = L.mk_mg_host_pcg(A.shape[0], _ptr(ptr, _I64), _ptr(col, _I32), _ptr(val, _D), int(cplx),
+                         _ptr(nv, _D) if nnull else None, nnull, _ptr(bb, _D), _ptr(x, _D), tol, maxit,
+                         C.byref(it), C.byref(lv), C.byref(rr))
+    if s:
+        raise MetrikoError(s, "mk_mg_host_pcg")
+    if cplx:
+        x = x[0::2] + 1j * x[1::2]
+    return x, it.value, lv.value, rr.value
+
+
 def combinatorics(V, F, compressed, N=4):
     """Host-only stages after the field solve (roots, Curl matching, singularities, seam,
     cut mesh, combed matching, IGM setup sizes); no GPU needed."""
@@ -246,6 +283,13 @@ class Context:
 
     def set_timing(self, on):
         self._check(self.L.mk_set_timing(self.h, int(on)), "mk_set_timing")
+
+    def timing_report(self, reset=True):
+        """Per-kernel SpMV timings (hipEvent) since the last reset, as a list of dicts."""
+        import json
+        buf = C.create_string_buffer(1 << 20)
+        self._check(self.L.mk_timing_report(self.h, int(reset), buf, 1 << 20), "mk_timing_report")
+        return json.loads(buf.value.decode())
 
     # --- operators -------------------------------------------------------------------
     def op_real(self, A, o=None):

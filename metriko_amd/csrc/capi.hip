@@ -137,25 +137,47 @@ void host_csr(mk_ctx* c, const mk_op* A, std::vector<int64_t>& rp, std::vector<i
     A->val64.download(hv.data(), hv.size(), c->stream);
     std::vector<int> perm;
     if (A->perm.p) {
-        perm.resize(size_t(A->n));
+        perm.resize(size_t(A->n_short));
         A->perm.download(perm.data(), perm.size(), c->stream);
     }
+    std::vector<int> lrows(size_t(A->nlong)), lcol;
+    std::vector<int64_t> lptr(size_t(A->nlong) + 1, 0);
+    std::vector<double> lval;
+    if (A->nlong) {
+        A->long_rows.download(lrows.data(), lrows.size(), c->stream);
+        A->long_ptr.download(lptr.data(), lptr.size(), c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        lcol.resize(size_t(lptr.back()));
+        lval.resize(size_t(lptr.back()) * vw);
+        A->long_col.download(lcol.data(), lcol.size(), c->stream);
+        A->long_val64.download(lval.data(), lval.size(), c->stream);
+    }
     MK_HIP(hipStreamSynchronize(c->stream));
-    std::vector<int> inv(size_t(A->n));
-    for (int r = 0; r < A->n; r++) inv[perm.empty() ? r : perm[r]] = r;
+    // row -> (kind, position): kind 0 = SELL position, 1 = long-row index
+    std::vector<int> kind(size_t(A->n), -1), pos(size_t(A->n), -1);
+    for (int r = 0; r < A->n_short; r++) { const int row = perm.empty() ? r : perm[r]; kind[row] = 0; pos[row] = r; }
+    for (int w = 0; w < A->nlong; w++) { kind[lrows[w]] = 1; pos[lrows[w]] = w; }
     rp.assign(size_t(A->n) + 1, 0);
     cc.clear();
     vv.clear();
     for (int row = 0; row < A->n; row++) {
-        const int r = inv[row];
-        const int s = r / A->C, lane = r % A->C;
-        for (int j = 0; j < sw[s]; j++) {
-            const int64_t e = sptr[s] + int64_t(j) * A->C + lane;
-            bool zero = true;
-            for (int q = 0; q < vw; q++) if (hv[e * vw + q] != 0.0) zero = false;
-            if (zero) continue;
-            cc.push_back(hc[e]);
-            for (int q = 0; q < vw; q++) vv.push_back(hv[e * vw + q]);
+        if (kind[row] == 0) {
+            const int r = pos[row];
+            const int s = r / A->C, lane = r % A->C;
+            for (int j = 0; j < sw[s]; j++) {
+                const int64_t e = sptr[s] + int64_t(j) * A->C + lane;
+                bool zero = true;
+                for (int q = 0; q < vw; q++) if (hv[e * vw + q] != 0.0) zero = false;
+                if (zero) continue;
+                cc.push_back(hc[e]);
+                for (int q = 0; q < vw; q++) vv.push_back(hv[e * vw + q]);
+            }
+        } else if (kind[row] == 1) {
+            const int w = pos[row];
+            for (int64_t k = lptr[w]; k < lptr[w + 1]; k++) {
+                cc.push_back(lcol[k]);
+                for (int q = 0; q < vw; q++) vv.push_back(lval[k * vw + q]);
+            }
         }
         rp[row + 1] = int64_t(cc.size());
     }
@@ -170,7 +192,9 @@ void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
     std::vector<double> vv;
     host_csr(c, A, rp, cc, vv);
     A->mg = A->cplx ? mg_build_complex(c, A->n, rp.data(), cc.data(), vv.data(), o)
-                    : mg_build_real(c, A->n, rp.data(), cc.data(), vv.data(), o);
+                    : mg_build_real(c, A->n, rp.data(), cc.data(), vv.data(), o,
+                                    A->nnull ? &A->nullvec : nullptr, A->nnull);
+    mg_set_labels(A->mg, A->label);
 }
 
 void stats_reset(mk_ctx* c) {
@@ -272,6 +296,27 @@ mk_status mk_set_timing(mk_ctx* c, int on) {
     if (!c) return MK_ERR_ARG;
     c->time_spmv = on != 0;
     return MK_OK;
+}
+
+mk_status mk_timing_report(mk_ctx* c, int reset, char* buf, int len) {
+    if (!c || !buf || len <= 0) return MK_ERR_ARG;
+    return guard(c, [&] {
+        mk::timing_flush(c);
+        std::string s = "[";
+        char tmp[512];
+        for (size_t i = 0; i < c->kstats.size(); i++) {
+            const auto& k = c->kstats[i].second;
+            std::snprintf(tmp, sizeof(tmp),
+                          "%s{\"key\":\"%s\",\"calls\":%lld,\"ms\":%.6f,\"bytes\":%.1f,\"bytes_per_call\":%.1f,\"rows\":%d}",
+                          i ? "," : "", c->kstats[i].first.c_str(), (long long)k.calls, k.ms, k.bytes,
+                          k.bytes_per_call, k.rows);
+            s += tmp;
+        }
+        s += "]";
+        MK_CHECK(int(s.size()) < len, MK_ERR_ARG, "timing report buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+        if (reset) c->kstats.clear();
+    });
 }
 
 static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
@@ -388,6 +433,9 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
             op->ctx = c;
             op->n = nF;
             op->m = nF;
+            op->label = "field_L";
+            op->n_short = nF;
+            op->nlong = 0;
             op->cplx = true;
             op->C = C;
             op->sigma = 1;

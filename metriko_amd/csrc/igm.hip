@@ -228,6 +228,105 @@ bool lm_solve(Ctx& K, size_t n, double* x0, double* x_out, int maxIter, double f
     return ret;
 }
 
+// Near-null vectors of E = X2F^T X2F for the multigrid setup: the constant parameterisations
+// (u, v) = (1, 0) and (0, 1).  RosyParameterization lays the variables out as n dofs per
+// vertex (component = index mod n) followed by n dofs per seam transition, and the transition
+// dofs are exactly integerIdcs (setup.ipp:163-170).  The vertex dofs take the constant; the
+// transition dofs solve min ||G2 y||^2 + ||Cfull y||^2 (consistent: the constant map is
+// seamless; transitions absent from G2 are fixed by the constraints), normal equations by
+// Jacobi-CG on the host.  The
+// compressed coordinates are read off UFull: identity rows for nonPart, and the unit rows
+// FullPivLU::kernel puts at each kernel column's free variable.  Returns n*k values
+// (column-major), or nothing when a step does not apply or the vectors are not null to
+// round-off; only the multigrid convergence rate depends on them.
+std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F, int n, const int* intIdx, int nint,
+                              const std::vector<int>& nonPart, const std::vector<int>& uniqueJ,
+                              const metriko::MatXd& US) {
+    const int nfull = G2.cols(), k = X2F.cols(), npn = int(nonPart.size());
+    if (n < 1 || n > 4) return {};
+    std::vector<int> tpos(size_t(nfull), -1);
+    for (int i = 0; i < nint; i++) if (tpos[intIdx[i]] < 0) tpos[intIdx[i]] = i;
+    // free-variable (unit) row of each kernel column
+    const int dimker = US.rows() ? US.cols() : 0;
+    std::vector<int> freeRow(size_t(dimker), -1);
+    for (int r = 0; r < US.rows(); r++) {
+        int nzj = -1, cnt = 0;
+        for (int j = 0; j < dimker && cnt < 2; j++) if (US(r, j) != 0.0) { nzj = j; cnt++; }
+        if (cnt == 1 && US(r, nzj) == 1.0 && freeRow[nzj] < 0) freeRow[nzj] = r;
+    }
+    for (int j = 0; j < dimker; j++) if (freeRow[j] < 0) return {};
+    // S = [G2; Cfull] split into transition columns (GT) and the rest
+    const int srows = G2.rows() + Cf.rows();
+    std::vector<TripD> tt;
+    for (int i = 0; i < G2.rows(); i++)
+        for (int64_t q = G2.ptr[i]; q < G2.ptr[i + 1]; q++)
+            if (tpos[G2.col[q]] >= 0) tt.emplace_back(i, tpos[G2.col[q]], G2.val[q]);
+    for (int i = 0; i < Cf.rows(); i++)
+        for (int64_t q = Cf.ptr[i]; q < Cf.ptr[i + 1]; q++)
+            if (tpos[Cf.col[q]] >= 0) tt.emplace_back(G2.rows() + i, tpos[Cf.col[q]], Cf.val[q]);
+    SprsD GT(srows, std::max(nint, 1));
+    GT.setFromTriplets(tt);
+    const SprsD GTt = GT.transpose();
+    const SprsD M = GTt * GT;
+    std::vector<double> dg(size_t(M.rows()), 0.0);
+    for (int i = 0; i < M.rows(); i++)
+        for (int64_t q = M.ptr[i]; q < M.ptr[i + 1]; q++) if (M.col[q] == i) dg[i] = M.val[q];
+    std::vector<double> out(size_t(k) * n, 0.0);
+    for (int c = 0; c < n; c++) {
+        std::vector<double> y(size_t(nfull), 0.0);
+        for (int i = 0; i < nfull; i++) if (tpos[i] < 0 && i % n == c) y[i] = 1.0;
+        if (nint > 0) {
+            std::vector<double> g(static_cast<size_t>(srows)), b(static_cast<size_t>(nint)), x(static_cast<size_t>(nint), 0.0);
+            G2.multiply(y.data(), g.data());
+            if (Cf.rows() > 0) Cf.multiply(y.data(), g.data() + G2.rows());
+            GTt.multiply(g.data(), b.data());
+            for (auto& v : b) v = -v;
+            // Jacobi-CG on the (possibly semidefinite, consistent) normal equations
+            std::vector<double> r(b), z(static_cast<size_t>(nint)), p, q(static_cast<size_t>(nint));
+            auto prec = [&](const std::vector<double>& a, std::vector<double>& o) {
+                for (int i = 0; i < nint; i++) o[i] = dg[i] > 0.0 ? a[i] / dg[i] : 0.0;
+            };
+            prec(r, z);
+            p = z;
+            double rz = 0.0, bb = 0.0;
+            for (int i = 0; i < nint; i++) { rz += r[i] * z[i]; bb += b[i] * b[i]; }
+            for (int it = 0; it < 20000 && bb > 0.0; it++) {
+                M.multiply(p.data(), q.data());
+                double pq = 0.0;
+                for (int i = 0; i < nint; i++) pq += p[i] * q[i];
+                if (!(pq > 0.0)) break;
+                const double al = rz / pq;
+                double rr = 0.0;
+                for (int i = 0; i < nint; i++) { x[i] += al * p[i]; r[i] -= al * q[i]; rr += r[i] * r[i]; }
+                if (rr <= 1e-30 * bb) break;
+                prec(r, z);
+                double rz2 = 0.0;
+                for (int i = 0; i < nint; i++) rz2 += r[i] * z[i];
+                const double be = rz2 / rz;
+                rz = rz2;
+                for (int i = 0; i < nint; i++) p[i] = z[i] + be * p[i];
+            }
+            for (int i = 0; i < nfull; i++) if (tpos[i] >= 0) y[i] = x[tpos[i]];
+        }
+        double* xs = out.data() + size_t(c) * k;
+        for (int i = 0; i < npn; i++) xs[i] = y[nonPart[i]];
+        for (int j = 0; j < dimker; j++) xs[npn + j] = y[uniqueJ[freeRow[j]]];
+        // null to round-off: ||X2F x||^2 against the same sum of |terms|
+        double e = 0.0, s = 0.0;
+        for (int i = 0; i < X2F.rows(); i++) {
+            double a = 0.0, m = 0.0;
+            for (int64_t q = X2F.ptr[i]; q < X2F.ptr[i + 1]; q++) {
+                a += X2F.val[q] * xs[X2F.col[q]];
+                m += std::abs(X2F.val[q] * xs[X2F.col[q]]);
+            }
+            e += a * a;
+            s += m * m;
+        }
+        if (!(s > 0.0) || !(e <= 1e-20 * s)) return {};
+    }
+    return out;
+}
+
 }  // namespace
 
 extern "C" mk_status mk_iterative_rounding(
@@ -339,10 +438,17 @@ extern "C" mk_status mk_iterative_rounding(
             for (auto& v : A0.val) v *= length * wI * wI;
             A0 = A0 + SprsD::identity(k, wC * wC);
             hA0.op = make_op(c, A0, o);
+            hA0.op->label = "igm_A0";
+            if (o.precond == MK_PRECOND_AMG) {
+                hA0.op->nullvec = near_null(G2, Cf, X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
+                hA0.op->nnull = hA0.op->nullvec.empty() ? 0 : n;
+                S.null_vectors = hA0.op->nnull;
+            }
             mk::ensure_mg(c, hA0.op, o);
             S.setup_ms += now_ms() - ts;
         }
         hE.op = make_op(c, Eaug, o);
+        hE.op->label = "igm_Eaug";
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
         drhs.upload(rhs.data(), rhs.size(), K.st());
         MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));

@@ -90,31 +90,89 @@ __device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, 
 // SELL SpMV.  T = float/double; CPLX: values/vectors interleaved (re, im).
 // MODE 0: y = A x; 1: y = A x, out = Re(x^H y); 2: y = b - A x, out = |y|^2;
 // 3: y = b - A x; 4: y += A x.  skip (optional): device flag, nonzero = do nothing.
-template <typename T, bool CPLX, int MODE>
-__global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __restrict__ slice_ptr,
-                                               const int* __restrict__ slice_w, const int* __restrict__ col,
-                                               const T* __restrict__ val, const int* __restrict__ perm,
-                                               const T* __restrict__ x, T* __restrict__ y,
-                                               const T* __restrict__ b, double* partials, unsigned* counter,
-                                               double* out, const int* skip) {
-    if (skip && *skip) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    double contrib = 0.0;
-    if (r < n) {
-        const int s = r / C, lane = r - s * C;
-        const int64_t base = slice_ptr[s] + lane;
-        const int w = slice_w[s];
-        const int row = perm ? perm[r] : r;
-        if constexpr (CPLX) {
-            T ar = 0, ai = 0;
-            for (int j = 0; j < w; j++) {
-                const int64_t e = base + int64_t(j) * C;
-                const int c = col[e];
-                const T vr = val[2 * e], vi = val[2 * e + 1];
+//
+// Hybrid layout: the first nb_sell blocks run the SELL part (short rows, thread per row);
+// the remaining blocks give each long row a whole wavefront (lanes stride the row, then a
+// shuffle reduction), so a few very long rows never serialise one lane.
+template <typename T>
+struct SellDev {
+    int n_short, C, nb_sell, nlong;
+    const int64_t* slice_ptr;
+    const int* slice_w;
+    const int* col;
+    const T* val;
+    const int* perm;
+    const int* long_rows;
+    const int64_t* long_ptr;
+    const int* long_col;
+    const T* long_val;
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_t(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Computes one row of A x.  Returns true in the thread that owns the result (row, ar, ai).
+template <typename T, bool CPLX>
+__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int& row, T& ar, T& ai) {
+    ar = 0;
+    ai = 0;
+    if (int(blockIdx.x) < A.nb_sell) {
+        const int r = blockIdx.x * blockDim.x + threadIdx.x;
+        if (r >= A.n_short) return false;
+        const int s = r / A.C, lane = r - s * A.C;
+        const int64_t base = A.slice_ptr[s] + lane;
+        const int w = A.slice_w[s];
+        row = A.perm ? A.perm[r] : r;
+        for (int j = 0; j < w; j++) {
+            const int64_t e = base + int64_t(j) * A.C;
+            const int c = A.col[e];
+            if constexpr (CPLX) {
+                const T vr = A.val[2 * e], vi = A.val[2 * e + 1];
                 const T xr = x[2 * c], xi = x[2 * c + 1];
                 ar += vr * xr - vi * xi;
                 ai += vr * xi + vi * xr;
+            } else {
+                ar += A.val[e] * x[c];
             }
+        }
+        return true;
+    }
+    const int w = (int(blockIdx.x) - A.nb_sell) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (w < A.nlong) {  // uniform inside the wavefront
+        for (int64_t k = A.long_ptr[w] + lane; k < A.long_ptr[w + 1]; k += 64) {
+            const int c = A.long_col[k];
+            if constexpr (CPLX) {
+                const T vr = A.long_val[2 * k], vi = A.long_val[2 * k + 1];
+                const T xr = x[2 * c], xi = x[2 * c + 1];
+                ar += vr * xr - vi * xi;
+                ai += vr * xi + vi * xr;
+            } else {
+                ar += A.long_val[k] * x[c];
+            }
+        }
+        ar = wave_sum_t(ar);
+        if constexpr (CPLX) ai = wave_sum_t(ai);
+        row = A.long_rows[w];
+        return lane == 0;
+    }
+    return false;
+}
+
+template <typename T, bool CPLX, int MODE>
+__global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restrict__ x, T* __restrict__ y,
+                                               const T* __restrict__ b, double* partials, unsigned* counter,
+                                               double* out, const int* skip) {
+    if (skip && *skip) return;
+    double contrib = 0.0;
+    int row = 0;
+    T ar, ai;
+    if (spmv_row<T, CPLX>(A, x, row, ar, ai)) {
+        if constexpr (CPLX) {
             if constexpr (MODE == 2 || MODE == 3) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
             if constexpr (MODE == 4) { ar += y[2 * row]; ai += y[2 * row + 1]; }
             y[2 * row] = ar;
@@ -122,11 +180,7 @@ __global__ void __launch_bounds__(256) sell_spmv(int n, int C, const int64_t* __
             if constexpr (MODE == 1) contrib = double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
             if constexpr (MODE == 2) contrib = double(ar) * double(ar) + double(ai) * double(ai);
         } else {
-            T a = 0;
-            for (int j = 0; j < w; j++) {
-                const int64_t e = base + int64_t(j) * C;
-                a += val[e] * x[col[e]];
-            }
+            T a = ar;
             if constexpr (MODE == 2 || MODE == 3) a = b[row] - a;
             if constexpr (MODE == 4) a += y[row];
             y[row] = a;
@@ -178,40 +232,20 @@ __global__ void __launch_bounds__(256) cg_init(int n, const T* __restrict__ r, c
 
 // q = A p ; pq = p^H q ; alpha = rz / pq
 template <typename T, bool CPLX>
-__global__ void __launch_bounds__(256) cg_spmv(int n, int C, const int64_t* __restrict__ slice_ptr,
-                                             const int* __restrict__ slice_w, const int* __restrict__ col,
-                                             const T* __restrict__ val, const int* __restrict__ perm,
-                                             const T* __restrict__ p, T* __restrict__ q, CgState* st,
-                                             double* partials, unsigned* counter) {
+__global__ void __launch_bounds__(256) cg_spmv(SellDev<T> A, const T* __restrict__ p, T* __restrict__ q,
+                                             CgState* st, double* partials, unsigned* counter) {
     if (st->done) return;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
     double contrib = 0.0;
-    if (r < n) {
-        const int s = r / C, lane = r - s * C;
-        const int64_t base = slice_ptr[s] + lane;
-        const int w = slice_w[s];
-        const int row = perm ? perm[r] : r;
+    int row = 0;
+    T ar, ai;
+    if (spmv_row<T, CPLX>(A, p, row, ar, ai)) {
         if constexpr (CPLX) {
-            T ar = 0, ai = 0;
-            for (int j = 0; j < w; j++) {
-                const int64_t e = base + int64_t(j) * C;
-                const int c = col[e];
-                const T vr = val[2 * e], vi = val[2 * e + 1];
-                const T xr = p[2 * c], xi = p[2 * c + 1];
-                ar += vr * xr - vi * xi;
-                ai += vr * xi + vi * xr;
-            }
             q[2 * row] = ar;
             q[2 * row + 1] = ai;
             contrib = double(p[2 * row]) * double(ar) + double(p[2 * row + 1]) * double(ai);
         } else {
-            T a = 0;
-            for (int j = 0; j < w; j++) {
-                const int64_t e = base + int64_t(j) * C;
-                a += val[e] * p[col[e]];
-            }
-            q[row] = a;
-            contrib = double(p[row]) * double(a);
+            q[row] = ar;
+            contrib = double(p[row]) * double(ar);
         }
     }
     double v[1] = {contrib}, tot[1];
@@ -415,29 +449,43 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     const int vw = op->cplx ? 2 : 1;
     MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
     MK_CHECK(sigma >= 1, MK_ERR_ARG, "sigma must be >= 1");
+    // long rows (> max(64, 4 x median length)) go to the wavefront-per-row part
+    std::vector<int> lens(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) lens[i] = int(rowptr[i + 1] - rowptr[i]);
+    int med = 0;
+    if (n > 0) {
+        std::vector<int> s(lens);
+        std::nth_element(s.begin(), s.begin() + n / 2, s.end());
+        med = s[size_t(n / 2)];
+    }
+    const int thr = std::max(64, 4 * med);
+    std::vector<int> order, lrows;
+    for (int i = 0; i < n; i++) (lens[i] > thr ? lrows : order).push_back(i);
+    const int ns_rows = int(order.size());
+    // remaining rows of uneven length: sort inside windows of 1024 rows
+    if (sigma <= 1 && ns_rows > 0) {
+        int mx = 0;
+        int64_t tot = 0;
+        for (int r : order) { mx = std::max(mx, lens[r]); tot += lens[r]; }
+        if (double(mx) > 2.0 * double(tot) / ns_rows + 8.0) sigma = 1024;
+    }
     op->C = C;
     op->sigma = sigma;
-    std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    bool permuted = false;
     if (sigma > 1) {
-        for (int w0 = 0; w0 < n; w0 += sigma) {
-            const int w1 = std::min(n, w0 + sigma);
-            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
-                return (rowptr[a + 1] - rowptr[a]) > (rowptr[b + 1] - rowptr[b]);
-            });
+        for (int w0 = 0; w0 < ns_rows; w0 += sigma) {
+            const int w1 = std::min(ns_rows, w0 + sigma);
+            std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) { return lens[a] > lens[b]; });
         }
-        for (int i = 0; i < n; i++) if (order[i] != i) { permuted = true; break; }
     }
-    const int ns = (n + C - 1) / C;
+    bool permuted = !lrows.empty();
+    for (int i = 0; i < ns_rows && !permuted; i++) if (order[i] != i) permuted = true;
+    const int n_rows = ns_rows;
+    const int ns = (n_rows + C - 1) / C;
     std::vector<int64_t> sptr(size_t(ns) + 1, 0);
     std::vector<int> sw(size_t(ns), 0);
     for (int s = 0; s < ns; s++) {
         int w = 0;
-        for (int l = 0; l < C && s * C + l < n; l++) {
-            const int row = order[s * C + l];
-            w = std::max<int>(w, int(rowptr[row + 1] - rowptr[row]));
-        }
+        for (int l = 0; l < C && s * C + l < n_rows; l++) w = std::max<int>(w, lens[order[s * C + l]]);
         sw[s] = w;
         sptr[s + 1] = sptr[s] + int64_t(w) * C;
     }
@@ -448,7 +496,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     for (int s = 0; s < ns; s++) {
         for (int l = 0; l < C; l++) {
             const int r = s * C + l;
-            const int row = r < n ? order[r] : -1;
+            const int row = r < n_rows ? order[r] : -1;
             const int64_t len = row >= 0 ? rowptr[row + 1] - rowptr[row] : 0;
             for (int j = 0; j < sw[s]; j++) {
                 const int64_t e = sptr[s] + int64_t(j) * C + l;
@@ -474,21 +522,48 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
         dinv32[i] = float(dinv[i]);
     }
+    // long-row part (CSR)
+    const int nlong = int(lrows.size());
+    std::vector<int64_t> lptr(size_t(nlong) + 1, 0);
+    std::vector<int> lcol;
+    std::vector<double> lval;
+    std::vector<float> lval32;
+    for (int w = 0; w < nlong; w++) {
+        const int row = lrows[w];
+        for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
+            lcol.push_back(col[k]);
+            for (int c = 0; c < vw; c++) { lval.push_back(val[k * vw + c]); lval32.push_back(float(val[k * vw + c])); }
+        }
+        lptr[w + 1] = int64_t(lcol.size());
+    }
+    // keep every uploaded array non-empty (an operator may consist of long rows only)
+    if (sw.empty()) sw.push_back(0);
+    if (hcol.empty()) { hcol.push_back(0); hval.assign(size_t(vw), 0.0); hval32.assign(size_t(vw), 0.0f); }
+    if (order.empty()) order.push_back(0);
     hipStream_t st = op->ctx->stream;
     op->nslices = ns;
-    op->stored = stored;
+    op->stored = stored;  // SELL entries only (the long part is long_ptr[nlong] entries)
+    op->n_short = n_rows;
+    op->nlong = nlong;
     op->slice_ptr.upload(sptr.data(), sptr.size(), st);
-    op->slice_w.upload(sw.data(), sw.size(), st);
-    op->col.upload(hcol.data(), hcol.size(), st);
-    op->val64.upload(hval.data(), hval.size(), st);
-    op->val32.upload(hval32.data(), hval32.size(), st);
+    op->slice_w.upload(sw.data(), std::max<size_t>(sw.size(), 1), st);
+    op->col.upload(hcol.data(), std::max<size_t>(hcol.size(), 1), st);
+    op->val64.upload(hval.data(), std::max<size_t>(hval.size(), 1), st);
+    op->val32.upload(hval32.data(), std::max<size_t>(hval32.size(), 1), st);
     op->dinv64.upload(dinv.data(), dinv.size(), st);
     op->dinv32.upload(dinv32.data(), dinv32.size(), st);
-    if (permuted) op->perm.upload(order.data(), order.size(), st);
+    if (permuted) op->perm.upload(order.data(), std::max<size_t>(order.size(), 1), st);
     else op->perm.release();
+    if (nlong) {
+        op->long_rows.upload(lrows.data(), lrows.size(), st);
+        op->long_ptr.upload(lptr.data(), lptr.size(), st);
+        op->long_col.upload(lcol.data(), lcol.size(), st);
+        op->long_val64.upload(lval.data(), lval.size(), st);
+        op->long_val32.upload(lval32.data(), lval32.size(), st);
+    }
     MK_HIP(hipStreamSynchronize(st));
     const double sv64 = 8.0 * vw, sv32 = 4.0 * vw;
-    const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0);
+    const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0) + double(nlong) * 12.0;
     op->bytes64 = double(op->nnz) * (sv64 + 4.0) + meta + double(n) * sv64 * 2.0;
     op->bytes32 = double(op->nnz) * (sv32 + 4.0) + meta + double(n) * sv32 * 2.0;
 }
@@ -509,17 +584,35 @@ void timing_flush(mk_ctx* c) {
         float ms = 0.f;
         MK_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
         c->spmv_ms += ms;
+        const std::string& key = c->ev_key[i / 2];
+        auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
+        if (it != c->kstats.end()) { it->second.calls++; it->second.ms += ms; }
     }
     c->ev_used = 0;
+    c->ev_key.clear();
 }
 
+// Counts an SpMV launch and, when timing is on, brackets it with events attributed to
+// "<operator label>:<kernel>:<fp32|fp64>".
 struct SpmvTimer {
     mk_ctx* c;
-    explicit SpmvTimer(mk_ctx* c_, double bytes) : c(c_) {
+    SpmvTimer(mk_ctx* c_, double bytes, const mk_op* A = nullptr, const char* kernel = "spmv", bool fp32 = false)
+        : c(c_) {
         c->spmv_calls++;
         c->spmv_bytes += bytes;
         if (c->time_spmv) {
             if (c->ev_used + 2 > 4096) timing_flush(c);
+            std::string key = std::string(A ? A->label : "op") + ":" + kernel + ":" + (fp32 ? "fp32" : "fp64");
+            auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
+            if (it == c->kstats.end()) {
+                mk_ctx::KStat s;
+                s.bytes_per_call = bytes;
+                s.rows = A ? A->n : 0;
+                c->kstats.emplace_back(key, s);
+                it = c->kstats.end() - 1;
+            }
+            it->second.bytes += bytes;
+            c->ev_key.push_back(key);
             MK_HIP(hipEventRecord(next_event(c), c->stream));
         }
     }
@@ -529,17 +622,38 @@ struct SpmvTimer {
 };
 
 template <typename T>
+static SellDev<T> sell_dev(const mk_op* A, const T* val, int block, unsigned* grid) {
+    SellDev<T> d;
+    d.n_short = A->n_short;
+    d.C = A->C;
+    d.nb_sell = int(blocks_for(size_t(A->n_short), block));
+    d.nlong = A->nlong;
+    d.slice_ptr = A->slice_ptr.p;
+    d.slice_w = A->slice_w.p;
+    d.col = A->col.p;
+    d.val = val;
+    d.perm = A->perm.p;
+    d.long_rows = A->long_rows.p;
+    d.long_ptr = A->long_ptr.p;
+    d.long_col = A->long_col.p;
+    if constexpr (std::is_same<T, double>::value) d.long_val = A->long_val64.p;
+    else d.long_val = A->long_val32.p;
+    const unsigned nb_long = blocks_for(size_t(A->nlong), block / 64);
+    *grid = std::max(1u, unsigned(d.nb_sell) + nb_long);
+    return d;
+}
+
+template <typename T>
 void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode, double* out,
                  double bytes, const int* skip) {
-    const int n = A->n;
-    const unsigned g = blocks_for(n, A->block);
+    unsigned g = 0;
+    const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
-    SpmvTimer tm(c, bytes);
-    const int* perm = A->perm.p;
+    static const char* kname[] = {"spmv", "spmv_dot", "spmv_resid", "spmv_resid_nored", "spmv_accum"};
+    SpmvTimer tm(c, bytes, A, kname[int(mode)], sizeof(T) == 4);
 #define MK_SPMV(CPLX, MODE)                                                                                \
-    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,        \
-                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, perm, x, y, b, c->partials.p,          \
-                       c->counter.p, out, skip)
+    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3(g), dim3(A->block), 0, c->stream, D, x, y, b,     \
+                       c->partials.p, c->counter.p, out, skip)
 #define MK_SPMV_ALL(CPLX)                            \
     switch (mode) {                                  \
         case SPMV_PLAIN: MK_SPMV(CPLX, 0); break;    \
@@ -605,6 +719,20 @@ void add32to64_scaled(mk_ctx* c, size_t len, const float* d, const double* s_dev
 static inline void mg_apply(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { mg_apply64(c, g, b, x, skip); }
 static inline void mg_apply(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { mg_apply32(c, g, b, x, skip); }
 
+// q = A p fused with p^H q and alpha (CG state on the device)
+template <typename T>
+static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, T* q, double bytes) {
+    unsigned g = 0;
+    const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
+    SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
+    if (A->cplx)
+        hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
+                           c->partials.p, c->counter.p);
+    else
+        hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
+                           c->partials.p, c->counter.p);
+}
+
 // one PCG iteration with an external (multigrid) preconditioner
 template <typename T>
 static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T* r, T* z, T* p, T* q, double bytes) {
@@ -612,15 +740,7 @@ static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T*
     const size_t len = size_t(n) * (A->cplx ? 2 : 1);
     const unsigned g = blocks_for(n);
     CgState* st = c->state.p;
-    {
-        SpmvTimer tm(c, bytes);
-        if (A->cplx)
-            hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, n, A->C, A->slice_ptr.p,
-                               A->slice_w.p, A->col.p, val, A->perm.p, p, q, st, c->partials.p, c->counter.p);
-        else
-            hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, n, A->C, A->slice_ptr.p,
-                               A->slice_w.p, A->col.p, val, A->perm.p, p, q, st, c->partials.p, c->counter.p);
-    }
+    launch_cg_spmv<T>(c, A, val, p, q, bytes);
     if (A->cplx)
         hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, p, q, st,
                            c->partials.p, c->counter.p);
@@ -686,17 +806,7 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
         if (h.done || h.it >= maxiter) break;
         const int batch = std::min(check_every, maxiter - h.it);
         for (int b = 0; b < batch; b++) {
-            {
-                SpmvTimer tm(c, bytes);
-                if (A->cplx)
-                    hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,
-                                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, A->perm.p, p, q, st,
-                                       c->partials.p, c->counter.p);
-                else
-                    hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, n, A->C,
-                                       A->slice_ptr.p, A->slice_w.p, A->col.p, val, A->perm.p, p, q, st,
-                                       c->partials.p, c->counter.p);
-            }
+            launch_cg_spmv<T>(c, A, val, p, q, bytes);
             if (A->cplx)
                 hipLaunchKernelGGL((cg_update<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
                                    st, c->partials.p, c->counter.p);

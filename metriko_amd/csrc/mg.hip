@@ -56,6 +56,31 @@ void mg_free(Mg* g) {
     delete g;
 }
 
+// Multigrid parameters (METRIKO_MG_* environment overrides, read once).
+struct MgTune {
+    double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 30.0, lmax_fac = 1.1;
+    int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1;
+};
+static const MgTune& tune() {
+    static const MgTune t = [] {
+        MgTune x;
+        auto f = [](const char* n, double& v) { if (const char* s = std::getenv(n)) v = std::atof(s); };
+        auto i = [](const char* n, int& v) { if (const char* s = std::getenv(n)) v = std::atoi(s); };
+        f("METRIKO_MG_THETA_REAL", x.theta_real);
+        f("METRIKO_MG_THETA_CPLX", x.theta_cplx);
+        f("METRIKO_MG_LMIN_DIV", x.lmin_div);
+        f("METRIKO_MG_LMAX_FAC", x.lmax_fac);
+        i("METRIKO_MG_DEG", x.deg);
+        i("METRIKO_MG_POWER", x.power);
+        i("METRIKO_MG_PHASE_REAL", x.phase_real);
+        i("METRIKO_MG_COARSE_REAL", x.coarse_real);
+        i("METRIKO_MG_COARSE_CPLX", x.coarse_cplx);
+        i("METRIKO_MG_SMOOTH_LONG", x.smooth_long);
+        return x;
+    }();
+    return t;
+}
+
 namespace {
 
 using metriko::Csr;
@@ -97,7 +122,7 @@ double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
         else v = cd(U(g), U(g));
     }
     double lam = 1.0;
-    for (int it = 0; it < 15; it++) {
+    for (int it = 0; it < tune().power; it++) {
         A.multiply(x.data(), y.data());
         double nrm = 0.0;
         for (int i = 0; i < n; i++) { y[i] = y[i] / (d[i] != 0.0 ? d[i] : 1.0); nrm += mag2(y[i]); }
@@ -112,16 +137,45 @@ double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
     return lam;
 }
 
+// rows much longer than typical: > max(32, 8 x median length)
 template <typename T>
-Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, int& nagg) {
+std::vector<uint8_t> long_rows(const Csr<T>& A) {
     const int n = A.nr;
-    std::vector<int64_t> sptr(size_t(n) + 1, 0);
+    std::vector<int> len(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) len[i] = int(A.ptr[i + 1] - A.ptr[i]);
+    std::vector<int> s(len);
+    std::nth_element(s.begin(), s.begin() + n / 2, s.end());
+    const int thr = std::max(32, 8 * (n ? s[size_t(n / 2)] : 1));
+    std::vector<uint8_t> lng(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; i++) lng[i] = len[i] > thr ? 1 : 0;
+    return lng;
+}
+
+// Strength graph and greedy aggregation (seed / attach / leftover passes).
+template <typename T>
+struct Aggregation {
+    std::vector<int64_t> sptr;
     std::vector<int> scol;
     std::vector<T> sval;
+    std::vector<int> agg, root;
+    int nagg = 0;
+};
+
+template <typename T>
+Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double theta) {
+    Aggregation<T> G;
+    const int n = A.nr;
+    std::vector<int64_t>& sptr = G.sptr;
+    std::vector<int>& scol = G.scol;
+    std::vector<T>& sval = G.sval;
+    sptr.assign(size_t(n) + 1, 0);
+    // long rows (kernel-basis / transition DOFs coupling to a whole seam) stay out of the
+    // strength graph: they become singleton aggregates and are not smoothed into others
+    std::vector<uint8_t> lng = long_rows(A);
     for (int i = 0; i < n; i++) {
         for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
             const int j = A.col[k];
-            if (j == i) continue;
+            if (j == i || lng[i] || lng[j]) continue;
             if (mag2(A.val[k]) >= theta * theta * std::abs(d[i]) * std::abs(d[j])) {
                 scol.push_back(j);
                 sval.push_back(A.val[k]);
@@ -129,8 +183,10 @@ Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, in
         }
         sptr[i + 1] = int64_t(scol.size());
     }
-    std::vector<int> agg(size_t(n), -1), root;
-    nagg = 0;
+    std::vector<int>& agg = G.agg;
+    std::vector<int>& root = G.root;
+    int& nagg = G.nagg;
+    agg.assign(size_t(n), -1);
     for (int i = 0; i < n; i++) {  // seeds: nodes whose strong neighbourhood is untouched
         if (agg[i] != -1) continue;
         bool free_nb = true;
@@ -159,7 +215,21 @@ Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, in
         root.push_back(i);
         nagg++;
     }
-    // near-null vector by BFS inside each aggregate following the operator's phases
+    return G;
+}
+
+// Tentative prolongator with one near-null vector built by BFS inside each aggregate that
+// follows the operator's phases (the connection Laplacian's parallel transport).
+template <typename T>
+Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, int& nagg) {
+    const int n = A.nr;
+    const Aggregation<T> G = aggregate(A, d, theta);
+    const std::vector<int64_t>& sptr = G.sptr;
+    const std::vector<int>& scol = G.scol;
+    const std::vector<T>& sval = G.sval;
+    const std::vector<int>& agg = G.agg;
+    const std::vector<int>& root = G.root;
+    nagg = G.nagg;
     std::vector<T> B(size_t(n), T(0));
     std::vector<uint8_t> seen(size_t(n), 0);
     std::vector<int> q;
@@ -174,7 +244,10 @@ Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, in
                 const int j = scol[k];
                 if (seen[j] || agg[j] != a) continue;
                 seen[j] = 1;
-                B[j] = -conjv(sval[k]) / mag(sval[k]) * B[u];
+                // complex (connection Laplacian): follow the transport phase; real (Laplacian
+                // of the uv coordinates): translations are the near-null space -> constants
+                if (std::is_same<T, double>::value && !tune().phase_real) B[j] = B[u];
+                else B[j] = -conjv(sval[k]) / mag(sval[k]) * B[u];
                 q.push_back(j);
             }
         }
@@ -190,6 +263,78 @@ Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, in
         P.val[i] = B[i] / std::sqrt(nrm[agg[i]]);
         P.ptr[i + 1] = i + 1;
     }
+    return P;
+}
+
+// Tentative prolongator for a given block of near-null vectors B (n x nb, column-major):
+// per aggregate, a thin QR of B's rows (modified Gram-Schmidt, twice, rank-revealing) gives
+// the columns of P; R stacked over the aggregates is the coarse level's B.  An aggregate on
+// which every vector vanishes keeps one normalised indicator column so no row is orphaned.
+template <typename T>
+Csr<T> tentative_qr(const Csr<T>& A, const std::vector<double>& d, double theta, const std::vector<T>& B, int nb,
+                    std::vector<T>& Bc, int& nc) {
+    const int n = A.nr;
+    const Aggregation<T> G = aggregate(A, d, theta);
+    std::vector<int> mptr(size_t(G.nagg) + 1, 0), mem(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) mptr[G.agg[i] + 1]++;
+    for (int a = 0; a < G.nagg; a++) mptr[a + 1] += mptr[a];
+    {
+        std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+        for (int i = 0; i < n; i++) mem[fill[G.agg[i]]++] = i;
+    }
+    std::vector<double> scale(static_cast<size_t>(nb), 0.0);
+    for (int c = 0; c < nb; c++)
+        for (int i = 0; i < n; i++) scale[c] = std::max(scale[c], mag(B[size_t(c) * n + i]));
+    std::vector<metriko::Triplet<T>> tr;
+    tr.reserve(size_t(n) * nb);
+    std::vector<std::vector<T>> bc_rows;  // coarse rows of B, one vector of nb entries per coarse dof
+    nc = 0;
+    std::vector<T> Q, v;
+    for (int a = 0; a < G.nagg; a++) {
+        const int m = mptr[a + 1] - mptr[a];
+        const int* rows = mem.data() + mptr[a];
+        Q.clear();
+        int r = 0;
+        for (int c = 0; c < nb; c++) {
+            v.assign(static_cast<size_t>(m), T(0));
+            for (int i = 0; i < m; i++) v[i] = B[size_t(c) * n + rows[i]];
+            double n0 = 0.0;
+            for (int i = 0; i < m; i++) n0 += mag2(v[i]);
+            n0 = std::sqrt(n0);
+            for (int pass = 0; pass < 2; pass++)
+                for (int q = 0; q < r; q++) {
+                    T s = T(0);
+                    for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * v[i];
+                    for (int i = 0; i < m; i++) v[i] -= s * Q[size_t(q) * m + i];
+                }
+            double nv = 0.0;
+            for (int i = 0; i < m; i++) nv += mag2(v[i]);
+            nv = std::sqrt(nv);
+            if (!(nv > 1e-10 * std::max(n0, scale[c] * 1e-6)) || nv == 0.0) continue;
+            for (int i = 0; i < m; i++) Q.push_back(v[i] / nv);
+            r++;
+        }
+        if (r == 0) {
+            for (int i = 0; i < m; i++) Q.push_back(T(1.0 / std::sqrt(double(m))));
+            r = 1;
+        }
+        for (int q = 0; q < r; q++) {
+            std::vector<T> row(static_cast<size_t>(nb), T(0));
+            for (int c = 0; c < nb; c++) {
+                T s = T(0);
+                for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * B[size_t(c) * n + rows[i]];
+                row[c] = s;
+            }
+            for (int i = 0; i < m; i++) tr.emplace_back(rows[i], nc + q, Q[size_t(q) * m + i]);
+            bc_rows.push_back(std::move(row));
+        }
+        nc += r;
+    }
+    Csr<T> P(n, nc);
+    P.setFromTriplets(tr);
+    Bc.assign(size_t(nc) * nb, T(0));
+    for (int i = 0; i < nc; i++)
+        for (int c = 0; c < nb; c++) Bc[size_t(c) * nc + i] = bc_rows[i][c];
     return P;
 }
 
@@ -254,54 +399,96 @@ mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o) {
     return op;
 }
 
+// Host multigrid hierarchy (FP64): level operators, prolongators, restrictions, the
+// Chebyshev upper bound per level and the dense inverse of the coarsest operator.
 template <typename T>
-Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta, int coarse_max) {
-    auto* g = new Mg();
-    try {
-        g->cplx = !std::is_same<T, double>::value;
-        Csr<T> A = A0;
-        const int maxLevels = 14;
-        for (int l = 0; l < maxLevels; l++) {
-            MgLevel L;
-            L.n = A.nr;
-            std::vector<double> d = diag_of(A);
-            L.lmax = 1.1 * rho_dinv_a(A, d);
-            L.A = upload(c, A, o);
-            g->nnz_total += A.nonZeros();
-            const bool last = A.nr <= coarse_max || l == maxLevels - 1;
-            if (last) {
-                std::vector<double> inv;
-                dense_hpd_inverse(A, inv, g->cplx);
-                g->nc = A.nr;
-                g->inv64.upload(inv.data(), inv.size(), c->stream);
-                std::vector<float> inv32(inv.begin(), inv.end());
-                g->inv32.upload(inv32.data(), inv32.size(), c->stream);
-                MK_HIP(hipStreamSynchronize(c->stream));
-                g->lv.push_back(std::move(L));
-                break;
-            }
-            int nagg = 0;
-            Csr<T> Pt = tentative(A, d, theta, nagg);
+struct HostHier {
+    std::vector<Csr<T>> A, P, R;
+    std::vector<double> lmax;
+    std::vector<std::vector<double>> dinv;
+    std::vector<double> inv;  // coarsest dense inverse (row-major, complex interleaved)
+    int nc = 0;
+};
+
+template <typename T>
+HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std::vector<T>* B0 = nullptr,
+                       int nb = 0) {
+    HostHier<T> H;
+    const bool cplx = !std::is_same<T, double>::value;
+    Csr<T> A = A0;
+    std::vector<T> B;
+    if (B0 && nb > 0) {
+        if (B0->size() != size_t(A0.nr) * size_t(nb)) throw Error(MK_ERR_ARG, "multigrid: near-null block size mismatch");
+        B = *B0;
+    } else {
+        nb = 0;
+    }
+    const int maxLevels = 14;
+    const bool verbose = std::getenv("METRIKO_VERBOSE") != nullptr;
+    for (int l = 0; l < maxLevels; l++) {
+        std::vector<double> d = diag_of(A);
+        if (l == 0)
+            for (int i = 0; i < A.nr; i++)
+                if (!(d[i] > 0.0)) throw Error(MK_ERR_NOT_SPD, "multigrid: non-positive diagonal (matrix not SPD/HPD)");
+        const double lmax = tune().lmax_fac * rho_dinv_a(A, d);
+        std::vector<double> di(d.size());
+        for (size_t i = 0; i < d.size(); i++) di[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+        if (verbose) {
+            int64_t mx = 0;
+            for (int i = 0; i < A.nr; i++) mx = std::max<int64_t>(mx, A.ptr[i + 1] - A.ptr[i]);
+            std::fprintf(stderr, "[mg] level %d: n=%d nnz=%lld maxrow=%lld lmax=%.3g\n", l, A.nr,
+                         (long long)A.nonZeros(), (long long)mx, lmax);
+        }
+        int nagg = 0;
+        Csr<T> Pt;
+        bool last = A.nr <= coarse_max || l == maxLevels - 1;
+        if (!last) {
+            std::vector<T> Bc;
+            if (nb > 0) Pt = tentative_qr(A, d, theta, B, nb, Bc, nagg);
+            else Pt = tentative(A, d, theta, nagg);
+            if (nb > 0) B = std::move(Bc);
             if (nagg >= A.nr * 0.9 || nagg < 1) {  // no coarsening possible: make this the coarsest level
                 if (A.nr > 4000) throw Error(MK_ERR_UNSUPPORTED, "multigrid: aggregation stalled on a large level");
-                std::vector<double> inv;
-                dense_hpd_inverse(A, inv, g->cplx);
-                g->nc = A.nr;
-                g->inv64.upload(inv.data(), inv.size(), c->stream);
-                std::vector<float> inv32(inv.begin(), inv.end());
-                g->inv32.upload(inv32.data(), inv32.size(), c->stream);
-                MK_HIP(hipStreamSynchronize(c->stream));
-                g->lv.push_back(std::move(L));
-                break;
+                last = true;
             }
-            const double rho = L.lmax / 1.1;
+        }
+        H.lmax.push_back(lmax);
+        H.dinv.push_back(std::move(di));
+        if (last) {
+            dense_hpd_inverse(A, H.inv, cplx);
+            H.nc = A.nr;
+            H.A.push_back(std::move(A));
+            break;
+        }
+        const double rho = lmax / tune().lmax_fac;
+            const bool keep_long = nb > 0 && tune().smooth_long;
             const double omega = 4.0 / (3.0 * rho);
-            Csr<T> AP = metriko::spgemm(A, Pt);
-            // P = Pt - omega D^-1 A Pt
+            // P = Pt - omega D^-1 Af Pt, Af = A without the couplings of long rows/columns
+            std::vector<uint8_t> lng = long_rows(A);
+            Csr<T> Af = A;
+            {
+                int64_t w = 0;
+                std::vector<int64_t> np(size_t(A.nr) + 1, 0);
+                for (int i = 0; i < A.nr; i++) {
+                    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
+                        const int j = A.col[k];
+                        if (!keep_long && j != i && (lng[i] || lng[j])) continue;
+                        Af.col[w] = j;
+                        Af.val[w] = A.val[k];
+                        w++;
+                    }
+                    np[i + 1] = w;
+                }
+                Af.col.resize(w);
+                Af.val.resize(w);
+                Af.ptr = np;
+            }
+            Csr<T> AP = metriko::spgemm(Af, Pt);
             std::vector<metriko::Triplet<T>> tr;
             tr.reserve(size_t(AP.nonZeros() + Pt.nonZeros()));
             for (int i = 0; i < A.nr; i++) {
                 for (int64_t k = Pt.ptr[i]; k < Pt.ptr[i + 1]; k++) tr.emplace_back(i, Pt.col[k], Pt.val[k]);
+                if (lng[i] && !keep_long) continue;
                 const double di = d[i] != 0.0 ? d[i] : 1.0;
                 for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++) tr.emplace_back(i, AP.col[k], -omega / di * AP.val[k]);
             }
@@ -314,11 +501,39 @@ Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta
             for (size_t k = 0; k < Ac.val.size(); k++) Ac.val[k] *= 0.5;
             for (auto& v : AcH.val) v *= 0.5;
             Ac = Ac + AcH;
-            L.P = upload(c, P, o);
-            L.R = upload(c, R, o);
-            g->lv.push_back(std::move(L));
+            H.A.push_back(std::move(A));
+            H.P.push_back(std::move(P));
+            H.R.push_back(std::move(R));
             A = std::move(Ac);
+    }
+    return H;
+}
+
+template <typename T>
+Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta, int coarse_max,
+               const std::vector<T>* nullvec = nullptr, int nnull = 0) {
+    HostHier<T> H = build_host<T>(A0, theta, coarse_max, nullvec, nnull);
+    auto* g = new Mg();
+    try {
+        g->cplx = !std::is_same<T, double>::value;
+        g->deg = tune().deg;
+        for (size_t l = 0; l < H.A.size(); l++) {
+            MgLevel L;
+            L.n = H.A[l].nr;
+            L.lmax = H.lmax[l];
+            L.A = upload(c, H.A[l], o);
+            g->nnz_total += H.A[l].nonZeros();
+            if (l < H.P.size()) {
+                L.P = upload(c, H.P[l], o);
+                L.R = upload(c, H.R[l], o);
+            }
+            g->lv.push_back(std::move(L));
         }
+        g->nc = H.nc;
+        g->inv64.upload(H.inv.data(), H.inv.size(), c->stream);
+        std::vector<float> inv32(H.inv.begin(), H.inv.end());
+        g->inv32.upload(inv32.data(), inv32.size(), c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
         for (auto& L : g->lv) {
             const size_t len = size_t(L.n) * (g->cplx ? 2 : 1);
             L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len);
@@ -422,7 +637,7 @@ struct Sel<float> {
 
 template <typename T>
 void chebyshev(mk_ctx* c, const Mg* g, MgLevel& L, const T* b, T* x, bool zero_start, const int* skip) {
-    const double lmax = L.lmax, lmin = L.lmax / 30.0;
+    const double lmax = L.lmax, lmin = L.lmax / tune().lmin_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
     T* r = Sel<T>::r(L);
@@ -475,14 +690,142 @@ void vcycle(mk_ctx* c, Mg* g, int l, const T* b, T* x, const int* skip) {
     chebyshev<T>(c, g, L, b, x, false, skip);
 }
 
+// ---- host V-cycle / PCG (test and tuning aid; same algorithm as the device V-cycle) ----------
+template <typename T>
+void host_cheb(const Csr<T>& A, const std::vector<double>& dinv, double lmax, int deg, const std::vector<T>& b,
+               std::vector<T>& x, bool zero) {
+    const size_t n = b.size();
+    const double lmin = lmax / tune().lmin_div, theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin),
+                 sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    std::vector<T> r(n), d(n), ax(n);
+    if (zero) { r = b; x.assign(n, T(0)); }
+    else { A.multiply(x.data(), ax.data()); for (size_t i = 0; i < n; i++) r[i] = b[i] - ax[i]; }
+    for (size_t i = 0; i < n; i++) { d[i] = (1.0 / theta) * dinv[i] * r[i]; x[i] += d[i]; }
+    for (int k = 1; k < deg; k++) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        A.multiply(x.data(), ax.data());
+        for (size_t i = 0; i < n; i++) {
+            d[i] = (rho1 * rho0) * d[i] + (2.0 * rho1 / delta) * dinv[i] * (b[i] - ax[i]);
+            x[i] += d[i];
+        }
+        rho0 = rho1;
+    }
+}
+
+template <typename T>
+void host_vcycle(const HostHier<T>& H, size_t l, const std::vector<T>& b, std::vector<T>& x) {
+    const size_t n = b.size();
+    if (l + 1 == H.A.size()) {
+        x.assign(n, T(0));
+        for (size_t i = 0; i < n; i++) {
+            T s = T(0);
+            for (size_t j = 0; j < n; j++) {
+                if constexpr (std::is_same<T, double>::value) s += H.inv[i * n + j] * b[j];
+                else s += cd(H.inv[2 * (i * n + j)], H.inv[2 * (i * n + j) + 1]) * b[j];
+            }
+            x[i] = s;
+        }
+        return;
+    }
+    host_cheb(H.A[l], H.dinv[l], H.lmax[l], tune().deg, b, x, true);
+    std::vector<T> ax(n), r(n);
+    H.A[l].multiply(x.data(), ax.data());
+    for (size_t i = 0; i < n; i++) r[i] = b[i] - ax[i];
+    std::vector<T> bc(size_t(H.R[l].nr)), xc;
+    H.R[l].multiply(r.data(), bc.data());
+    host_vcycle(H, l + 1, bc, xc);
+    std::vector<T> px(n);
+    H.P[l].multiply(xc.data(), px.data());
+    for (size_t i = 0; i < n; i++) x[i] += px[i];
+    host_cheb(H.A[l], H.dinv[l], H.lmax[l], tune().deg, b, x, false);
+}
+
+inline double redot(double a, double b) { return a * b; }
+inline double redot(const cd& a, const cd& b) { return (std::conj(a) * b).real(); }
+
+template <typename T>
+int host_pcg(const Csr<T>& A, const HostHier<T>& H, const T* b, T* x, double tol, int maxit, double* relres) {
+    const size_t n = size_t(A.nr);
+    std::vector<T> r(n), z, p, q(n), bb(b, b + n);
+    A.multiply(x, q.data());
+    double bn = 0.0;
+    for (size_t i = 0; i < n; i++) { r[i] = b[i] - q[i]; bn += redot(b[i], b[i]); }
+    host_vcycle(H, 0, r, z);
+    p = z;
+    double rz = 0.0, rr = 0.0;
+    for (size_t i = 0; i < n; i++) { rz += redot(r[i], z[i]); rr += redot(r[i], r[i]); }
+    int it = 0;
+    while (it < maxit && rr > tol * tol * bn) {
+        A.multiply(p.data(), q.data());
+        double pq = 0.0;
+        for (size_t i = 0; i < n; i++) pq += redot(p[i], q[i]);
+        const double alpha = rz / pq;
+        rr = 0.0;
+        for (size_t i = 0; i < n; i++) { x[i] += alpha * p[i]; r[i] -= alpha * q[i]; rr += redot(r[i], r[i]); }
+        it++;
+        if (rr <= tol * tol * bn) break;
+        host_vcycle(H, 0, r, z);
+        double rz2 = 0.0;
+        for (size_t i = 0; i < n; i++) rz2 += redot(r[i], z[i]);
+        const double beta = rz2 / rz;
+        rz = rz2;
+        for (size_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+    }
+    if (relres) *relres = std::sqrt(rr / (bn > 0 ? bn : 1.0));
+    return it;
+}
+
 }  // namespace
 
-Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o) {
+// Host-only multigrid PCG (no GPU): builds the same hierarchy the device uses and runs PCG
+// with a host V-cycle; returns iterations, levels and the final relative residual.
+extern "C" mk_status mk_mg_host_pcg(int n, const int64_t* ptr, const int32_t* col, const double* val, int cplx,
+                                    const double* nullvec, int nnull, const double* b, double* x, double tol,
+                                    int maxit, int* iters, int* levels, double* relres) {
+    if (n <= 0 || !ptr || !col || !val || !b || !x || nnull < 0 || (nnull > 0 && !nullvec)) return MK_ERR_ARG;
+    if (cplx && nnull > 0) return MK_ERR_UNSUPPORTED;
+    try {
+        if (cplx) {
+            Csr<cd> A(n, n);
+            A.ptr.assign(ptr, ptr + n + 1);
+            A.col.assign(col, col + ptr[n]);
+            A.val.resize(size_t(ptr[n]));
+            for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
+            HostHier<cd> H = build_host<cd>(A, tune().theta_cplx, tune().coarse_cplx);
+            if (levels) *levels = int(H.A.size());
+            int it = host_pcg<cd>(A, H, reinterpret_cast<const cd*>(b), reinterpret_cast<cd*>(x), tol, maxit, relres);
+            if (iters) *iters = it;
+        } else {
+            Csr<double> A(n, n);
+            A.ptr.assign(ptr, ptr + n + 1);
+            A.col.assign(col, col + ptr[n]);
+            A.val.assign(val, val + ptr[n]);
+            std::vector<double> B;
+            if (nnull > 0) B.assign(nullvec, nullvec + size_t(n) * nnull);
+            HostHier<double> H = build_host<double>(A, tune().theta_real, tune().coarse_real, nnull ? &B : nullptr, nnull);
+            if (levels) *levels = int(H.A.size());
+            int it = host_pcg<double>(A, H, b, x, tol, maxit, relres);
+            if (iters) *iters = it;
+        }
+        return MK_OK;
+    } catch (const Error& e) {
+        return e.code;
+    } catch (const std::exception&) {
+        return MK_ERR_ARG;
+    }
+}
+
+namespace {
+}  // namespace
+
+Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o,
+                  const std::vector<double>* nullvec, int nnull) {
     Csr<double> A(n, n);
     A.ptr.assign(ptr, ptr + n + 1);
     A.col.assign(col, col + ptr[n]);
     A.val.assign(val, val + ptr[n]);
-    return build_impl<double>(c, A, o, 0.08, 500);
+    return build_impl<double>(c, A, o, tune().theta_real, tune().coarse_real, nullvec, nnull);
 }
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val,
                      const mk_solve_opts& o) {
@@ -491,11 +834,19 @@ Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const
     A.col.assign(col, col + ptr[n]);
     A.val.resize(size_t(ptr[n]));
     for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
-    return build_impl<cd>(c, A, o, 0.08, 400);
+    return build_impl<cd>(c, A, o, tune().theta_cplx, tune().coarse_cplx);
 }
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { vcycle<float>(c, g, 0, b, x, skip); }
 int mg_levels(const Mg* g) { return int(g->lv.size()); }
+void mg_set_labels(Mg* g, const std::string& base) {
+    for (size_t l = 0; l < g->lv.size(); l++) {
+        const std::string s = base + "_mg" + std::to_string(l);
+        if (g->lv[l].A) g->lv[l].A->label = s;
+        if (g->lv[l].P) g->lv[l].P->label = s + "_P";
+        if (g->lv[l].R) g->lv[l].R->label = s + "_R";
+    }
+}
 double mg_setup_ms(const Mg* g) { return g->setup_ms; }
 void mg_set_setup_ms(Mg* g, double ms) { g->setup_ms = ms; }
 int64_t mg_nnz(const Mg* g) { return g->nnz_total; }

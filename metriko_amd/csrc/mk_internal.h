@@ -79,11 +79,13 @@ struct Work {
 };
 
 // multigrid (mg.hip)
-Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
+Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o,
+                  const std::vector<double>* nullvec = nullptr, int nnull = 0);
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip);
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
 int mg_levels(const Mg* g);
+void mg_set_labels(Mg* g, const std::string& base);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
 void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
 int64_t mg_nnz(const Mg* g);
@@ -101,10 +103,14 @@ struct mk_ctx {
     mk::DevBuf<unsigned int> counter;
     mk::DevBuf<mk::CgState> state;
     mk::DevBuf<double> scal;   // general device scalars
-    // SpMV timing (events around each SpMV launch when enabled)
+    // SpMV timing (events around each SpMV launch when enabled), attributed per
+    // "operator label:kernel:precision" key
     bool time_spmv = false;
     std::vector<hipEvent_t> ev_pool;
+    std::vector<std::string> ev_key;   // key of each recorded event pair
     size_t ev_used = 0;
+    struct KStat { int64_t calls = 0; double ms = 0.0, bytes = 0.0, bytes_per_call = 0.0; int rows = 0; };
+    std::vector<std::pair<std::string, KStat>> kstats;
     double spmv_ms = 0.0;
     int64_t spmv_calls = 0;
     double spmv_bytes = 0.0;
@@ -131,10 +137,21 @@ struct mk_op {
     mk::DevBuf<double> val64;   // complex: interleaved
     mk::DevBuf<float> val32;
     mk::DevBuf<int> perm;       // SELL row -> original row (empty if identity)
+    int n_short = 0;            // rows in the SELL part
+    int nlong = 0;              // rows in the wavefront-per-row part
+    mk::DevBuf<int> long_rows;
+    mk::DevBuf<int64_t> long_ptr;
+    mk::DevBuf<int> long_col;
+    mk::DevBuf<double> long_val64;
+    mk::DevBuf<float> long_val32;
     mk::DevBuf<double> dinv64;  // inverse diagonal (real part for complex)
     mk::DevBuf<float> dinv32;
     int block = 256;
     double bytes64 = 0, bytes32 = 0;  // algorithmic bytes per SpMV
+    std::string label = "op";          // timing attribution
+    // near-null vectors (n x nnull, column-major) handed to the multigrid setup; empty = phase rule
+    std::vector<double> nullvec;
+    int nnull = 0;
     // host CSR copy (sorted columns)
     std::vector<int64_t> h_ptr;
     std::vector<int> h_col;

@@ -1,0 +1,37 @@
+"""Writes the oracle's outputs on the reference's own input fixture as regression vectors.
+
+The reference ships no expected outputs; these fixtures freeze the oracle (pinned by
+tests/test_oracle_pinning.py) on example/models/icosphere.obj with the example_igm flag
+set and ratio 0.03, so later changes to the oracle or the GPU path are caught exactly.
+Run: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from metriko_amd.meshgen import read_obj  # noqa: E402
+from oracle.field import eigen_random_complex  # noqa: E402
+from oracle.pipeline import run_pipeline  # noqa: E402
+
+
+def main():
+    V, F = read_obj(os.path.join(HERE, "icosphere.obj"))
+    u0 = eigen_random_complex(F.shape[0])
+    r = run_pipeline(V, F, ratio=0.03, u0=u0)
+    dec = np.array([(i, v) for (i, v, _) in r["decisions"]], dtype=np.int64)
+    margin = np.array([m for (_, _, m) in r["decisions"]])
+    np.savez_compressed(
+        os.path.join(HERE, "icosphere_obj_oracle.npz"),
+        compressed=r["compressed"], singular=r["singular"].astype(np.int32),
+        seam_edges=np.nonzero(r["seam"])[0].astype(np.int32), cut_idx=r["cut"].idx.astype(np.int32),
+        combed_matching=r["combed_matching"].astype(np.int32), decisions=dec, decision_margin=margin,
+        uv=r["uv"], nS=r["nS"], nT=r["nT"])
+    print("wrote icosphere_obj_oracle.npz", r["nS"], r["nT"], len(dec))
+
+
+if __name__ == "__main__":
+    main()

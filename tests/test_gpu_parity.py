@@ -104,8 +104,9 @@ def test_spd_solve_real_and_breakdown(gpu_ctx):
     for prec in (mk.PREC_FP64, mk.PREC_MIXED):
         x, st = op.solve(b, o=mk.opts(precision=prec))
         assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-10
-    # negated matrix: CG breakdown reported as MK_ERR_NOT_SPD
+    # negated matrix: CG breakdown (Jacobi) and the multigrid setup both report MK_ERR_NOT_SPD
     opn = gpu_ctx.op_real(-A)
-    with pytest.raises(mk.MetrikoError) as ei:
-        opn.solve(b, o=mk.opts(precision=mk.PREC_FP64))
-    assert ei.value.status == 3
+    for pc in (mk.PRECOND_JACOBI, mk.PRECOND_AMG):
+        with pytest.raises(mk.MetrikoError) as ei:
+            opn.solve(b, o=mk.opts(precision=mk.PREC_FP64, precond=pc))
+        assert ei.value.status == 3
