@@ -34,7 +34,11 @@ enum {
 typedef struct mk_ctx mk_ctx;
 typedef struct mk_op mk_op;
 
-enum { MK_PREC_FP64 = 0, MK_PREC_MIXED = 1 };     /* FP64, or FP32 inner + FP64 refinement */
+/* MK_PREC_FP64: FP64 PCG.  MK_PREC_MIXED: FP32 PCG corrections inside FP64 iterative
+ * refinement.  MK_PREC_MIXED_MG: FP64 (flexible) PCG whose multigrid V-cycle runs in FP32;
+ * the FP64 Krylov loop refines the FP32 preconditioner's corrections every iteration (with a
+ * non-multigrid preconditioner it is plain FP64 PCG). */
+enum { MK_PREC_FP64 = 0, MK_PREC_MIXED = 1, MK_PREC_MIXED_MG = 2 };
 enum { MK_FMT_CSR = 0, MK_FMT_SELL = 1 };
 enum { MK_PRECOND_NONE = 0, MK_PRECOND_JACOBI = 1, MK_PRECOND_AMG = 2 };  /* AMG: smoothed aggregation V-cycle */
 
@@ -96,6 +100,14 @@ mk_status mk_op_apply(mk_ctx* ctx, const mk_op* op, const double* x, double* y);
 mk_status mk_spd_solve(mk_ctx* ctx, const mk_op* op, const double* b, double* x, const mk_solve_opts* opts,
                        mk_stats* stats);
 
+/* A X = B for m right-hand sides (B, X: n x m column-major host arrays; X starts from zero).
+ * Real operators with the multigrid preconditioner: batched PCG, up to 64 columns per block
+ * (one matrix read serves every column), FP64 Krylov vectors and FP32 V-cycle.  Other
+ * configurations solve the columns one by one with mk_spd_solve's algorithm.  Replaces
+ * repeated CholeskyWrapper::solve calls (iter_rounding.h:19). */
+mk_status mk_spd_solve_many(mk_ctx* ctx, const mk_op* op, int m, const double* B, double* X,
+                            const mk_solve_opts* opts, mk_stats* stats);
+
 /* --- field (a2-a4) --------------------------------------------------------------------
  * Connection Laplacian of FaceRosyField::connectionLaplacian (vectorfield/face_rosy_field.h:46-76)
  * assembled on the device from halfedge topology: face_halfs[3f+k] = k-th halfedge of face f
@@ -128,6 +140,7 @@ typedef struct {
 typedef struct {
     int rounds, lm_rounds, k, kernel_rank, n_fixed;
     int null_vectors;  /* near-null vectors given to the A0 multigrid (0 = one-vector rule) */
+    int batched_solves, batch_iterations;  /* Woodbury columns solved in blocks; their PCG iterations */
     double kkt_rel_residual, setup_ms, kkt_ms, naive_ms, rounding_ms;
     mk_stats solve;
 } mk_igm_stats;

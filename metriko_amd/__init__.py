@@ -14,7 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmetriko_hip.so")
 _lib = None
 
-PREC_FP64, PREC_MIXED = 0, 1
+PREC_FP64, PREC_MIXED, PREC_MIXED_MG = 0, 1, 2
 FMT_CSR, FMT_SELL = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_AMG = 0, 1, 2
 
@@ -23,6 +23,7 @@ EXPORTED = [
     "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
+    "mk_spd_solve_many",
 ]
 
 
@@ -46,7 +47,8 @@ class IgmFlags(C.Structure):
 
 class IgmStats(C.Structure):
     _fields_ = [("rounds", C.c_int), ("lm_rounds", C.c_int), ("k", C.c_int), ("kernel_rank", C.c_int),
-                ("n_fixed", C.c_int), ("null_vectors", C.c_int), ("kkt_rel_residual", C.c_double),
+                ("n_fixed", C.c_int), ("null_vectors", C.c_int), ("batched_solves", C.c_int),
+                ("batch_iterations", C.c_int), ("kkt_rel_residual", C.c_double),
                 ("setup_ms", C.c_double),
                 ("kkt_ms", C.c_double), ("naive_ms", C.c_double), ("rounding_ms", C.c_double), ("solve", Stats)]
 
@@ -94,6 +96,7 @@ def lib():
     L.mk_op_destroy.argtypes = [_P]
     L.mk_op_apply.argtypes = [_P, _P, _D, _D]
     L.mk_spd_solve.argtypes = [_P, _P, _D, _D, C.POINTER(SolveOpts), C.POINTER(Stats)]
+    L.mk_spd_solve_many.argtypes = [_P, _P, C.c_int, _D, _D, C.POINTER(SolveOpts), C.POINTER(Stats)]
     L.mk_field_laplacian.argtypes = [_P, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, C.POINTER(SolveOpts),
                                      C.POINTER(_P)]
     L.mk_op_export_csr.argtypes = [_P, _P, _I64, _I64, _I32, _D]
@@ -408,6 +411,19 @@ class Op:
         self.ctx._check(self.ctx.L.mk_spd_solve(self.ctx.h, self.h, _ptr(bp, _D), _ptr(x, _D), C.byref(o or opts()),
                                                 C.byref(st)), "mk_spd_solve")
         return self._unpack(x), struct_dict(st)
+
+    def solve_many(self, B, o=None):
+        """Solves A X = B for the columns of B (n x m, real operators)."""
+        B = np.asarray(B, np.float64)
+        if B.ndim == 1:
+            B = B[:, None]
+        m = B.shape[1]
+        Bc = np.ascontiguousarray(B.T)  # column-major n x m == row-major m x n
+        X = np.zeros_like(Bc)
+        st = Stats()
+        self.ctx._check(self.ctx.L.mk_spd_solve_many(self.ctx.h, self.h, m, _ptr(Bc, _D), _ptr(X, _D),
+                                                     C.byref(o or opts()), C.byref(st)), "mk_spd_solve_many")
+        return X.T.copy(), struct_dict(st)
 
     def export_csr(self):
         import scipy.sparse as sp

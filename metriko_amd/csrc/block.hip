@@ -1,0 +1,398 @@
+// Multi-right-hand-side kernels: SpMM on the hybrid SELL layout and a batched PCG that
+// solves A X = B for up to 64 right-hand sides at once (one matrix read serves every
+// column).  The IGM rounding loop needs A0^{-1} u_j for every candidate constraint row u_j;
+// those solves are independent of the rounding decisions, so they run as one batch.
+//
+// Block layout: MW right-hand sides (MW in {8, 16, 32, 64}) stored row-major, X[i*MW + j].
+// SpMM: a wavefront covers 64/MW rows at a time, lane = sub*MW + j (sub = row of the group,
+// j = column): each matrix entry is loaded once per row and shared by its MW lanes, and every
+// access to X is one contiguous MW-wide segment.
+//
+// Batched PCG: independent CG recurrences per column (own alpha, beta, convergence flag),
+// FP64 Krylov vectors, flexible beta, the operator's multigrid V-cycle applied to the whole
+// block in FP32.  Column reductions are deterministic: fixed grid, per-block partials, the last
+// block to arrive sums them in a fixed order.
+#include "sell.h"
+
+namespace mk {
+
+static constexpr int kBlkParts = 2048;  // grid of the column-reduction kernels
+
+template <typename T, int MW, int MODE>
+__global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
+                                               T* __restrict__ Y, const T* __restrict__ B, const int* skip) {
+    if (skip && *skip) return;
+    constexpr int R = 64 / MW;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane / MW, j = lane - sub * MW;
+    const int gw = int(blockIdx.x) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+    T acc = 0;
+    int row = -1;
+    if (gw < nw_sell) {
+        const int r = gw * R + sub;
+        if (r >= A.n_short) return;
+        const int s = r / A.C, lr = r - s * A.C;
+        const int64_t base = A.slice_ptr[s] + lr;
+        const int w = A.slice_w[s];
+        row = A.perm ? A.perm[r] : r;
+        for (int k = 0; k < w; k++) {
+            const int64_t e = base + int64_t(k) * A.C;
+            acc += A.val[e] * X[size_t(A.col[e]) * MW + j];
+        }
+    } else {
+        const int lw = gw - nw_sell;
+        if (lw >= A.nlong) return;  // uniform inside the wavefront
+        for (int64_t k = A.long_ptr[lw] + sub; k < A.long_ptr[lw + 1]; k += R)
+            acc += A.long_val[k] * X[size_t(A.long_col[k]) * MW + j];
+#pragma unroll
+        for (int off = MW; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+        if (sub != 0) return;
+        row = A.long_rows[lw];
+    }
+    const size_t o = size_t(row) * MW + j;
+    if constexpr (MODE == 0) Y[o] = acc;
+    if constexpr (MODE == 3) Y[o] = B[o] - acc;
+    if constexpr (MODE == 4) Y[o] += acc;
+}
+
+template <typename T, int MW>
+static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, SpmvMode mode,
+                    const int* skip) {
+    unsigned g0 = 0;
+    const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
+    const int nw_sell = int((int64_t(A->n_short) + 64 / MW - 1) / (64 / MW));
+    const int64_t waves = int64_t(nw_sell) + A->nlong;
+    const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
+    switch (mode) {
+        case SPMV_PLAIN:
+            hipLaunchKernelGGL((sell_spmm<T, MW, 0>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
+            break;
+        case SPMV_RESID_NORED:
+            hipLaunchKernelGGL((sell_spmm<T, MW, 3>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
+            break;
+        case SPMV_ACCUM:
+            hipLaunchKernelGGL((sell_spmm<T, MW, 4>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
+            break;
+        default: throw Error(MK_ERR_ARG, "spmm: unsupported mode");
+    }
+}
+
+template <typename T>
+void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip) {
+    MK_CHECK(!A->cplx, MK_ERR_UNSUPPORTED, "spmm: real operators only");
+    const T* val;
+    if constexpr (std::is_same<T, double>::value) val = A->val64.p;
+    else val = A->val32.p;
+    const double vb = double(sizeof(T)) * mw;
+    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * sizeof(T) +
+                         double(A->m) * vb + double(A->n) * vb * (mode == SPMV_PLAIN ? 1.0 : 2.0);
+    SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64" : mw == 32 ? "spmm32" : mw == 16 ? "spmm16" : "spmm8",
+                 sizeof(T) == 4);
+    switch (mw) {
+        case 8: spmm_mw<T, 8>(c, A, val, X, Y, B, mode, skip); break;
+        case 16: spmm_mw<T, 16>(c, A, val, X, Y, B, mode, skip); break;
+        case 32: spmm_mw<T, 32>(c, A, val, X, Y, B, mode, skip); break;
+        case 64: spmm_mw<T, 64>(c, A, val, X, Y, B, mode, skip); break;
+        default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
+    }
+    MK_HIP(hipGetLastError());
+}
+template void spmm_launch<double>(mk_ctx*, const mk_op*, int, const double*, double*, const double*, SpmvMode,
+                                  const int*);
+template void spmm_launch<float>(mk_ctx*, const mk_op*, int, const float*, float*, const float*, SpmvMode,
+                                 const int*);
+
+// ---- batched PCG --------------------------------------------------------------------------
+
+// Per-column grand totals of K quantities (the thread's column is threadIdx.x % MW, blockDim
+// 256, grid <= kBlkParts).  Returns true in the last block to arrive, with tot[k][j] filled.
+template <int K, int MW>
+__device__ __forceinline__ bool colsum_last(double (&v)[K], double* partials, unsigned* counter,
+                                            double (*tot)[64]) {
+    __shared__ double sh[K][4][64];
+    __shared__ int is_last;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double a = v[k];
+#pragma unroll
+        for (int off = MW; off < 64; off <<= 1) a += __shfl_xor(a, off, 64);
+        if (lane < MW) sh[k][wid][lane] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < MW) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int t = threadIdx.x;
+            partials[(size_t(k) * 64 + t) * kBlkParts + blockIdx.x] = sh[k][0][t] + sh[k][1][t] + sh[k][2][t] + sh[k][3][t];
+        }
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1 : 0;
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    const int j = threadIdx.x % MW;
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        acc[k] = 0.0;
+        const volatile double* pk = partials + (size_t(k) * 64 + j) * kBlkParts;
+        for (unsigned b = threadIdx.x / MW; b < gridDim.x; b += 256 / MW) acc[k] += pk[b];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double a = acc[k];
+#pragma unroll
+        for (int off = MW; off < 64; off <<= 1) a += __shfl_xor(a, off, 64);
+        if (lane < MW) sh[k][wid][lane] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < MW) {
+        const int t = threadIdx.x;
+#pragma unroll
+        for (int k = 0; k < K; k++) tot[k][t] = sh[k][0][t] + sh[k][1][t] + sh[k][2][t] + sh[k][3][t];
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+    __syncthreads();
+    return true;
+}
+
+__device__ __forceinline__ void bstate_all_done(BState* st, int mw) {
+    int all = 1;
+    for (int j = 0; j < mw; j++) all &= st->done[j];
+    st->all_done = all;
+}
+
+// X = 0 ; R = B ; R32 = float(B) ; bb_j = |B_j|^2
+template <int MW>
+__global__ void __launch_bounds__(256) bk_init0(size_t len, const double* __restrict__ B, double* __restrict__ X,
+                                              double* __restrict__ R, float* __restrict__ R32, BState* st,
+                                              double* partials, unsigned* counter) {
+    __shared__ double tot[1][64];
+    double v[1] = {0.0};
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
+        const double b = B[e];
+        X[e] = 0.0;
+        R[e] = b;
+        R32[e] = float(b);
+        v[0] += b * b;
+    }
+    if (colsum_last<1, MW>(v, partials, counter, tot) && threadIdx.x < MW) {
+        const int j = threadIdx.x;
+        st->bb[j] = tot[0][j];
+        st->it[j] = 0;
+        st->status[j] = MK_OK;
+    }
+}
+
+// P = Zold = Z32 ; rz_j = R.Z32 ; rr_j = R.R ; done flags
+template <int MW>
+__global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
+                                              double* __restrict__ P, double* __restrict__ Zold, BState* st,
+                                              double* partials, unsigned* counter) {
+    __shared__ double tot[2][64];
+    double v[2] = {0.0, 0.0};
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
+        const double z = double(Z32[e]), r = R[e];
+        P[e] = z;
+        Zold[e] = z;
+        v[0] += r * z;
+        v[1] += r * r;
+    }
+    if (colsum_last<2, MW>(v, partials, counter, tot)) {
+        if (threadIdx.x < MW) {
+            const int j = threadIdx.x;
+            const double rz = tot[0][j], rr = tot[1][j];
+            st->rz[j] = rz;
+            st->rr[j] = rr;
+            int d = (rr <= st->tol2 * st->bb[j]) ? 1 : 0;
+            if (!(rz == rz) || !(rr == rr)) { st->status[j] = MK_ERR_NAN; d = 1; }
+            else if (!d && !(rz > 0.0)) { st->status[j] = MK_ERR_NOT_SPD; d = 1; }
+            st->done[j] = d;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) bstate_all_done(st, MW);
+    }
+}
+
+// pq_j = P.Q ; alpha_j = rz_j / pq_j (0 for finished columns)
+template <int MW>
+__global__ void __launch_bounds__(256) bk_alpha(size_t len, const double* __restrict__ P, const double* __restrict__ Q,
+                                              BState* st, double* partials, unsigned* counter) {
+    if (st->all_done) return;
+    __shared__ double tot[1][64];
+    double v[1] = {0.0};
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) v[0] += P[e] * Q[e];
+    if (colsum_last<1, MW>(v, partials, counter, tot) && threadIdx.x < MW) {
+        const int j = threadIdx.x;
+        const double pq = tot[0][j];
+        if (st->done[j]) { st->alpha[j] = 0.0; }
+        else if (!(pq > 0.0)) {
+            st->status[j] = (pq == pq) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
+            st->done[j] = 1;
+            st->alpha[j] = 0.0;
+        } else {
+            st->alpha[j] = st->rz[j] / pq;
+        }
+    }
+}
+
+// X += alpha P ; R -= alpha Q ; R32 = float(R) ; rr_j ; convergence
+template <int MW>
+__global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict__ X, double* __restrict__ R,
+                                               float* __restrict__ R32, const double* __restrict__ P,
+                                               const double* __restrict__ Q, BState* st, double* partials,
+                                               unsigned* counter) {
+    if (st->all_done) return;
+    __shared__ double tot[1][64];
+    const double alpha = st->alpha[threadIdx.x % MW];
+    double v[1] = {0.0};
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
+        double r = R[e];
+        if (alpha != 0.0) {
+            X[e] += alpha * P[e];
+            r -= alpha * Q[e];
+            R[e] = r;
+            R32[e] = float(r);
+        }
+        v[0] += r * r;
+    }
+    if (colsum_last<1, MW>(v, partials, counter, tot)) {
+        if (threadIdx.x < MW) {
+            const int j = threadIdx.x;
+            const double rr = tot[0][j];
+            if (!st->done[j]) {
+                st->rr[j] = rr;
+                st->it[j] += 1;
+                if (!(rr == rr)) { st->status[j] = MK_ERR_NAN; st->done[j] = 1; }
+                else if (rr <= st->tol2 * st->bb[j]) st->done[j] = 1;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) bstate_all_done(st, MW);
+    }
+}
+
+// flexible beta_j = (R.Z32 - R.Zold) / rz_j ; rz_j = R.Z32
+template <int MW>
+__global__ void __launch_bounds__(256) bk_beta(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
+                                             const double* __restrict__ Zold, BState* st, double* partials,
+                                             unsigned* counter) {
+    if (st->all_done) return;
+    __shared__ double tot[2][64];
+    double v[2] = {0.0, 0.0};
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
+        const double r = R[e];
+        v[0] += r * double(Z32[e]);
+        v[1] += r * Zold[e];
+    }
+    if (colsum_last<2, MW>(v, partials, counter, tot)) {
+        if (threadIdx.x < MW) {
+            const int j = threadIdx.x;
+            if (!st->done[j]) {
+                const double a = tot[0][j];
+                if (!(a > 0.0)) {
+                    st->status[j] = (a == a) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
+                    st->done[j] = 1;
+                } else {
+                    st->beta[j] = (a - tot[1][j]) / st->rz[j];
+                    st->rz[j] = a;
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) bstate_all_done(st, MW);
+    }
+}
+
+// P = Z32 + beta P ; Zold = Z32 (running columns only)
+template <int MW>
+__global__ void __launch_bounds__(256) bk_pupdate(size_t len, double* __restrict__ P, const float* __restrict__ Z32,
+                                                double* __restrict__ Zold, const BState* st) {
+    if (st->all_done) return;
+    const int j = threadIdx.x % MW;
+    if (st->done[j]) return;
+    const double beta = st->beta[j];
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
+        const double z = double(Z32[e]);
+        P[e] = z + beta * P[e];
+        Zold[e] = z;
+    }
+}
+
+__global__ void k_set_bstate(BState* st, double tol2) {
+    st->tol2 = tol2;
+    st->all_done = 0;
+}
+
+template <int MW>
+static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double* B, double* X, double rel_tol,
+                                int maxiter, int check_every) {
+    const size_t len = size_t(A->n) * MW;
+    DevBuf<double> R(len), P(len), Q(len), Zold(len), part(size_t(2) * 64 * kBlkParts);
+    DevBuf<float> R32(len), Z32(len);
+    DevBuf<unsigned> cnt(1);
+    DevBuf<BState> st(1);
+    MK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), c->stream));
+    MK_HIP(hipMemsetAsync(st.p, 0, sizeof(BState), c->stream));
+    const unsigned g = unsigned(std::min<size_t>(kBlkParts, std::max<size_t>(1, (len + 255) / 256)));
+    // the recurrence residual is driven to half the target so the true residual meets it
+    hipLaunchKernelGGL(k_set_bstate, dim3(1), dim3(1), 0, c->stream, st.p, 0.25 * rel_tol * rel_tol);
+    hipLaunchKernelGGL((bk_init0<MW>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, R32.p, st.p, part.p, cnt.p);
+    mg_apply32_block(c, mg, MW, R32.p, Z32.p, nullptr);
+    hipLaunchKernelGGL((bk_init1<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
+                       (const float*)Z32.p, P.p, Zold.p, st.p, part.p, cnt.p);
+    MK_HIP(hipGetLastError());
+    BState h{};
+    int iters = 0;
+    for (;;) {
+        MK_HIP(hipMemcpyAsync(&h, st.p, sizeof(BState), hipMemcpyDeviceToHost, c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        timing_flush(c);
+        if (h.all_done || iters >= maxiter) break;
+        const int batch = std::min(check_every, maxiter - iters);
+        for (int k = 0; k < batch; k++) {
+            spmm_launch<double>(c, A, MW, P.p, Q.p, nullptr, SPMV_PLAIN, &st.p->all_done);
+            hipLaunchKernelGGL((bk_alpha<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)P.p,
+                               (const double*)Q.p, st.p, part.p, cnt.p);
+            hipLaunchKernelGGL((bk_update<MW>), dim3(g), dim3(256), 0, c->stream, len, X, R.p, R32.p,
+                               (const double*)P.p, (const double*)Q.p, st.p, part.p, cnt.p);
+            mg_apply32_block(c, mg, MW, R32.p, Z32.p, &st.p->all_done);
+            hipLaunchKernelGGL((bk_beta<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
+                               (const float*)Z32.p, (const double*)Zold.p, st.p, part.p, cnt.p);
+            hipLaunchKernelGGL((bk_pupdate<MW>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const float*)Z32.p,
+                               Zold.p, (const BState*)st.p);
+        }
+        iters += batch;
+        MK_HIP(hipGetLastError());
+    }
+    BlockResult res;
+    res.status = MK_OK;
+    res.iters = 0;
+    res.max_rel = 0.0;
+    for (int j = 0; j < MW; j++) {
+        res.iters = std::max(res.iters, h.it[j]);
+        if (h.status[j] != MK_OK && res.status == MK_OK) res.status = h.status[j];
+        if (h.bb[j] > 0.0) res.max_rel = std::max(res.max_rel, std::sqrt(h.rr[j] / h.bb[j]));
+    }
+    if (res.status == MK_OK && !h.all_done) res.status = MK_ERR_NO_CONVERGENCE;
+    return res;
+}
+
+BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
+                      int maxiter, int check_every) {
+    MK_CHECK(mg, MK_ERR_ARG, "block PCG needs the operator's multigrid hierarchy");
+    MK_CHECK(!A->cplx && A->n == A->m, MK_ERR_ARG, "block PCG: square real operator");
+    switch (mw) {
+        case 8: return block_pcg_mw<8>(c, A, mg, B, X, rel_tol, maxiter, check_every);
+        case 16: return block_pcg_mw<16>(c, A, mg, B, X, rel_tol, maxiter, check_every);
+        case 32: return block_pcg_mw<32>(c, A, mg, B, X, rel_tol, maxiter, check_every);
+        case 64: return block_pcg_mw<64>(c, A, mg, B, X, rel_tol, maxiter, check_every);
+        default: throw Error(MK_ERR_ARG, "block PCG: block width must be 8, 16, 32 or 64");
+    }
+}
+
+}  // namespace mk

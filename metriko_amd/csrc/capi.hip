@@ -413,6 +413,84 @@ mk_status mk_spd_solve(mk_ctx* c, const mk_op* A, const double* b, double* x, co
     return s != MK_OK ? s : inner;
 }
 
+__global__ void k_pack_cols(size_t n, int mw, int cnt, const double* src, double* dst) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * size_t(mw)) return;
+    const size_t i = e / size_t(mw);
+    const int j = int(e - i * size_t(mw));
+    dst[e] = j < cnt ? src[size_t(j) * n + i] : 0.0;
+}
+__global__ void k_unpack_block(size_t n, int mw, int cnt, const double* src, double* dst) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * size_t(mw)) return;
+    const size_t i = e / size_t(mw);
+    const int j = int(e - i * size_t(mw));
+    if (j < cnt) dst[size_t(j) * n + i] = src[e];
+}
+
+mk_status mk_spd_solve_many(mk_ctx* c, const mk_op* A, int m, const double* B, double* X, const mk_solve_opts* o_,
+                            mk_stats* st) {
+    if (!c || !A || m < 0 || (m > 0 && (!B || !X))) return MK_ERR_ARG;
+    mk_status inner = MK_OK;
+    mk_status s = guard(c, [&] {
+        const mk_solve_opts o = resolve(o_);
+        MK_HIP(hipSetDevice(c->device));
+        const double t0 = now_ms();
+        mk::ensure_mg(c, A, o);
+        const double t_setup = now_ms() - t0;
+        mk::stats_reset(c);
+        const size_t n = size_t(A->n), len = n * (A->cplx ? 2 : 1);
+        int iters = 0, solves = 0;
+        double worst = 0.0;
+        const bool batched = !A->cplx && A->mg && o.precond == MK_PRECOND_AMG && o.precision != MK_PREC_FP64;
+        if (batched && m > 0) {
+            const int mw = m > 32 ? 64 : m > 16 ? 32 : m > 8 ? 16 : 8;
+            DevBuf<double> dB(n * size_t(std::min(m, mw))), dX(n * size_t(std::min(m, mw)));
+            DevBuf<double> Bp(n * mw), Xp(n * mw);
+            for (int b0 = 0; b0 < m && inner == MK_OK; b0 += mw) {
+                const int cnt = std::min(mw, m - b0);
+                dB.upload(B + size_t(b0) * n, size_t(cnt) * n, c->stream);
+                hipLaunchKernelGGL(k_pack_cols, dim3(unsigned((n * mw + 255) / 256)), dim3(256), 0, c->stream, n, mw,
+                                   cnt, (const double*)dB.p, Bp.p);
+                mk::BlockResult r = mk::block_pcg(c, A, A->mg, mw, Bp.p, Xp.p, o.tol, o.maxiter, o.check_every);
+                hipLaunchKernelGGL(k_unpack_block, dim3(unsigned((n * mw + 255) / 256)), dim3(256), 0, c->stream, n,
+                                   mw, cnt, (const double*)Xp.p, dX.p);
+                dX.download(X + size_t(b0) * n, size_t(cnt) * n, c->stream);
+                MK_HIP(hipStreamSynchronize(c->stream));
+                iters += r.iters;
+                solves += cnt;
+                worst = std::max(worst, r.max_rel);
+                inner = r.status;
+            }
+        } else {
+            DevBuf<double> db, dx(len);
+            for (int j = 0; j < m && inner == MK_OK; j++) {
+                db.upload(B + size_t(j) * len, len, c->stream);
+                MK_HIP(hipMemsetAsync(dx.p, 0, len * sizeof(double), c->stream));
+                mk::SolveResult r = mk::solve_spd(c, A, db.p, dx.p, o);
+                dx.download(X + size_t(j) * len, len, c->stream);
+                MK_HIP(hipStreamSynchronize(c->stream));
+                iters += r.iters;
+                solves++;
+                worst = std::max(worst, r.rel_residual);
+                inner = r.status;
+            }
+        }
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->iterations = iters;
+            st->solves = solves;
+            st->rel_residual = worst;
+            mk::stats_fill(c, st);
+            st->ms = now_ms() - t0;
+            st->setup_ms = t_setup;
+            st->mg_levels = A->mg ? mk::mg_levels(A->mg) : 0;
+        }
+        if (inner != MK_OK) c->last_error = "mk_spd_solve_many: " + std::string(mk_status_str(inner));
+    });
+    return s != MK_OK ? s : inner;
+}
+
 mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_halfs, const int32_t* twin,
                              const int32_t* half_face, const double* farg, int N, const mk_solve_opts* o_,
                              mk_op** out) {

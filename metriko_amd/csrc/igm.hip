@@ -66,13 +66,41 @@ __global__ void k_zero(size_t n, double* y) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) y[i] = 0.0;
 }
-// y = x + sum_j c[j] * Z[:, j]   (Z column-major, k rows)
-__global__ void k_combine(size_t k, int r, const double* x, const double* Z, const double* c, double* y) {
+// y = x + sum_j c[j] * Z[:, idx[j]]   (Z column-major, k rows)
+__global__ void k_combine(size_t k, int r, const double* x, const double* Z, const int* idx, const double* c,
+                          double* y) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= k) return;
     double s = x[i];
-    for (int j = 0; j < r; j++) s += c[j] * Z[size_t(j) * k + i];
+    for (int j = 0; j < r; j++) s += c[j] * Z[size_t(idx[j]) * k + i];
     y[i] = s;
+}
+// packed right-hand sides: B[col * mw + j] = row sel[j] of a RowSet (sel[j] < 0: empty column)
+__global__ void k_scatter_rows_packed(int mw, const int* sel, const int64_t* ptr, const int* col, const double* val,
+                                      double* B) {
+    const int j = blockIdx.x;
+    const int p = sel[j];
+    if (p < 0) return;
+    for (int64_t e = ptr[p] + threadIdx.x; e < ptr[p + 1]; e += blockDim.x) B[size_t(col[e]) * mw + j] = val[e];
+}
+// Z[:, sel[j]] = X[:, j] for the packed n x mw block X
+__global__ void k_unpack_cols(size_t n, int mw, const int* sel, const double* X, double* Z) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n * size_t(mw)) return;
+    const size_t i = e / size_t(mw);
+    const int j = int(e - i * size_t(mw));
+    const int p = sel[j];
+    if (p >= 0) Z[size_t(p) * n + i] = X[e];
+}
+// UZ[c * nc + p] = row c of the RowSet . Z[:, p]
+__global__ void k_uz(int nc, size_t n, const int64_t* ptr, const int* col, const double* val, const double* Z,
+                     double* UZ) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= size_t(nc) * size_t(nc)) return;
+    const size_t c = t / size_t(nc), p = t - c * size_t(nc);
+    double s = 0.0;
+    for (int64_t e = ptr[c]; e < ptr[c + 1]; e++) s += val[e] * Z[p * n + size_t(col[e])];
+    UZ[t] = s;
 }
 // y = a*x + b*z
 __global__ void k_lin2(size_t n, double a, const double* x, double b, const double* z, double* y) {
@@ -169,6 +197,24 @@ mk_op* make_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
     mk_op* op = nullptr;
     mk_status s = mk_op_create_real(c, A.rows(), A.ptr.data(), A.col.data(), A.val.data(), &o, &op);
     if (s != MK_OK) throw Error(s, "operator upload failed: " + c->last_error);
+    return op;
+}
+// rectangular operator (no multigrid, no diagonal): hybrid SELL upload of a host CSR
+mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
+    auto* op = new mk_op();
+    try {
+        op->ctx = c;
+        op->n = A.rows();
+        op->m = A.cols();
+        op->cplx = false;
+        op->nnz = A.nonZeros();
+        op->block = o.block_size;
+        const bool csr = o.format == MK_FMT_CSR;
+        mk::build_sell(op, A.ptr.data(), A.col.data(), A.val.data(), csr ? 64 : o.slice_height, csr ? 1 : o.sigma);
+    } catch (...) {
+        delete op;
+        throw;
+    }
     return op;
 }
 struct OpHolder {
@@ -539,13 +585,14 @@ extern "C" mk_status mk_iterative_rounding(
             K.sync();
             X2Ft.multiply(hF2_.data(), bint.data());
             for (auto& v : bint) v *= length * wI * wI;
-            // device operator for field = X2F x (energies): upload X2F rows as a RowSet-like CSR op
-            DevBuf<int64_t> xptr;
-            DevBuf<int> xcol;
-            DevBuf<double> xval;
-            xptr.upload(X2F.ptr.data(), X2F.ptr.size(), K.st());
-            xcol.upload(X2F.col.data(), std::max<size_t>(X2F.col.size(), 1), K.st());
-            xval.upload(X2F.val.data(), std::max<size_t>(X2F.val.size(), 1), K.st());
+            // device operator length * X2F (rectangular, hybrid SELL) for the energies
+            OpHolder hX2F;
+            {
+                SprsD Xs = X2F;
+                for (auto& v : Xs.val) v *= length;
+                hX2F.op = make_rect_op(c, Xs, o);
+                hX2F.op->label = "igm_X2F";
+            }
             DevBuf<double> dfield(l1), dbint, dxcur(static_cast<size_t>(k)), dxprev(static_cast<size_t>(k)), dxt(static_cast<size_t>(k)), dxstar(static_cast<size_t>(k)),
                 dxlm(static_cast<size_t>(k)), db0(static_cast<size_t>(k)), drow(static_cast<size_t>(k)), dcand, dfixv, dcoef;
             dbint.upload(bint.data(), bint.size(), K.st());
@@ -564,16 +611,59 @@ extern "C" mk_status mk_iterative_rounding(
             std::vector<int> fixedI;
             std::vector<double> fixedV;
             const size_t ncand = std::max<size_t>(candIdx.size(), 1);
-            DevBuf<double> Zall(size_t(k) * ncand);       // Woodbury columns z_j = A0^{-1} u_j
-            std::vector<std::vector<double>> candDots;    // u_c . z_j for every candidate c
-            std::vector<double> UZ;  // r x r, row-major, UZ[i][j] = u_i . z_j
+            const size_t nc_ = candIdx.size();
+            DevBuf<double> Zall(size_t(k) * ncand);       // Woodbury column z_p = A0^{-1} u_p of candidate p
+            std::vector<double> UZall(nc_ * nc_, 0.0);    // UZall[c * nc_ + p] = u_c . z_p
+            std::vector<uint8_t> haveZ(nc_, 0);
             std::vector<int> left = flags->round_seams ? integer : singular;
             bool roundedSingulars = false, rounded = false;
+            // Batched path: every candidate the loop can fix gets its column up front, in blocks
+            // of up to 64 right-hand sides (FP64 PCG, FP32 V-cycle on the whole block).
+            if (o.precision != MK_PREC_FP64 && o.precond == MK_PRECOND_AMG && hA0.op->mg && nc_ > 0) {
+                std::vector<int> need;  // candidate positions, in candidate order
+                std::vector<uint8_t> mark(nc_, 0);
+                for (int i : left) mark[size_t(candPos[i])] = 1;
+                if (!flags->round_seams) for (int i : integer) mark[size_t(candPos[i])] = 1;
+                for (size_t p = 0; p < nc_; p++) if (mark[p]) need.push_back(int(p));
+                const int mw = need.size() > 32 ? 64 : need.size() > 16 ? 32 : need.size() > 8 ? 16 : 8;
+                DevBuf<double> Bp(size_t(k) * mw), Xp(size_t(k) * mw);
+                DevBuf<int> dsel(static_cast<size_t>(mw));
+                for (size_t b0 = 0; b0 < need.size(); b0 += size_t(mw)) {
+                    const int cnt = int(std::min<size_t>(size_t(mw), need.size() - b0));
+                    std::vector<int> sel(size_t(mw), -1);
+                    for (int j = 0; j < cnt; j++) sel[size_t(j)] = need[b0 + size_t(j)];
+                    dsel.upload(sel.data(), sel.size(), K.st());
+                    hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k) * mw)), dim3(256), 0, K.st(), size_t(k) * mw, Bp.p);
+                    hipLaunchKernelGGL(k_scatter_rows_packed, dim3(unsigned(mw)), dim3(64), 0, K.st(), mw, dsel.p,
+                                       cand.ptr.p, cand.col.p, cand.val.p, Bp.p);
+                    mk::BlockResult br = mk::block_pcg(c, hA0.op, hA0.op->mg, mw, Bp.p, Xp.p, o.tol, o.maxiter,
+                                                       o.check_every);
+                    K.acc.iterations += br.iters;
+                    K.acc.solves += cnt;
+                    S.batched_solves += cnt;
+                    S.batch_iterations += br.iters;
+                    if (br.status != MK_OK)
+                        throw Error(br.status, "IGM batched solve failed: " + std::string(mk_status_str(br.status)));
+                    hipLaunchKernelGGL(k_unpack_cols, dim3(nblk(size_t(k) * mw)), dim3(256), 0, K.st(), size_t(k), mw,
+                                       dsel.p, (const double*)Xp.p, Zall.p);
+                    for (int j = 0; j < cnt; j++) haveZ[size_t(need[b0 + size_t(j)])] = 1;
+                }
+                // U Z for every candidate pair, on the device
+                DevBuf<double> duz(nc_ * nc_);
+                hipLaunchKernelGGL(k_uz, dim3(nblk(nc_ * nc_)), dim3(256), 0, K.st(), int(nc_), size_t(k), cand.ptr.p,
+                                   cand.col.p, cand.val.p, (const double*)Zall.p, duz.p);
+                duz.download(UZall.data(), UZall.size(), K.st());
+                K.sync();
+            }
+            DevBuf<int> dzidx;
             std::vector<double> Xc(candIdx.size());
-            auto spmv_field = [&](const double* x, double* y) {
-                // y = length * X2F x   (CSR row kernel, one block per 256 rows would be better; use rowset kernel)
-                hipLaunchKernelGGL(k_rowset_dot, dim3(unsigned(l1)), dim3(64), 0, K.st(), int(l1), xptr.p, xcol.p,
-                                   xval.p, x, y);
+            // ||F2_ - length X2F x||^2 (residual SpMV with fused norm)
+            auto field_energy = [&](const double* x) {
+                mk::spmv64(c, hX2F.op, x, dfield.p, F2_.p, mk::SPMV_RESID, K.part.p + 2048, nullptr);
+                double h = 0.0;
+                MK_HIP(hipMemcpyAsync(&h, K.part.p + 2048, sizeof(double), hipMemcpyDeviceToHost, K.st()));
+                K.sync();
+                return h;
             };
             std::vector<double> dotbuf;
             DevBuf<double> ddots;
@@ -608,29 +698,23 @@ extern "C" mk_status mk_iterative_rounding(
                 left.erase(left.begin() + minIdx);
                 fixedRows.push(UF, pick);
                 fixedRows.sync(K.st());
-                // new Woodbury column z = A0^{-1} u_pick
-                {
-                    const int j = int(fixedI.size()) - 1;
-                    double* z = Zall.p + size_t(j) * size_t(k);
+                // Woodbury column z = A0^{-1} u_pick (computed now unless the batch has it)
+                const int cp = candPos[pick];
+                if (!haveZ[size_t(cp)]) {
+                    double* z = Zall.p + size_t(cp) * size_t(k);
                     hipLaunchKernelGGL(k_zero, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), drow.p);
-                    const int cp = candPos[pick];
                     const int64_t b = cand.hptr[size_t(cp)], e = cand.hptr[size_t(cp) + 1];
                     if (e > b)
                         hipLaunchKernelGGL(k_scatter_row, dim3(nblk(size_t(e - b))), dim3(256), 0, K.st(), b, e,
                                            cand.col.p, cand.val.p, drow.p);
-                    MK_HIP(hipMemsetAsync(z, 0, size_t(k) * sizeof(double), K.st()));
+                    hipLaunchKernelGGL(k_zero, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), z);
                     K.solve(hA0.op, drow.p, z);
-                    std::vector<double> cd(candIdx.size());
+                    std::vector<double> cd(nc_);
                     K.rowdots(cand, z, dcand.p);
                     dcand.download(cd.data(), cd.size(), K.st());
                     K.sync();
-                    candDots.push_back(std::move(cd));
-                    const int r = j + 1;
-                    std::vector<double> nz(size_t(r) * size_t(r), 0.0);
-                    for (int a = 0; a < r; a++)
-                        for (int bq = 0; bq < r; bq++)
-                            nz[size_t(a * r + bq)] = candDots[size_t(bq)][size_t(candPos[fixedI[size_t(a)]])];
-                    UZ.swap(nz);
+                    for (size_t q = 0; q < nc_; q++) UZall[q * nc_ + size_t(cp)] = cd[q];
+                    haveZ[size_t(cp)] = 1;
                 }
                 if (left.empty() && !flags->round_seams && !roundedSingulars) {
                     left = integer;
@@ -641,6 +725,14 @@ extern "C" mk_status mk_iterative_rounding(
                 rounded = true;
                 lm_rounds++;
                 const int r = int(fixedI.size());
+                std::vector<double> UZ(size_t(r) * size_t(r));
+                std::vector<int> zidx(static_cast<size_t>(r));
+                for (int a = 0; a < r; a++) {
+                    zidx[size_t(a)] = candPos[fixedI[size_t(a)]];
+                    for (int bq = 0; bq < r; bq++)
+                        UZ[size_t(a * r + bq)] = UZall[size_t(candPos[fixedI[size_t(a)]]) * nc_ + size_t(candPos[fixedI[size_t(bq)]])];
+                }
+                dzidx.upload(zidx.data(), zidx.size(), K.st());
                 // xt = A0^{-1} (b_int + wC^2 xPrev)  (warm start: previous xt)
                 hipLaunchKernelGGL(k_lin2, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), 1.0, dbint.p,
                                    wC * wC, dxprev.p, db0.p);
@@ -657,15 +749,14 @@ extern "C" mk_status mk_iterative_rounding(
                 // x* = xt + Z c
                 dcoef.upload(cc.data(), cc.size(), K.st());
                 hipLaunchKernelGGL(k_combine, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), r, dxt.p, Zall.p,
-                                   dcoef.p, dxstar.p);
+                                   (const int*)dzidx.p, dcoef.p, dxstar.p);
                 std::vector<double> fr_(fixedV);
                 auto dir = [&](const double* prev, double* d) {
                     hipLaunchKernelGGL(k_lin2, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), 1.0, dxstar.p,
                                        -1.0, prev, d);
                 };
                 auto energy = [&](const double* x) {
-                    spmv_field(x, dfield.p);
-                    const double a = K.sqdiff(l1, length, dfield.p, F2_.p);
+                    const double a = field_energy(x);
                     const double b = K.sqdiff(size_t(k), 1.0, x, dxprev.p);
                     std::vector<double> ux;
                     fixed_dots(x, ux);

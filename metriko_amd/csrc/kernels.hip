@@ -15,153 +15,34 @@
 #include <numeric>
 #include "mk_internal.h"
 
+#include "sell.h"
+
 namespace mk {
 
-static constexpr int kMaxWaves = 16;        // 1024 threads / 64
-static constexpr size_t kMaxParts = 1 << 17;  // partial-sum slots per reduced quantity
-static constexpr int kMaxK = 4;
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+hipEvent_t next_event(mk_ctx* c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        MK_HIP(hipEventCreate(&e));
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
 }
 
-// Block-reduce K values; thread 0 gets the block totals.
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K]) {
-    __shared__ double sh[K][kMaxWaves];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double s = wave_sum(v[k]);
-        if (lane == 0) sh[k][wid] = s;
+void timing_flush(mk_ctx* c) {
+    if (!c->time_spmv || c->ev_used == 0) { c->ev_used = 0; return; }
+    MK_HIP(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
+        float ms = 0.f;
+        MK_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
+        c->spmv_ms += ms;
+        const std::string& key = c->ev_key[i / 2];
+        auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
+        if (it != c->kstats.end()) { it->second.calls++; it->second.ms += ms; }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            double s = 0.0;
-            for (int w = 0; w < nw; w++) s += sh[k][w];
-            v[k] = s;
-        }
-    }
-    __syncthreads();
+    c->ev_used = 0;
+    c->ev_key.clear();
 }
 
-// Writes the block partials and returns true in the last block to arrive, with the
-// grand totals in tot (valid in all threads of that block).
-template <int K>
-__device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
-    __shared__ int is_last;
-    __shared__ double shtot[K];
-    block_sum<K>(v);
-    const unsigned G = gridDim.x;
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) partials[k * kMaxParts + blockIdx.x] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == G - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        acc[k] = 0.0;
-        const volatile double* pk = partials + k * kMaxParts;
-        for (unsigned i = threadIdx.x; i < G; i += blockDim.x) acc[k] += pk[i];
-    }
-    block_sum<K>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) shtot[k] = acc[k];
-        *counter = 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) tot[k] = shtot[k];
-    return true;
-}
-
-// ---------------------------------------------------------------------------------------
-// SELL SpMV.  T = float/double; CPLX: values/vectors interleaved (re, im).
-// MODE 0: y = A x; 1: y = A x, out = Re(x^H y); 2: y = b - A x, out = |y|^2;
-// 3: y = b - A x; 4: y += A x.  skip (optional): device flag, nonzero = do nothing.
-//
-// Hybrid layout: the first nb_sell blocks run the SELL part (short rows, thread per row);
-// the remaining blocks give each long row a whole wavefront (lanes stride the row, then a
-// shuffle reduction), so a few very long rows never serialise one lane.
-template <typename T>
-struct SellDev {
-    int n_short, C, nb_sell, nlong;
-    const int64_t* slice_ptr;
-    const int* slice_w;
-    const int* col;
-    const T* val;
-    const int* perm;
-    const int* long_rows;
-    const int64_t* long_ptr;
-    const int* long_col;
-    const T* long_val;
-};
-
-template <typename T>
-__device__ __forceinline__ T wave_sum_t(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Computes one row of A x.  Returns true in the thread that owns the result (row, ar, ai).
-template <typename T, bool CPLX>
-__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int& row, T& ar, T& ai) {
-    ar = 0;
-    ai = 0;
-    if (int(blockIdx.x) < A.nb_sell) {
-        const int r = blockIdx.x * blockDim.x + threadIdx.x;
-        if (r >= A.n_short) return false;
-        const int s = r / A.C, lane = r - s * A.C;
-        const int64_t base = A.slice_ptr[s] + lane;
-        const int w = A.slice_w[s];
-        row = A.perm ? A.perm[r] : r;
-        for (int j = 0; j < w; j++) {
-            const int64_t e = base + int64_t(j) * A.C;
-            const int c = A.col[e];
-            if constexpr (CPLX) {
-                const T vr = A.val[2 * e], vi = A.val[2 * e + 1];
-                const T xr = x[2 * c], xi = x[2 * c + 1];
-                ar += vr * xr - vi * xi;
-                ai += vr * xi + vi * xr;
-            } else {
-                ar += A.val[e] * x[c];
-            }
-        }
-        return true;
-    }
-    const int w = (int(blockIdx.x) - A.nb_sell) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (w < A.nlong) {  // uniform inside the wavefront
-        for (int64_t k = A.long_ptr[w] + lane; k < A.long_ptr[w + 1]; k += 64) {
-            const int c = A.long_col[k];
-            if constexpr (CPLX) {
-                const T vr = A.long_val[2 * k], vi = A.long_val[2 * k + 1];
-                const T xr = x[2 * c], xi = x[2 * c + 1];
-                ar += vr * xr - vi * xi;
-                ai += vr * xi + vi * xr;
-            } else {
-                ar += A.long_val[k] * x[c];
-            }
-        }
-        ar = wave_sum_t(ar);
-        if constexpr (CPLX) ai = wave_sum_t(ai);
-        row = A.long_rows[w];
-        return lane == 0;
-    }
-    return false;
-}
 
 template <typename T, bool CPLX, int MODE>
 __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restrict__ x, T* __restrict__ y,
@@ -391,6 +272,98 @@ __global__ void __launch_bounds__(256) cg_pupdate(size_t len, T* __restrict__ p,
 }
 
 // ---------------------------------------------------------------------------------------
+// FP64 PCG with an FP32 multigrid preconditioner (vectors as real arrays of len entries;
+// complex interleaved gives Re(r^H z) as the plain sum).  The preconditioner runs in FP32,
+// so it is not exactly symmetric: beta is the flexible (Polak-Ribiere) form
+// beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k, with z_k kept in FP64 (zold).
+
+// x += alpha p ; r -= alpha q ; r32 = float(r) ; rr
+__global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict__ x, double* __restrict__ r,
+                                                float* __restrict__ r32, const double* __restrict__ p,
+                                                const double* __restrict__ q, CgState* st, double* partials,
+                                                unsigned* counter) {
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < len) {
+        x[i] += alpha * p[i];
+        const double rk = r[i] - alpha * q[i];
+        r[i] = rk;
+        r32[i] = float(rk);
+        rr = rk * rk;
+    }
+    double v[1] = {rr}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rr = tot[0];
+        st->it += 1;
+        if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (tot[0] <= st->tol2 * st->bb) st->done = 1;
+    }
+}
+
+// rz_new = r.z32, rz_mix = r.zold ; beta = (rz_new - rz_mix) / rz
+__global__ void __launch_bounds__(256) cgm_beta(size_t len, const double* __restrict__ r,
+                                              const float* __restrict__ z32, const double* __restrict__ zold,
+                                              CgState* st, double* partials, unsigned* counter) {
+    if (st->done) return;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double a = 0.0, b = 0.0;
+    if (i < len) {
+        a = r[i] * double(z32[i]);
+        b = r[i] * zold[i];
+    }
+    double v[2] = {a, b}, tot[2];
+    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+        if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
+        else { st->beta = (tot[0] - tot[1]) / st->rz; st->rz = tot[0]; }
+    }
+}
+
+// p = z32 + beta p ; zold = z32
+__global__ void __launch_bounds__(256) cgm_pupdate(size_t len, double* __restrict__ p, const float* __restrict__ z32,
+                                                 double* __restrict__ zold, const CgState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) {
+        const double z = double(z32[i]);
+        p[i] = z + beta * p[i];
+        zold[i] = z;
+    }
+}
+
+// p = zold = z32 ; rz = r.z32 ; rr = r.r
+__global__ void __launch_bounds__(256) cgm_init(size_t len, const double* __restrict__ r, const float* __restrict__ z32,
+                                              double* __restrict__ p, double* __restrict__ zold, CgState* st,
+                                              double* partials, unsigned* counter) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (i < len) {
+        const double z = double(z32[i]);
+        p[i] = z;
+        zold[i] = z;
+        rz = r[i] * z;
+        rr = r[i] * r[i];
+    }
+    double v[2] = {rz, rr}, tot[2];
+    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it = 0;
+        st->status = MK_OK;
+        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
+        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (!st->done && !(tot[0] > 0.0)) { st->status = MK_ERR_NOT_SPD; st->done = 1; }
+    }
+}
+
+__global__ void k_to32(size_t len, const double* __restrict__ x, float* __restrict__ y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = float(x[i]);
+}
+
+// ---------------------------------------------------------------------------------------
 // misc vector kernels
 __global__ void k_axpby(size_t len, double a, const double* x, double b, double* y) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -442,7 +415,6 @@ __global__ void k_set_state_dev(CgState* st, const double* bb, double tol2) {
 // ---------------------------------------------------------------------------------------
 // host side
 
-static inline unsigned blocks_for(size_t len, int bs = 256) { return unsigned((len + bs - 1) / bs); }
 
 void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma) {
     const int n = op->n;
@@ -568,80 +540,6 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->bytes32 = double(op->nnz) * (sv32 + 4.0) + meta + double(n) * sv32 * 2.0;
 }
 
-static hipEvent_t next_event(mk_ctx* c) {
-    if (c->ev_used == c->ev_pool.size()) {
-        hipEvent_t e;
-        MK_HIP(hipEventCreate(&e));
-        c->ev_pool.push_back(e);
-    }
-    return c->ev_pool[c->ev_used++];
-}
-
-void timing_flush(mk_ctx* c) {
-    if (!c->time_spmv || c->ev_used == 0) { c->ev_used = 0; return; }
-    MK_HIP(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i + 1 < c->ev_used; i += 2) {
-        float ms = 0.f;
-        MK_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
-        c->spmv_ms += ms;
-        const std::string& key = c->ev_key[i / 2];
-        auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
-        if (it != c->kstats.end()) { it->second.calls++; it->second.ms += ms; }
-    }
-    c->ev_used = 0;
-    c->ev_key.clear();
-}
-
-// Counts an SpMV launch and, when timing is on, brackets it with events attributed to
-// "<operator label>:<kernel>:<fp32|fp64>".
-struct SpmvTimer {
-    mk_ctx* c;
-    SpmvTimer(mk_ctx* c_, double bytes, const mk_op* A = nullptr, const char* kernel = "spmv", bool fp32 = false)
-        : c(c_) {
-        c->spmv_calls++;
-        c->spmv_bytes += bytes;
-        if (c->time_spmv) {
-            if (c->ev_used + 2 > 4096) timing_flush(c);
-            std::string key = std::string(A ? A->label : "op") + ":" + kernel + ":" + (fp32 ? "fp32" : "fp64");
-            auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
-            if (it == c->kstats.end()) {
-                mk_ctx::KStat s;
-                s.bytes_per_call = bytes;
-                s.rows = A ? A->n : 0;
-                c->kstats.emplace_back(key, s);
-                it = c->kstats.end() - 1;
-            }
-            it->second.bytes += bytes;
-            c->ev_key.push_back(key);
-            MK_HIP(hipEventRecord(next_event(c), c->stream));
-        }
-    }
-    ~SpmvTimer() {
-        if (c->time_spmv) (void)hipEventRecord(next_event(c), c->stream);
-    }
-};
-
-template <typename T>
-static SellDev<T> sell_dev(const mk_op* A, const T* val, int block, unsigned* grid) {
-    SellDev<T> d;
-    d.n_short = A->n_short;
-    d.C = A->C;
-    d.nb_sell = int(blocks_for(size_t(A->n_short), block));
-    d.nlong = A->nlong;
-    d.slice_ptr = A->slice_ptr.p;
-    d.slice_w = A->slice_w.p;
-    d.col = A->col.p;
-    d.val = val;
-    d.perm = A->perm.p;
-    d.long_rows = A->long_rows.p;
-    d.long_ptr = A->long_ptr.p;
-    d.long_col = A->long_col.p;
-    if constexpr (std::is_same<T, double>::value) d.long_val = A->long_val64.p;
-    else d.long_val = A->long_val32.p;
-    const unsigned nb_long = blocks_for(size_t(A->nlong), block / 64);
-    *grid = std::max(1u, unsigned(d.nb_sell) + nb_long);
-    return d;
-}
 
 template <typename T>
 void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, const T* b, SpmvMode mode, double* out,
@@ -852,15 +750,63 @@ CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_to
                          c->scal.p + 2, rel_tol, maxiter, check_every, A->bytes32, mg);
 }
 
+// FP64 flexible PCG whose multigrid V-cycle runs in FP32 (vectors converted at its ends).
+CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_tol, int maxiter,
+                   int check_every, Mg* mg) {
+    Work& w = work(c);
+    const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
+    w.r.ensure(len); w.z.ensure(len); w.p.ensure(len); w.q.ensure(len);
+    w.r32.ensure(len); w.z32.ensure(len);
+    const unsigned g = blocks_for(len);
+    MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
+    double* bb = c->scal.p + 0;
+    double* rr0 = c->scal.p + 1;
+    dot64(c, len, b, b, bb);
+    spmv64(c, A, x, w.r.p, b, SPMV_RESID, rr0, nullptr);
+    CgState* st = c->state.p;
+    hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, (const double*)bb, rel_tol * rel_tol);
+    hipLaunchKernelGGL(k_to32, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.r32.p);
+    mg_apply32(c, mg, w.r32.p, w.z32.p, nullptr);
+    hipLaunchKernelGGL(cgm_init, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, (const float*)w.z32.p,
+                       w.p.p, w.z.p, st, c->partials.p, c->counter.p);
+    MK_HIP(hipGetLastError());
+    CgState h{};
+    for (;;) {
+        MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
+        MK_HIP(hipStreamSynchronize(c->stream));
+        timing_flush(c);
+        if (h.done || h.it >= maxiter) break;
+        const int batch = std::min(check_every, maxiter - h.it);
+        for (int k = 0; k < batch; k++) {
+            launch_cg_spmv<double>(c, A, A->val64.p, w.p.p, w.q.p, A->bytes64);
+            hipLaunchKernelGGL(cgm_update, dim3(g), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
+                               (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
+            mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
+            hipLaunchKernelGGL(cgm_beta, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p,
+                               (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
+            hipLaunchKernelGGL(cgm_pupdate, dim3(g), dim3(256), 0, c->stream, len, w.p.p, (const float*)w.z32.p,
+                               w.z.p, (const CgState*)st);
+        }
+        MK_HIP(hipGetLastError());
+    }
+    CgResult res{h.status, h.it, h.rr, h.bb};
+    if (res.status == MK_OK && !h.done) res.status = MK_ERR_NO_CONVERGENCE;
+    return res;
+}
+
 // Iterative refinement: FP64 residual r = b - A x, FP32 PCG correction on r/||r||.
 SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o, Mg* pre) {
+    MK_CHECK(o.precision >= MK_PREC_FP64 && o.precision <= MK_PREC_MIXED_MG, MK_ERR_ARG, "unknown precision mode");
     const bool jac = o.precond == MK_PRECOND_JACOBI;
     Mg* mg = (o.precond == MK_PRECOND_AMG) ? (pre ? pre : A->mg) : nullptr;
     MK_CHECK(o.precond != MK_PRECOND_AMG || mg, MK_ERR_ARG, "AMG preconditioner requested but not built");
     SolveResult res{MK_OK, 0, 0, 0.0};
     const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
-    if (o.precision == MK_PREC_FP64) {
-        CgResult cr = cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false, mg);
+    // FP32 multigrid inside FP64 PCG; without multigrid there is no FP32 work to do: plain FP64
+    const bool mg32 = o.precision == MK_PREC_MIXED_MG && mg;
+    if (o.precision == MK_PREC_FP64 || o.precision == MK_PREC_MIXED_MG) {
+        CgResult cr = mg32 ? cg64_mg32(c, A, b, x, o.tol, o.maxiter, o.check_every, mg)
+                           : cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false, mg);
         res.iters = cr.iters;
         // true FP64 residual
         Work& w = work(c);

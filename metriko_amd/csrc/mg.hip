@@ -33,6 +33,7 @@ struct MgLevel {
     int n = 0;
     DevBuf<double> b64, x64, r64, d64;
     DevBuf<float> b32, x32, r32, d32;
+    DevBuf<float> bb32, bx32, br32, bd32;  // n x mw blocks for the batched V-cycle
 };
 
 struct Mg {
@@ -690,6 +691,79 @@ void vcycle(mk_ctx* c, Mg* g, int l, const T* b, T* x, const int* skip) {
     chebyshev<T>(c, g, L, b, x, false, skip);
 }
 
+// ---- batched V-cycle (n x mw row-major blocks, FP32) -------------------------------------------
+__global__ void k_bcheb_first(size_t len, int sh, const float* __restrict__ dinv, const float* __restrict__ r,
+                              float* __restrict__ d, float* __restrict__ x, float c0, int zero_start, const int* skip) {
+    if (skip && *skip) return;
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    const float v = c0 * dinv[e >> sh] * r[e];
+    d[e] = v;
+    if (zero_start) x[e] = v;
+    else x[e] += v;
+}
+__global__ void k_bcheb_next(size_t len, int sh, const float* __restrict__ dinv, const float* __restrict__ r,
+                             float* __restrict__ d, float* __restrict__ x, float a, float bcoef, const int* skip) {
+    if (skip && *skip) return;
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= len) return;
+    const float v = a * d[e] + bcoef * dinv[e >> sh] * r[e];
+    d[e] = v;
+    x[e] += v;
+}
+// X = Inv B on the coarsest level, one thread per (row, column)
+__global__ void k_bdense(int nc, int sh, const float* __restrict__ inv, const float* __restrict__ b,
+                         float* __restrict__ x, const int* skip) {
+    if (skip && *skip) return;
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t mw = size_t(1) << sh;
+    if (e >= size_t(nc) * mw) return;
+    const size_t i = e >> sh, j = e & (mw - 1);
+    float a = 0.f;
+    for (int k = 0; k < nc; k++) a += inv[i * size_t(nc) + k] * b[size_t(k) * mw + j];
+    x[e] = a;
+}
+
+static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, const float* b, float* x,
+                            bool zero_start, const int* skip) {
+    const double lmax = L.lmax, lmin = L.lmax / tune().lmin_div;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho0 = 1.0 / sigma;
+    const size_t len = size_t(L.n) * mw;
+    float* r = L.br32.p;
+    float* d = L.bd32.p;
+    const float* src = b;
+    if (!zero_start) {
+        spmm_launch<float>(c, L.A, mw, x, r, b, SPMV_RESID_NORED, skip);
+        src = r;
+    }
+    hipLaunchKernelGGL(k_bcheb_first, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p, src,
+                       d, x, float(1.0 / theta), zero_start ? 1 : 0, skip);
+    for (int k = 1; k < g->deg; k++) {
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        spmm_launch<float>(c, L.A, mw, x, r, b, SPMV_RESID_NORED, skip);
+        hipLaunchKernelGGL(k_bcheb_next, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p,
+                           (const float*)r, d, x, float(rho1 * rho0), float(2.0 * rho1 / delta), skip);
+        rho0 = rho1;
+    }
+}
+
+static void vcycle_block(mk_ctx* c, Mg* g, int l, int mw, int sh, const float* b, float* x, const int* skip) {
+    MgLevel& L = g->lv[size_t(l)];
+    if (l + 1 == int(g->lv.size())) {
+        hipLaunchKernelGGL(k_bdense, dim3(nb(size_t(g->nc) * mw)), dim3(256), 0, c->stream, g->nc, sh,
+                           (const float*)g->inv32.p, b, x, skip);
+        return;
+    }
+    chebyshev_block(c, g, L, mw, sh, b, x, true, skip);
+    spmm_launch<float>(c, L.A, mw, x, L.br32.p, b, SPMV_RESID_NORED, skip);
+    MgLevel& Cn = g->lv[size_t(l) + 1];
+    spmm_launch<float>(c, L.R, mw, L.br32.p, Cn.bb32.p, nullptr, SPMV_PLAIN, skip);
+    vcycle_block(c, g, l + 1, mw, sh, Cn.bb32.p, Cn.bx32.p, skip);
+    spmm_launch<float>(c, L.P, mw, Cn.bx32.p, x, nullptr, SPMV_ACCUM, skip);
+    chebyshev_block(c, g, L, mw, sh, b, x, false, skip);
+}
+
 // ---- host V-cycle / PCG (test and tuning aid; same algorithm as the device V-cycle) ----------
 template <typename T>
 void host_cheb(const Csr<T>& A, const std::vector<double>& dinv, double lmax, int deg, const std::vector<T>& b,
@@ -838,6 +912,18 @@ Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const
 }
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { vcycle<float>(c, g, 0, b, x, skip); }
+void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const int* skip) {
+    MK_CHECK(!g->cplx, MK_ERR_UNSUPPORTED, "batched V-cycle: real hierarchies only");
+    MK_CHECK(mw == 8 || mw == 16 || mw == 32 || mw == 64, MK_ERR_ARG, "batched V-cycle: block width");
+    const int sh = mw == 8 ? 3 : mw == 16 ? 4 : mw == 32 ? 5 : 6;
+    for (size_t l = 0; l < g->lv.size(); l++) {
+        MgLevel& L = g->lv[l];
+        const size_t len = size_t(L.n) * mw;
+        if (l > 0) { L.bb32.ensure(len); L.bx32.ensure(len); }
+        if (l + 1 < g->lv.size()) { L.br32.ensure(len); L.bd32.ensure(len); }
+    }
+    vcycle_block(c, g, 0, mw, sh, b, x, skip);
+}
 int mg_levels(const Mg* g) { return int(g->lv.size()); }
 void mg_set_labels(Mg* g, const std::string& base) {
     for (size_t l = 0; l < g->lv.size(); l++) {

@@ -78,12 +78,26 @@ struct Work {
     DevBuf<float> r32, z32, p32, q32, x32, d32;
 };
 
+// batched (multi-right-hand-side) solves (block.hip): up to 64 columns, row-major blocks
+struct BState {
+    double rz[64], rr[64], bb[64], alpha[64], beta[64];
+    int done[64], status[64], it[64];
+    int all_done, pad;
+    double tol2;
+};
+struct BlockResult {
+    mk_status status;
+    int iters;        // max over the columns
+    double max_rel;   // max relative residual (recurrence) over the columns
+};
 // multigrid (mg.hip)
 Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o,
                   const std::vector<double>* nullvec = nullptr, int nnull = 0);
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip);
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
+// V-cycle on an n x mw row-major block (FP32)
+void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const int* skip);
 int mg_levels(const Mg* g);
 void mg_set_labels(Mg* g, const std::string& base);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
@@ -164,6 +178,12 @@ namespace mk {
 // PLAIN: y = A x; DOT: y = A x and x^H y; RESID: y = b - A x and |y|^2;
 // RESID_NORED: y = b - A x; ACCUM: y += A x
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2, SPMV_RESID_NORED = 3, SPMV_ACCUM = 4 };
+template <typename T>
+void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip);
+// A X = B for mw right-hand sides (B, X: n x mw row-major, device); X starts from zero.
+BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
+                      int maxiter, int check_every);
+
 
 void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma);
 // y = A x (mode PLAIN); y = A x and partial x^H y (DOT); y = b - A x and partial |y|^2 (RESID)

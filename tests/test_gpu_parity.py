@@ -8,6 +8,7 @@ residual (the library default).
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 import metriko_amd as mk
 from oracle.field import connection_laplacian, eigen_random_complex, solve_smallest_eig
@@ -35,7 +36,7 @@ def test_field_laplacian_device_assembly(gpu_ctx, name):
     assert abs(Lg - Lg.getH()).max() < 1e-14
 
 
-@pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED])
+@pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED, mk.PREC_MIXED_MG])
 @pytest.mark.parametrize("name", ["ico2", "icosphere_obj", "torus_small"])
 def test_smallest_eig_matches_oracle(gpu_ctx, name, precision):
     V, F = mesh(name)
@@ -60,7 +61,7 @@ def _ordered_real_decisions(dec_gpu, dec_ref):
     return a, b
 
 
-@pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED])
+@pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED, mk.PREC_MIXED_MG])
 @pytest.mark.parametrize("name", ["ico2", "icosphere_obj", "torus_small"])
 def test_pipeline_matches_oracle(gpu_ctx, name, precision):
     V, F = mesh(name)
@@ -101,7 +102,7 @@ def test_spd_solve_real_and_breakdown(gpu_ctx):
     A = sp.diags([-np.ones(n - 1), 2.0 * np.ones(n) + 0.01, -np.ones(n - 1)], [-1, 0, 1], format="csr")
     b = rng.standard_normal(n)
     op = gpu_ctx.op_real(A)
-    for prec in (mk.PREC_FP64, mk.PREC_MIXED):
+    for prec in (mk.PREC_FP64, mk.PREC_MIXED, mk.PREC_MIXED_MG):
         x, st = op.solve(b, o=mk.opts(precision=prec))
         assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1e-10
     # negated matrix: CG breakdown (Jacobi) and the multigrid setup both report MK_ERR_NOT_SPD
@@ -110,3 +111,27 @@ def test_spd_solve_real_and_breakdown(gpu_ctx):
         with pytest.raises(mk.MetrikoError) as ei:
             opn.solve(b, o=mk.opts(precision=mk.PREC_FP64, precond=pc))
         assert ei.value.status == 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("m", [5, 37, 70])
+def test_spd_solve_many_matches_direct(gpu_ctx, m):
+    """Batched multi-right-hand-side PCG (blocks of up to 64 columns) against SuperLU, with the
+    sequential FP64 path as a second check; every column to a 1e-10 relative residual."""
+    nx = 70
+    T = sp.diags([-np.ones(nx - 1), 2.0 * np.ones(nx), -np.ones(nx - 1)], [-1, 0, 1])
+    A = (sp.kron(T, sp.identity(nx)) + sp.kron(sp.identity(nx), T) + 1e-3 * sp.identity(nx * nx)).tocsr()
+    rng = np.random.default_rng(m)
+    B = rng.standard_normal((A.shape[0], m))
+    B[:, 1] = 0.0  # an empty column converges at once
+    op = gpu_ctx.op_real(A)
+    Xref = spla.splu(A.tocsc()).solve(B)
+    for prec in (mk.PREC_MIXED_MG, mk.PREC_FP64):
+        X, st = op.solve_many(B, o=mk.opts(precision=prec))
+        print(m, prec, st)
+        assert st["solves"] == m
+        res = np.linalg.norm(A @ X - B, axis=0) / np.maximum(np.linalg.norm(B, axis=0), 1e-300)
+        assert res.max() <= 1e-10
+        assert np.abs(X[:, 1]).max() == 0.0
+        err = np.linalg.norm(X - Xref, axis=0) / np.maximum(np.linalg.norm(Xref, axis=0), 1e-300)
+        assert err.max() <= 1e-6
