@@ -185,7 +185,8 @@ mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, in
 /* Host-only multigrid PCG (no GPU; test and tuning aid): builds the same smoothed-aggregation
  * hierarchy the device path uses for the CSR matrix (complex when cplx != 0, values
  * interleaved) and solves A x = b with a host V-cycle.  Returns iterations, levels and the
- * final relative residual. */
+ * final relative residual; maxit = 0 applies one V-cycle (x = M^-1 b).  nullvec (n x nnull,
+ * column-major, real operators) are near-null vectors for the aggregation. */
 mk_status mk_mg_host_pcg(int n, const int64_t* rowptr, const int32_t* col, const double* val, int cplx,
                          const double* nullvec, int nnull, const double* b, double* x, double tol, int maxit,
                          int* iters, int* levels, double* relres);

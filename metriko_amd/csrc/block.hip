@@ -39,15 +39,25 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             const int64_t e = base + int64_t(k) * A.C;
             acc += A.val[e] * X[size_t(A.col[e]) * MW + j];
         }
-    } else {
-        const int lw = gw - nw_sell;
-        if (lw >= A.nlong) return;  // uniform inside the wavefront
-        for (int64_t k = A.long_ptr[lw] + sub; k < A.long_ptr[lw + 1]; k += R)
-            acc += A.long_val[k] * X[size_t(A.long_col[k]) * MW + j];
+    } else {  // medium and long rows: a wavefront per row, its 64/MW row groups split the entries
+        int lw = gw - nw_sell;
+        const int* rows = A.mid_rows;
+        const int64_t* ptr = A.mid_ptr;
+        const int* cols = A.mid_col;
+        const T* vals = A.mid_val;
+        if (lw >= A.nmid) {
+            lw -= A.nmid;
+            if (lw >= A.nlong) return;  // uniform inside the wavefront
+            rows = A.long_rows;
+            ptr = A.long_ptr;
+            cols = A.long_col;
+            vals = A.long_val;
+        }
+        for (int64_t k = ptr[lw] + sub; k < ptr[lw + 1]; k += R) acc += vals[k] * X[size_t(cols[k]) * MW + j];
 #pragma unroll
         for (int off = MW; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
         if (sub != 0) return;
-        row = A.long_rows[lw];
+        row = rows[lw];
     }
     const size_t o = size_t(row) * MW + j;
     if constexpr (MODE == 0) Y[o] = acc;
@@ -61,7 +71,7 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
     unsigned g0 = 0;
     const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
     const int nw_sell = int((int64_t(A->n_short) + 64 / MW - 1) / (64 / MW));
-    const int64_t waves = int64_t(nw_sell) + A->nlong;
+    const int64_t waves = int64_t(nw_sell) + A->nmid + A->nlong;
     const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
     switch (mode) {
         case SPMV_PLAIN:
@@ -338,7 +348,7 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
     DevBuf<BState> st(1);
     MK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), c->stream));
     MK_HIP(hipMemsetAsync(st.p, 0, sizeof(BState), c->stream));
-    const unsigned g = unsigned(std::min<size_t>(kBlkParts, std::max<size_t>(1, (len + 255) / 256)));
+    const unsigned g = unsigned(std::min<size_t>(kRedGrid, std::max<size_t>(1, (len + 255) / 256)));
     // the recurrence residual is driven to half the target so the true residual meets it
     hipLaunchKernelGGL(k_set_bstate, dim3(1), dim3(1), 0, c->stream, st.p, 0.25 * rel_tol * rel_tol);
     hipLaunchKernelGGL((bk_init0<MW>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, R32.p, st.p, part.p, cnt.p);

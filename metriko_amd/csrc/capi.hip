@@ -128,6 +128,7 @@ void host_csr(mk_ctx* c, const mk_op* A, std::vector<int64_t>& rp, std::vector<i
         vv = A->h_val;
         return;
     }
+    MK_CHECK(A->nmid == 0, MK_ERR_UNSUPPORTED, "operator export: no host copy of the medium-row part");
     std::vector<int64_t> sptr(size_t(A->nslices) + 1);
     std::vector<int> sw(size_t(A->nslices)), hc(size_t(A->stored));
     std::vector<double> hv(size_t(A->stored) * vw);
@@ -254,9 +255,11 @@ mk_status mk_init(mk_ctx** out, int device) {
         c->device = device;
         MK_HIP(hipSetDevice(device));
         MK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        c->partials.alloc(size_t(1 << 17) * 4);
-        c->counter.alloc(16);
-        MK_HIP(hipMemsetAsync(c->counter.p, 0, 16 * sizeof(unsigned), c->stream));
+        // per-block partials (4 quantities x 2^17 blocks) + group slots (4 x 2^12), and the
+        // global + per-group arrival counters of the grid reductions
+        c->partials.alloc(size_t(1 << 17) * 4 + size_t(1 << 12) * 4);
+        c->counter.alloc(1 + (1 << 12) + 16);
+        MK_HIP(hipMemsetAsync(c->counter.p, 0, (1 + (1 << 12) + 16) * sizeof(unsigned), c->stream));
         c->state.alloc(1);
         c->scal.alloc(64);
         MK_HIP(hipMemsetAsync(c->scal.p, 0, 64 * sizeof(double), c->stream));
@@ -619,6 +622,7 @@ mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, c
             total_it += r.iters;
             total_ref += r.refinements;
             last_rel = r.rel_residual;
+            if (it % 10 == 9) MK_LOG("[field] inverse iteration %d: %d PCG iterations so far\n", it + 1, total_it);
             if (r.status != MK_OK) {
                 inner = r.status;
                 c->last_error = "inverse iteration " + std::to_string(it) + ": " + mk_status_str(r.status);

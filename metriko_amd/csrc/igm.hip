@@ -424,7 +424,10 @@ extern "C" mk_status mk_iterative_rounding(
             metriko::MatXd CS = metriko::MatXd::Zero(c_rows, int(uniqueJ.size()));
             for (int i = 0; i < c_rows; i++)
                 for (int64_t k = Cf.ptr[i]; k < Cf.ptr[i + 1]; k++) CS(i, jmask[Cf.col[k]]) = Cf.val[k];
+            MK_LOG("[igm] constraints %d x %zu (of %d variables), FullPivLU...\n", c_rows, uniqueJ.size(), nfull);
             US = metriko::fullpiv_lu_kernel(CS, &crank);
+            MK_LOG("[igm] kernel basis %d x %d, rank %d, %.0f ms\n", int(US.rows()), int(US.cols()), crank,
+                   now_ms() - t_all);
         } else {
             nonPart.resize(size_t(nfull));
             std::iota(nonPart.begin(), nonPart.end(), 0);
@@ -444,7 +447,9 @@ extern "C" mk_status mk_iterative_rounding(
                     if (US(r, j) != 0.0) T.emplace_back(uniqueJ[r], npn + j, US(r, j));
             UF.setFromTriplets(T);
         }
+        MK_LOG("[igm] UFull %d x %d nnz %lld, %.0f ms\n", nfull, k, (long long)UF.nonZeros(), now_ms() - t_all);
         SprsD X2F = (G2 * UF).pruned();      // G2 * UFull (naive, without length)
+        MK_LOG("[igm] X2F nnz %lld, %.0f ms\n", (long long)X2F.nonZeros(), now_ms() - t_all);
         SprsD X2Ft = X2F.transpose();
         SprsD E = X2Ft * X2F;
         for (auto& v : E.val) v *= length;   // E = X2F^T X2F * length
@@ -455,6 +460,7 @@ extern "C" mk_status mk_iterative_rounding(
         S.k = k;
         S.kernel_rank = crank;
         S.setup_ms = now_ms() - t_all;
+        MK_LOG("[igm] host setup %.0f ms (k=%d, kernel rank %d)\n", S.setup_ms, k, crank);
 
         // ---- Poisson KKT [[E, C^T],[C, 0]] [x; l] = [f; g] (iter_rounding_init.h:270-307) ----
         // f lies in range(E) and C only pins the translations E annihilates, so the
@@ -515,6 +521,7 @@ extern "C" mk_status mk_iterative_rounding(
             S.kkt_rel_residual = fn > 0 ? std::sqrt(rn / fn) : std::sqrt(rn);
         }
         S.kkt_ms = now_ms() - t0;
+        MK_LOG("[igm] KKT solve %.0f ms\n", S.kkt_ms);
         std::vector<double> x0(static_cast<size_t>(nfull));
         UF.multiply(XSmall.data(), x0.data());
 
@@ -576,6 +583,7 @@ extern "C" mk_status mk_iterative_rounding(
             DevBuf<double> F2_(l1);
             lm_solve(K, l1, fcur.p, F2_.p, 100, 10e-10, dir, energy, pv, dd, tr);
             S.naive_ms = now_ms() - t0;
+            MK_LOG("[igm] naive LM %.0f ms\n", S.naive_ms);
 
             // ---- seamless rounding loop (iter_rounding.h:89-112, iter_rounding_loop.h) ----
             t0 = now_ms();
@@ -647,6 +655,8 @@ extern "C" mk_status mk_iterative_rounding(
                     hipLaunchKernelGGL(k_unpack_cols, dim3(nblk(size_t(k) * mw)), dim3(256), 0, K.st(), size_t(k), mw,
                                        dsel.p, (const double*)Xp.p, Zall.p);
                     for (int j = 0; j < cnt; j++) haveZ[size_t(need[b0 + size_t(j)])] = 1;
+                    MK_LOG("[igm] batched Woodbury columns %zu/%zu: %d iterations\n", b0 + size_t(cnt), need.size(),
+                           br.iters);
                 }
                 // U Z for every candidate pair, on the device
                 DevBuf<double> duz(nc_ * nc_);
@@ -721,6 +731,7 @@ extern "C" mk_status mk_iterative_rounding(
                     roundedSingulars = true;
                 }
                 rounds++;
+                if (rounds % 10 == 0) MK_LOG("[igm] rounding: %d rounds, %d left\n", rounds, int(left.size()));
                 if (!(minDif > 10e-7)) continue;
                 rounded = true;
                 lm_rounds++;

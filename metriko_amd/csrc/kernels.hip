@@ -50,23 +50,25 @@ __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restri
                                                double* out, const int* skip) {
     if (skip && *skip) return;
     double contrib = 0.0;
-    int row = 0;
-    T ar, ai;
-    if (spmv_row<T, CPLX>(A, x, row, ar, ai)) {
+    // reducing modes run a capped grid that strides over the virtual blocks
+    for (int vb = blockIdx.x; vb < A.nvb; vb += gridDim.x) {
+        int row = 0;
+        T ar, ai;
+        if (!spmv_row<T, CPLX>(A, x, vb, row, ar, ai)) continue;
         if constexpr (CPLX) {
             if constexpr (MODE == 2 || MODE == 3) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
             if constexpr (MODE == 4) { ar += y[2 * row]; ai += y[2 * row + 1]; }
             y[2 * row] = ar;
             y[2 * row + 1] = ai;
-            if constexpr (MODE == 1) contrib = double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
-            if constexpr (MODE == 2) contrib = double(ar) * double(ar) + double(ai) * double(ai);
+            if constexpr (MODE == 1) contrib += double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
+            if constexpr (MODE == 2) contrib += double(ar) * double(ar) + double(ai) * double(ai);
         } else {
             T a = ar;
             if constexpr (MODE == 2 || MODE == 3) a = b[row] - a;
             if constexpr (MODE == 4) a += y[row];
             y[row] = a;
-            if constexpr (MODE == 1) contrib = double(x[row]) * double(a);
-            if constexpr (MODE == 2) contrib = double(a) * double(a);
+            if constexpr (MODE == 1) contrib += double(x[row]) * double(a);
+            if constexpr (MODE == 2) contrib += double(a) * double(a);
         }
     }
     if constexpr (MODE == 1 || MODE == 2) {
@@ -83,21 +85,20 @@ template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_init(int n, const T* __restrict__ r, const T* __restrict__ dinv,
                                              T* __restrict__ z, T* __restrict__ p, CgState* st, double* partials,
                                              unsigned* counter) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const T d = dinv ? dinv[i] : T(1);
         if constexpr (CPLX) {
             const T r0 = r[2 * i], r1 = r[2 * i + 1];
             const T z0 = d * r0, z1 = d * r1;
             z[2 * i] = z0; z[2 * i + 1] = z1; p[2 * i] = z0; p[2 * i + 1] = z1;
-            rz = double(r0) * z0 + double(r1) * z1;
-            rr = double(r0) * r0 + double(r1) * r1;
+            rz += double(r0) * z0 + double(r1) * z1;
+            rr += double(r0) * r0 + double(r1) * r1;
         } else {
             const T r0 = r[i], z0 = d * r0;
             z[i] = z0; p[i] = z0;
-            rz = double(r0) * z0;
-            rr = double(r0) * r0;
+            rz += double(r0) * z0;
+            rr += double(r0) * r0;
         }
     }
     double v[2] = {rz, rr}, tot[2];
@@ -117,16 +118,17 @@ __global__ void __launch_bounds__(256) cg_spmv(SellDev<T> A, const T* __restrict
                                              CgState* st, double* partials, unsigned* counter) {
     if (st->done) return;
     double contrib = 0.0;
-    int row = 0;
-    T ar, ai;
-    if (spmv_row<T, CPLX>(A, p, row, ar, ai)) {
+    for (int vb = blockIdx.x; vb < A.nvb; vb += gridDim.x) {
+        int row = 0;
+        T ar, ai;
+        if (!spmv_row<T, CPLX>(A, p, vb, row, ar, ai)) continue;
         if constexpr (CPLX) {
             q[2 * row] = ar;
             q[2 * row + 1] = ai;
-            contrib = double(p[2 * row]) * double(ar) + double(p[2 * row + 1]) * double(ai);
+            contrib += double(p[2 * row]) * double(ar) + double(p[2 * row + 1]) * double(ai);
         } else {
             q[row] = ar;
-            contrib = double(p[row]) * double(ar);
+            contrib += double(p[row]) * double(ar);
         }
     }
     double v[1] = {contrib}, tot[1];
@@ -149,9 +151,8 @@ __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __
                                                unsigned* counter) {
     if (st->done) return;
     const T alpha = T(st->alpha);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const T d = dinv ? dinv[i] : T(1);
         if constexpr (CPLX) {
 #pragma unroll
@@ -171,8 +172,8 @@ __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __
             r[i] = rk;
             const T zk = d * rk;
             z[i] = zk;
-            rz = double(rk) * zk;
-            rr = double(rk) * rk;
+            rz += double(rk) * zk;
+            rr += double(rk) * rk;
         }
     }
     double v[2] = {rz, rr}, tot[2];
@@ -190,19 +191,18 @@ __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_init_z(int n, const T* __restrict__ r, const T* __restrict__ z,
                                                T* __restrict__ p, CgState* st, double* partials, unsigned* counter) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if constexpr (CPLX) {
             const T r0 = r[2 * i], r1 = r[2 * i + 1], z0 = z[2 * i], z1 = z[2 * i + 1];
             p[2 * i] = z0; p[2 * i + 1] = z1;
-            rz = double(r0) * z0 + double(r1) * z1;
-            rr = double(r0) * r0 + double(r1) * r1;
+            rz += double(r0) * z0 + double(r1) * z1;
+            rr += double(r0) * r0 + double(r1) * r1;
         } else {
             const T r0 = r[i], z0 = z[i];
             p[i] = z0;
-            rz = double(r0) * z0;
-            rr = double(r0) * r0;
+            rz += double(r0) * z0;
+            rr += double(r0) * r0;
         }
     }
     double v[2] = {rz, rr}, tot[2];
@@ -223,9 +223,8 @@ __global__ void __launch_bounds__(256) cg_update_noz(int n, T* __restrict__ x, T
                                                    double* partials, unsigned* counter) {
     if (st->done) return;
     const T alpha = T(st->alpha);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int w = CPLX ? 2 : 1;
 #pragma unroll
         for (int c = 0; c < w; c++) {
@@ -249,11 +248,10 @@ template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_beta(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st,
                                              double* partials, unsigned* counter) {
     if (st->done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0;
-    if (i < n) {
-        if constexpr (CPLX) rz = double(r[2 * i]) * z[2 * i] + double(r[2 * i + 1]) * z[2 * i + 1];
-        else rz = double(r[i]) * z[i];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if constexpr (CPLX) rz += double(r[2 * i]) * z[2 * i] + double(r[2 * i + 1]) * z[2 * i + 1];
+        else rz += double(r[i]) * z[i];
     }
     double v[1] = {rz}, tot[1];
     if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
@@ -284,14 +282,13 @@ __global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict
                                                 unsigned* counter) {
     if (st->done) return;
     const double alpha = st->alpha;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double rr = 0.0;
-    if (i < len) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
         x[i] += alpha * p[i];
         const double rk = r[i] - alpha * q[i];
         r[i] = rk;
         r32[i] = float(rk);
-        rr = rk * rk;
+        rr += rk * rk;
     }
     double v[1] = {rr}, tot[1];
     if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
@@ -307,11 +304,10 @@ __global__ void __launch_bounds__(256) cgm_beta(size_t len, const double* __rest
                                               const float* __restrict__ z32, const double* __restrict__ zold,
                                               CgState* st, double* partials, unsigned* counter) {
     if (st->done) return;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double a = 0.0, b = 0.0;
-    if (i < len) {
-        a = r[i] * double(z32[i]);
-        b = r[i] * zold[i];
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
+        a += r[i] * double(z32[i]);
+        b += r[i] * zold[i];
     }
     double v[2] = {a, b}, tot[2];
     if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
@@ -337,14 +333,13 @@ __global__ void __launch_bounds__(256) cgm_pupdate(size_t len, double* __restric
 __global__ void __launch_bounds__(256) cgm_init(size_t len, const double* __restrict__ r, const float* __restrict__ z32,
                                               double* __restrict__ p, double* __restrict__ zold, CgState* st,
                                               double* partials, unsigned* counter) {
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     double rz = 0.0, rr = 0.0;
-    if (i < len) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
         const double z = double(z32[i]);
         p[i] = z;
         zold[i] = z;
-        rz = r[i] * z;
-        rr = r[i] * r[i];
+        rz += r[i] * z;
+        rr += r[i] * r[i];
     }
     double v[2] = {rz, rr}, tot[2];
     if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
@@ -380,9 +375,8 @@ __global__ void k_mass_mul(int n, const double* mass, const double* u, double* b
 }
 __global__ void k_mass_dot(int n, const double* mass, const double* x, double* partials, unsigned* counter,
                            double* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double c = 0.0;
-    if (i < n) c = mass[i] * (x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += mass[i] * (x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
     double v[1] = {c}, tot[1];
     if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
 }
@@ -431,8 +425,10 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         med = s[size_t(n / 2)];
     }
     const int thr = std::max(64, 4 * med);
-    std::vector<int> order, lrows;
-    for (int i = 0; i < n; i++) (lens[i] > thr ? lrows : order).push_back(i);
+    // medium rows (more than kMidLen entries) get a group of lanes each instead of one lane
+    constexpr int kMidLen = 12;
+    std::vector<int> order, lrows, mrows;
+    for (int i = 0; i < n; i++) (lens[i] > thr ? lrows : lens[i] > kMidLen ? mrows : order).push_back(i);
     const int ns_rows = int(order.size());
     // remaining rows of uneven length: sort inside windows of 1024 rows
     if (sigma <= 1 && ns_rows > 0) {
@@ -449,7 +445,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
             std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) { return lens[a] > lens[b]; });
         }
     }
-    bool permuted = !lrows.empty();
+    bool permuted = !lrows.empty() || !mrows.empty();
     for (int i = 0; i < ns_rows && !permuted; i++) if (order[i] != i) permuted = true;
     const int n_rows = ns_rows;
     const int ns = (n_rows + C - 1) / C;
@@ -494,6 +490,28 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
         dinv32[i] = float(dinv[i]);
     }
+    // medium-row part (CSR, mid_lanes lanes per row: about 4 entries per lane)
+    const int nmid = int(mrows.size());
+    int mlanes = 4;
+    if (nmid) {
+        std::vector<int> ml;
+        for (int r : mrows) ml.push_back(lens[r]);
+        std::nth_element(ml.begin(), ml.begin() + nmid / 2, ml.end());
+        const int mm = ml[size_t(nmid / 2)];
+        while (mlanes < 16 && mlanes * 4 < mm) mlanes *= 2;
+    }
+    std::vector<int64_t> mptr(size_t(nmid) + 1, 0);
+    std::vector<int> mcol;
+    std::vector<double> mval;
+    std::vector<float> mval32;
+    for (int w = 0; w < nmid; w++) {
+        const int row = mrows[w];
+        for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
+            mcol.push_back(col[k]);
+            for (int c = 0; c < vw; c++) { mval.push_back(val[k * vw + c]); mval32.push_back(float(val[k * vw + c])); }
+        }
+        mptr[w + 1] = int64_t(mcol.size());
+    }
     // long-row part (CSR)
     const int nlong = int(lrows.size());
     std::vector<int64_t> lptr(size_t(nlong) + 1, 0);
@@ -526,6 +544,15 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->dinv32.upload(dinv32.data(), dinv32.size(), st);
     if (permuted) op->perm.upload(order.data(), std::max<size_t>(order.size(), 1), st);
     else op->perm.release();
+    op->nmid = nmid;
+    op->mid_lanes = mlanes;
+    if (nmid) {
+        op->mid_rows.upload(mrows.data(), mrows.size(), st);
+        op->mid_ptr.upload(mptr.data(), mptr.size(), st);
+        op->mid_col.upload(mcol.data(), mcol.size(), st);
+        op->mid_val64.upload(mval.data(), mval.size(), st);
+        op->mid_val32.upload(mval32.data(), mval32.size(), st);
+    }
     if (nlong) {
         op->long_rows.upload(lrows.data(), lrows.size(), st);
         op->long_ptr.upload(lptr.data(), lptr.size(), st);
@@ -535,7 +562,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     }
     MK_HIP(hipStreamSynchronize(st));
     const double sv64 = 8.0 * vw, sv32 = 4.0 * vw;
-    const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0) + double(nlong) * 12.0;
+    const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0) + double(nlong + nmid) * 12.0;
     op->bytes64 = double(op->nnz) * (sv64 + 4.0) + meta + double(n) * sv64 * 2.0;
     op->bytes32 = double(op->nnz) * (sv32 + 4.0) + meta + double(n) * sv32 * 2.0;
 }
@@ -549,8 +576,10 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
     static const char* kname[] = {"spmv", "spmv_dot", "spmv_resid", "spmv_resid_nored", "spmv_accum"};
     SpmvTimer tm(c, bytes, A, kname[int(mode)], sizeof(T) == 4);
+    const unsigned gr = std::min(g, kRedGrid);  // reducing modes: capped grid
 #define MK_SPMV(CPLX, MODE)                                                                                \
-    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3(g), dim3(A->block), 0, c->stream, D, x, y, b,     \
+    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
+                       c->stream, D, x, y, b,                                                              \
                        c->partials.p, c->counter.p, out, skip)
 #define MK_SPMV_ALL(CPLX)                            \
     switch (mode) {                                  \
@@ -565,6 +594,71 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
 #undef MK_SPMV
     MK_HIP(hipGetLastError());
 }
+// One Chebyshev step fused with the residual it needs, row-local:
+//   r = b - A x_in ; d = a d + s D^-1 r (first step: d = s D^-1 r) ; x_out = x_in + d.
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) sell_cheb(SellDev<T> A, const T* __restrict__ x_in, T* __restrict__ x_out,
+                                               const T* __restrict__ b, T* __restrict__ d,
+                                               const T* __restrict__ dinv, T a, T s, int first, const int* skip) {
+    if (skip && *skip) return;
+    int row = 0;
+    T ar, ai;
+    if (!spmv_row<T, CPLX>(A, x_in, blockIdx.x, row, ar, ai)) return;
+    const T sd = s * dinv[row];
+    if constexpr (CPLX) {
+        T d0 = sd * (b[2 * row] - ar), d1 = sd * (b[2 * row + 1] - ai);
+        if (!first) { d0 += a * d[2 * row]; d1 += a * d[2 * row + 1]; }
+        d[2 * row] = d0;
+        d[2 * row + 1] = d1;
+        x_out[2 * row] = x_in[2 * row] + d0;
+        x_out[2 * row + 1] = x_in[2 * row + 1] + d1;
+    } else {
+        T d0 = sd * (b[row] - ar);
+        if (!first) d0 += a * d[row];
+        d[row] = d0;
+        x_out[row] = x_in[row] + d0;
+    }
+}
+
+template <typename T>
+void cheb_launch(mk_ctx* c, const mk_op* A, const T* x_in, T* x_out, const T* b, T* d, T a, T s, int first,
+                 const int* skip) {
+    unsigned g = 0;
+    const T* val;
+    const T* dinv;
+    if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
+    else { val = A->val32.p; dinv = A->dinv32.p; }
+    const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
+    const double vb = double(sizeof(T)) * (A->cplx ? 2 : 1);
+    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) + double(A->n) * (vb * (first ? 2.0 : 3.0) +
+                                                                                      double(sizeof(T)));
+    SpmvTimer tm(c, bytes, A, "cheb", sizeof(T) == 4);
+    if (A->cplx)
+        hipLaunchKernelGGL((sell_cheb<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv, a,
+                           s, first, skip);
+    else
+        hipLaunchKernelGGL((sell_cheb<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
+                           a, s, first, skip);
+    MK_HIP(hipGetLastError());
+}
+template void cheb_launch<double>(mk_ctx*, const mk_op*, const double*, double*, const double*, double*, double,
+                                  double, int, const int*);
+template void cheb_launch<float>(mk_ctx*, const mk_op*, const float*, float*, const float*, float*, float, float,
+                                 int, const int*);
+
+template <typename T>
+__global__ void k_copy(size_t len, const T* __restrict__ x, T* __restrict__ y, const int* skip) {
+    if (skip && *skip) return;
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < len) y[i] = x[i];
+}
+template <typename T>
+void copy_launch(mk_ctx* c, size_t len, const T* x, T* y, const int* skip) {
+    hipLaunchKernelGGL(k_copy<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, x, y, skip);
+}
+template void copy_launch<double>(mk_ctx*, size_t, const double*, double*, const int*);
+template void copy_launch<float>(mk_ctx*, size_t, const float*, float*, const int*);
+
 template void spmv_launch<double>(mk_ctx*, const mk_op*, const double*, const double*, double*, const double*,
                                   SpmvMode, double*, double, const int*);
 template void spmv_launch<float>(mk_ctx*, const mk_op*, const float*, const float*, float*, const float*, SpmvMode,
@@ -589,7 +683,7 @@ void scale_dev64(mk_ctx* c, size_t len, const double* s_dev, int invert_sqrt, do
     MK_HIP(hipGetLastError());
 }
 void dot64(mk_ctx* c, size_t len, const double* x, const double* y, double* out_dev) {
-    unsigned g = std::min<unsigned>(blocks_for(len), 2048u);
+    unsigned g = rgrid(len);
     g = std::max(g, 1u);
     hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, c->stream, len, x, y, c->partials.p, c->counter.p, out_dev);
     MK_HIP(hipGetLastError());
@@ -599,7 +693,7 @@ void mass_mul(mk_ctx* c, int n, const double* mass, const double* u, double* b) 
     MK_HIP(hipGetLastError());
 }
 void mass_dot(mk_ctx* c, int n, const double* mass, const double* x, double* out_dev) {
-    hipLaunchKernelGGL(k_mass_dot, dim3(blocks_for(n)), dim3(256), 0, c->stream, n, mass, x, c->partials.p,
+    hipLaunchKernelGGL(k_mass_dot, dim3(rgrid(n)), dim3(256), 0, c->stream, n, mass, x, c->partials.p,
                        c->counter.p, out_dev);
     MK_HIP(hipGetLastError());
 }
@@ -624,10 +718,10 @@ static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, 
     const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
     SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
     if (A->cplx)
-        hipLaunchKernelGGL((cg_spmv<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
+        hipLaunchKernelGGL((cg_spmv<T, true>), dim3(std::min(g, kRedGrid)), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
                            c->partials.p, c->counter.p);
     else
-        hipLaunchKernelGGL((cg_spmv<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
+        hipLaunchKernelGGL((cg_spmv<T, false>), dim3(std::min(g, kRedGrid)), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
                            c->partials.p, c->counter.p);
 }
 
@@ -640,17 +734,17 @@ static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T*
     CgState* st = c->state.p;
     launch_cg_spmv<T>(c, A, val, p, q, bytes);
     if (A->cplx)
-        hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, p, q, st,
+        hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, p, q, st,
                            c->partials.p, c->counter.p);
     else
-        hipLaunchKernelGGL((cg_update_noz<T, false>), dim3(g), dim3(256), 0, c->stream, n, x, r, p, q, st,
+        hipLaunchKernelGGL((cg_update_noz<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, p, q, st,
                            c->partials.p, c->counter.p);
     mg_apply(c, mg, r, z, &st->done);
     if (A->cplx)
-        hipLaunchKernelGGL((cg_beta<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
+        hipLaunchKernelGGL((cg_beta<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
                            c->counter.p);
     else
-        hipLaunchKernelGGL((cg_beta<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
+        hipLaunchKernelGGL((cg_beta<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
                            c->counter.p);
     hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z, (const CgState*)st);
 }
@@ -668,16 +762,16 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
     if (mg) {
         mg_apply(c, mg, r, z, nullptr);
         if (A->cplx)
-            hipLaunchKernelGGL((cg_init_z<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
+            hipLaunchKernelGGL((cg_init_z<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
                                c->counter.p);
         else
-            hipLaunchKernelGGL((cg_init_z<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
+            hipLaunchKernelGGL((cg_init_z<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
                                c->counter.p);
     } else if (A->cplx)
-        hipLaunchKernelGGL((cg_init<T, true>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
+        hipLaunchKernelGGL((cg_init<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
                            c->partials.p, c->counter.p);
     else
-        hipLaunchKernelGGL((cg_init<T, false>), dim3(g), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
+        hipLaunchKernelGGL((cg_init<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
                            c->partials.p, c->counter.p);
     MK_HIP(hipGetLastError());
     if (mg) {
@@ -706,10 +800,10 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
         for (int b = 0; b < batch; b++) {
             launch_cg_spmv<T>(c, A, val, p, q, bytes);
             if (A->cplx)
-                hipLaunchKernelGGL((cg_update<T, true>), dim3(g), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                hipLaunchKernelGGL((cg_update<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
                                    st, c->partials.p, c->counter.p);
             else
-                hipLaunchKernelGGL((cg_update<T, false>), dim3(g), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                hipLaunchKernelGGL((cg_update<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
                                    st, c->partials.p, c->counter.p);
             hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z,
                                (const CgState*)st);
@@ -767,7 +861,7 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
     hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, (const double*)bb, rel_tol * rel_tol);
     hipLaunchKernelGGL(k_to32, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.r32.p);
     mg_apply32(c, mg, w.r32.p, w.z32.p, nullptr);
-    hipLaunchKernelGGL(cgm_init, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, (const float*)w.z32.p,
+    hipLaunchKernelGGL(cgm_init, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p, (const float*)w.z32.p,
                        w.p.p, w.z.p, st, c->partials.p, c->counter.p);
     MK_HIP(hipGetLastError());
     CgState h{};
@@ -779,10 +873,10 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         const int batch = std::min(check_every, maxiter - h.it);
         for (int k = 0; k < batch; k++) {
             launch_cg_spmv<double>(c, A, A->val64.p, w.p.p, w.q.p, A->bytes64);
-            hipLaunchKernelGGL(cgm_update, dim3(g), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
+            hipLaunchKernelGGL(cgm_update, dim3(rgrid(len)), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
                                (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
             mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
-            hipLaunchKernelGGL(cgm_beta, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p,
+            hipLaunchKernelGGL(cgm_beta, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p,
                                (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
             hipLaunchKernelGGL(cgm_pupdate, dim3(g), dim3(256), 0, c->stream, len, w.p.p, (const float*)w.z32.p,
                                w.z.p, (const CgState*)st);

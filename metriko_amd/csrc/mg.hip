@@ -31,8 +31,8 @@ struct MgLevel {
     mk_op* R = nullptr;  // n_{l+1} x n_l
     double lmax = 2.0;
     int n = 0;
-    DevBuf<double> b64, x64, r64, d64;
-    DevBuf<float> b32, x32, r32, d32;
+    DevBuf<double> b64, x64, r64, d64, y64;  // y: second iterate buffer of the fused smoother
+    DevBuf<float> b32, x32, r32, d32, y32;
     DevBuf<float> bb32, bx32, br32, bd32;  // n x mw blocks for the batched V-cycle
 };
 
@@ -59,8 +59,9 @@ void mg_free(Mg* g) {
 
 // Multigrid parameters (METRIKO_MG_* environment overrides, read once).
 struct MgTune {
-    double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 30.0, lmax_fac = 1.1;
-    int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1;
+    double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 10.0, lmax_fac = 1.2;
+    int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1, compat = 1,
+        l1 = 1;
 };
 static const MgTune& tune() {
     static const MgTune t = [] {
@@ -77,6 +78,8 @@ static const MgTune& tune() {
         i("METRIKO_MG_COARSE_REAL", x.coarse_real);
         i("METRIKO_MG_COARSE_CPLX", x.coarse_cplx);
         i("METRIKO_MG_SMOOTH_LONG", x.smooth_long);
+        i("METRIKO_MG_COMPAT", x.compat);
+        i("METRIKO_MG_L1", x.l1);
         return x;
     }();
     return t;
@@ -152,6 +155,25 @@ std::vector<uint8_t> long_rows(const Csr<T>& A) {
     return lng;
 }
 
+// With near-null vectors B (n x nb, column-major): a coupling joins an aggregate only if it is
+// negative and the two rows of B point the same way (cosine >= 0.9).  Couplings across a seam
+// (a rotation maps u to +-v or u to -u) fail this and stay between aggregates, where the
+// Galerkin product carries them; an aggregate never straddles a change of chart.
+inline bool compatible(double a, const std::vector<double>& B, int nb, int n, int i, int j) {
+    if (!(a < 0.0) || !tune().compat) return tune().compat ? false : true;
+    double bi = 0.0, bj = 0.0, bij = 0.0;
+    for (int c = 0; c < nb; c++) {
+        const double x = B[size_t(c) * n + i], y = B[size_t(c) * n + j];
+        bi += x * x;
+        bj += y * y;
+        bij += x * y;
+    }
+    return bij >= 0.9 * std::sqrt(bi * bj);
+}
+inline bool compatible(const std::complex<double>&, const std::vector<std::complex<double>>&, int, int, int, int) {
+    return true;
+}
+
 // Strength graph and greedy aggregation (seed / attach / leftover passes).
 template <typename T>
 struct Aggregation {
@@ -163,7 +185,8 @@ struct Aggregation {
 };
 
 template <typename T>
-Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double theta) {
+Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double theta,
+                         const std::vector<T>* B = nullptr, int nb = 0) {
     Aggregation<T> G;
     const int n = A.nr;
     std::vector<int64_t>& sptr = G.sptr;
@@ -177,6 +200,7 @@ Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double t
         for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
             const int j = A.col[k];
             if (j == i || lng[i] || lng[j]) continue;
+            if (B && nb > 0 && !compatible(A.val[k], *B, nb, n, i, j)) continue;
             if (mag2(A.val[k]) >= theta * theta * std::abs(d[i]) * std::abs(d[j])) {
                 scol.push_back(j);
                 sval.push_back(A.val[k]);
@@ -275,7 +299,7 @@ template <typename T>
 Csr<T> tentative_qr(const Csr<T>& A, const std::vector<double>& d, double theta, const std::vector<T>& B, int nb,
                     std::vector<T>& Bc, int& nc) {
     const int n = A.nr;
-    const Aggregation<T> G = aggregate(A, d, theta);
+    const Aggregation<T> G = aggregate(A, d, theta, &B, nb);
     std::vector<int> mptr(size_t(G.nagg) + 1, 0), mem(static_cast<size_t>(n));
     for (int i = 0; i < n; i++) mptr[G.agg[i] + 1]++;
     for (int a = 0; a < G.nagg; a++) mptr[a + 1] += mptr[a];
@@ -431,9 +455,20 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
         if (l == 0)
             for (int i = 0; i < A.nr; i++)
                 if (!(d[i] > 0.0)) throw Error(MK_ERR_NOT_SPD, "multigrid: non-positive diagonal (matrix not SPD/HPD)");
-        const double lmax = tune().lmax_fac * rho_dinv_a(A, d);
+        // smoothing diagonal: d_i, raised to half the absolute row sum where the row is not
+        // diagonally dominant, so every Gershgorin disc of Ds^-1 A lies in [0, 2] and a few
+        // strongly coupled rows cannot stretch the Chebyshev interval of all the others
+        std::vector<double> ds(d);
+        if (tune().l1) {
+            for (int i = 0; i < A.nr; i++) {
+                double sa = 0.0;
+                for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) sa += mag(A.val[k]);
+                ds[i] = std::max(d[i], 0.5 * sa);
+            }
+        }
+        const double lmax = tune().lmax_fac * rho_dinv_a(A, ds);
         std::vector<double> di(d.size());
-        for (size_t i = 0; i < d.size(); i++) di[i] = d[i] != 0.0 ? 1.0 / d[i] : 1.0;
+        for (size_t i = 0; i < d.size(); i++) di[i] = ds[i] != 0.0 ? 1.0 / ds[i] : 1.0;
         if (verbose) {
             int64_t mx = 0;
             for (int i = 0; i < A.nr; i++) mx = std::max<int64_t>(mx, A.ptr[i + 1] - A.ptr[i]);
@@ -462,9 +497,13 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             break;
         }
         const double rho = lmax / tune().lmax_fac;
+            // With exact near-null vectors the ordinary rows see their couplings to long rows
+            // (so P B_c = B holds there too); long rows themselves are never smoothed: a smoothed
+            // long row would couple every aggregate along its seam in R A P.
             const bool keep_long = nb > 0 && tune().smooth_long;
             const double omega = 4.0 / (3.0 * rho);
-            // P = Pt - omega D^-1 Af Pt, Af = A without the couplings of long rows/columns
+            // P = Pt - omega D^-1 Af Pt, Af = A without the couplings of long rows (and, without
+            // exact near-null vectors, of long columns)
             std::vector<uint8_t> lng = long_rows(A);
             Csr<T> Af = A;
             {
@@ -473,7 +512,7 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
                 for (int i = 0; i < A.nr; i++) {
                     for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
                         const int j = A.col[k];
-                        if (!keep_long && j != i && (lng[i] || lng[j])) continue;
+                        if (j != i && (lng[i] || (lng[j] && !keep_long))) continue;
                         Af.col[w] = j;
                         Af.val[w] = A.val[k];
                         w++;
@@ -489,8 +528,8 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             tr.reserve(size_t(AP.nonZeros() + Pt.nonZeros()));
             for (int i = 0; i < A.nr; i++) {
                 for (int64_t k = Pt.ptr[i]; k < Pt.ptr[i + 1]; k++) tr.emplace_back(i, Pt.col[k], Pt.val[k]);
-                if (lng[i] && !keep_long) continue;
-                const double di = d[i] != 0.0 ? d[i] : 1.0;
+                if (lng[i]) continue;
+                const double di = ds[i] != 0.0 ? ds[i] : 1.0;
                 for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++) tr.emplace_back(i, AP.col[k], -omega / di * AP.val[k]);
             }
             Csr<T> P(A.nr, nagg);
@@ -523,6 +562,13 @@ Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta
             L.n = H.A[l].nr;
             L.lmax = H.lmax[l];
             L.A = upload(c, H.A[l], o);
+            {  // the smoother's inverse diagonal replaces the plain one
+                const std::vector<double>& dv = H.dinv[l];
+                std::vector<float> dv32(dv.begin(), dv.end());
+                L.A->dinv64.upload(dv.data(), dv.size(), c->stream);
+                L.A->dinv32.upload(dv32.data(), dv32.size(), c->stream);
+                MK_HIP(hipStreamSynchronize(c->stream));
+            }
             g->nnz_total += H.A[l].nonZeros();
             if (l < H.P.size()) {
                 L.P = upload(c, H.P[l], o);
@@ -537,8 +583,8 @@ Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta
         MK_HIP(hipStreamSynchronize(c->stream));
         for (auto& L : g->lv) {
             const size_t len = size_t(L.n) * (g->cplx ? 2 : 1);
-            L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len);
-            L.b32.alloc(len); L.x32.alloc(len); L.r32.alloc(len); L.d32.alloc(len);
+            L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len); L.y64.alloc(len);
+            L.b32.alloc(len); L.x32.alloc(len); L.r32.alloc(len); L.d32.alloc(len); L.y32.alloc(len);
         }
     } catch (...) {
         mg_free(g);
@@ -566,23 +612,6 @@ __global__ void k_cheb_first(int n, const T* __restrict__ dinv, const T* __restr
         d[i] = d0;
         if (zero_start) x[i] = d0;
         else x[i] += d0;
-    }
-}
-template <typename T, bool CPLX>
-__global__ void k_cheb_next(int n, const T* __restrict__ dinv, const T* __restrict__ r, T* __restrict__ d,
-                            T* __restrict__ x, T a, T bcoef, const int* skip) {
-    if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const T s = bcoef * dinv[i];
-    if constexpr (CPLX) {
-        const T d0 = a * d[2 * i] + s * r[2 * i], d1 = a * d[2 * i + 1] + s * r[2 * i + 1];
-        d[2 * i] = d0; d[2 * i + 1] = d1;
-        x[2 * i] += d0; x[2 * i + 1] += d1;
-    } else {
-        const T d0 = a * d[i] + s * r[i];
-        d[i] = d0;
-        x[i] += d0;
     }
 }
 // x = Inv b (dense n x n, row-major), one thread per row
@@ -621,6 +650,7 @@ struct Sel<double> {
     static double* x(MgLevel& L) { return L.x64.p; }
     static double* r(MgLevel& L) { return L.r64.p; }
     static double* d(MgLevel& L) { return L.d64.p; }
+    static double* y(MgLevel& L) { return L.y64.p; }
     static const double* inv(const Mg* g) { return g->inv64.p; }
     static double bytes(const mk_op* A) { return A->bytes64; }
 };
@@ -632,41 +662,51 @@ struct Sel<float> {
     static float* x(MgLevel& L) { return L.x32.p; }
     static float* r(MgLevel& L) { return L.r32.p; }
     static float* d(MgLevel& L) { return L.d32.p; }
+    static float* y(MgLevel& L) { return L.y32.p; }
     static const float* inv(const Mg* g) { return g->inv32.p; }
     static double bytes(const mk_op* A) { return A->bytes32; }
 };
 
+// Chebyshev smoothing of A x = b (degree g->deg, interval [lmax/lmin_div, lmax] of D^-1 A).
+// Each step after the first of a zero start is one fused kernel (residual, direction and
+// update of a row together) that reads one iterate and writes the other (x <-> L.y), so
+// every step is a single launch; the buffers are chosen so the result lands in x.
 template <typename T>
 void chebyshev(mk_ctx* c, const Mg* g, MgLevel& L, const T* b, T* x, bool zero_start, const int* skip) {
     const double lmax = L.lmax, lmin = L.lmax / tune().lmin_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
-    T* r = Sel<T>::r(L);
+    T* y = Sel<T>::y(L);
     T* d = Sel<T>::d(L);
     const int n = L.n;
-    const T* src = b;
-    if (!zero_start) {
-        spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
-        src = r;
-    }
-    if (g->cplx)
-        hipLaunchKernelGGL((k_cheb_first<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), src,
-                           d, x, T(1.0 / theta), zero_start ? 1 : 0, skip);
-    else
-        hipLaunchKernelGGL((k_cheb_first<T, false>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), src,
-                           d, x, T(1.0 / theta), zero_start ? 1 : 0, skip);
-    for (int k = 1; k < g->deg; k++) {
-        const double rho1 = 1.0 / (2.0 * sigma - rho0);
-        spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
-        const T a = T(rho1 * rho0), bc = T(2.0 * rho1 / delta);
+    const int steps = zero_start ? g->deg - 1 : g->deg;  // fused residual steps
+    T* cur = x;
+    if (zero_start) {
+        cur = (steps % 2 == 0) ? x : y;
         if (g->cplx)
-            hipLaunchKernelGGL((k_cheb_next<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), r,
-                               d, x, a, bc, skip);
+            hipLaunchKernelGGL((k_cheb_first<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), b,
+                               d, cur, T(1.0 / theta), 1, skip);
         else
-            hipLaunchKernelGGL((k_cheb_next<T, false>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), r,
-                               d, x, a, bc, skip);
-        rho0 = rho1;
+            hipLaunchKernelGGL((k_cheb_first<T, false>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A),
+                               b, d, cur, T(1.0 / theta), 1, skip);
     }
+    for (int k = 0; k < steps; k++) {
+        T* nxt = (cur == x) ? y : x;
+        double a = 0.0, sc;
+        int first = 0;
+        if (!zero_start && k == 0) {
+            first = 1;
+            sc = 1.0 / theta;
+        } else {
+            const double rho1 = 1.0 / (2.0 * sigma - rho0);
+            a = rho1 * rho0;
+            sc = 2.0 * rho1 / delta;
+            rho0 = rho1;
+        }
+        cheb_launch<T>(c, L.A, cur, nxt, b, d, T(a), T(sc), first, skip);
+        cur = nxt;
+    }
+    if (cur != x) copy_launch<T>(c, size_t(n) * (g->cplx ? 2 : 1), cur, x, skip);
 }
 
 template <typename T>
@@ -821,6 +861,12 @@ inline double redot(const cd& a, const cd& b) { return (std::conj(a) * b).real()
 template <typename T>
 int host_pcg(const Csr<T>& A, const HostHier<T>& H, const T* b, T* x, double tol, int maxit, double* relres) {
     const size_t n = size_t(A.nr);
+    if (maxit == 0) {  // one V-cycle: x = M^-1 b
+        std::vector<T> bv(b, b + n), xv;
+        host_vcycle(H, 0, bv, xv);
+        std::copy(xv.begin(), xv.end(), x);
+        return 0;
+    }
     std::vector<T> r(n), z, p, q(n), bb(b, b + n);
     A.multiply(x, q.data());
     double bn = 0.0;

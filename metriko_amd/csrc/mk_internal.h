@@ -6,9 +6,24 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <cstdio>
+#include <cstdlib>
 #include "metriko_hip.h"
 
 namespace mk {
+
+// Progress lines on stderr when METRIKO_VERBOSE is set (long runs; no effect otherwise).
+inline bool verbose() {
+    static const bool v = std::getenv("METRIKO_VERBOSE") != nullptr;
+    return v;
+}
+#define MK_LOG(...)                              \
+    do {                                         \
+        if (mk::verbose()) {                     \
+            std::fprintf(stderr, __VA_ARGS__);   \
+            std::fflush(stderr);                 \
+        }                                        \
+    } while (0)
 
 struct Mg;
 
@@ -152,6 +167,13 @@ struct mk_op {
     mk::DevBuf<float> val32;
     mk::DevBuf<int> perm;       // SELL row -> original row (empty if identity)
     int n_short = 0;            // rows in the SELL part
+    int nmid = 0;               // rows in the group-of-lanes-per-row part (medium length)
+    int mid_lanes = 8;          // lanes per medium row
+    mk::DevBuf<int> mid_rows;
+    mk::DevBuf<int64_t> mid_ptr;
+    mk::DevBuf<int> mid_col;
+    mk::DevBuf<double> mid_val64;
+    mk::DevBuf<float> mid_val32;
     int nlong = 0;              // rows in the wavefront-per-row part
     mk::DevBuf<int> long_rows;
     mk::DevBuf<int64_t> long_ptr;
@@ -178,6 +200,12 @@ namespace mk {
 // PLAIN: y = A x; DOT: y = A x and x^H y; RESID: y = b - A x and |y|^2;
 // RESID_NORED: y = b - A x; ACCUM: y += A x
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2, SPMV_RESID_NORED = 3, SPMV_ACCUM = 4 };
+// fused Chebyshev step: r = b - A x_in ; d = a d + s D^-1 r (first: d = s D^-1 r) ; x_out = x_in + d
+template <typename T>
+void cheb_launch(mk_ctx* c, const mk_op* A, const T* x_in, T* x_out, const T* b, T* d, T a, T s, int first,
+                 const int* skip);
+template <typename T>
+void copy_launch(mk_ctx* c, size_t len, const T* x, T* y, const int* skip);
 template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip);
 // A X = B for mw right-hand sides (B, X: n x mw row-major, device); X starts from zero.

@@ -177,6 +177,7 @@ mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int3
         rawf.compressed = solveSmallestEig(L, mass, u, n_iter);
         info->field = rt.last_field;
         info->ms_field_solve = now_ms() - t;
+        MK_LOG("[pipeline] field solve %.0f ms\n", info->ms_field_solve);
         rawf.field.resize(nF, N);
         for (int f = 0; f < nF; f++) {
             const double th = std::arg(rawf.compressed[f]);
@@ -188,6 +189,7 @@ mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int3
         info->ms_matching = H.ms_matching;
         info->ms_seam = H.ms_seam;
         info->ms_cut = H.ms_cut;
+        MK_LOG("[pipeline] matching/seam/cut %.0f/%.0f/%.0f ms\n", H.ms_matching, H.ms_seam, H.ms_cut);
         t = now_ms();
         RosyParameterization rp(mesh, *H.cut, H.ext, rawf.singular, H.combed->matching, H.seam, N, ratio);
         rp.seamless = true;
@@ -196,6 +198,7 @@ mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int3
         rp.verbose = false;
         rp.setup();
         info->ms_setup = now_ms() - t;
+        MK_LOG("[pipeline] parameterization setup %.0f ms\n", info->ms_setup);
         t = now_ms();
         const bool ok = rp.integ();
         info->ms_integ = now_ms() - t;

@@ -39,20 +39,46 @@ __device__ __forceinline__ void block_sum(double (&v)[K]) {
     __syncthreads();
 }
 
-// Writes the block partials and returns true in the last block to arrive, with the
-// grand totals in tot (valid in all threads of that block).
+// Deterministic grid reduction with a two-level arrival ticket.  Each block writes its
+// partial sums; blocks arrive on the counter of their group of kGroup blocks, the last
+// block of a group sums the group's partials (fixed order) into a group slot and arrives on
+// the global counter, and the last group to finish sums the group slots (fixed order).  A
+// single global ticket would serialise every block of a large grid on one atomic address.
+// Returns true in the last block, with the grand totals in tot (all threads of that block).
+// Layout: partials[k*kMaxParts + block], group slots at partials[kMaxK*kMaxParts + k*kMaxGroups
+// + group]; counter[0] global, counter[1 + group] per group; counters are left at zero.
+static constexpr int kGroup = 32;
+static constexpr size_t kMaxGroups = kMaxParts / kGroup;
 template <int K>
 __device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
     __shared__ int is_last;
     __shared__ double shtot[K];
     block_sum<K>(v);
     const unsigned G = gridDim.x;
+    const unsigned grp = blockIdx.x / kGroup;
+    const unsigned ngrp = (G + kGroup - 1) / kGroup;
+    const unsigned gsize = (grp + 1 == ngrp) ? G - grp * kGroup : kGroup;
+    double* gslot = partials + size_t(kMaxK) * kMaxParts;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; k++) partials[k * kMaxParts + blockIdx.x] = v[k];
         __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        is_last = (t == G - 1);
+        const unsigned t = atomicAdd(counter + 1 + grp, 1u);
+        is_last = 0;
+        if (t == gsize - 1) {  // last of its group: group sum, then the global ticket
+            __threadfence();
+            counter[1 + grp] = 0u;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const volatile double* pk = partials + k * kMaxParts + grp * kGroup;
+                double s = 0.0;
+                for (unsigned i = 0; i < gsize; i++) s += pk[i];
+                gslot[k * kMaxGroups + grp] = s;
+            }
+            __threadfence();
+            const unsigned t2 = atomicAdd(counter, 1u);
+            is_last = (t2 == ngrp - 1) ? 1 : 0;
+        }
     }
     __syncthreads();
     if (!is_last) return false;
@@ -61,8 +87,8 @@ __device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, 
 #pragma unroll
     for (int k = 0; k < K; k++) {
         acc[k] = 0.0;
-        const volatile double* pk = partials + k * kMaxParts;
-        for (unsigned i = threadIdx.x; i < G; i += blockDim.x) acc[k] += pk[i];
+        const volatile double* pk = gslot + k * kMaxGroups;
+        for (unsigned i = threadIdx.x; i < ngrp; i += blockDim.x) acc[k] += pk[i];
     }
     block_sum<K>(acc);
     if (threadIdx.x == 0) {
@@ -76,17 +102,15 @@ __device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, 
     return true;
 }
 
-// ---------------------------------------------------------------------------------------
-// SELL SpMV.  T = float/double; CPLX: values/vectors interleaved (re, im).
-// MODE 0: y = A x; 1: y = A x, out = Re(x^H y); 2: y = b - A x, out = |y|^2;
-// 3: y = b - A x; 4: y += A x.  skip (optional): device flag, nonzero = do nothing.
-//
-// Hybrid layout: the first nb_sell blocks run the SELL part (short rows, thread per row);
-// the remaining blocks give each long row a whole wavefront (lanes stride the row, then a
-// shuffle reduction), so a few very long rows never serialise one lane.
 template <typename T>
 struct SellDev {
     int n_short, C, nb_sell, nlong;
+    int nvb;  // virtual blocks: nb_sell SELL blocks, nb_mid medium-row blocks, then long-row blocks
+    int nb_mid, nmid, mid_lanes;
+    const int* mid_rows;
+    const int64_t* mid_ptr;
+    const int* mid_col;
+    const T* mid_val;
     const int64_t* slice_ptr;
     const int* slice_w;
     const int* col;
@@ -105,13 +129,15 @@ __device__ __forceinline__ T wave_sum_t(T v) {
     return v;
 }
 
-// Computes one row of A x.  Returns true in the thread that owns the result (row, ar, ai).
+// Computes one row of A x for virtual block vb (uniform in the block).  Returns true in the
+// thread that owns the result (row, ar, ai).
 template <typename T, bool CPLX>
-__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int& row, T& ar, T& ai) {
+__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int vb, int& row, T& ar,
+                                         T& ai) {
     ar = 0;
     ai = 0;
-    if (int(blockIdx.x) < A.nb_sell) {
-        const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (vb < A.nb_sell) {
+        const int r = vb * blockDim.x + threadIdx.x;
         if (r >= A.n_short) return false;
         const int s = r / A.C, lane = r - s * A.C;
         const int64_t base = A.slice_ptr[s] + lane;
@@ -131,7 +157,33 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
         }
         return true;
     }
-    const int w = (int(blockIdx.x) - A.nb_sell) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+    if (vb < A.nb_sell + A.nb_mid) {  // medium rows: a group of mid_lanes lanes per row
+        const int L = A.mid_lanes;
+        const int t = (vb - A.nb_sell) * int(blockDim.x) + int(threadIdx.x);
+        const int m = t / L, sub = t - m * L;
+        if (m < A.nmid) {
+            for (int64_t k = A.mid_ptr[m] + sub; k < A.mid_ptr[m + 1]; k += L) {
+                const int c = A.mid_col[k];
+                if constexpr (CPLX) {
+                    const T vr = A.mid_val[2 * k], vi = A.mid_val[2 * k + 1];
+                    const T xr = x[2 * c], xi = x[2 * c + 1];
+                    ar += vr * xr - vi * xi;
+                    ai += vr * xi + vi * xr;
+                } else {
+                    ar += A.mid_val[k] * x[c];
+                }
+            }
+        }
+        // reduce inside the group (every lane takes part; L divides 64)
+        for (int off = L >> 1; off > 0; off >>= 1) {
+            ar += __shfl_xor(ar, off, 64);
+            if constexpr (CPLX) ai += __shfl_xor(ai, off, 64);
+        }
+        if (m >= A.nmid || sub != 0) return false;
+        row = A.mid_rows[m];
+        return true;
+    }
+    const int w = (vb - A.nb_sell - A.nb_mid) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w < A.nlong) {  // uniform inside the wavefront
         for (int64_t k = A.long_ptr[w] + lane; k < A.long_ptr[w + 1]; k += 64) {
@@ -155,6 +207,10 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
 
 
 static inline unsigned blocks_for(size_t len, int bs = 256) { return unsigned((len + bs - 1) / bs); }
+// grid of the grid-stride reduction kernels: few enough blocks that the per-block arrival
+// (partial store, device fence, ticket) stays a small part of the kernel
+static constexpr unsigned kRedGrid = 512;
+static inline unsigned rgrid(size_t len) { return std::max(1u, std::min(blocks_for(len), kRedGrid)); }
 
 hipEvent_t next_event(mk_ctx* c);
 void timing_flush(mk_ctx* c);
@@ -205,8 +261,17 @@ static SellDev<T> sell_dev(const mk_op* A, const T* val, int block, unsigned* gr
     d.long_col = A->long_col.p;
     if constexpr (std::is_same<T, double>::value) d.long_val = A->long_val64.p;
     else d.long_val = A->long_val32.p;
+    d.nmid = A->nmid;
+    d.mid_lanes = A->mid_lanes;
+    d.nb_mid = int(blocks_for(size_t(A->nmid) * size_t(A->mid_lanes), block));
+    d.mid_rows = A->mid_rows.p;
+    d.mid_ptr = A->mid_ptr.p;
+    d.mid_col = A->mid_col.p;
+    if constexpr (std::is_same<T, double>::value) d.mid_val = A->mid_val64.p;
+    else d.mid_val = A->mid_val32.p;
     const unsigned nb_long = blocks_for(size_t(A->nlong), block / 64);
-    *grid = std::max(1u, unsigned(d.nb_sell) + nb_long);
+    *grid = std::max(1u, unsigned(d.nb_sell) + unsigned(d.nb_mid) + nb_long);
+    d.nvb = int(*grid);
     return d;
 }
 
