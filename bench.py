@@ -1,17 +1,23 @@
-"""Benchmark: field + IGM solve on the 1.31M-face bumpy icosphere, FP32+FP64-refinement vs FP64.
+"""Benchmark: field + IGM solve on the 1.31M-face bumpy icosphere, FP32 multigrid inside
+FP64 PCG vs plain FP64, both to a 1e-10 relative residual.
 
 Workload (one "step"): icosphere level 8 (655,362 v / 1,310,720 f), radial displacement
-0.05 sin(6 theta) sin(5 phi) plus N(0, 1e-3) radial jitter (seed 5); the example_igm
-pipeline end to end: Smoothest 4-RoSy field (50 inverse-iteration solves), Curl matching,
-seam, cut, combing, seamless IGM with iterative rounding (ratio 0.03).
+0.05 sin(6 theta) sin(5 phi) plus N(0, sigma) radial jitter (seed 5) with sigma = 1e-3 * 2^(4-8)
+= 6.25e-5: the level-4 jitter of 1e-3 scaled with the edge length, so every level carries the
+same relative noise (at the unscaled 1e-3, a quarter of an L8 edge, the surface folds and the
+smoothest field has 17,229 singularities; see --as-specified).  The example_igm pipeline end to
+end: Smoothest 4-RoSy field (50 inverse-iteration solves), Curl matching, seam, cut, combing,
+seamless IGM with iterative rounding (ratio 0.03).
 
-value = field + IGM solve ms per step (median over the timed steps), FP32 inner solves with
-FP64 refinement to a 1e-10 FP64 relative residual.  "Solve" = everything in
+value = field + IGM solve ms per step (median of the timed steps): everything in
 mk_field_smallest_eig (multigrid setup + 50 PCG solves + normalisations) plus everything in
 mk_iterative_rounding (kernel basis, operator assembly, multigrid setup, KKT, LM passes,
-rounding loop); the host-side mesh/seam/cut combinatorics are reported separately in
-ms_per_step (whole pipeline).  Inputs enter through host memory (the C ABI takes host
-arrays), so the timed region includes their transfer.
+rounding loop).  ms_per_step is the whole pipeline (host mesh/seam/cut combinatorics too).
+Inputs enter through host memory (the C ABI takes host arrays).
+
+roofline: the kernel with the largest total time in one extra, event-timed step; achieved =
+its algorithmic bytes per launch / its average launch time (hipEvents on the library stream);
+traffic = HBM bytes per launch from the committed rocprofv3 --pmc summary when present.
 
 Prints ONE JSON line.  Only --gpus 1 is supported (one process, one GPU).
 """
@@ -28,11 +34,16 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+LEVEL = 8
 
 
-def workload():
+def jitter_for(level):
+    return 1e-3 * 2.0 ** (4 - level)
+
+
+def workload(level=LEVEL, jitter=None):
     from metriko_amd.meshgen import bumpy_icosphere
-    V, F = bumpy_icosphere(8, amp=0.05, jitter=1e-3, seed=5)
+    V, F = bumpy_icosphere(level, amp=0.05, jitter=jitter_for(level) if jitter is None else jitter, seed=5)
     g = np.random.Generator(np.random.PCG64(5))
     u0 = g.uniform(-1, 1, F.shape[0]) + 1j * g.uniform(-1, 1, F.shape[0])
     return V, F, u0
@@ -43,15 +54,12 @@ def solve_ms(info):
 
 
 def cpu_baseline():
-    """The CPU oracle (numpy/scipy SuperLU, one core) on a bounded sample: the same
-    pipeline on the level-6 bumpy icosphere (81,920 faces), field + IGM solve stages timed,
-    scaled to 1,310,720 faces with the n^1.5 cost of sparse direct factorisation on a
+    """The CPU oracle (numpy/scipy SuperLU, one core) on a bounded sample: the same pipeline on
+    the level-6 bumpy icosphere (81,920 faces, same relative jitter), field + IGM solve stages
+    timed, scaled to 1,310,720 faces with the n^1.5 cost of sparse direct factorisation on a
     2-manifold mesh."""
-    from metriko_amd.meshgen import bumpy_icosphere
     from oracle.pipeline import run_pipeline
-    V, F = bumpy_icosphere(6, amp=0.05, jitter=1e-3, seed=5)
-    g = np.random.Generator(np.random.PCG64(5))
-    u0 = g.uniform(-1, 1, F.shape[0]) + 1j * g.uniform(-1, 1, F.shape[0])
+    V, F, u0 = workload(6)
     t0 = time.perf_counter()
     r = run_pipeline(V, F, ratio=0.03, u0=u0)
     wall = time.perf_counter() - t0
@@ -65,25 +73,22 @@ def cpu_baseline():
     }
 
 
-def traffic_for(kernel_symbol):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def traffic_for(key):
+    """HBM bytes per launch of the kernel behind `key`, from profiles/traffic.json (written from
+    the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), if present."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
         return None
     try:
-        d = json.load(open(p))
-        return d.get(kernel_symbol)
-    except Exception:
+        return json.load(open(p)).get(key)
+    except (OSError, ValueError):
         return None
 
 
-# kernel symbol each timing key maps to (rocprofv3 names)
-SYMBOL = {
-    ("cg_spmv", True, "fp32"): "cg_spmv<float, true>",
-    ("cg_spmv", False, "fp32"): "cg_spmv<float, false>",
-    ("cg_spmv", True, "fp64"): "cg_spmv<double, true>",
-    ("cg_spmv", False, "fp64"): "cg_spmv<double, false>",
-}
+def run(ctx, V, F, u0, o):
+    t0 = time.perf_counter()
+    out = ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=o, igm_opts=o)
+    return out, 1e3 * (time.perf_counter() - t0)
 
 
 def main():
@@ -93,63 +98,75 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--level", type=int, default=LEVEL)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy a 1 GiB buffer three times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: only --gpus 1 is supported (one process drives one MI355X); got --gpus {args.gpus}. "
-              f"The batched and multi-mesh paths shard by mesh with no communication; see DESIGN.md.",
-              file=sys.stderr)
+              f"The path shards by mesh with no communication; see DESIGN.md.", file=sys.stderr)
         sys.exit(2)
     import metriko_amd as mk
-    V, F, u0 = workload()
-    mixed = mk.opts(precision=mk.PREC_MIXED)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    V, F, u0 = workload(args.level)
+    mixed = mk.opts(precision=mk.PREC_MIXED_MG)
     fp64 = mk.opts(precision=mk.PREC_FP64)
     ctx = mk.Context(0)
-    for _ in range(args.warmup):
-        ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=mixed, igm_opts=mixed)
-        print(f"warmup done", file=sys.stderr, flush=True)
-    ctx.set_timing(True)
-    ctx.timing_report(reset=True)
-    walls, solves, infos, outs = [], [], [], []
+    for w in range(args.warmup):
+        run(ctx, V, F, u0, mixed)
+        print(f"warmup {w} done", file=sys.stderr, flush=True)
+    walls, solves, outs = [], [], []
     for s in range(args.steps):
-        t0 = time.perf_counter()
-        out = ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=mixed, igm_opts=mixed)
-        walls.append(1e3 * (time.perf_counter() - t0))
+        out, wall = run(ctx, V, F, u0, mixed)
+        walls.append(wall)
         solves.append(solve_ms(out["info"]))
-        infos.append(out["info"])
         outs.append(out)
-        print(f"step {s}: wall {walls[-1]:.0f} ms solve {solves[-1]:.0f} ms", file=sys.stderr, flush=True)
-    rep = ctx.timing_report(reset=True)
-    ctx.set_timing(False)
-    # dominant SpMV kernel by total event time
-    dom = max(rep, key=lambda r: r["ms"])
-    op, kern, prec = dom["key"].split(":")
-    avg_ms = dom["ms"] / max(dom["calls"], 1)
-    achieved = dom["bytes_per_call"] / (avg_ms * 1e-3) / 1e9
-    cplx = op.startswith("field")
-    sym = SYMBOL.get((kern, cplx, prec), kern)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(sym),
-                "kernel": sym, "key": dom["key"], "avg_us": round(avg_ms * 1e3, 2),
-                "bytes_per_launch": dom["bytes_per_call"], "launches": dom["calls"]}
-    info = infos[-1]
+        print(f"step {s}: wall {wall:.0f} ms solve {solves[-1]:.0f} ms", file=sys.stderr, flush=True)
+    info = outs[-1]["info"]
     result = {
         "metric": "field+IGM solve ms & SpMV HBM GB/s @1.31M faces, FP32+refine vs FP64 @1e-10",
         "value": round(statistics.median(solves), 1), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(statistics.median(walls), 1), "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64-refine", "data": "synthetic",
-        "config": {"workload": "bumpy icosphere L8 (655,362 v / 1,310,720 f), field + seamless IGM, ratio 0.03",
-                   "nS": info["nS"], "nT": info["nT"], "k": info["k"],
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64 (fp32 multigrid)", "data": "synthetic",
+        "config": {"workload": f"bumpy icosphere L{args.level} ({V.shape[0]:,} v / {F.shape[0]:,} f, jitter "
+                               f"{jitter_for(args.level):g}), field + seamless IGM, ratio 0.03",
+                   "precision": "MK_PREC_MIXED_MG: FP64 flexible PCG, FP32 multigrid V-cycle, tol 1e-10",
+                   "nS": info["nS"], "nT": info["nT"], "k": info["k"], "success": info["success"],
                    "field_ms": round(info["field"]["ms"] + info["field"]["setup_ms"], 1),
                    "field_iters": info["field"]["iterations"],
                    "igm_ms": round(info["igm"]["solve"]["ms"], 1), "igm_solves": info["igm"]["solve"]["solves"],
                    "igm_iters": info["igm"]["solve"]["iterations"],
-                   "rel_residual": info["igm"]["solve"]["rel_residual"]},
-        "roofline": roofline,
+                   "host_stages_ms": round(info["ms_hmesh"] + info["ms_matching"] + info["ms_seam"] +
+                                           info["ms_cut"] + info["ms_setup"], 1)},
     }
+    if not args.no_roofline:
+        ctx.set_timing(True)
+        ctx.timing_report(reset=True)
+        run(ctx, V, F, u0, mixed)
+        rep = ctx.timing_report(reset=True)
+        ctx.set_timing(False)
+        dom = max(rep, key=lambda r: r["ms"])
+        avg_ms = dom["ms"] / max(dom["calls"], 1)
+        achieved = dom["bytes_per_call"] / (avg_ms * 1e-3) / 1e9
+        result["roofline"] = {
+            "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(dom["key"]),
+            "kernel": dom["key"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_call"],
+            "launches": dom["calls"], "rows": dom["rows"]}
+        top = sorted(rep, key=lambda r: -r["ms"])[:6]
+        result["top_kernels"] = [{"key": r["key"], "ms": round(r["ms"], 1), "calls": r["calls"],
+                                  "GBps": round(r["bytes_per_call"] / (r["ms"] / max(r["calls"], 1) * 1e-3) / 1e9, 1)}
+                                 for r in top]
     if not args.no_fp64:
-        t0 = time.perf_counter()
-        o64 = ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=fp64, igm_opts=fp64)
-        w64 = 1e3 * (time.perf_counter() - t0)
+        o64, w64 = run(ctx, V, F, u0, fp64)
         a, b = outs[-1], o64
         c = np.vdot(b["compressed"], a["compressed"])
         c = c / abs(c)

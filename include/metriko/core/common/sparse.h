@@ -8,6 +8,10 @@
 #define METRIKO_SPARSE_H
 #include <algorithm>
 #include <numeric>
+#include <stdexcept>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "typedef.h"
 
 namespace metriko {
@@ -35,7 +39,7 @@ namespace metriko {
         int64_t nonZeros() const { return ptr.empty() ? 0 : ptr.back(); }
         void resize(int r, int c) { nr = r; nc = c; ptr.assign(size_t(r) + 1, 0); col.clear(); val.clear(); }
 
-        // Eigen setFromTriplets semantics: duplicates are summed.
+        // Eigen setFromTriplets semantics: duplicates are summed (in input order within a row).
         void setFromTriplets(const std::vector<Triplet<T>>& T_) {
             std::vector<int64_t> cnt(size_t(nr) + 1, 0);
             for (auto& t : T_) cnt[size_t(t.r) + 1]++;
@@ -44,21 +48,44 @@ namespace metriko {
             std::vector<T> v(T_.size());
             std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
             for (auto& t : T_) { int64_t p = pos[t.r]++; c[p] = t.c; v[p] = t.v; }
-            ptr.assign(size_t(nr) + 1, 0);
-            col.clear(); val.clear();
-            col.reserve(T_.size()); val.reserve(T_.size());
-            std::vector<int> idx;
-            for (int i = 0; i < nr; i++) {
-                idx.resize(size_t(cnt[i + 1] - cnt[i]));
-                std::iota(idx.begin(), idx.end(), 0);
-                const int64_t b = cnt[i];
-                std::stable_sort(idx.begin(), idx.end(), [&](int a, int bb) { return c[b + a] < c[b + bb]; });
-                int last = -1;
-                for (int k : idx) {
-                    if (c[b + k] == last) val.back() += v[b + k];
-                    else { col.push_back(c[b + k]); val.push_back(v[b + k]); last = c[b + k]; }
+            // rows are independent: sort + merge each (two passes: sizes, then values)
+            std::vector<int64_t> uq(size_t(nr) + 1, 0);
+#pragma omp parallel if (T_.size() > 100000)
+            {
+                std::vector<int> idx, cs;
+                std::vector<T> vs;
+#pragma omp for schedule(dynamic, 512)
+                for (int i = 0; i < nr; i++) {
+                    const int64_t b = cnt[i], e = cnt[i + 1];
+                    if (e - b <= 1) { uq[i + 1] = e - b; continue; }
+                    idx.resize(size_t(e - b));
+                    std::iota(idx.begin(), idx.end(), 0);
+                    std::stable_sort(idx.begin(), idx.end(), [&](int a, int bb) { return c[b + a] < c[b + bb]; });
+                    // reorder the row in place (sorted, stable) and count distinct columns
+                    cs.resize(idx.size());
+                    vs.resize(idx.size());
+                    for (size_t k = 0; k < idx.size(); k++) { cs[k] = c[b + idx[k]]; vs[k] = v[b + idx[k]]; }
+                    int64_t u = 0;
+                    for (size_t k = 0; k < cs.size(); k++) {
+                        c[b + int64_t(k)] = cs[k];
+                        v[b + int64_t(k)] = vs[k];
+                        if (k == 0 || cs[k] != cs[k - 1]) u++;
+                    }
+                    uq[i + 1] = u;
                 }
-                ptr[i + 1] = int64_t(col.size());
+            }
+            for (int i = 0; i < nr; i++) uq[i + 1] += uq[i];
+            ptr = uq;
+            col.assign(size_t(uq[nr]), 0);
+            val.assign(size_t(uq[nr]), T());
+#pragma omp parallel for schedule(dynamic, 512) if (T_.size() > 100000)
+            for (int i = 0; i < nr; i++) {
+                int64_t w = ptr[i] - 1;
+                int last = -1;
+                for (int64_t k = cnt[i]; k < cnt[i + 1]; k++) {
+                    if (c[k] == last) val[w] += v[k];
+                    else { ++w; col[w] = c[k]; val[w] = v[k]; last = c[k]; }
+                }
             }
         }
 
@@ -79,23 +106,46 @@ namespace metriko {
 
         Csr transpose() const {
             Csr t(nc, nr);
-            std::vector<int64_t> cnt(size_t(nc) + 1, 0);
-            for (int64_t k = 0; k < nonZeros(); k++) cnt[size_t(col[k]) + 1]++;
-            for (int i = 0; i < nc; i++) cnt[i + 1] += cnt[i];
-            t.ptr = cnt;
-            t.col.resize(col.size());
-            t.val.resize(val.size());
-            std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-            for (int i = 0; i < nr; i++)
-                for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) {
-                    int64_t p = pos[col[k]]++;
-                    t.col[p] = i;
-                    t.val[p] = val[k];
-                }
+            const int64_t nz = nonZeros();
+            t.col.resize(size_t(nz));
+            t.val.resize(size_t(nz));
+            // rows split into chunks: per-chunk column counts give every entry its slot, so the
+            // result (rows ascending inside each column) does not depend on the chunking
+            int nch = 1;
+#ifdef _OPENMP
+            nch = std::max(1, std::min(omp_get_max_threads(), int(nz / 200000) + 1));
+#endif
+            std::vector<std::vector<int64_t>> cc(static_cast<size_t>(nch));
+            const int step = (nr + nch - 1) / std::max(nch, 1);
+#pragma omp parallel for schedule(static, 1)
+            for (int q = 0; q < nch; q++) {
+                cc[size_t(q)].assign(size_t(nc), 0);
+                const int r1 = std::min(nr, (q + 1) * step);
+                for (int i = q * step; i < r1; i++)
+                    for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) cc[size_t(q)][size_t(col[k])]++;
+            }
+            t.ptr.assign(size_t(nc) + 1, 0);
+            for (int j = 0; j < nc; j++) {
+                int64_t s = t.ptr[j];
+                for (int q = 0; q < nch; q++) { const int64_t x = cc[size_t(q)][size_t(j)]; cc[size_t(q)][size_t(j)] = s; s += x; }
+                t.ptr[j + 1] = s;
+            }
+#pragma omp parallel for schedule(static, 1)
+            for (int q = 0; q < nch; q++) {
+                std::vector<int64_t>& pos = cc[size_t(q)];
+                const int r1 = std::min(nr, (q + 1) * step);
+                for (int i = q * step; i < r1; i++)
+                    for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) {
+                        const int64_t p = pos[size_t(col[k])]++;
+                        t.col[p] = i;
+                        t.val[p] = val[k];
+                    }
+            }
             return t;
         }
 
         void multiply(const T* x, T* y) const {
+#pragma omp parallel for schedule(static, 4096) if (nonZeros() > 200000)
             for (int i = 0; i < nr; i++) {
                 T s = T();
                 for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) s += val[k] * x[col[k]];
@@ -124,28 +174,54 @@ namespace metriko {
         }
     };
 
-    // C = A * B (Gustavson with a dense accumulator; sorted output columns)
+    // C = A * B (Gustavson with a dense accumulator; sorted output columns).  Rows are
+    // independent: a symbolic pass sizes them, a numeric pass fills them (same arithmetic
+    // order per row as a sequential product).
     template<typename T>
     Csr<T> spgemm(const Csr<T>& A, const Csr<T>& B) {
         if (A.nc != B.nr) throw std::invalid_argument("spgemm: shape mismatch");
         Csr<T> C(A.nr, B.nc);
-        std::vector<T> acc(size_t(B.nc), T());
-        std::vector<int> mark(size_t(B.nc), -1);
-        std::vector<int> cols;
-        for (int i = 0; i < A.nr; i++) {
-            cols.clear();
-            for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
-                const int j = A.col[ka];
-                const T a = A.val[ka];
-                for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) {
-                    const int c = B.col[kb];
-                    if (mark[c] != i) { mark[c] = i; acc[c] = a * B.val[kb]; cols.push_back(c); }
-                    else acc[c] += a * B.val[kb];
+        const bool par = A.nonZeros() > 100000;
+        std::vector<int64_t> cnt(size_t(A.nr) + 1, 0);
+#pragma omp parallel if (par)
+        {
+            std::vector<int> mark(size_t(B.nc), -1);
+#pragma omp for schedule(dynamic, 1024)
+            for (int i = 0; i < A.nr; i++) {
+                int64_t n = 0;
+                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+                    const int j = A.col[ka];
+                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
+                        if (mark[B.col[kb]] != i) { mark[B.col[kb]] = i; n++; }
                 }
+                cnt[size_t(i) + 1] = n;
             }
-            std::sort(cols.begin(), cols.end());
-            for (int c : cols) { C.col.push_back(c); C.val.push_back(acc[c]); }
-            C.ptr[i + 1] = int64_t(C.col.size());
+        }
+        for (int i = 0; i < A.nr; i++) cnt[size_t(i) + 1] += cnt[size_t(i)];
+        C.ptr = cnt;
+        C.col.resize(size_t(cnt[size_t(A.nr)]));
+        C.val.resize(size_t(cnt[size_t(A.nr)]));
+#pragma omp parallel if (par)
+        {
+            std::vector<T> acc(size_t(B.nc), T());
+            std::vector<int> mark(size_t(B.nc), -1);
+            std::vector<int> cols;
+#pragma omp for schedule(dynamic, 1024)
+            for (int i = 0; i < A.nr; i++) {
+                cols.clear();
+                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+                    const int j = A.col[ka];
+                    const T a = A.val[ka];
+                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) {
+                        const int c = B.col[kb];
+                        if (mark[c] != i) { mark[c] = i; acc[c] = a * B.val[kb]; cols.push_back(c); }
+                        else acc[c] += a * B.val[kb];
+                    }
+                }
+                std::sort(cols.begin(), cols.end());
+                int64_t w = C.ptr[size_t(i)];
+                for (int c : cols) { C.col[size_t(w)] = c; C.val[size_t(w)] = acc[c]; w++; }
+            }
         }
         return C;
     }
@@ -153,16 +229,48 @@ namespace metriko {
     template<typename T>
     Csr<T> operator*(const Csr<T>& A, const Csr<T>& B) { return spgemm(A, B); }
 
+    // C = a A + b B by merging the sorted rows (entries present in either operand are kept,
+    // zeros included, like a triplet assembly of both).  Rows are merged in parallel.
     template<typename T>
-    Csr<T> operator+(const Csr<T>& A, const Csr<T>& B) {
-        std::vector<Triplet<T>> t;
-        t.reserve(size_t(A.nonZeros() + B.nonZeros()));
-        for (int i = 0; i < A.nr; i++) for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) t.emplace_back(i, A.col[k], A.val[k]);
-        for (int i = 0; i < B.nr; i++) for (int64_t k = B.ptr[i]; k < B.ptr[i + 1]; k++) t.emplace_back(i, B.col[k], B.val[k]);
+    Csr<T> add(const Csr<T>& A, T a, const Csr<T>& B, T b) {
+        if (A.nr != B.nr || A.nc != B.nc) throw std::invalid_argument("sparse add: shape mismatch");
         Csr<T> C(A.nr, A.nc);
-        C.setFromTriplets(t);
+        const bool par = A.nonZeros() + B.nonZeros() > 200000;
+        std::vector<int64_t> cnt(size_t(A.nr) + 1, 0);
+#pragma omp parallel for schedule(static, 4096) if (par)
+        for (int i = 0; i < A.nr; i++) {
+            int64_t p = A.ptr[i], q = B.ptr[i], n = 0;
+            while (p < A.ptr[i + 1] || q < B.ptr[i + 1]) {
+                if (q >= B.ptr[i + 1] || (p < A.ptr[i + 1] && A.col[p] < B.col[q])) p++;
+                else if (p >= A.ptr[i + 1] || B.col[q] < A.col[p]) q++;
+                else { p++; q++; }
+                n++;
+            }
+            cnt[size_t(i) + 1] = n;
+        }
+        for (int i = 0; i < A.nr; i++) cnt[size_t(i) + 1] += cnt[size_t(i)];
+        C.ptr = cnt;
+        C.col.resize(size_t(cnt[size_t(A.nr)]));
+        C.val.resize(size_t(cnt[size_t(A.nr)]));
+#pragma omp parallel for schedule(static, 4096) if (par)
+        for (int i = 0; i < A.nr; i++) {
+            int64_t p = A.ptr[i], q = B.ptr[i], w = C.ptr[i];
+            while (p < A.ptr[i + 1] || q < B.ptr[i + 1]) {
+                if (q >= B.ptr[i + 1] || (p < A.ptr[i + 1] && A.col[p] < B.col[q])) {
+                    C.col[w] = A.col[p]; C.val[w] = a * A.val[p]; p++;
+                } else if (p >= A.ptr[i + 1] || B.col[q] < A.col[p]) {
+                    C.col[w] = B.col[q]; C.val[w] = b * B.val[q]; q++;
+                } else {
+                    C.col[w] = A.col[p]; C.val[w] = a * A.val[p] + b * B.val[q]; p++; q++;
+                }
+                w++;
+            }
+        }
         return C;
     }
+
+    template<typename T>
+    Csr<T> operator+(const Csr<T>& A, const Csr<T>& B) { return add(A, T(1), B, T(1)); }
 
     using SprsD = Csr<double>;
     using SprsC = Csr<complex>;

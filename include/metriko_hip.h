@@ -74,8 +74,12 @@ void mk_default_opts(mk_solve_opts* o);
 mk_status mk_init(mk_ctx** ctx, int device);
 mk_status mk_finalize(mk_ctx* ctx);
 mk_status mk_last_error(mk_ctx* ctx, char* buf, int len);
-/* Enable per-launch hipEvent timing of SpMV kernels (reported in mk_stats.spmv_ms). */
+/* Enable per-launch hipEvent timing of SpMV kernels (reported in mk_stats.spmv_ms).  While
+ * timing is on, the PCG iterations run as individual launches (no graph replay). */
 mk_status mk_set_timing(mk_ctx* ctx, int on);
+/* Multigrid PCG iterations are captured once per operator and buffer set as a hipGraph of
+ * check_every iterations and replayed (default on; off = individual launches, same results). */
+mk_status mk_set_graphs(mk_ctx* ctx, int on);
 /* JSON array of per-kernel SpMV timings recorded since the last reset: one object per
  * "<operator>:<kernel>:<fp32|fp64>" key with calls, ms (hipEvent), algorithmic bytes,
  * bytes per call and rows.  reset != 0 clears the table afterwards. */

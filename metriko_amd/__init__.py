@@ -19,7 +19,7 @@ FMT_CSR, FMT_SELL = 0, 1
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_AMG = 0, 1, 2
 
 EXPORTED = [
-    "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing", "mk_timing_report",
+    "mk_status_str", "mk_default_opts", "mk_init", "mk_finalize", "mk_last_error", "mk_set_timing", "mk_set_graphs", "mk_timing_report",
     "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
@@ -90,6 +90,7 @@ def lib():
     L.mk_finalize.argtypes = [_P]
     L.mk_last_error.argtypes = [_P, C.c_char_p, C.c_int]
     L.mk_set_timing.argtypes = [_P, C.c_int]
+    L.mk_set_graphs.argtypes = [_P, C.c_int]
     L.mk_timing_report.argtypes = [_P, C.c_int, C.c_char_p, C.c_int]
     L.mk_op_create_real.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
     L.mk_op_create_complex.argtypes = [_P, C.c_int, _I64, _I32, _D, C.POINTER(SolveOpts), C.POINTER(_P)]
@@ -286,6 +287,9 @@ class Context:
 
     def set_timing(self, on):
         self._check(self.L.mk_set_timing(self.h, int(on)), "mk_set_timing")
+
+    def set_graphs(self, on):
+        self._check(self.L.mk_set_graphs(self.h, int(on)), "mk_set_graphs")
 
     def timing_report(self, reset=True):
         """Per-kernel SpMV timings (hipEvent) since the last reset, as a list of dicts."""

@@ -18,15 +18,62 @@ namespace mk {
 
 static constexpr int kBlkParts = 2048;  // grid of the column-reduction kernels
 
+// 16-byte vectors of T (4 floats or 2 doubles) carried by one lane
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+    static constexpr int N = 4;
+    using type = float4;
+};
+template <>
+struct Vec16<double> {
+    static constexpr int N = 2;
+    using type = double2;
+};
+template <typename V>
+__device__ __forceinline__ void vfma(V& acc, float a, const V& x) {
+    acc.x += a * x.x; acc.y += a * x.y; acc.z += a * x.z; acc.w += a * x.w;
+}
+__device__ __forceinline__ void vfma(double2& acc, double a, const double2& x) {
+    acc.x += a * x.x; acc.y += a * x.y;
+}
+__device__ __forceinline__ void vzero(float4& a) { a.x = a.y = a.z = a.w = 0.f; }
+__device__ __forceinline__ void vzero(double2& a) { a.x = a.y = 0.0; }
+__device__ __forceinline__ void vshfl_add(float4& a, int off) {
+    a.x += __shfl_xor(a.x, off, 64); a.y += __shfl_xor(a.y, off, 64);
+    a.z += __shfl_xor(a.z, off, 64); a.w += __shfl_xor(a.w, off, 64);
+}
+__device__ __forceinline__ void vshfl_add(double2& a, int off) {
+    a.x += __shfl_xor(a.x, off, 64); a.y += __shfl_xor(a.y, off, 64);
+}
+__device__ __forceinline__ float4 vsub(const float4& b, const float4& a) {
+    return make_float4(b.x - a.x, b.y - a.y, b.z - a.z, b.w - a.w);
+}
+__device__ __forceinline__ double2 vsub(const double2& b, const double2& a) { return make_double2(b.x - a.x, b.y - a.y); }
+__device__ __forceinline__ float4 vadd(const float4& b, const float4& a) {
+    return make_float4(b.x + a.x, b.y + a.y, b.z + a.z, b.w + a.w);
+}
+__device__ __forceinline__ double2 vadd(const double2& b, const double2& a) { return make_double2(b.x + a.x, b.y + a.y); }
+
+// Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  A lane carries
+// one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront covers 64 / LPR
+// rows of the SELL part at a time; medium and long rows take a whole wavefront whose row
+// groups split the entries.
 template <typename T, int MW, int MODE>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
                                                T* __restrict__ Y, const T* __restrict__ B, const int* skip) {
     if (skip && *skip) return;
-    constexpr int R = 64 / MW;
+    using V = typename Vec16<T>::type;
+    constexpr int N = Vec16<T>::N;
+    constexpr int LPR = MW / N;   // lanes per row
+    constexpr int R = 64 / LPR;   // rows per wavefront
+    const V* __restrict__ Xv = reinterpret_cast<const V*>(X);
     const int lane = threadIdx.x & 63;
-    const int sub = lane / MW, j = lane - sub * MW;
+    const int sub = lane / LPR, j = lane - sub * LPR;
     const int gw = int(blockIdx.x) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
-    T acc = 0;
+    V acc;
+    vzero(acc);
     int row = -1;
     if (gw < nw_sell) {
         const int r = gw * R + sub;
@@ -35,11 +82,20 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
         const int64_t base = A.slice_ptr[s] + lr;
         const int w = A.slice_w[s];
         row = A.perm ? A.perm[r] : r;
-        for (int k = 0; k < w; k++) {
-            const int64_t e = base + int64_t(k) * A.C;
-            acc += A.val[e] * X[size_t(A.col[e]) * MW + j];
+        int k = 0;
+        for (; k + 1 < w; k += 2) {  // two independent gathers in flight
+            const int64_t e0 = base + int64_t(k) * A.C, e1 = e0 + A.C;
+            const int c0 = A.col[e0], c1 = A.col[e1];
+            const T v0 = A.val[e0], v1 = A.val[e1];
+            const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
+            vfma(acc, v0, x0);
+            vfma(acc, v1, x1);
         }
-    } else {  // medium and long rows: a wavefront per row, its 64/MW row groups split the entries
+        if (k < w) {
+            const int64_t e = base + int64_t(k) * A.C;
+            vfma(acc, A.val[e], Xv[size_t(A.col[e]) * LPR + j]);
+        }
+    } else {
         int lw = gw - nw_sell;
         const int* rows = A.mid_rows;
         const int64_t* ptr = A.mid_ptr;
@@ -53,16 +109,17 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             cols = A.long_col;
             vals = A.long_val;
         }
-        for (int64_t k = ptr[lw] + sub; k < ptr[lw + 1]; k += R) acc += vals[k] * X[size_t(cols[k]) * MW + j];
+        for (int64_t k = ptr[lw] + sub; k < ptr[lw + 1]; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
 #pragma unroll
-        for (int off = MW; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
+        for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
         if (sub != 0) return;
         row = rows[lw];
     }
-    const size_t o = size_t(row) * MW + j;
-    if constexpr (MODE == 0) Y[o] = acc;
-    if constexpr (MODE == 3) Y[o] = B[o] - acc;
-    if constexpr (MODE == 4) Y[o] += acc;
+    V* Yv = reinterpret_cast<V*>(Y);
+    const size_t o = size_t(row) * LPR + j;
+    if constexpr (MODE == 0) Yv[o] = acc;
+    if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
+    if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
 }
 
 template <typename T, int MW>
@@ -70,7 +127,8 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
                     const int* skip) {
     unsigned g0 = 0;
     const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
-    const int nw_sell = int((int64_t(A->n_short) + 64 / MW - 1) / (64 / MW));
+    constexpr int R = 64 / (MW / Vec16<T>::N);
+    const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
     const int64_t waves = int64_t(nw_sell) + A->nmid + A->nlong;
     const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
     switch (mode) {

@@ -253,6 +253,7 @@ mk_status mk_init(mk_ctx** out, int device) {
         MK_HIP(hipGetDeviceCount(&n));
         MK_CHECK(device >= 0 && device < n, MK_ERR_ARG, "no such device");
         c->device = device;
+        if (const char* g = std::getenv("METRIKO_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
         MK_HIP(hipSetDevice(device));
         MK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         // per-block partials (4 quantities x 2^17 blocks) + group slots (4 x 2^12), and the
@@ -301,6 +302,12 @@ mk_status mk_set_timing(mk_ctx* c, int on) {
     return MK_OK;
 }
 
+mk_status mk_set_graphs(mk_ctx* c, int on) {
+    if (!c) return MK_ERR_ARG;
+    c->use_graphs = on != 0;
+    return MK_OK;
+}
+
 mk_status mk_timing_report(mk_ctx* c, int reset, char* buf, int len) {
     if (!c || !buf || len <= 0) return MK_ERR_ARG;
     return guard(c, [&] {
@@ -338,7 +345,10 @@ static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_
             op->block = o.block_size;
             MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 1024, MK_ERR_ARG, "block_size");
             for (int i = 0; i < n; i++) MK_CHECK(rowptr[i + 1] >= rowptr[i], MK_ERR_ARG, "rowptr not monotone");
-            for (int64_t k = 0; k < op->nnz; k++) MK_CHECK(col[k] >= 0 && col[k] < n, MK_ERR_ARG, "column index out of range");
+            int64_t bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static, 65536) if (op->nnz > 1000000)
+            for (int64_t k = 0; k < op->nnz; k++) bad += (col[k] >= 0 && col[k] < n) ? 0 : 1;
+            MK_CHECK(bad == 0, MK_ERR_ARG, "column index out of range");
             const int vw = cplx ? 2 : 1;
             op->h_ptr.assign(rowptr, rowptr + n + 1);
             op->h_col.assign(col, col + op->nnz);

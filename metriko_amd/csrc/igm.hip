@@ -451,7 +451,9 @@ extern "C" mk_status mk_iterative_rounding(
         SprsD X2F = (G2 * UF).pruned();      // G2 * UFull (naive, without length)
         MK_LOG("[igm] X2F nnz %lld, %.0f ms\n", (long long)X2F.nonZeros(), now_ms() - t_all);
         SprsD X2Ft = X2F.transpose();
+        MK_LOG("[igm] X2F^T %.0f ms\n", now_ms() - t_all);
         SprsD E = X2Ft * X2F;
+        MK_LOG("[igm] E nnz %lld, %.0f ms\n", (long long)E.nonZeros(), now_ms() - t_all);
         for (auto& v : E.val) v *= length;   // E = X2F^T X2F * length
         std::vector<double> fr(static_cast<size_t>(k));
         X2Ft.multiply(F2, fr.data());        // f = X2F^T F2
@@ -490,13 +492,16 @@ extern "C" mk_status mk_iterative_rounding(
             for (auto& v : A0.val) v *= length * wI * wI;
             A0 = A0 + SprsD::identity(k, wC * wC);
             hA0.op = make_op(c, A0, o);
+            MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
             hA0.op->label = "igm_A0";
             if (o.precond == MK_PRECOND_AMG) {
                 hA0.op->nullvec = near_null(G2, Cf, X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
                 hA0.op->nnull = hA0.op->nullvec.empty() ? 0 : n;
                 S.null_vectors = hA0.op->nnull;
             }
+            MK_LOG("[igm] near-null vectors %d, %.0f ms\n", hA0.op->nnull, now_ms() - t_all);
             mk::ensure_mg(c, hA0.op, o);
+            MK_LOG("[igm] A0 multigrid %.0f ms\n", now_ms() - t_all);
             S.setup_ms += now_ms() - ts;
         }
         hE.op = make_op(c, Eaug, o);
@@ -504,7 +509,10 @@ extern "C" mk_status mk_iterative_rounding(
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
         drhs.upload(rhs.data(), rhs.size(), K.st());
         MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));
-        K.solve(hE.op, drhs.p, dXS.p, hA0.op->mg);
+        {
+            const mk::SolveResult kr = K.solve(hE.op, drhs.p, dXS.p, hA0.op->mg);
+            MK_LOG("[igm] KKT PCG %d iterations, residual %.2e\n", kr.iters, kr.rel_residual);
+        }
         std::vector<double> XSmall(static_cast<size_t>(k));
         dXS.download(XSmall.data(), XSmall.size(), K.st());
         K.sync();
@@ -633,11 +641,14 @@ extern "C" mk_status mk_iterative_rounding(
                 for (int i : left) mark[size_t(candPos[i])] = 1;
                 if (!flags->round_seams) for (int i : integer) mark[size_t(candPos[i])] = 1;
                 for (size_t p = 0; p < nc_; p++) if (mark[p]) need.push_back(int(p));
-                const int mw = need.size() > 32 ? 64 : need.size() > 16 ? 32 : need.size() > 8 ? 16 : 8;
-                DevBuf<double> Bp(size_t(k) * mw), Xp(size_t(k) * mw);
-                DevBuf<int> dsel(static_cast<size_t>(mw));
-                for (size_t b0 = 0; b0 < need.size(); b0 += size_t(mw)) {
-                    const int cnt = int(std::min<size_t>(size_t(mw), need.size() - b0));
+                // blocks of at most 64 columns, each as narrow as its column count allows
+                const size_t mwmax = need.size() > 32 ? 64 : need.size() > 16 ? 32 : need.size() > 8 ? 16 : 8;
+                DevBuf<double> Bp(size_t(k) * mwmax), Xp(size_t(k) * mwmax);
+                DevBuf<int> dsel(mwmax);
+                for (size_t b0 = 0, cnt_ = 0; b0 < need.size(); b0 += cnt_) {
+                    cnt_ = std::min<size_t>(mwmax, need.size() - b0);
+                    const int cnt = int(cnt_);
+                    const int mw = cnt > 32 ? 64 : cnt > 16 ? 32 : cnt > 8 ? 16 : 8;
                     std::vector<int> sel(size_t(mw), -1);
                     for (int j = 0; j < cnt; j++) sel[size_t(j)] = need[b0 + size_t(j)];
                     dsel.upload(sel.data(), sel.size(), K.st());
@@ -687,6 +698,8 @@ extern "C" mk_status mk_iterative_rounding(
             };
             int lm_rounds = 0, rounds = 0;
             DevBuf<double> pv2, dd2, tr2;
+            double tm_xt = 0.0, tm_lm = 0.0, tm_all0 = now_ms();
+            int xt_iters = 0;
             while (!left.empty()) {
                 // initFixedIndices (iter_rounding_loop.h:29-74)
                 MK_HIP(hipMemcpyAsync(dxprev.p, dxcur.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
@@ -747,7 +760,11 @@ extern "C" mk_status mk_iterative_rounding(
                 // xt = A0^{-1} (b_int + wC^2 xPrev)  (warm start: previous xt)
                 hipLaunchKernelGGL(k_lin2, dim3(nblk(static_cast<size_t>(k))), dim3(256), 0, K.st(), size_t(k), 1.0, dbint.p,
                                    wC * wC, dxprev.p, db0.p);
-                K.solve(hA0.op, db0.p, dxt.p);
+                {
+                    const double t1 = now_ms();
+                    xt_iters += K.solve(hA0.op, db0.p, dxt.p).iters;
+                    tm_xt += now_ms() - t1;
+                }
                 std::vector<double> Uxt;
                 fixed_dots(dxt.p, Uxt);
                 std::vector<double> M(size_t(r) * size_t(r)), rhs2(static_cast<size_t>(r));
@@ -775,7 +792,11 @@ extern "C" mk_status mk_iterative_rounding(
                     for (int i = 0; i < r; i++) q += (ux[size_t(i)] - fr_[size_t(i)]) * (ux[size_t(i)] - fr_[size_t(i)]);
                     return wI * wI * a + wC * wC * b + wK * wK * q;
                 };
-                lm_solve(K, size_t(k), dxcur.p, dxlm.p, 100, 1e-7, dir, energy, pv2, dd2, tr2);
+                {
+                    const double t1 = now_ms();
+                    lm_solve(K, size_t(k), dxcur.p, dxlm.p, 100, 1e-7, dir, energy, pv2, dd2, tr2);
+                    tm_lm += now_ms() - t1;
+                }
                 // post_checking (iter_rounding_loop.h:89-100)
                 std::vector<double> ux;
                 fixed_dots(dxlm.p, ux);
@@ -784,6 +805,8 @@ extern "C" mk_status mk_iterative_rounding(
                 MK_HIP(hipMemcpyAsync(dxcur.p, dxlm.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
                 if (!(mx <= 10e-7)) { *success = 0; break; }
             }
+            MK_LOG("[igm] rounding loop %.0f ms: %d rounds, %d with LM; xt solves %.0f ms (%d iterations), LM "
+                   "passes %.0f ms\n", now_ms() - tm_all0, rounds, lm_rounds, tm_xt, xt_iters, tm_lm);
             S.rounds = rounds;
             S.lm_rounds = lm_rounds;
             S.n_fixed = int(fixedI.size());

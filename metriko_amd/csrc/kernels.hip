@@ -461,6 +461,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     std::vector<int> hcol(static_cast<size_t>(stored));
     std::vector<double> hval(size_t(stored) * vw, 0.0);
     std::vector<float> hval32(size_t(stored) * vw, 0.0f);
+#pragma omp parallel for schedule(dynamic, 64) if (stored > 200000)
     for (int s = 0; s < ns; s++) {
         for (int l = 0; l < C; l++) {
             const int r = s * C + l;
@@ -484,7 +485,9 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     // inverse diagonal (real part); rectangular operators keep 1
     std::vector<double> dinv(n, 1.0);
     std::vector<float> dinv32(n, 1.0f);
-    for (int i = 0; i < n && op->m == n; i++) {
+#pragma omp parallel for schedule(static, 4096) if (n > 100000 && op->m == n)
+    for (int i = 0; i < n; i++) {
+        if (op->m != n) continue;
         double d = 0.0;
         for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += val[k * vw];
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
@@ -725,6 +728,56 @@ static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, 
                            c->partials.p, c->counter.p);
 }
 
+// Runs `batch` calls of body() on the context stream: eagerly, or -- when graphs are on, timing
+// is off and batch is the operator's usual check interval -- by replaying a hipGraph captured
+// from the same calls the first time this key (buffers, mode, batch) is seen.  body() must
+// only launch kernels on c->stream with arguments fixed by the key.
+template <typename F>
+static void run_batch(mk_ctx* c, const mk_op* A_, std::vector<uintptr_t> key, int batch, bool graphable, F&& body) {
+    mk_op* A = const_cast<mk_op*>(A_);
+    if (!c->use_graphs || c->time_spmv || !graphable) {
+        for (int b = 0; b < batch; b++) body();
+        return;
+    }
+    key.push_back(uintptr_t(batch));
+    mk_op::Graph* G = nullptr;
+    for (auto& g : A->graphs) if (g.key == key) { G = &g; break; }
+    if (!G) {
+        if (A->graphs.size() >= 8) {  // bounded cache: drop the oldest
+            if (A->graphs.front().exec) MK_HIP(hipGraphExecDestroy(A->graphs.front().exec));
+            A->graphs.erase(A->graphs.begin());
+        }
+        const int64_t calls0 = c->spmv_calls;
+        const double bytes0 = c->spmv_bytes;
+        MK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        hipGraph_t graph = nullptr;
+        try {
+            for (int b = 0; b < batch; b++) body();
+        } catch (...) {
+            (void)hipStreamEndCapture(c->stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        MK_HIP(hipStreamEndCapture(c->stream, &graph));
+        mk_op::Graph ng;
+        ng.key = key;
+        hipError_t e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) throw Error(MK_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        ng.spmv_calls = c->spmv_calls - calls0;
+        ng.spmv_bytes = c->spmv_bytes - bytes0;
+        c->spmv_calls = calls0;
+        c->spmv_bytes = bytes0;
+        A->graphs.push_back(std::move(ng));
+        G = &A->graphs.back();
+        c->graph_captures++;
+    }
+    MK_HIP(hipGraphLaunch(G->exec, c->stream));
+    c->spmv_calls += G->spmv_calls;
+    c->spmv_bytes += G->spmv_bytes;
+    c->graph_replays++;
+}
+
 // one PCG iteration with an external (multigrid) preconditioner
 template <typename T>
 static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T* r, T* z, T* p, T* q, double bytes) {
@@ -782,7 +835,9 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
             timing_flush(c);
             if (h.done || h.it >= maxiter) break;
             const int batch = std::min(check_every, maxiter - h.it);
-            for (int b = 0; b < batch; b++) cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes);
+            run_batch(c, A, {uintptr_t(sizeof(T)), uintptr_t(mg), uintptr_t(val), uintptr_t(x), uintptr_t(r),
+                             uintptr_t(z), uintptr_t(p), uintptr_t(q)},
+                      batch, batch == check_every, [&] { cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes); });
             MK_HIP(hipGetLastError());
         }
         CgResult res{h.status, h.it, h.rr, h.bb};
@@ -871,7 +926,9 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         timing_flush(c);
         if (h.done || h.it >= maxiter) break;
         const int batch = std::min(check_every, maxiter - h.it);
-        for (int k = 0; k < batch; k++) {
+        run_batch(c, A, {uintptr_t(3), uintptr_t(mg), uintptr_t(x), uintptr_t(w.r.p), uintptr_t(w.r32.p),
+                         uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p), uintptr_t(w.q.p)},
+                  batch, batch == check_every, [&] {
             launch_cg_spmv<double>(c, A, A->val64.p, w.p.p, w.q.p, A->bytes64);
             hipLaunchKernelGGL(cgm_update, dim3(rgrid(len)), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
                                (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
@@ -880,7 +937,7 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
                                (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
             hipLaunchKernelGGL(cgm_pupdate, dim3(g), dim3(256), 0, c->stream, len, w.p.p, (const float*)w.z32.p,
                                w.z.p, (const CgState*)st);
-        }
+        });
         MK_HIP(hipGetLastError());
     }
     CgResult res{h.status, h.it, h.rr, h.bb};

@@ -12,6 +12,7 @@
 // in D^-1 A on [rho/30, 1.1 rho]; every operator is a SELL-C-sigma matrix and every
 // kernel honours the CG "done" flag so a converged solve costs nothing further.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <numeric>
@@ -61,7 +62,7 @@ void mg_free(Mg* g) {
 struct MgTune {
     double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 10.0, lmax_fac = 1.2;
     int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1, compat = 1,
-        l1 = 1;
+        l1 = 1, attach = 1;
 };
 static const MgTune& tune() {
     static const MgTune t = [] {
@@ -80,6 +81,7 @@ static const MgTune& tune() {
         i("METRIKO_MG_SMOOTH_LONG", x.smooth_long);
         i("METRIKO_MG_COMPAT", x.compat);
         i("METRIKO_MG_L1", x.l1);
+        i("METRIKO_MG_ATTACH", x.attach);
         return x;
     }();
     return t;
@@ -109,9 +111,28 @@ Csr<T> conj_transpose(const Csr<T>& A) {
 template <typename T>
 std::vector<double> diag_of(const Csr<T>& A) {
     std::vector<double> d(size_t(A.nr), 0.0);
+#pragma omp parallel for schedule(static, 4096) if (A.nr > 100000)
     for (int i = 0; i < A.nr; i++)
         for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) if (A.col[k] == i) d[i] = re(A.val[k]);
     return d;
+}
+
+// sum of f(i) over [0, n) in 64 fixed chunks (parallel, same result for any thread count)
+template <typename F>
+double chunked_sum(int n, F&& f) {
+    constexpr int kChunks = 64;
+    double part[kChunks];
+    const int step = (n + kChunks - 1) / kChunks;
+#pragma omp parallel for schedule(static, 1) if (n > 100000)
+    for (int c = 0; c < kChunks; c++) {
+        double s = 0.0;
+        const int e = std::min(n, (c + 1) * step);
+        for (int i = c * step; i < e; i++) s += f(i);
+        part[c] = s;
+    }
+    double t = 0.0;
+    for (int c = 0; c < kChunks; c++) t += part[c];
+    return t;
 }
 
 // spectral radius of D^-1 A by power iteration (fixed seed)
@@ -128,14 +149,15 @@ double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
     double lam = 1.0;
     for (int it = 0; it < tune().power; it++) {
         A.multiply(x.data(), y.data());
-        double nrm = 0.0;
-        for (int i = 0; i < n; i++) { y[i] = y[i] / (d[i] != 0.0 ? d[i] : 1.0); nrm += mag2(y[i]); }
+        double nrm = chunked_sum(n, [&](int i) {
+            y[i] = y[i] / (d[i] != 0.0 ? d[i] : 1.0);
+            return mag2(y[i]);
+        });
         nrm = std::sqrt(nrm);
-        double xn = 0.0;
-        for (auto& v : x) xn += mag2(v);
-        xn = std::sqrt(xn);
+        double xn = std::sqrt(chunked_sum(n, [&](int i) { return mag2(x[i]); }));
         lam = nrm / (xn > 0 ? xn : 1.0);
         if (nrm == 0.0) break;
+#pragma omp parallel for schedule(static, 4096) if (n > 100000)
         for (int i = 0; i < n; i++) x[i] = y[i] / nrm;
     }
     return lam;
@@ -196,17 +218,26 @@ Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double t
     // long rows (kernel-basis / transition DOFs coupling to a whole seam) stay out of the
     // strength graph: they become singleton aggregates and are not smoothed into others
     std::vector<uint8_t> lng = long_rows(A);
+    auto strong = [&](int i, int64_t k) {
+        const int j = A.col[k];
+        if (j == i || lng[i] || lng[j]) return false;
+        if (B && nb > 0 && !compatible(A.val[k], *B, nb, n, i, j)) return false;
+        return mag2(A.val[k]) >= theta * theta * std::abs(d[i]) * std::abs(d[j]);
+    };
+#pragma omp parallel for schedule(static, 4096) if (n > 100000)
     for (int i = 0; i < n; i++) {
-        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
-            const int j = A.col[k];
-            if (j == i || lng[i] || lng[j]) continue;
-            if (B && nb > 0 && !compatible(A.val[k], *B, nb, n, i, j)) continue;
-            if (mag2(A.val[k]) >= theta * theta * std::abs(d[i]) * std::abs(d[j])) {
-                scol.push_back(j);
-                sval.push_back(A.val[k]);
-            }
-        }
-        sptr[i + 1] = int64_t(scol.size());
+        int64_t c = 0;
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) c += strong(i, k) ? 1 : 0;
+        sptr[size_t(i) + 1] = c;
+    }
+    for (int i = 0; i < n; i++) sptr[size_t(i) + 1] += sptr[size_t(i)];
+    scol.resize(size_t(sptr[size_t(n)]));
+    sval.resize(size_t(sptr[size_t(n)]));
+#pragma omp parallel for schedule(static, 4096) if (n > 100000)
+    for (int i = 0; i < n; i++) {
+        int64_t w = sptr[size_t(i)];
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+            if (strong(i, k)) { scol[size_t(w)] = A.col[k]; sval[size_t(w)] = A.val[k]; w++; }
     }
     std::vector<int>& agg = G.agg;
     std::vector<int>& root = G.root;
@@ -223,7 +254,7 @@ Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double t
         nagg++;
     }
     std::vector<int> agg1 = agg;
-    for (int i = 0; i < n; i++) {  // attach to the strongest seeded neighbour
+    for (int i = 0; i < n && tune().attach; i++) {  // attach to the strongest seeded neighbour
         if (agg[i] != -1) continue;
         int best = -1;
         double bv = -1.0;
@@ -450,7 +481,16 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
     }
     const int maxLevels = 14;
     const bool verbose = std::getenv("METRIKO_VERBOSE") != nullptr;
+    double t_lap = 0.0;
+    auto lap = [&](const char* what) {
+        if (!verbose) return;
+        const double t = std::chrono::duration<double, std::milli>(
+                             std::chrono::steady_clock::now().time_since_epoch()).count();
+        if (t_lap > 0.0) std::fprintf(stderr, "[mg]   %s %.0f ms\n", what, t - t_lap);
+        t_lap = t;
+    };
     for (int l = 0; l < maxLevels; l++) {
+        lap("level start");
         std::vector<double> d = diag_of(A);
         if (l == 0)
             for (int i = 0; i < A.nr; i++)
@@ -460,16 +500,25 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
         // strongly coupled rows cannot stretch the Chebyshev interval of all the others
         std::vector<double> ds(d);
         if (tune().l1) {
+#pragma omp parallel for schedule(static, 4096) if (A.nr > 100000)
             for (int i = 0; i < A.nr; i++) {
                 double sa = 0.0;
                 for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) sa += mag(A.val[k]);
                 ds[i] = std::max(d[i], 0.5 * sa);
             }
         }
+        lap("diagonals");
         const double lmax = tune().lmax_fac * rho_dinv_a(A, ds);
+        lap("spectral bound");
         std::vector<double> di(d.size());
         for (size_t i = 0; i < d.size(); i++) di[i] = ds[i] != 0.0 ? 1.0 / ds[i] : 1.0;
         if (verbose) {
+            static double t_prev = 0.0;
+            const double t_now = std::chrono::duration<double, std::milli>(
+                                     std::chrono::steady_clock::now().time_since_epoch()).count();
+            if (l == 0) t_prev = t_now;
+            std::fprintf(stderr, "[mg] +%.0f ms\n", t_now - t_prev);
+            t_prev = t_now;
             int64_t mx = 0;
             for (int i = 0; i < A.nr; i++) mx = std::max<int64_t>(mx, A.ptr[i + 1] - A.ptr[i]);
             std::fprintf(stderr, "[mg] level %d: n=%d nnz=%lld maxrow=%lld lmax=%.3g\n", l, A.nr,
@@ -482,6 +531,7 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             std::vector<T> Bc;
             if (nb > 0) Pt = tentative_qr(A, d, theta, B, nb, Bc, nagg);
             else Pt = tentative(A, d, theta, nagg);
+            lap("aggregation + tentative");
             if (nb > 0) B = std::move(Bc);
             if (nagg >= A.nr * 0.9 || nagg < 1) {  // no coarsening possible: make this the coarsest level
                 if (A.nr > 4000) throw Error(MK_ERR_UNSUPPORTED, "multigrid: aggregation stalled on a large level");
@@ -505,42 +555,54 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             // P = Pt - omega D^-1 Af Pt, Af = A without the couplings of long rows (and, without
             // exact near-null vectors, of long columns)
             std::vector<uint8_t> lng = long_rows(A);
-            Csr<T> Af = A;
+            auto keep = [&](int i, int j) { return j == i || !(lng[i] || (lng[j] && !keep_long)); };
+            Csr<T> Af(A.nr, A.nc);
             {
-                int64_t w = 0;
-                std::vector<int64_t> np(size_t(A.nr) + 1, 0);
+                std::vector<int64_t>& np = Af.ptr;
+#pragma omp parallel for schedule(static, 4096)
                 for (int i = 0; i < A.nr; i++) {
-                    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) {
-                        const int j = A.col[k];
-                        if (j != i && (lng[i] || (lng[j] && !keep_long))) continue;
-                        Af.col[w] = j;
-                        Af.val[w] = A.val[k];
-                        w++;
-                    }
-                    np[i + 1] = w;
+                    int64_t c = 0;
+                    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++) c += keep(i, A.col[k]) ? 1 : 0;
+                    np[size_t(i) + 1] = c;
                 }
-                Af.col.resize(w);
-                Af.val.resize(w);
-                Af.ptr = np;
+                for (int i = 0; i < A.nr; i++) np[size_t(i) + 1] += np[size_t(i)];
+                Af.col.resize(size_t(np[size_t(A.nr)]));
+                Af.val.resize(size_t(np[size_t(A.nr)]));
+#pragma omp parallel for schedule(static, 4096)
+                for (int i = 0; i < A.nr; i++) {
+                    int64_t w = np[size_t(i)];
+                    for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; k++)
+                        if (keep(i, A.col[k])) { Af.col[size_t(w)] = A.col[k]; Af.val[size_t(w)] = A.val[k]; w++; }
+                }
             }
+            lap("filtered operator");
             Csr<T> AP = metriko::spgemm(Af, Pt);
-            std::vector<metriko::Triplet<T>> tr;
-            tr.reserve(size_t(AP.nonZeros() + Pt.nonZeros()));
+            lap("A Pt");
+            // P = Pt + rows of A Pt scaled by -omega / ds_i (long rows: none)
+            Csr<T> APs(A.nr, AP.nc);
+            for (int i = 0; i < A.nr; i++)
+                APs.ptr[size_t(i) + 1] = APs.ptr[size_t(i)] + (lng[i] ? 0 : AP.ptr[i + 1] - AP.ptr[i]);
+            APs.col.resize(size_t(APs.ptr[size_t(A.nr)]));
+            APs.val.resize(size_t(APs.ptr[size_t(A.nr)]));
+#pragma omp parallel for schedule(static, 4096)
             for (int i = 0; i < A.nr; i++) {
-                for (int64_t k = Pt.ptr[i]; k < Pt.ptr[i + 1]; k++) tr.emplace_back(i, Pt.col[k], Pt.val[k]);
                 if (lng[i]) continue;
                 const double di = ds[i] != 0.0 ? ds[i] : 1.0;
-                for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++) tr.emplace_back(i, AP.col[k], -omega / di * AP.val[k]);
+                int64_t w = APs.ptr[size_t(i)];
+                for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++, w++) {
+                    APs.col[size_t(w)] = AP.col[k];
+                    APs.val[size_t(w)] = -omega / di * AP.val[k];
+                }
             }
-            Csr<T> P(A.nr, nagg);
-            P.setFromTriplets(tr);
+            Csr<T> P = metriko::add(Pt, T(1), APs, T(1));
+            lap("smoothed P");
             Csr<T> R = conj_transpose(P);
+            lap("R");
             Csr<T> Ac = metriko::spgemm(R, metriko::spgemm(A, P));
+            lap("R A P");
             // exact Hermitian symmetry of the coarse operator
             Csr<T> AcH = conj_transpose(Ac);
-            for (size_t k = 0; k < Ac.val.size(); k++) Ac.val[k] *= 0.5;
-            for (auto& v : AcH.val) v *= 0.5;
-            Ac = Ac + AcH;
+            Ac = metriko::add(Ac, T(0.5), AcH, T(0.5));
             H.A.push_back(std::move(A));
             H.P.push_back(std::move(P));
             H.R.push_back(std::move(R));

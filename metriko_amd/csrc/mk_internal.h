@@ -143,6 +143,8 @@ struct mk_ctx {
     double spmv_ms = 0.0;
     int64_t spmv_calls = 0;
     double spmv_bytes = 0.0;
+    bool use_graphs = true;  // replay captured PCG iteration batches
+    int64_t graph_replays = 0, graph_captures = 0;
 };
 
 namespace mk { struct Mg; void mg_free(Mg*); }
@@ -152,7 +154,19 @@ struct mk_op {
     int n = 0;          // rows
     int m = 0;          // columns (== n for square operators)
     mk::Mg* mg = nullptr;  // multigrid hierarchy used as preconditioner (owned)
-    ~mk_op() { if (mg) mk::mg_free(mg); }
+    // captured PCG iteration batches: key (buffers, mode, batch) -> executable graph, and the
+    // SpMV counters one replay adds
+    struct Graph {
+        std::vector<uintptr_t> key;
+        hipGraphExec_t exec = nullptr;
+        int64_t spmv_calls = 0;
+        double spmv_bytes = 0.0;
+    };
+    std::vector<Graph> graphs;
+    ~mk_op() {
+        for (auto& g : graphs) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (mg) mk::mg_free(mg);
+    }
     bool cplx = false;
     int64_t nnz = 0;
     // SELL-C-sigma layout (C = slice height; rows sorted by length inside windows of sigma)
