@@ -5,7 +5,8 @@ Workload (one "step"): icosphere level 8 (655,362 v / 1,310,720 f), radial displ
 0.05 sin(6 theta) sin(5 phi) plus N(0, sigma) radial jitter (seed 5) with sigma = 1e-3 * 2^(4-8)
 = 6.25e-5: the level-4 jitter of 1e-3 scaled with the edge length, so every level carries the
 same relative noise (at the unscaled 1e-3, a quarter of an L8 edge, the surface folds and the
-smoothest field has 17,229 singularities; see --as-specified).  The example_igm pipeline end to
+smoothest field has 17,229 singularities: the "as_specified" entry reports that mesh's field
+stage and the size of its IGM problem).  The example_igm pipeline end to
 end: Smoothest 4-RoSy field (50 inverse-iteration solves), Curl matching, seam, cut, combing,
 seamless IGM with iterative rounding (ratio 0.03).
 
@@ -85,6 +86,23 @@ def traffic_for(key):
         return None
 
 
+def as_specified(ctx, mk):
+    """The level-8 mesh with the unscaled N(0, 1e-3) jitter: field stage timed, then the size of
+    the IGM problem it leads to (host combinatorics)."""
+    V, F, u0 = workload(LEVEL, jitter=1e-3)
+    he = mk.hmesh_halfedges(V, F)
+    t0 = time.perf_counter()
+    L = ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
+    x, st = ctx.smallest_eig(L, he["face_area"], u0, 50, mk.opts(precision=mk.PREC_MIXED_MG))
+    field_ms = 1e3 * (time.perf_counter() - t0)
+    c = mk.combinatorics(V, F, x)
+    return {"jitter": 1e-3, "field_ms": round(field_ms, 1), "field_iters": st["iterations"],
+            "singularities": c["nS"], "transitions": c["nT"], "constraint_rows": c["constraint_rows"],
+            "seam_edges": int(c["seam"].sum()),
+            "igm": "not run: the dense FullPivLU kernel of the constraint block and one greedy rounding "
+                   "round per integer variable (reference algorithm) do not finish at this size"}
+
+
 def run(ctx, V, F, u0, o):
     t0 = time.perf_counter()
     out = ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=o, igm_opts=o)
@@ -99,6 +117,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-as-specified", action="store_true")
     ap.add_argument("--level", type=int, default=LEVEL)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 1 GiB buffer three times (HBM counter calibration for rocprofv3 --pmc)")
@@ -182,6 +201,8 @@ def main():
                 "decisions_equal": a["decisions"] == b["decisions"]},
         }
         result["speedup_mixed_vs_fp64"] = round(result["fp64"]["solve_ms"] / result["value"], 3)
+    if not args.no_as_specified and args.level == LEVEL:
+        result["as_specified"] = as_specified(ctx, mk)
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
