@@ -43,14 +43,14 @@ enum { MK_FMT_CSR = 0, MK_FMT_SELL = 1 };
 enum { MK_PRECOND_NONE = 0, MK_PRECOND_JACOBI = 1, MK_PRECOND_AMG = 2 };  /* AMG: smoothed aggregation V-cycle */
 
 typedef struct {
-    int precision;      /* MK_PREC_* */
+    int precision;      /* MK_PREC_*, default MK_PREC_MIXED_MG */
     double tol;         /* FP64 relative residual ||b - A x|| / ||b||, default 1e-10 */
     int maxiter;        /* max inner iterations per solve (all refinement steps), default 20000 */
     int format;         /* MK_FMT_* (device operator layout), default SELL */
     int slice_height;   /* SELL slice height C (multiple of 64), default 64 */
     int sigma;          /* SELL sorting window (rows), 1 = unsorted, default 1 */
     int block_size;     /* threads per block for SpMV, default 256 */
-    int precond;        /* MK_PRECOND_*, default JACOBI */
+    int precond;        /* MK_PRECOND_*, default AMG */
     double inner_tol;   /* FP32 inner relative tolerance per refinement step, default 1e-5 */
     int max_refine;     /* max refinement steps, default 12 */
     int check_every;    /* iterations between host convergence checks, default 16 */

@@ -231,7 +231,7 @@ const char* mk_status_str(mk_status s) {
 }
 
 void mk_default_opts(mk_solve_opts* o) {
-    o->precision = MK_PREC_MIXED;
+    o->precision = MK_PREC_MIXED_MG;
     o->tol = 1e-10;
     o->maxiter = 20000;
     o->format = MK_FMT_SELL;

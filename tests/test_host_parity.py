@@ -30,7 +30,7 @@ def test_status_strings_and_defaults():
     assert L.mk_status_str(0) == b"ok"
     assert b"SPD" in L.mk_status_str(3)
     o = mk.opts()
-    assert o.tol == 1e-10 and o.precision == mk.PREC_MIXED and o.precond == mk.PRECOND_AMG
+    assert o.tol == 1e-10 and o.precision == mk.PREC_MIXED_MG and o.precond == mk.PRECOND_AMG
 
 
 @pytest.mark.parametrize("name", ["ico2", "icosphere_obj", "torus_small", "torus_mid"])
