@@ -174,25 +174,45 @@ namespace metriko {
         }
     };
 
-    // C = A * B (Gustavson with a dense accumulator; sorted output columns).  Rows are
-    // independent: a symbolic pass sizes them, a numeric pass fills them (same arithmetic
-    // order per row as a sequential product).
+    // C = A * B (Gustavson; sorted output columns).  Rows are independent: a symbolic pass
+    // sizes them, a numeric pass fills them.  Short rows (few products) accumulate by a
+    // stable sort of their (column, product) list, long rows in a dense accumulator; both
+    // add the products of a column in the same order (A's row order, then B's), so the
+    // result does not depend on the path or on the thread count.
     template<typename T>
     Csr<T> spgemm(const Csr<T>& A, const Csr<T>& B) {
         if (A.nc != B.nr) throw std::invalid_argument("spgemm: shape mismatch");
         Csr<T> C(A.nr, B.nc);
         const bool par = A.nonZeros() > 100000;
+        constexpr int64_t kShort = 32;  // products per row up to which the sort path is used
+        auto work = [&](int i) {
+            int64_t w = 0;
+            for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) w += B.ptr[A.col[ka] + 1] - B.ptr[A.col[ka]];
+            return w;
+        };
         std::vector<int64_t> cnt(size_t(A.nr) + 1, 0);
 #pragma omp parallel if (par)
         {
-            std::vector<int> mark(size_t(B.nc), -1);
+            std::vector<int> mark, cols;
 #pragma omp for schedule(dynamic, 1024)
             for (int i = 0; i < A.nr; i++) {
                 int64_t n = 0;
-                for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
-                    const int j = A.col[ka];
-                    for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
-                        if (mark[B.col[kb]] != i) { mark[B.col[kb]] = i; n++; }
+                if (work(i) <= kShort) {
+                    cols.clear();
+                    for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+                        const int j = A.col[ka];
+                        for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) cols.push_back(B.col[kb]);
+                    }
+                    for (size_t q = 1; q < cols.size(); q++)  // insertion sort (a few entries)
+                        for (size_t r = q; r > 0 && cols[r - 1] > cols[r]; r--) std::swap(cols[r - 1], cols[r]);
+                    n = int64_t(std::unique(cols.begin(), cols.end()) - cols.begin());
+                } else {
+                    if (mark.empty()) mark.assign(size_t(B.nc), -1);
+                    for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+                        const int j = A.col[ka];
+                        for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++)
+                            if (mark[B.col[kb]] != i) { mark[B.col[kb]] = i; n++; }
+                    }
                 }
                 cnt[size_t(i) + 1] = n;
             }
@@ -203,11 +223,29 @@ namespace metriko {
         C.val.resize(size_t(cnt[size_t(A.nr)]));
 #pragma omp parallel if (par)
         {
-            std::vector<T> acc(size_t(B.nc), T());
-            std::vector<int> mark(size_t(B.nc), -1);
+            std::vector<T> acc;
+            std::vector<int> mark;
             std::vector<int> cols;
+            std::vector<std::pair<int, T>> pr;
 #pragma omp for schedule(dynamic, 1024)
             for (int i = 0; i < A.nr; i++) {
+                int64_t w = C.ptr[size_t(i)];
+                if (work(i) <= kShort) {
+                    pr.clear();
+                    for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
+                        const int j = A.col[ka];
+                        const T a = A.val[ka];
+                        for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) pr.emplace_back(B.col[kb], a * B.val[kb]);
+                    }
+                    for (size_t q = 1; q < pr.size(); q++)  // stable insertion sort by column
+                        for (size_t r = q; r > 0 && pr[r - 1].first > pr[r].first; r--) std::swap(pr[r - 1], pr[r]);
+                    for (size_t q = 0; q < pr.size(); q++) {
+                        if (q > 0 && pr[q].first == pr[q - 1].first) C.val[size_t(w - 1)] += pr[q].second;
+                        else { C.col[size_t(w)] = pr[q].first; C.val[size_t(w)] = pr[q].second; w++; }
+                    }
+                    continue;
+                }
+                if (acc.empty()) { acc.assign(size_t(B.nc), T()); mark.assign(size_t(B.nc), -1); }
                 cols.clear();
                 for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
                     const int j = A.col[ka];
@@ -219,7 +257,6 @@ namespace metriko {
                     }
                 }
                 std::sort(cols.begin(), cols.end());
-                int64_t w = C.ptr[size_t(i)];
                 for (int c : cols) { C.col[size_t(w)] = c; C.val[size_t(w)] = acc[c]; w++; }
             }
         }

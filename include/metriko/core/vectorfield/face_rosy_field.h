@@ -67,6 +67,23 @@ namespace metriko {
             return S;
         }
 
+        // principalCurvatureDir (face_rosy_field.h:86-98): per face, a quarter of the sum over
+        // its halfedges of -l e^{2i farg} darg (c = l e^{i farg}, -c^2 / l), in the face basis.
+        VecXc principalCurvatureDir() const {
+            VecXc D(mesh.nF);
+            for (int f = 0; f < mesh.nF; f++) {
+                complex dir(0, 0);
+                int h = mesh.face2half[f];
+                for (int k = 0; k < 3; k++, h = mesh.next[h]) {
+                    const double l = mesh.hvec(h).norm();
+                    const complex c = std::polar(1.0, mesh.heArgOnFace[h]) * l;
+                    dir += -c * c / l * mesh.dihedralArg[h];
+                }
+                D[f] = dir * 0.25;
+            }
+            return D;
+        }
+
         void computeMatching(const MatchingType type) {
             computeConnection();
             matching = VecXi::Constant(mesh.nE, -1);
@@ -112,8 +129,26 @@ namespace metriko {
             compressed = solveSmallestEig(L, mass, u0, nIter);
             roots();
         }
+        // FieldType::CurvatureAligned (face_rosy_field.h:25-34) with lambda = 0: D = the principal
+        // curvature direction (squared for N = 4), rhs = M D / sqrt(|D^H M D|), one Hermitian
+        // positive definite solve L x = rhs on the device operator (multigrid PCG).
         void solve_curvature_aligned() {
-            throw std::runtime_error("FaceRosyField: CurvatureAligned is not built in this tier");
+            if (rosyN != 2 && rosyN != 4)
+                throw std::invalid_argument("FaceRosyField: CurvatureAligned needs N = 2 or 4");
+            VecXc D = principalCurvatureDir();
+            if (rosyN == 4) for (int f = 0; f < mesh.nF; f++) D[f] = D[f] * D[f];
+            double dmd = 0.0;
+            for (int f = 0; f < mesh.nF; f++) dmd += mesh.faceArea[f] * std::norm(D[f]);
+            const double s = std::sqrt(std::abs(dmd));
+            VecXc rhs(mesh.nF);
+            for (int f = 0; f < mesh.nF; f++) rhs[f] = mesh.faceArea[f] * D[f] / s;
+            DeviceOp L = connectionLaplacianDevice();
+            auto& rt = hip::runtime();
+            compressed = VecXc::Zero(mesh.nF);
+            hip::check(mk_spd_solve(rt.get(), L.op, reinterpret_cast<const double*>(rhs.data()),
+                                    reinterpret_cast<double*>(compressed.data()), &rt.field_opts, &rt.last_field),
+                       "mk_spd_solve");
+            roots();
         }
         // face_rosy_field.h:37-43: principal N-th root times the N rotations
         void roots() {

@@ -186,6 +186,26 @@ mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, in
                            int32_t* singular, int32_t* raw_matching, int32_t* matching, uint8_t* seam,
                            int32_t* cut_idx, int32_t* sizes, double* ext);
 
+/* --- field types and matchings beyond the example_igm flow -------------------------------
+ * mk_field_solve: the FaceRosyField(mesh, N, type) constructor (face_rosy_field.h:17-44) in
+ * one call.  MK_FIELD_SMOOTHEST: 50-step inverse iteration as mk_field_smallest_eig (u0 NULL:
+ * glibc rand() stream).  MK_FIELD_CURVATURE_ALIGNED (N = 2 or 4): D = principal curvature
+ * direction (face_rosy_field.h:86-98), squared for N = 4, and one Hermitian positive definite
+ * solve L x = M D / sqrt|D^H M D| (face_rosy_field.h:25-34; solveSquare,
+ * sparse_linear_solver.h:16-27).  compressed[2nF] out (interleaved). */
+enum { MK_FIELD_SMOOTHEST = 0, MK_FIELD_CURVATURE_ALIGNED = 1 };
+mk_status mk_field_solve(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, int N, int field_type,
+                         const double* u0, int n_iter, const mk_solve_opts* opts, double* compressed,
+                         mk_stats* stats);
+/* principalCurvatureDir (face_rosy_field.h:86-98), host only: D[2nF] interleaved. */
+mk_status mk_curvature_dir(int nV, int nF, const double* V, const int32_t* F, double* D);
+/* computeMatching(MatchingType) (face_rosy_field.h:119-135) on a given compressed field, host
+ * only: Principal (pMatching, :138-167) or Curl (cMatching, :169-202), then computeSingular
+ * (:100-107).  singular[nV], matching[nE], effort[nE] out (any may be NULL). */
+enum { MK_MATCH_PRINCIPAL = 0, MK_MATCH_CURL = 1 };
+mk_status mk_field_matching(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
+                            int matching_type, int32_t* singular, int32_t* matching, double* effort);
+
 /* Host-only multigrid PCG (no GPU; test and tuning aid): builds the same smoothed-aggregation
  * hierarchy the device path uses for the CSR matrix (complex when cplx != 0, values
  * interleaved) and solves A x = b with a host V-cycle.  Returns iterations, levels and the

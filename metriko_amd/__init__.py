@@ -23,8 +23,10 @@ EXPORTED = [
     "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
-    "mk_spd_solve_many",
+    "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching",
 ]
+FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
+MATCH_PRINCIPAL, MATCH_CURL = 0, 1
 
 
 class SolveOpts(C.Structure):
@@ -113,6 +115,10 @@ def lib():
     L.mk_combinatorics.argtypes = [C.c_int, C.c_int, _D, _I32, C.c_int, _D, _I32, _I32, _I32, _U8, _I32, _I32, _D]
     L.mk_hmesh_export.argtypes = [C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), _I32, _I32, _I32, _I32, _D, _D]
     L.mk_hmesh_halfedges.argtypes = [C.c_int, C.c_int, _D, _I32, _I32, _I32, _I32, _D, _D]
+    L.mk_field_solve.argtypes = [_P, C.c_int, C.c_int, _D, _I32, C.c_int, C.c_int, _D, C.c_int,
+                                 C.POINTER(SolveOpts), _D, C.POINTER(Stats)]
+    L.mk_curvature_dir.argtypes = [C.c_int, C.c_int, _D, _I32, _D]
+    L.mk_field_matching.argtypes = [C.c_int, C.c_int, _D, _I32, C.c_int, _D, C.c_int, _I32, _I32, _D]
     L.mk_mg_host_pcg.argtypes = [C.c_int, _I64, _I32, _D, C.c_int, _D, C.c_int, _D, _D, C.c_double, C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     for name in EXPORTED:
@@ -211,8 +217,13 @@ def mg_host_pcg(A, b, tol=1e-10, maxit=500, nullvec=None):
     x = np.zeros_like(bb)
     nnull, nv = 0, None
     if nullvec is not None:
-        nv = np.ascontiguousarray(np.asarray(nullvec, np.float64).reshape(A.shape[0], -1).T)
-        nnull = nv.shape[0]
+        z = np.asarray(nullvec).reshape(A.shape[0], -1)
+        nnull = z.shape[1]
+        if cplx:
+            z = z.astype(np.complex128).T  # column-major, complex interleaved
+            nv = np.ascontiguousarray(np.stack([z.real, z.imag], -1).reshape(-1))
+        else:
+            nv = np.ascontiguousarray(z.astype(np.float64).T)
     it, lv, rr = C.c_int(0), C.c_int(0), C.c_double(0)
     s = L.mk_mg_host_pcg(A.shape[0], _ptr(ptr, _I64), _ptr(col, _I32), _ptr(val, _D), int(cplx),
                          _ptr(nv, _D) if nnull else None, nnull, _ptr(bb, _D), _ptr(x, _D), tol, maxit,
@@ -247,6 +258,39 @@ def combinatorics(V, F, compressed, N=4):
     return {"singular": sing, "raw_matching": rawm, "matching": mat, "seam": seam.astype(bool), "cut_idx": cut,
             "cutV": int(sizes[0]), "nS": int(sizes[1]), "nT": int(sizes[2]), "constraint_rows": int(sizes[3]),
             "ext": ext}
+
+
+def _cplx_in(z):
+    return np.ascontiguousarray(np.stack([np.real(z), np.imag(z)], 1).reshape(-1))
+
+
+def curvature_dir(V, F):
+    """principalCurvatureDir of the library's Hmesh (host only): complex per face."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    D = np.zeros(2 * F.shape[0])
+    s = L.mk_curvature_dir(V.shape[0], F.shape[0], _ptr(V, _D), _ptr(F, _I32), _ptr(D, _D))
+    if s:
+        raise MetrikoError(s, "mk_curvature_dir")
+    return D[0::2] + 1j * D[1::2]
+
+
+def field_matching(V, F, compressed, N=4, kind="curl"):
+    """computeMatching(Principal | Curl) + computeSingular on a given compressed field (host only)."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    nE = hmesh_export(V, F)["nE"]
+    comp = _cplx_in(compressed)
+    sing = np.zeros(nV, np.int32)
+    mat = np.zeros(nE, np.int32)
+    eff = np.zeros(nE)
+    s = L.mk_field_matching(nV, nF, _ptr(V, _D), _ptr(F, _I32), N, _ptr(comp, _D),
+                            MATCH_CURL if kind == "curl" else MATCH_PRINCIPAL, _ptr(sing, _I32), _ptr(mat, _I32),
+                            _ptr(eff, _D))
+    if s:
+        raise MetrikoError(s, "mk_field_matching")
+    return {"singular": sing, "matching": mat, "effort": eff}
 
 
 class Context:
@@ -343,6 +387,19 @@ class Context:
         self._check(self.L.mk_field_smallest_eig(self.h, Lop.h, _ptr(mass, _D), _ptr(u, _D), n_iter,
                                                  C.byref(o or opts()), _ptr(x, _D), C.byref(st)),
                     "mk_field_smallest_eig")
+        return x[0::2] + 1j * x[1::2], struct_dict(st)
+
+    def field(self, V, F, N=4, kind="smoothest", u0=None, n_iter=50, o=None):
+        """FaceRosyField(mesh, N, Smoothest | CurvatureAligned) (mk_field_solve): compressed field."""
+        V, F = _mesh_arrays(V, F)
+        nF = F.shape[0]
+        u = None if u0 is None else _cplx_in(u0)
+        x = np.zeros(2 * nF)
+        st = Stats()
+        ft = FIELD_CURVATURE_ALIGNED if kind == "curvature" else FIELD_SMOOTHEST
+        self._check(self.L.mk_field_solve(self.h, V.shape[0], nF, _ptr(V, _D), _ptr(F, _I32), N, ft,
+                                          None if u is None else _ptr(u, _D), n_iter, C.byref(o or opts()),
+                                          _ptr(x, _D), C.byref(st)), "mk_field_solve")
         return x[0::2] + 1j * x[1::2], struct_dict(st)
 
     def pipeline(self, V, F, ratio=0.03, u0=None, n_iter=50, field_opts=None, igm_opts=None):

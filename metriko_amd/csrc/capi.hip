@@ -5,6 +5,7 @@
 #include <cstring>
 #include <numeric>
 #include "mk_internal.h"
+#include "sell.h"
 
 using mk::DevBuf;
 using mk::Error;
@@ -115,6 +116,133 @@ __global__ void k_dinv_from_diag(int n, int C, const int* col, const double* val
 }
 
 inline unsigned nblk(size_t n, int bs = 256) { return unsigned((n + bs - 1) / bs); }
+
+// ---- Galerkin start of the inverse iteration -------------------------------------------------
+// The window holds the last kWin iterates u (complex, 2n doubles each, column j at W + 2n j).
+constexpr int kWin = 4;
+constexpr int kWinDots = 4 * kWin;  // per column: w^H a (re, im), w^H b (re, im)
+
+// Per-block partial sums of w_j^H a and w_j^H b for the nw <= kWin window columns.
+__global__ void k_window_dots(int n, int nw, const double* __restrict__ W, const double* __restrict__ a,
+                              const double* __restrict__ b, double* __restrict__ part) {
+    double acc[kWinDots];
+#pragma unroll
+    for (int k = 0; k < kWinDots; k++) acc[k] = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+#pragma unroll
+        for (int j = 0; j < kWin; j++) {
+            if (j < nw) {
+                const double wr = W[size_t(j) * 2 * n + 2 * i], wi = W[size_t(j) * 2 * n + 2 * i + 1];
+                acc[4 * j] += wr * ar + wi * ai;
+                acc[4 * j + 1] += wr * ai - wi * ar;
+                acc[4 * j + 2] += wr * br + wi * bi;
+                acc[4 * j + 3] += wr * bi - wi * br;
+            }
+        }
+    }
+    mk::block_sum<kWinDots>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kWinDots; k++) part[size_t(blockIdx.x) * kWinDots + k] = acc[k];
+}
+// Fixed-order sum of the per-block partials (one thread per value).
+__global__ void k_window_sum(int nparts, const double* __restrict__ part, double* __restrict__ out) {
+    const int k = threadIdx.x;
+    if (k >= kWinDots) return;
+    double s = 0.0;
+    for (int p = 0; p < nparts; p++) s += part[size_t(p) * kWinDots + k];
+    out[k] = s;
+}
+// x = sum_j y_j w_j (complex coefficients y[2j], y[2j+1]).
+__global__ void k_window_combine(int n, int nw, const double* __restrict__ W, const double* __restrict__ y,
+                                 double* __restrict__ x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xr = 0.0, xi = 0.0;
+    for (int j = 0; j < nw; j++) {
+        const double wr = W[size_t(j) * 2 * n + 2 * i], wi = W[size_t(j) * 2 * n + 2 * i + 1];
+        xr += y[2 * j] * wr - y[2 * j + 1] * wi;
+        xi += y[2 * j] * wi + y[2 * j + 1] * wr;
+    }
+    x[2 * i] = xr;
+    x[2 * i + 1] = xi;
+}
+
+// Eigen-decomposition of a real symmetric m x m matrix (cyclic Jacobi); A is destroyed, V
+// receives the eigenvectors as columns, w the eigenvalues.
+void jacobi_eig(int m, std::vector<double>& A, std::vector<double>& V, std::vector<double>& w) {
+    V.assign(size_t(m) * m, 0.0);
+    for (int i = 0; i < m; i++) V[size_t(i) * m + i] = 1.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0.0, tot = 0.0;
+        for (int p = 0; p < m; p++)
+            for (int q = 0; q < m; q++) {
+                const double v = A[size_t(p) * m + q] * A[size_t(p) * m + q];
+                tot += v;
+                if (p != q) off += v;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < m; p++)
+            for (int q = p + 1; q < m; q++) {
+                const double apq = A[size_t(p) * m + q];
+                if (std::abs(apq) < 1e-300) continue;
+                const double th = (A[size_t(q) * m + q] - A[size_t(p) * m + p]) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::abs(th) + std::sqrt(th * th + 1.0));
+                const double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+                for (int k = 0; k < m; k++) {  // A P
+                    const double akp = A[size_t(k) * m + p], akq = A[size_t(k) * m + q];
+                    A[size_t(k) * m + p] = cs * akp - sn * akq;
+                    A[size_t(k) * m + q] = sn * akp + cs * akq;
+                }
+                for (int k = 0; k < m; k++) {  // P^T (A P)
+                    const double apk = A[size_t(p) * m + k], aqk = A[size_t(q) * m + k];
+                    A[size_t(p) * m + k] = cs * apk - sn * aqk;
+                    A[size_t(q) * m + k] = sn * apk + cs * aqk;
+                }
+                for (int k = 0; k < m; k++) {  // V P
+                    const double vkp = V[size_t(k) * m + p], vkq = V[size_t(k) * m + q];
+                    V[size_t(k) * m + p] = cs * vkp - sn * vkq;
+                    V[size_t(k) * m + q] = sn * vkp + cs * vkq;
+                }
+            }
+    }
+    w.resize(size_t(m));
+    for (int i = 0; i < m; i++) w[size_t(i)] = A[size_t(i) * m + i];
+}
+
+// y = G^+ g for the Hermitian positive semi-definite G (m x m, row-major), through its real
+// embedding [[Re G, -Im G], [Im G, Re G]]; eigenvalues below 1e-13 of the largest are
+// dropped (the window's columns become nearly parallel as the iteration converges).
+std::vector<std::complex<double>> herm_pinv_solve(int m, const std::vector<std::complex<double>>& G,
+                                                  const std::vector<std::complex<double>>& g) {
+    const int r = 2 * m;
+    std::vector<double> A(static_cast<size_t>(r) * r), V, w, rhs(static_cast<size_t>(r));
+    for (int i = 0; i < m; i++) {
+        for (int j = 0; j < m; j++) {
+            const std::complex<double> z = G[size_t(i) * m + j];
+            A[size_t(i) * r + j] = z.real();
+            A[size_t(i) * r + j + m] = -z.imag();
+            A[size_t(i + m) * r + j] = z.imag();
+            A[size_t(i + m) * r + j + m] = z.real();
+        }
+        rhs[size_t(i)] = g[size_t(i)].real();
+        rhs[size_t(i + m)] = g[size_t(i)].imag();
+    }
+    jacobi_eig(r, A, V, w);
+    double wmax = 0.0;
+    for (double v : w) wmax = std::max(wmax, v);
+    std::vector<double> y(static_cast<size_t>(r), 0.0);
+    for (int k = 0; k < r; k++) {
+        if (!(w[size_t(k)] > 1e-13 * wmax)) continue;
+        double c = 0.0;
+        for (int i = 0; i < r; i++) c += V[size_t(i) * r + k] * rhs[size_t(i)];
+        c /= w[size_t(k)];
+        for (int i = 0; i < r; i++) y[size_t(i)] += c * V[size_t(i) * r + k];
+    }
+    std::vector<std::complex<double>> out(static_cast<size_t>(m));
+    for (int i = 0; i < m; i++) out[size_t(i)] = {y[size_t(i)], y[size_t(i + m)]};
+    return out;
+}
 
 }  // namespace
 
@@ -605,29 +733,57 @@ mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, c
         mk::stats_reset(c);
         const int n = L->n;
         const size_t len = size_t(n) * 2;
-        DevBuf<double> dm, du, dx, db, dLu;
+        DevBuf<double> dm, du, dx, db, dW, dLW, dpart, dy;
         dm.upload(mass, size_t(n), c->stream);
         du.upload(u0, len, c->stream);
         dx.alloc(len);
         db.alloc(len);
-        dLu.alloc(len);
+        dW.alloc(len * kWin);
+        dLW.alloc(len * kWin);
+        const unsigned gw = mk::rgrid(size_t(n));
+        dpart.alloc(size_t(gw) * kWinDots + kWinDots);
+        dy.alloc(2 * kWin);
         double* sc = c->scal.p;
         int total_it = 0, total_ref = 0;
         double last_rel = 0.0;
-        MK_HIP(hipMemcpyAsync(dx.p, du.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        // Galerkin matrix G = W^H L W of the window (slot order), kept across iterations
+        std::vector<std::complex<double>> G(static_cast<size_t>(kWin) * kWin), g(static_cast<size_t>(kWin));
+        double hd[kWinDots], hy[2 * kWin];
         for (int it = 0; it < n_iter; it++) {
             // b = M u
             mk::mass_mul(c, n, dm.p, du.p, db.p);
-            // warm start x = u * (u^H M u) / (u^H L u)  (Rayleigh quotient of the current iterate)
-            mk::spmv64(c, L, du.p, dLu.p, nullptr, mk::SPMV_DOT, sc + 10, nullptr);
-            mk::mass_dot(c, n, dm.p, du.p, sc + 11);
-            double h[2];
-            MK_HIP(hipMemcpyAsync(h, sc + 10, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            // The iterate enters the window (slot s) with L u, then the start is the Galerkin
+            // solution on span(W): x0 = W (W^H L W)^+ W^H b.  With the window {u} alone this is
+            // the Rayleigh-quotient start u (u^H M u) / (u^H L u); the last kWin iterates
+            // span the slowly converging eigen-directions, so x0 starts PCG many digits closer.
+            const int s = it % kWin, nw = std::min(it + 1, kWin);
+            double* ws = dW.p + len * size_t(s);
+            double* lws = dLW.p + len * size_t(s);
+            MK_HIP(hipMemcpyAsync(ws, du.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            mk::spmv64(c, L, du.p, lws, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+            hipLaunchKernelGGL(k_window_dots, dim3(gw), dim3(256), 0, c->stream, n, nw, (const double*)dW.p,
+                               (const double*)lws, (const double*)db.p, dpart.p);
+            hipLaunchKernelGGL(k_window_sum, dim3(1), dim3(64), 0, c->stream, int(gw), (const double*)dpart.p,
+                               dpart.p + size_t(gw) * kWinDots);
+            MK_HIP(hipMemcpyAsync(hd, dpart.p + size_t(gw) * kWinDots, sizeof(hd), hipMemcpyDeviceToHost,
+                                  c->stream));
             MK_HIP(hipStreamSynchronize(c->stream));
-            const double theta = h[0] / h[1];
-            MK_CHECK(theta == theta && theta > 0.0, MK_ERR_NAN, "Rayleigh quotient not positive");
-            MK_HIP(hipMemcpyAsync(dx.p, du.p, len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-            mk::axpby64(c, len, 0.0, dx.p, 1.0 / theta, dx.p);
+            for (int j = 0; j < nw; j++) {
+                const std::complex<double> gjs(hd[4 * j], hd[4 * j + 1]);  // w_j^H L w_s
+                G[size_t(j) * kWin + s] = gjs;
+                G[size_t(s) * kWin + j] = std::conj(gjs);
+                g[size_t(j)] = {hd[4 * j + 2], hd[4 * j + 3]};            // w_j^H b
+            }
+            G[size_t(s) * kWin + s] = hd[4 * s];
+            MK_CHECK(hd[4 * s] == hd[4 * s] && hd[4 * s] > 0.0, MK_ERR_NAN, "u^H L u not positive");
+            std::vector<std::complex<double>> Gn(size_t(nw) * nw), gn(g.begin(), g.begin() + nw);
+            for (int i = 0; i < nw; i++)
+                for (int j = 0; j < nw; j++) Gn[size_t(i) * nw + j] = G[size_t(i) * kWin + j];
+            const std::vector<std::complex<double>> y = herm_pinv_solve(nw, Gn, gn);
+            for (int j = 0; j < nw; j++) { hy[2 * j] = y[size_t(j)].real(); hy[2 * j + 1] = y[size_t(j)].imag(); }
+            MK_HIP(hipMemcpyAsync(dy.p, hy, 2 * size_t(nw) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+            hipLaunchKernelGGL(k_window_combine, dim3(nblk(size_t(n))), dim3(256), 0, c->stream, n, nw,
+                               (const double*)dW.p, (const double*)dy.p, dx.p);
             mk::SolveResult r = mk::solve_spd(c, L, db.p, dx.p, o);
             total_it += r.iters;
             total_ref += r.refinements;

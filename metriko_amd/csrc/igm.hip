@@ -200,7 +200,7 @@ mk_op* make_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
     return op;
 }
 // rectangular operator (no multigrid, no diagonal): hybrid SELL upload of a host CSR
-mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
+mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o, double scale = 1.0) {
     auto* op = new mk_op();
     try {
         op->ctx = c;
@@ -210,7 +210,8 @@ mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
         op->nnz = A.nonZeros();
         op->block = o.block_size;
         const bool csr = o.format == MK_FMT_CSR;
-        mk::build_sell(op, A.ptr.data(), A.col.data(), A.val.data(), csr ? 64 : o.slice_height, csr ? 1 : o.sigma);
+        mk::build_sell(op, A.ptr.data(), A.col.data(), A.val.data(), csr ? 64 : o.slice_height, csr ? 1 : o.sigma,
+                       scale);
     } catch (...) {
         delete op;
         throw;
@@ -301,43 +302,60 @@ std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F
         if (cnt == 1 && US(r, nzj) == 1.0 && freeRow[nzj] < 0) freeRow[nzj] = r;
     }
     for (int j = 0; j < dimker; j++) if (freeRow[j] < 0) return {};
-    // S = [G2; Cfull] split into transition columns (GT) and the rest
-    const int srows = G2.rows() + Cf.rows();
-    std::vector<TripD> tt;
-    for (int i = 0; i < G2.rows(); i++)
-        for (int64_t q = G2.ptr[i]; q < G2.ptr[i + 1]; q++)
-            if (tpos[G2.col[q]] >= 0) tt.emplace_back(i, tpos[G2.col[q]], G2.val[q]);
-    for (int i = 0; i < Cf.rows(); i++)
-        for (int64_t q = Cf.ptr[i]; q < Cf.ptr[i + 1]; q++)
-            if (tpos[Cf.col[q]] >= 0) tt.emplace_back(G2.rows() + i, tpos[Cf.col[q]], Cf.val[q]);
-    SprsD GT(srows, std::max(nint, 1));
-    GT.setFromTriplets(tt);
-    const SprsD GTt = GT.transpose();
-    const SprsD M = GTt * GT;
-    std::vector<double> dg(size_t(M.rows()), 0.0);
-    for (int i = 0; i < M.rows(); i++)
-        for (int64_t q = M.ptr[i]; q < M.ptr[i + 1]; q++) if (M.col[q] == i) dg[i] = M.val[q];
+    // S = [G2; Cfull]: only its rows that touch a transition DOF matter for the transition
+    // solve; the normal matrix over the transition DOFs is small and dense
+    if (nint > 4096) return {};
+    struct SRow { const SprsD* A; int row; };
+    std::vector<SRow> trows;
+    for (const SprsD* A : {&G2, &Cf})
+        for (int i = 0; i < A->rows(); i++)
+            for (int64_t q = A->ptr[i]; q < A->ptr[i + 1]; q++)
+                if (tpos[A->col[q]] >= 0) { trows.push_back({A, i}); break; }
+    const size_t nt = size_t(std::max(nint, 1));
+    std::vector<double> Mt(nt * nt, 0.0);  // sum over those rows of the outer product of transition entries
+    for (const SRow& t : trows) {
+        const SprsD& A = *t.A;
+        for (int64_t p = A.ptr[t.row]; p < A.ptr[t.row + 1]; p++) {
+            const int a = tpos[A.col[p]];
+            if (a < 0) continue;
+            for (int64_t q = A.ptr[t.row]; q < A.ptr[t.row + 1]; q++) {
+                const int b = tpos[A.col[q]];
+                if (b >= 0) Mt[size_t(a) * nt + size_t(b)] += A.val[p] * A.val[q];
+            }
+        }
+    }
     std::vector<double> out(size_t(k) * n, 0.0);
     for (int c = 0; c < n; c++) {
         std::vector<double> y(size_t(nfull), 0.0);
         for (int i = 0; i < nfull; i++) if (tpos[i] < 0 && i % n == c) y[i] = 1.0;
         if (nint > 0) {
-            std::vector<double> g(static_cast<size_t>(srows)), b(static_cast<size_t>(nint)), x(static_cast<size_t>(nint), 0.0);
-            G2.multiply(y.data(), g.data());
-            if (Cf.rows() > 0) Cf.multiply(y.data(), g.data() + G2.rows());
-            GTt.multiply(g.data(), b.data());
-            for (auto& v : b) v = -v;
+            // b = -GT^T (S y), over the rows that touch a transition DOF
+            std::vector<double> b(static_cast<size_t>(nint), 0.0), x(static_cast<size_t>(nint), 0.0);
+            for (const SRow& t : trows) {
+                const SprsD& A = *t.A;
+                double sy = 0.0;
+                for (int64_t q = A.ptr[t.row]; q < A.ptr[t.row + 1]; q++) sy += A.val[q] * y[A.col[q]];
+                for (int64_t q = A.ptr[t.row]; q < A.ptr[t.row + 1]; q++)
+                    if (tpos[A.col[q]] >= 0) b[size_t(tpos[A.col[q]])] -= A.val[q] * sy;
+            }
             // Jacobi-CG on the (possibly semidefinite, consistent) normal equations
             std::vector<double> r(b), z(static_cast<size_t>(nint)), p, q(static_cast<size_t>(nint));
             auto prec = [&](const std::vector<double>& a, std::vector<double>& o) {
-                for (int i = 0; i < nint; i++) o[i] = dg[i] > 0.0 ? a[i] / dg[i] : 0.0;
+                for (int i = 0; i < nint; i++) {
+                    const double dgi = Mt[size_t(i) * nt + size_t(i)];
+                    o[i] = dgi > 0.0 ? a[i] / dgi : 0.0;
+                }
             };
             prec(r, z);
             p = z;
             double rz = 0.0, bb = 0.0;
             for (int i = 0; i < nint; i++) { rz += r[i] * z[i]; bb += b[i] * b[i]; }
             for (int it = 0; it < 20000 && bb > 0.0; it++) {
-                M.multiply(p.data(), q.data());
+                for (int i = 0; i < nint; i++) {
+                    double sq = 0.0;
+                    for (int j = 0; j < nint; j++) sq += Mt[size_t(i) * nt + size_t(j)] * p[j];
+                    q[i] = sq;
+                }
                 double pq = 0.0;
                 for (int i = 0; i < nint; i++) pq += p[i] * q[i];
                 if (!(pq > 0.0)) break;
@@ -357,18 +375,28 @@ std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F
         double* xs = out.data() + size_t(c) * k;
         for (int i = 0; i < npn; i++) xs[i] = y[nonPart[i]];
         for (int j = 0; j < dimker; j++) xs[npn + j] = y[uniqueJ[freeRow[j]]];
-        // null to round-off: ||X2F x||^2 against the same sum of |terms|
-        double e = 0.0, s = 0.0;
-        for (int i = 0; i < X2F.rows(); i++) {
-            double a = 0.0, m = 0.0;
-            for (int64_t q = X2F.ptr[i]; q < X2F.ptr[i + 1]; q++) {
-                a += X2F.val[q] * xs[X2F.col[q]];
-                m += std::abs(X2F.val[q] * xs[X2F.col[q]]);
+        // null to round-off: ||X2F x||^2 against the same sum of |terms| (64 fixed row chunks)
+        constexpr int kCh = 64;
+        double pe[kCh], ps[kCh];
+        const int rows = X2F.rows(), step = (rows + kCh - 1) / kCh;
+#pragma omp parallel for schedule(static, 1)
+        for (int ch = 0; ch < kCh; ch++) {
+            double e = 0.0, m2 = 0.0;
+            for (int i = ch * step; i < std::min(rows, (ch + 1) * step); i++) {
+                double a = 0.0, m = 0.0;
+                for (int64_t q = X2F.ptr[i]; q < X2F.ptr[i + 1]; q++) {
+                    a += X2F.val[q] * xs[X2F.col[q]];
+                    m += std::abs(X2F.val[q] * xs[X2F.col[q]]);
+                }
+                e += a * a;
+                m2 += m * m;
             }
-            e += a * a;
-            s += m * m;
+            pe[ch] = e;
+            ps[ch] = m2;
         }
-        if (!(s > 0.0) || !(e <= 1e-20 * s)) return {};
+        double e = 0.0, s2 = 0.0;
+        for (int ch = 0; ch < kCh; ch++) { e += pe[ch]; s2 += ps[ch]; }
+        if (!(s2 > 0.0) || !(e <= 1e-20 * s2)) return {};
     }
     return out;
 }
@@ -468,6 +496,7 @@ extern "C" mk_status mk_iterative_rounding(
         // f lies in range(E) and C only pins the translations E annihilates, so the
         // multiplier vanishes and x solves the SPD system (E + w C^T C) x = f + w C^T g.
         double t0 = now_ms();
+        MK_LOG("[igm] KKT: start\n");
         double dmean = 0.0;
         for (int i = 0; i < k; i++)
             for (int64_t q = E.ptr[i]; q < E.ptr[i + 1]; q++) if (E.col[q] == i) dmean += E.val[q];
@@ -503,8 +532,11 @@ extern "C" mk_status mk_iterative_rounding(
             mk::ensure_mg(c, hA0.op, o);
             MK_LOG("[igm] A0 multigrid %.0f ms\n", now_ms() - t_all);
             S.setup_ms += now_ms() - ts;
+            t0 += now_ms() - ts;  // the KKT time excludes the A0 operator and hierarchy set-up
         }
+        MK_LOG("[igm] KKT: Eaug assembled %.0f ms\n", now_ms() - t0);
         hE.op = make_op(c, Eaug, o);
+        MK_LOG("[igm] KKT: Eaug uploaded %.0f ms\n", now_ms() - t0);
         hE.op->label = "igm_Eaug";
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
         drhs.upload(rhs.data(), rhs.size(), K.st());
@@ -603,12 +635,9 @@ extern "C" mk_status mk_iterative_rounding(
             for (auto& v : bint) v *= length * wI * wI;
             // device operator length * X2F (rectangular, hybrid SELL) for the energies
             OpHolder hX2F;
-            {
-                SprsD Xs = X2F;
-                for (auto& v : Xs.val) v *= length;
-                hX2F.op = make_rect_op(c, Xs, o);
-                hX2F.op->label = "igm_X2F";
-            }
+            hX2F.op = make_rect_op(c, X2F, o, length);
+            hX2F.op->label = "igm_X2F";
+            MK_LOG("[igm] rounding set-up (b_int, X2F operator) %.0f ms\n", now_ms() - t0);
             DevBuf<double> dfield(l1), dbint, dxcur(static_cast<size_t>(k)), dxprev(static_cast<size_t>(k)), dxt(static_cast<size_t>(k)), dxstar(static_cast<size_t>(k)),
                 dxlm(static_cast<size_t>(k)), db0(static_cast<size_t>(k)), drow(static_cast<size_t>(k)), dcand, dfixv, dcoef;
             dbint.upload(bint.data(), bint.size(), K.st());
@@ -642,7 +671,9 @@ extern "C" mk_status mk_iterative_rounding(
                 if (!flags->round_seams) for (int i : integer) mark[size_t(candPos[i])] = 1;
                 for (size_t p = 0; p < nc_; p++) if (mark[p]) need.push_back(int(p));
                 // blocks of at most 64 columns, each as narrow as its column count allows
-                const size_t mwmax = need.size() > 32 ? 64 : need.size() > 16 ? 32 : need.size() > 8 ? 16 : 8;
+                size_t mwmax = need.size() > 32 ? 64 : need.size() > 16 ? 32 : need.size() > 8 ? 16 : 8;
+                if (const char* e = std::getenv("METRIKO_MAX_BLOCK"))
+                    mwmax = std::min<size_t>(mwmax, size_t(std::max(8, std::atoi(e))));
                 DevBuf<double> Bp(size_t(k) * mwmax), Xp(size_t(k) * mwmax);
                 DevBuf<int> dsel(mwmax);
                 for (size_t b0 = 0, cnt_ = 0; b0 < need.size(); b0 += cnt_) {
@@ -675,6 +706,7 @@ extern "C" mk_status mk_iterative_rounding(
                                    cand.col.p, cand.val.p, (const double*)Zall.p, duz.p);
                 duz.download(UZall.data(), UZall.size(), K.st());
                 K.sync();
+                MK_LOG("[igm] batched Woodbury columns done %.0f ms\n", now_ms() - t0);
             }
             DevBuf<int> dzidx;
             std::vector<double> Xc(candIdx.size());

@@ -299,57 +299,99 @@ __global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict
     }
 }
 
-// rz_new = r.z32, rz_mix = r.zold ; beta = (rz_new - rz_mix) / rz
-__global__ void __launch_bounds__(256) cgm_beta(size_t len, const double* __restrict__ r,
-                                              const float* __restrict__ z32, const double* __restrict__ zold,
-                                              CgState* st, double* partials, unsigned* counter) {
+// beta of the flexible recurrence: a = r.z32, m = r.zold; beta = (a - m) / rz (0 on the first
+// iteration); rz = a
+__global__ void __launch_bounds__(256) cgm_beta2(size_t len, const double* __restrict__ r,
+                                               const float* __restrict__ z32, const double* __restrict__ zold,
+                                               CgState* st, double* partials, unsigned* counter) {
     if (st->done) return;
-    double a = 0.0, b = 0.0;
+    double a = 0.0, m = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
         a += r[i] * double(z32[i]);
-        b += r[i] * zold[i];
+        m += r[i] * zold[i];
     }
-    double v[2] = {a, b}, tot[2];
+    double v[2] = {a, m}, tot[2];
     if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
         if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
-        else { st->beta = (tot[0] - tot[1]) / st->rz; st->rz = tot[0]; }
+        else {
+            st->beta = st->first ? 0.0 : (tot[0] - tot[1]) / st->rz;
+            st->rz = tot[0];
+            st->first = 0;
+        }
     }
 }
 
-// p = z32 + beta p ; zold = z32
-__global__ void __launch_bounds__(256) cgm_pupdate(size_t len, double* __restrict__ p, const float* __restrict__ z32,
-                                                 double* __restrict__ zold, const CgState* st) {
+// p = z + beta p ; q = A z + beta q (q tracks A p without a second product) ; zold = z ;
+// pq = p^H q ; alpha = rz / pq.  z is the FP32 preconditioned residual, A and p, q FP64.
+template <bool CPLX>
+struct Fp32Gather {
+    const float* __restrict__ z;
+    __device__ __forceinline__ void operator()(int c, double& xr, double& xi) const {
+        if constexpr (CPLX) { xr = double(z[2 * c]); xi = double(z[2 * c + 1]); }
+        else { xr = double(z[c]); xi = 0.0; }
+    }
+};
+template <bool CPLX>
+__global__ void __launch_bounds__(256) cgm_spmv(SellDev<double> A, const float* __restrict__ z32,
+                                              double* __restrict__ p, double* __restrict__ q,
+                                              double* __restrict__ zold, CgState* st, double* partials,
+                                              unsigned* counter) {
     if (st->done) return;
     const double beta = st->beta;
-    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < len) {
-        const double z = double(z32[i]);
-        p[i] = z + beta * p[i];
-        zold[i] = z;
+    double contrib = 0.0;
+    for (int vb = blockIdx.x; vb < A.nvb; vb += gridDim.x) {
+        int row = 0;
+        double ar, ai;
+        if (!spmv_row_g<double, CPLX>(A, Fp32Gather<CPLX>{z32}, vb, row, ar, ai)) continue;
+        if constexpr (CPLX) {
+            const double z0 = double(z32[2 * row]), z1 = double(z32[2 * row + 1]);
+            const double p0 = z0 + beta * p[2 * row], p1 = z1 + beta * p[2 * row + 1];
+            const double q0 = ar + beta * q[2 * row], q1 = ai + beta * q[2 * row + 1];
+            p[2 * row] = p0; p[2 * row + 1] = p1;
+            q[2 * row] = q0; q[2 * row + 1] = q1;
+            zold[2 * row] = z0; zold[2 * row + 1] = z1;
+            contrib += p0 * q0 + p1 * q1;
+        } else {
+            const double z0 = double(z32[row]);
+            const double p0 = z0 + beta * p[row], q0 = ar + beta * q[row];
+            p[row] = p0;
+            q[row] = q0;
+            zold[row] = z0;
+            contrib += p0 * q0;
+        }
+    }
+    double v[1] = {contrib}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->pq = tot[0];
+        if (!(tot[0] > 0.0)) {
+            st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
+            st->done = 1;
+        } else {
+            st->alpha = st->rz / tot[0];
+        }
     }
 }
 
-// p = zold = z32 ; rz = r.z32 ; rr = r.r
-__global__ void __launch_bounds__(256) cgm_init(size_t len, const double* __restrict__ r, const float* __restrict__ z32,
-                                              double* __restrict__ p, double* __restrict__ zold, CgState* st,
-                                              double* partials, unsigned* counter) {
-    double rz = 0.0, rr = 0.0;
+// p = q = zold = 0 ; rr = r.r ; state for the first iteration
+__global__ void __launch_bounds__(256) cgm_init2(size_t len, const double* __restrict__ r, double* __restrict__ p,
+                                               double* __restrict__ q, double* __restrict__ zold, CgState* st,
+                                               double* partials, unsigned* counter) {
+    double rr = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
-        const double z = double(z32[i]);
-        p[i] = z;
-        zold[i] = z;
-        rz += r[i] * z;
+        p[i] = 0.0;
+        q[i] = 0.0;
+        zold[i] = 0.0;
         rr += r[i] * r[i];
     }
-    double v[2] = {rz, rr}, tot[2];
-    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->rz = tot[0];
-        st->rr = tot[1];
+    double v[1] = {rr}, tot[1];
+    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+        st->rr = tot[0];
+        st->rz = 1.0;
         st->it = 0;
+        st->first = 1;
         st->status = MK_OK;
-        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
-        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
-        else if (!st->done && !(tot[0] > 0.0)) { st->status = MK_ERR_NOT_SPD; st->done = 1; }
+        st->done = (tot[0] <= st->tol2 * st->bb) ? 1 : 0;
+        if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
     }
 }
 
@@ -410,7 +452,7 @@ __global__ void k_set_state_dev(CgState* st, const double* bb, double tol2) {
 // host side
 
 
-void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma) {
+void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma, double scale) {
     const int n = op->n;
     const int vw = op->cplx ? 2 : 1;
     MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
@@ -440,6 +482,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->C = C;
     op->sigma = sigma;
     if (sigma > 1) {
+#pragma omp parallel for schedule(dynamic, 16) if (ns_rows > 100000)
         for (int w0 = 0; w0 < ns_rows; w0 += sigma) {
             const int w1 = std::min(ns_rows, w0 + sigma);
             std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) { return lens[a] > lens[b]; });
@@ -473,8 +516,8 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
                     const int64_t k = rowptr[row] + j;
                     hcol[e] = col[k];
                     for (int c = 0; c < vw; c++) {
-                        hval[e * vw + c] = val[k * vw + c];
-                        hval32[e * vw + c] = float(val[k * vw + c]);
+                        hval[e * vw + c] = scale * val[k * vw + c];
+                        hval32[e * vw + c] = float(scale * val[k * vw + c]);
                     }
                 } else {
                     hcol[e] = (row >= 0 && row < op->m) ? row : 0;  // padding: valid column, zero value
@@ -489,7 +532,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     for (int i = 0; i < n; i++) {
         if (op->m != n) continue;
         double d = 0.0;
-        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += val[k * vw];
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += scale * val[k * vw];
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
         dinv32[i] = float(dinv[i]);
     }
@@ -511,7 +554,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         const int row = mrows[w];
         for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
             mcol.push_back(col[k]);
-            for (int c = 0; c < vw; c++) { mval.push_back(val[k * vw + c]); mval32.push_back(float(val[k * vw + c])); }
+            for (int c = 0; c < vw; c++) { mval.push_back(scale * val[k * vw + c]); mval32.push_back(float(scale * val[k * vw + c])); }
         }
         mptr[w + 1] = int64_t(mcol.size());
     }
@@ -525,7 +568,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         const int row = lrows[w];
         for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
             lcol.push_back(col[k]);
-            for (int c = 0; c < vw; c++) { lval.push_back(val[k * vw + c]); lval32.push_back(float(val[k * vw + c])); }
+            for (int c = 0; c < vw; c++) { lval.push_back(scale * val[k * vw + c]); lval32.push_back(float(scale * val[k * vw + c])); }
         }
         lptr[w + 1] = int64_t(lcol.size());
     }
@@ -648,6 +691,110 @@ template void cheb_launch<double>(mk_ctx*, const mk_op*, const double*, double*,
                                   double, int, const int*);
 template void cheb_launch<float>(mk_ctx*, const mk_op*, const float*, float*, const float*, float*, float, float,
                                  int, const int*);
+
+// First two Chebyshev steps from a zero start in one launch: x1 = c0 D^-1 b is formed on the
+// fly inside the product (x1 never stored), then r = b - A x1, d = a x1 + s D^-1 r,
+// x_out = x1 + d.
+template <typename T, bool CPLX>
+struct ScaledGather {
+    const T* __restrict__ b;
+    const T* __restrict__ dinv;
+    T c0;
+    __device__ __forceinline__ void operator()(int c, T& xr, T& xi) const {
+        const T f = c0 * dinv[c];
+        if constexpr (CPLX) { xr = f * b[2 * c]; xi = f * b[2 * c + 1]; }
+        else { xr = f * b[c]; xi = T(0); }
+    }
+};
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) sell_cheb0(SellDev<T> A, const T* __restrict__ b, const T* __restrict__ dinv,
+                                                T* __restrict__ x_out, T* __restrict__ d, T c0, T a, T s,
+                                                const int* skip) {
+    if (skip && *skip) return;
+    int row = 0;
+    T ar, ai;
+    if (!spmv_row_g<T, CPLX>(A, ScaledGather<T, CPLX>{b, dinv, c0}, blockIdx.x, row, ar, ai)) return;
+    const T di = dinv[row], f = c0 * di, sd = s * di;
+    if constexpr (CPLX) {
+        const T x0 = f * b[2 * row], x1 = f * b[2 * row + 1];
+        const T d0 = a * x0 + sd * (b[2 * row] - ar), d1 = a * x1 + sd * (b[2 * row + 1] - ai);
+        d[2 * row] = d0;
+        d[2 * row + 1] = d1;
+        x_out[2 * row] = x0 + d0;
+        x_out[2 * row + 1] = x1 + d1;
+    } else {
+        const T x0 = f * b[row];
+        const T d0 = a * x0 + sd * (b[row] - ar);
+        d[row] = d0;
+        x_out[row] = x0 + d0;
+    }
+}
+
+template <typename T>
+void cheb0_launch(mk_ctx* c, const mk_op* A, const T* b, T* x_out, T* d, T c0, T a, T s, const int* skip) {
+    unsigned g = 0;
+    const T* val;
+    const T* dinv;
+    if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
+    else { val = A->val32.p; dinv = A->dinv32.p; }
+    const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
+    const double vb = double(sizeof(T)) * (A->cplx ? 2 : 1);
+    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) + double(A->n) * (vb + double(sizeof(T)));
+    SpmvTimer tm(c, bytes, A, "cheb0", sizeof(T) == 4);
+    if (A->cplx)
+        hipLaunchKernelGGL((sell_cheb0<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0, a,
+                           s, skip);
+    else
+        hipLaunchKernelGGL((sell_cheb0<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0, a,
+                           s, skip);
+    MK_HIP(hipGetLastError());
+}
+template void cheb0_launch<double>(mk_ctx*, const mk_op*, const double*, double*, double*, double, double, double,
+                                  const int*);
+template void cheb0_launch<float>(mk_ctx*, const mk_op*, const float*, float*, float*, float, float, float,
+                                 const int*);
+
+// y = M [u; v] for an operator whose columns [0, m1) act on u and [m1, m) on v (the
+// multigrid restriction of a residual: M = [R | -R A], y = R (b - A x) in one launch)
+template <typename T, bool CPLX>
+struct TwoSourceGather {
+    const T* __restrict__ u;
+    const T* __restrict__ v;
+    int m1;
+    __device__ __forceinline__ void operator()(int c, T& xr, T& xi) const {
+        const T* s = c < m1 ? u : v;
+        const int k = c < m1 ? c : c - m1;
+        if constexpr (CPLX) { xr = s[2 * k]; xi = s[2 * k + 1]; }
+        else { xr = s[k]; xi = T(0); }
+    }
+};
+template <typename T, bool CPLX>
+__global__ void __launch_bounds__(256) sell_spmv2(SellDev<T> A, int m1, const T* __restrict__ u, const T* __restrict__ v,
+                                                T* __restrict__ y, const int* skip) {
+    if (skip && *skip) return;
+    int row = 0;
+    T ar, ai;
+    if (!spmv_row_g<T, CPLX>(A, TwoSourceGather<T, CPLX>{u, v, m1}, blockIdx.x, row, ar, ai)) return;
+    if constexpr (CPLX) { y[2 * row] = ar; y[2 * row + 1] = ai; }
+    else y[row] = ar;
+}
+
+template <typename T>
+void spmv2_launch(mk_ctx* c, const mk_op* A, int m1, const T* u, const T* v, T* y, const int* skip) {
+    unsigned g = 0;
+    const T* val;
+    if constexpr (std::is_same<T, double>::value) val = A->val64.p;
+    else val = A->val32.p;
+    const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
+    SpmvTimer tm(c, sizeof(T) == 8 ? A->bytes64 : A->bytes32, A, "restrict_resid", sizeof(T) == 4);
+    if (A->cplx)
+        hipLaunchKernelGGL((sell_spmv2<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+    else
+        hipLaunchKernelGGL((sell_spmv2<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+    MK_HIP(hipGetLastError());
+}
+template void spmv2_launch<double>(mk_ctx*, const mk_op*, int, const double*, const double*, double*, const int*);
+template void spmv2_launch<float>(mk_ctx*, const mk_op*, int, const float*, const float*, float*, const int*);
 
 template <typename T>
 __global__ void k_copy(size_t len, const T* __restrict__ x, T* __restrict__ y, const int* skip) {
@@ -915,10 +1062,30 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
     CgState* st = c->state.p;
     hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, (const double*)bb, rel_tol * rel_tol);
     hipLaunchKernelGGL(k_to32, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.r32.p);
-    mg_apply32(c, mg, w.r32.p, w.z32.p, nullptr);
-    hipLaunchKernelGGL(cgm_init, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p, (const float*)w.z32.p,
-                       w.p.p, w.z.p, st, c->partials.p, c->counter.p);
+    hipLaunchKernelGGL(cgm_init2, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.p.p, w.q.p,
+                       w.z.p, st, c->partials.p, c->counter.p);
+    mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
     MK_HIP(hipGetLastError());
+    unsigned gs = 0;
+    const SellDev<double> D = sell_dev<double>(A, A->val64.p, A->block, &gs);
+    const unsigned gsr = std::min(gs, kRedGrid);
+    // one iteration: beta, p/q update with A z (fused), x/r update, V-cycle on the new residual
+    auto iteration = [&] {
+        hipLaunchKernelGGL(cgm_beta2, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p,
+                           (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
+        {
+            SpmvTimer tm(c, A->bytes64 + double(len) * (4.0 + 8.0 * 5.0), A, "cg_spmv", false);
+            if (A->cplx)
+                hipLaunchKernelGGL((cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
+                                   w.p.p, w.q.p, w.z.p, st, c->partials.p, c->counter.p);
+            else
+                hipLaunchKernelGGL((cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
+                                   w.p.p, w.q.p, w.z.p, st, c->partials.p, c->counter.p);
+        }
+        hipLaunchKernelGGL(cgm_update, dim3(rgrid(len)), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
+                           (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
+        mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
+    };
     CgState h{};
     for (;;) {
         MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
@@ -926,18 +1093,9 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         timing_flush(c);
         if (h.done || h.it >= maxiter) break;
         const int batch = std::min(check_every, maxiter - h.it);
-        run_batch(c, A, {uintptr_t(3), uintptr_t(mg), uintptr_t(x), uintptr_t(w.r.p), uintptr_t(w.r32.p),
+        run_batch(c, A, {uintptr_t(4), uintptr_t(mg), uintptr_t(x), uintptr_t(w.r.p), uintptr_t(w.r32.p),
                          uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p), uintptr_t(w.q.p)},
-                  batch, batch == check_every, [&] {
-            launch_cg_spmv<double>(c, A, A->val64.p, w.p.p, w.q.p, A->bytes64);
-            hipLaunchKernelGGL(cgm_update, dim3(rgrid(len)), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
-                               (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
-            mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
-            hipLaunchKernelGGL(cgm_beta, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p,
-                               (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
-            hipLaunchKernelGGL(cgm_pupdate, dim3(g), dim3(256), 0, c->stream, len, w.p.p, (const float*)w.z32.p,
-                               w.z.p, (const CgState*)st);
-        });
+                  batch, batch == check_every, iteration);
         MK_HIP(hipGetLastError());
     }
     CgResult res{h.status, h.it, h.rr, h.bb};
@@ -956,21 +1114,28 @@ SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, con
     // FP32 multigrid inside FP64 PCG; without multigrid there is no FP32 work to do: plain FP64
     const bool mg32 = o.precision == MK_PREC_MIXED_MG && mg;
     if (o.precision == MK_PREC_FP64 || o.precision == MK_PREC_MIXED_MG) {
-        CgResult cr = mg32 ? cg64_mg32(c, A, b, x, o.tol, o.maxiter, o.check_every, mg)
-                           : cg64(c, A, b, x, o.tol, o.maxiter, jac, o.check_every, false, mg);
-        res.iters = cr.iters;
-        // true FP64 residual
-        Work& w = work(c);
-        double h[2];
-        w.t.ensure(len);
-        spmv64(c, A, x, w.t.p, b, SPMV_RESID, c->scal.p + 3, nullptr);
-        MK_HIP(hipMemcpyAsync(h, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        double rr;
-        MK_HIP(hipMemcpyAsync(&rr, c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        MK_HIP(hipStreamSynchronize(c->stream));
-        res.rel_residual = h[0] > 0 ? std::sqrt(rr / h[0]) : std::sqrt(rr);
-        res.status = cr.status;
-        if (res.status == MK_ERR_NO_CONVERGENCE && res.rel_residual <= o.tol) res.status = MK_OK;
+        // the recurrences can drift from the true residual: it is recomputed in FP64 and the
+        // solve restarts from x (at most twice) while it is above the tolerance
+        for (int attempt = 0;; attempt++) {
+            const CgResult cr = mg32 ? cg64_mg32(c, A, b, x, o.tol, o.maxiter - res.iters, o.check_every, mg)
+                                     : cg64(c, A, b, x, o.tol, o.maxiter - res.iters, jac, o.check_every, false, mg);
+            res.iters += cr.iters;
+            Work& w = work(c);
+            double h[2];
+            w.t.ensure(len);
+            spmv64(c, A, x, w.t.p, b, SPMV_RESID, c->scal.p + 3, nullptr);
+            MK_HIP(hipMemcpyAsync(h, c->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            double rr;
+            MK_HIP(hipMemcpyAsync(&rr, c->scal.p + 3, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            MK_HIP(hipStreamSynchronize(c->stream));
+            res.rel_residual = h[0] > 0 ? std::sqrt(rr / h[0]) : std::sqrt(rr);
+            res.status = cr.status;
+            if (res.status == MK_ERR_NO_CONVERGENCE && res.rel_residual <= o.tol) res.status = MK_OK;
+            if (!(res.status == MK_OK && res.rel_residual > o.tol && attempt < 2 && res.iters < o.maxiter))
+                break;
+            res.refinements++;
+        }
+        if (res.status == MK_OK && res.rel_residual > o.tol) res.status = MK_ERR_NO_CONVERGENCE;
         return res;
     }
     Work& w = work(c);

@@ -30,6 +30,7 @@ struct MgLevel {
     mk_op* A = nullptr;
     mk_op* P = nullptr;  // n_l x n_{l+1}
     mk_op* R = nullptr;  // n_{l+1} x n_l
+    mk_op* RR = nullptr; // n_{l+1} x 2 n_l: [R | -R A] (restriction of the residual in one product)
     double lmax = 2.0;
     int n = 0;
     DevBuf<double> b64, x64, r64, d64, y64;  // y: second iterate buffer of the fused smoother
@@ -54,6 +55,7 @@ void mg_free(Mg* g) {
         delete L.A;
         delete L.P;
         delete L.R;
+        delete L.RR;
     }
     delete g;
 }
@@ -62,7 +64,7 @@ void mg_free(Mg* g) {
 struct MgTune {
     double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 10.0, lmax_fac = 1.2;
     int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1, compat = 1,
-        l1 = 1, attach = 1;
+        l1 = 1, attach = 1, rr = 0;
 };
 static const MgTune& tune() {
     static const MgTune t = [] {
@@ -82,6 +84,7 @@ static const MgTune& tune() {
         i("METRIKO_MG_COMPAT", x.compat);
         i("METRIKO_MG_L1", x.l1);
         i("METRIKO_MG_ATTACH", x.attach);
+        i("METRIKO_MG_RR", x.rr);
         return x;
     }();
     return t;
@@ -459,7 +462,7 @@ mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o) {
 // Chebyshev upper bound per level and the dense inverse of the coarsest operator.
 template <typename T>
 struct HostHier {
-    std::vector<Csr<T>> A, P, R;
+    std::vector<Csr<T>> A, P, R, RR;
     std::vector<double> lmax;
     std::vector<std::vector<double>> dinv;
     std::vector<double> inv;  // coarsest dense inverse (row-major, complex interleaved)
@@ -598,8 +601,31 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             lap("smoothed P");
             Csr<T> R = conj_transpose(P);
             lap("R");
-            Csr<T> Ac = metriko::spgemm(R, metriko::spgemm(A, P));
+            Csr<T> AP2 = metriko::spgemm(A, P);
+            Csr<T> Ac = metriko::spgemm(R, AP2);
             lap("R A P");
+            // [R | -R A] with R A = (A P)^H (A Hermitian): the residual restriction operator,
+            // built on request only (its transpose costs more host time than the fused
+            // restriction saves over the 50 inverse-iteration solves)
+            if (tune().rr) {
+                Csr<T> RA = conj_transpose(AP2);
+                Csr<T> RR(R.nr, 2 * A.nc);
+                for (int i = 0; i < R.nr; i++)
+                    RR.ptr[size_t(i) + 1] = RR.ptr[size_t(i)] + (R.ptr[i + 1] - R.ptr[i]) + (RA.ptr[i + 1] - RA.ptr[i]);
+                RR.col.resize(size_t(RR.ptr[size_t(R.nr)]));
+                RR.val.resize(size_t(RR.ptr[size_t(R.nr)]));
+#pragma omp parallel for schedule(static, 1024)
+                for (int i = 0; i < R.nr; i++) {
+                    int64_t w = RR.ptr[size_t(i)];
+                    for (int64_t k = R.ptr[i]; k < R.ptr[i + 1]; k++, w++) { RR.col[size_t(w)] = R.col[k]; RR.val[size_t(w)] = R.val[k]; }
+                    for (int64_t k = RA.ptr[i]; k < RA.ptr[i + 1]; k++, w++) {
+                        RR.col[size_t(w)] = A.nc + RA.col[k];
+                        RR.val[size_t(w)] = -RA.val[k];
+                    }
+                }
+                H.RR.push_back(std::move(RR));
+                lap("residual restriction");
+            }
             // exact Hermitian symmetry of the coarse operator
             Csr<T> AcH = conj_transpose(Ac);
             Ac = metriko::add(Ac, T(0.5), AcH, T(0.5));
@@ -635,6 +661,7 @@ Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta
             if (l < H.P.size()) {
                 L.P = upload(c, H.P[l], o);
                 L.R = upload(c, H.R[l], o);
+                if (l < H.RR.size()) L.RR = upload(c, H.RR[l], o);
             }
             g->lv.push_back(std::move(L));
         }
@@ -676,27 +703,34 @@ __global__ void k_cheb_first(int n, const T* __restrict__ dinv, const T* __restr
         else x[i] += d0;
     }
 }
-// x = Inv b (dense n x n, row-major), one thread per row
+// x = Inv b (dense n x n, row-major): one wavefront per row, lanes stride the row (coalesced
+// reads of Inv), shuffle reduction; 4 rows per 256-thread block
 template <typename T, bool CPLX>
-__global__ void k_dense_gemv(int n, const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x,
-                             const int* skip) {
+__global__ void __launch_bounds__(256) k_dense_gemv(int n, const T* __restrict__ inv, const T* __restrict__ b,
+                                                    T* __restrict__ x, const int* skip) {
     if (skip && *skip) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if constexpr (CPLX) {
-        T ar = 0, ai = 0;
-        for (int j = 0; j < n; j++) {
+    const int i = blockIdx.x * 4 + int(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;  // uniform inside the wavefront
+    T ar = 0, ai = 0;
+    for (int j = lane; j < n; j += 64) {
+        if constexpr (CPLX) {
             const T mr = inv[2 * (size_t(i) * n + j)], mi = inv[2 * (size_t(i) * n + j) + 1];
             const T br = b[2 * j], bi = b[2 * j + 1];
             ar += mr * br - mi * bi;
             ai += mr * bi + mi * br;
+        } else {
+            ar += inv[size_t(i) * n + j] * b[j];
         }
-        x[2 * i] = ar;
-        x[2 * i + 1] = ai;
-    } else {
-        T a = 0;
-        for (int j = 0; j < n; j++) a += inv[size_t(i) * n + j] * b[j];
-        x[i] = a;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        ar += __shfl_xor(ar, off, 64);
+        if constexpr (CPLX) ai += __shfl_xor(ai, off, 64);
+    }
+    if (lane == 0) {
+        if constexpr (CPLX) { x[2 * i] = ar; x[2 * i + 1] = ai; }
+        else x[i] = ar;
     }
 }
 
@@ -741,9 +775,17 @@ void chebyshev(mk_ctx* c, const Mg* g, MgLevel& L, const T* b, T* x, bool zero_s
     T* y = Sel<T>::y(L);
     T* d = Sel<T>::d(L);
     const int n = L.n;
-    const int steps = zero_start ? g->deg - 1 : g->deg;  // fused residual steps
+    // fused residual steps after the start; a zero start of degree >= 2 does its first two
+    // steps in one launch (cheb0) and continues from there
+    int steps = zero_start ? g->deg - 1 : g->deg;
     T* cur = x;
-    if (zero_start) {
+    if (zero_start && g->deg >= 2) {
+        steps = g->deg - 2;
+        cur = (steps % 2 == 0) ? x : y;
+        const double rho1 = 1.0 / (2.0 * sigma - rho0);
+        cheb0_launch<T>(c, L.A, b, cur, d, T(1.0 / theta), T(rho1 * rho0), T(2.0 * rho1 / delta), skip);
+        rho0 = rho1;
+    } else if (zero_start) {
         cur = (steps % 2 == 0) ? x : y;
         if (g->cplx)
             hipLaunchKernelGGL((k_cheb_first<T, true>), dim3(nb(n)), dim3(256), 0, c->stream, n, Sel<T>::dinv(L.A), b,
@@ -756,7 +798,7 @@ void chebyshev(mk_ctx* c, const Mg* g, MgLevel& L, const T* b, T* x, bool zero_s
         T* nxt = (cur == x) ? y : x;
         double a = 0.0, sc;
         int first = 0;
-        if (!zero_start && k == 0) {
+        if (!zero_start && k == 0) {  // first step of a warm start: d = D^-1 r / theta
             first = 1;
             sc = 1.0 / theta;
         } else {
@@ -776,18 +818,23 @@ void vcycle(mk_ctx* c, Mg* g, int l, const T* b, T* x, const int* skip) {
     MgLevel& L = g->lv[size_t(l)];
     if (l + 1 == int(g->lv.size())) {
         if (g->cplx)
-            hipLaunchKernelGGL((k_dense_gemv<T, true>), dim3(nb(g->nc)), dim3(256), 0, c->stream, g->nc,
+            hipLaunchKernelGGL((k_dense_gemv<T, true>), dim3(unsigned((g->nc + 3) / 4)), dim3(256), 0, c->stream, g->nc,
                                Sel<T>::inv(g), b, x, skip);
         else
-            hipLaunchKernelGGL((k_dense_gemv<T, false>), dim3(nb(g->nc)), dim3(256), 0, c->stream, g->nc,
+            hipLaunchKernelGGL((k_dense_gemv<T, false>), dim3(unsigned((g->nc + 3) / 4)), dim3(256), 0, c->stream, g->nc,
                                Sel<T>::inv(g), b, x, skip);
         return;
     }
     chebyshev<T>(c, g, L, b, x, true, skip);
-    T* r = Sel<T>::r(L);
-    spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
     MgLevel& Cn = g->lv[size_t(l) + 1];
-    spmv_launch<T>(c, L.R, Sel<T>::val(L.R), r, Sel<T>::b(Cn), nullptr, SPMV_PLAIN, nullptr, Sel<T>::bytes(L.R), skip);
+    if (L.RR) {  // b_c = [R | -R A] [b; x] = R (b - A x) in one launch
+        spmv2_launch<T>(c, L.RR, L.n, b, x, Sel<T>::b(Cn), skip);
+    } else {
+        T* r = Sel<T>::r(L);
+        spmv_launch<T>(c, L.A, Sel<T>::val(L.A), x, r, b, SPMV_RESID_NORED, nullptr, Sel<T>::bytes(L.A), skip);
+        spmv_launch<T>(c, L.R, Sel<T>::val(L.R), r, Sel<T>::b(Cn), nullptr, SPMV_PLAIN, nullptr, Sel<T>::bytes(L.R),
+                       skip);
+    }
     vcycle<T>(c, g, l + 1, Sel<T>::b(Cn), Sel<T>::x(Cn), skip);
     spmv_launch<T>(c, L.P, Sel<T>::val(L.P), Sel<T>::x(Cn), x, nullptr, SPMV_ACCUM, nullptr, Sel<T>::bytes(L.P), skip);
     chebyshev<T>(c, g, L, b, x, false, skip);
@@ -966,7 +1013,6 @@ extern "C" mk_status mk_mg_host_pcg(int n, const int64_t* ptr, const int32_t* co
                                     const double* nullvec, int nnull, const double* b, double* x, double tol,
                                     int maxit, int* iters, int* levels, double* relres) {
     if (n <= 0 || !ptr || !col || !val || !b || !x || nnull < 0 || (nnull > 0 && !nullvec)) return MK_ERR_ARG;
-    if (cplx && nnull > 0) return MK_ERR_UNSUPPORTED;
     try {
         if (cplx) {
             Csr<cd> A(n, n);
@@ -974,7 +1020,9 @@ extern "C" mk_status mk_mg_host_pcg(int n, const int64_t* ptr, const int32_t* co
             A.col.assign(col, col + ptr[n]);
             A.val.resize(size_t(ptr[n]));
             for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
-            HostHier<cd> H = build_host<cd>(A, tune().theta_cplx, tune().coarse_cplx);
+            std::vector<cd> B;
+            for (size_t q = 0; q < size_t(n) * size_t(nnull); q++) B.emplace_back(nullvec[2 * q], nullvec[2 * q + 1]);
+            HostHier<cd> H = build_host<cd>(A, tune().theta_cplx, tune().coarse_cplx, nnull ? &B : nullptr, nnull);
             if (levels) *levels = int(H.A.size());
             int it = host_pcg<cd>(A, H, reinterpret_cast<const cd*>(b), reinterpret_cast<cd*>(x), tol, maxit, relres);
             if (iters) *iters = it;
@@ -1039,6 +1087,7 @@ void mg_set_labels(Mg* g, const std::string& base) {
         if (g->lv[l].A) g->lv[l].A->label = s;
         if (g->lv[l].P) g->lv[l].P->label = s + "_P";
         if (g->lv[l].R) g->lv[l].R->label = s + "_R";
+        if (g->lv[l].RR) g->lv[l].RR->label = s + "_RR";
     }
 }
 double mg_setup_ms(const Mg* g) { return g->setup_ms; }

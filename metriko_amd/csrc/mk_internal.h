@@ -84,7 +84,7 @@ struct CgState {
     int done;       // 1 = converged or failed
     int status;     // MK_OK / MK_ERR_NOT_SPD / MK_ERR_NAN
     int it;         // iterations taken
-    int pad;
+    int first;      // 1 until the first beta of the fused FP32-multigrid iteration
 };
 
 // per-context scratch vectors
@@ -218,6 +218,12 @@ enum SpmvMode { SPMV_PLAIN = 0, SPMV_DOT = 1, SPMV_RESID = 2, SPMV_RESID_NORED =
 template <typename T>
 void cheb_launch(mk_ctx* c, const mk_op* A, const T* x_in, T* x_out, const T* b, T* d, T a, T s, int first,
                  const int* skip);
+// zero-start Chebyshev steps 1+2 fused: x_out = x1 + d, x1 = c0 D^-1 b, d = a x1 + s D^-1 (b - A x1)
+template <typename T>
+void cheb0_launch(mk_ctx* c, const mk_op* A, const T* b, T* x_out, T* d, T c0, T a, T s, const int* skip);
+// y = M [u; v], columns [0, m1) of M act on u, the rest on v
+template <typename T>
+void spmv2_launch(mk_ctx* c, const mk_op* A, int m1, const T* u, const T* v, T* y, const int* skip);
 template <typename T>
 void copy_launch(mk_ctx* c, size_t len, const T* x, T* y, const int* skip);
 template <typename T>
@@ -227,7 +233,9 @@ BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B
                       int maxiter, int check_every);
 
 
-void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma);
+// values are multiplied by scale on upload
+void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma,
+                double scale = 1.0);
 // y = A x (mode PLAIN); y = A x and partial x^H y (DOT); y = b - A x and partial |y|^2 (RESID)
 void spmv64(mk_ctx* c, const mk_op* A, const double* x, double* y, const double* b, SpmvMode mode, double* partials,
             int* nparts);

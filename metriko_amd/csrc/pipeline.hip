@@ -139,6 +139,71 @@ mk_status mk_combinatorics(int nV, int nF, const double* V, const int32_t* F, in
     });
 }
 
+mk_status mk_curvature_dir(int nV, int nF, const double* V, const int32_t* F, double* D) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !D) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        FaceRosyField f(mesh, 4);
+        VecXc d = f.principalCurvatureDir();
+        std::memcpy(D, d.data(), sizeof(double) * 2 * size_t(nF));
+    });
+}
+
+mk_status mk_field_solve(mk_ctx* c, int nV, int nF, const double* V, const int32_t* F, int N, int field_type,
+                         const double* u0, int n_iter, const mk_solve_opts* opts, double* compressed,
+                         mk_stats* stats) {
+    if (!c || nV <= 0 || nF <= 0 || !V || !F || N <= 0 || !compressed) return MK_ERR_ARG;
+    if (field_type != MK_FIELD_SMOOTHEST && field_type != MK_FIELD_CURVATURE_ALIGNED) return MK_ERR_ARG;
+    if (field_type == MK_FIELD_CURVATURE_ALIGNED && N != 2 && N != 4) return MK_ERR_ARG;
+    return guarded(c, [&] {
+        using namespace metriko;
+        auto& rt = hip::runtime();
+        struct Detach {
+            hip::Runtime& r;
+            ~Detach() { r.ctx = nullptr; r.external = false; }
+        } detach{rt};
+        rt.attach(c);
+        if (opts) rt.field_opts = *opts; else mk_default_opts(&rt.field_opts);
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        std::unique_ptr<FaceRosyField> f;
+        if (field_type == MK_FIELD_CURVATURE_ALIGNED) {
+            f = std::make_unique<FaceRosyField>(mesh, N, FieldType::CurvatureAligned);
+        } else {
+            VecXc u(nF);
+            if (u0) std::memcpy(u.data(), u0, sizeof(double) * 2 * size_t(nF));
+            else u = random_complex_vector(nF);
+            f = std::make_unique<FaceRosyField>(mesh, N, u, n_iter);
+        }
+        std::memcpy(compressed, f->compressed.data(), sizeof(double) * 2 * size_t(nF));
+        if (stats) *stats = rt.last_field;
+    });
+}
+
+mk_status mk_field_matching(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
+                            int matching_type, int32_t* singular, int32_t* matching, double* effort) {
+    if (nV <= 0 || nF <= 0 || !V || !F || N <= 0 || !compressed) return MK_ERR_ARG;
+    if (matching_type != MK_MATCH_PRINCIPAL && matching_type != MK_MATCH_CURL) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        FaceRosyField f(mesh, N);
+        f.compressed = VecXc(nF);
+        std::memcpy(f.compressed.data(), compressed, sizeof(double) * 2 * size_t(nF));
+        f.field.resize(nF, N);
+        for (int i = 0; i < nF; i++) {
+            const complex base = std::polar(1.0, std::arg(f.compressed[i]) / N);
+            for (int j = 0; j < N; j++) f.field(i, j) = base * std::polar(1.0, TwoPI / N * j);
+        }
+        f.computeMatching(matching_type == MK_MATCH_CURL ? MatchingType::Curl : MatchingType::Principal);
+        for (int v = 0; v < nV; v++) if (singular) singular[v] = f.singular[v];
+        for (int e = 0; e < mesh.nE; e++) {
+            if (matching) matching[e] = f.matching[e];
+            if (effort) effort[e] = f.effort[e];
+        }
+    });
+}
+
 mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int32_t* F, int N, double ratio,
                           const double* u0, int n_iter, const mk_solve_opts* field_opts,
                           const mk_solve_opts* igm_opts, double* compressed, int32_t* singular, int32_t* matching,

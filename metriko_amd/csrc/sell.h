@@ -129,11 +129,21 @@ __device__ __forceinline__ T wave_sum_t(T v) {
     return v;
 }
 
-// Computes one row of A x for virtual block vb (uniform in the block).  Returns true in the
-// thread that owns the result (row, ar, ai).
+// Vector access of the row kernels: a gather g(c, xr, xi) loads entry c of the vector the
+// operator multiplies (xi unused for real operators).
 template <typename T, bool CPLX>
-__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int vb, int& row, T& ar,
-                                         T& ai) {
+struct PlainGather {
+    const T* __restrict__ x;
+    __device__ __forceinline__ void operator()(int c, T& xr, T& xi) const {
+        if constexpr (CPLX) { xr = x[2 * c]; xi = x[2 * c + 1]; }
+        else { xr = x[c]; xi = T(0); }
+    }
+};
+
+// Computes one row of A x for virtual block vb (uniform in the block), x given by a gather.
+// Returns true in the thread that owns the result (row, ar, ai).
+template <typename T, bool CPLX, typename G>
+__device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int vb, int& row, T& ar, T& ai) {
     ar = 0;
     ai = 0;
     if (vb < A.nb_sell) {
@@ -146,13 +156,14 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
         for (int j = 0; j < w; j++) {
             const int64_t e = base + int64_t(j) * A.C;
             const int c = A.col[e];
+            T xr, xi;
+            g(c, xr, xi);
             if constexpr (CPLX) {
                 const T vr = A.val[2 * e], vi = A.val[2 * e + 1];
-                const T xr = x[2 * c], xi = x[2 * c + 1];
                 ar += vr * xr - vi * xi;
                 ai += vr * xi + vi * xr;
             } else {
-                ar += A.val[e] * x[c];
+                ar += A.val[e] * xr;
             }
         }
         return true;
@@ -164,13 +175,14 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
         if (m < A.nmid) {
             for (int64_t k = A.mid_ptr[m] + sub; k < A.mid_ptr[m + 1]; k += L) {
                 const int c = A.mid_col[k];
+                T xr, xi;
+                g(c, xr, xi);
                 if constexpr (CPLX) {
                     const T vr = A.mid_val[2 * k], vi = A.mid_val[2 * k + 1];
-                    const T xr = x[2 * c], xi = x[2 * c + 1];
                     ar += vr * xr - vi * xi;
                     ai += vr * xi + vi * xr;
                 } else {
-                    ar += A.mid_val[k] * x[c];
+                    ar += A.mid_val[k] * xr;
                 }
             }
         }
@@ -188,13 +200,14 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
     if (w < A.nlong) {  // uniform inside the wavefront
         for (int64_t k = A.long_ptr[w] + lane; k < A.long_ptr[w + 1]; k += 64) {
             const int c = A.long_col[k];
+            T xr, xi;
+            g(c, xr, xi);
             if constexpr (CPLX) {
                 const T vr = A.long_val[2 * k], vi = A.long_val[2 * k + 1];
-                const T xr = x[2 * c], xi = x[2 * c + 1];
                 ar += vr * xr - vi * xi;
                 ai += vr * xi + vi * xr;
             } else {
-                ar += A.long_val[k] * x[c];
+                ar += A.long_val[k] * xr;
             }
         }
         ar = wave_sum_t(ar);
@@ -205,6 +218,12 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
     return false;
 }
 
+
+template <typename T, bool CPLX>
+__device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restrict__ x, int vb, int& row, T& ar,
+                                         T& ai) {
+    return spmv_row_g<T, CPLX>(A, PlainGather<T, CPLX>{x}, vb, row, ar, ai);
+}
 
 static inline unsigned blocks_for(size_t len, int bs = 256) { return unsigned((len + bs - 1) / bs); }
 // grid of the grid-stride reduction kernels: few enough blocks that the per-block arrival

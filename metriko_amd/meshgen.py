@@ -84,6 +84,22 @@ def torus(n_tube, n_ring, R=1.0, r=0.35, jitter=0.0, seed=0):
     return V, F.astype(np.int64)
 
 
+def cylinder(n_around, n_rows, r=1.0, dz=0.1):
+    """Open cylinder: n_rows + 1 rings of n_around vertices, ring j at z = j dz, vertex (i, j)
+    at angle 2 pi i / n_around with id j*n_around + i; quad (i,j),(i+1,j),(i+1,j+1),(i,j+1)
+    split on its (i,j)-(i+1,j+1) diagonal; outward orientation."""
+    j, i = np.meshgrid(np.arange(n_rows + 1), np.arange(n_around), indexing="ij")
+    th = 2 * np.pi * i / n_around
+    V = np.stack([r * np.cos(th), r * np.sin(th), dz * j], -1).reshape(-1, 3).astype(np.float64)
+    jj, ii = np.meshgrid(np.arange(n_rows), np.arange(n_around), indexing="ij")
+    a = jj * n_around + ii
+    b = jj * n_around + (ii + 1) % n_around
+    c = (jj + 1) * n_around + (ii + 1) % n_around
+    d = (jj + 1) * n_around + ii
+    F = np.stack([np.stack([a, b, c], -1), np.stack([a, c, d], -1)], -2).reshape(-1, 3)
+    return V, F.astype(np.int64)
+
+
 def bumpy_icosphere(level, amp=0.05, jitter=1e-3, seed=5):
     """Icosphere with radial displacement r = 1 + amp*sin(6 theta)*sin(5 phi)
     (theta = acos z, phi = atan2(y, x)) plus N(0, jitter) radial jitter, seed ``seed``."""

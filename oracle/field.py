@@ -124,6 +124,35 @@ def solve_smallest_eig(L, mass, u0, n_iter=50, return_history=False):
     return (x, hist) if return_history else x
 
 
+def principal_curvature_dir(m):
+    """``FaceRosyField::principalCurvatureDir`` (face_rosy_field.h:86-98).
+
+    Per face, over its halfedges h (face2half, next, next.next): with l = |h| and
+    c = l e^{i farg(h)}, dir += -c^2 / l * darg(h); D = dir / 4.  Complex, face basis.
+    """
+    D = np.zeros(m.nF, dtype=np.complex128)
+    h = m.face2half.copy()
+    for _ in range(3):
+        vec = m.pos[m.head[h]] - m.pos[m.tail[h]]
+        length = np.sqrt((vec * vec).sum(axis=1))
+        c = np.exp(1j * m.heArgOnFace[h]) * length
+        D = D + (-c * c / length * m.dihedralArg[h])
+        h = m.next[h]
+    return D * 0.25
+
+
+def solve_curvature_aligned(L, mass, D, N):
+    """``FieldType::CurvatureAligned`` (face_rosy_field.h:25-34) with lambda = 0: D squared for
+    N = 4, rhs = M D / sqrt(|D^H M D|), compressed = solveSquare(L, rhs) (sparse direct LU,
+    sparse_linear_solver.h:16-27)."""
+    assert N in (2, 4)
+    if N == 4:
+        D = D * D
+    s = np.sqrt(abs(np.vdot(D, mass * D)))
+    rhs = mass * D / s
+    return spla.spsolve(L.tocsc(), rhs)
+
+
 def field_roots(compressed, N):
     """Field from the compressed representation (face_rosy_field.h:37-43):
     field(f,i) = (e^{i arg c_f})^{1/N} * (e^{2 pi i/N})^i, principal branch."""
@@ -234,14 +263,18 @@ def compute_singular(m, effort, N):
 class FaceRosyField:
     """``FaceRosyField(mesh, N, FieldType::Smoothest)`` + ``computeMatching`` (face_rosy_field.h:17-44,119-135)."""
 
-    def __init__(self, m, N, u0=None, n_iter=50, compute=True):
+    def __init__(self, m, N, u0=None, n_iter=50, compute=True, kind="smoothest"):
         self.mesh, self.rosyN = m, N
         if not compute:
             return
         self.L = connection_laplacian(m, N)
         self.M = mass_matrix(m)
-        self.u0 = eigen_random_complex(m.nF) if u0 is None else u0
-        self.compressed = solve_smallest_eig(self.L, self.M, self.u0, n_iter)
+        if kind == "curvature":
+            self.D = principal_curvature_dir(m)
+            self.compressed = solve_curvature_aligned(self.L, self.M, self.D, N)
+        else:
+            self.u0 = eigen_random_complex(m.nF) if u0 is None else u0
+            self.compressed = solve_smallest_eig(self.L, self.M, self.u0, n_iter)
         self.field = field_roots(self.compressed, N)
 
     def compute_matching(self, kind="curl"):

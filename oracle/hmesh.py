@@ -254,6 +254,17 @@ class Hmesh:
             self.heArgOnFace[h] = s
             h = nxt[h]
 
+        # dihedral angle per halfedge (hmesh.h:508-519): atan2(v . (n1 x n2), n1 . n2), v the
+        # unit halfedge direction, n1 / n2 the normals of its face and of its twin's face; 0 on
+        # boundary edges
+        self.dihedralArg = np.zeros(nH)
+        hh = np.nonzero((face >= 0) & (face[twin] >= 0))[0]
+        n1 = self.faceNormal[face[hh]]
+        n2 = self.faceNormal[face[twin[hh]]]
+        vh = P[head[hh]] - P[tail[hh]]
+        vh = vh / np.sqrt((vh * vh).sum(axis=1))[:, None]
+        self.dihedralArg[hh] = np.arctan2((vh * np.cross(n1, n2)).sum(axis=1), (n1 * n2).sum(axis=1))
+
         # vertex normals / bases (hmesh.h:492-506)
         vn = np.zeros((nV, 3))
         for j in range(3):

@@ -3,7 +3,8 @@
 The reference ships no expected outputs; these fixtures freeze the oracle (pinned by
 tests/test_oracle_pinning.py) on example/models/icosphere.obj with the example_igm flag
 set and ratio 0.03, so later changes to the oracle or the GPU path are caught exactly.
-Run: python tests/golden/make_golden.py
+Run: python tests/golden/make_golden.py            (icosphere.obj fixture)
+     python tests/golden/make_golden.py --torus    (204,800-face torus, about two minutes)
 """
 import os
 import sys
@@ -16,9 +17,30 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 from metriko_amd.meshgen import read_obj  # noqa: E402
 from oracle.field import eigen_random_complex  # noqa: E402
 from oracle.pipeline import run_pipeline  # noqa: E402
+from tests.helpers import cut_hash, torus_config  # noqa: E402
+
+
+def make_torus():
+    V, F, u0 = torus_config()
+    r = run_pipeline(V, F, ratio=0.03, u0=u0)
+    dec = np.array([(i, v) for (i, v, _) in r["decisions"]], dtype=np.int64)
+    margin = np.array([m for (_, _, m) in r["decisions"]])
+    sf = np.arange(0, F.shape[0], 50)          # sampled faces
+    sc = np.arange(0, 3 * F.shape[0], 150)     # sampled corners
+    sing = np.nonzero(r["singular"])[0]
+    np.savez_compressed(
+        os.path.join(HERE, "torus_256x400_oracle.npz"),
+        field_faces=sf, field=r["compressed"][sf], singular_v=sing.astype(np.int32),
+        singular_idx=r["singular"][sing].astype(np.int32), seam_edges=np.nonzero(r["seam"])[0].astype(np.int32),
+        cut_hash=np.int64(cut_hash(r["cut"].idx)), cutV=np.int64(r["cut"].nV), uv_corners=sc,
+        uv=r["uv"].reshape(-1, 2)[sc], decisions=dec, decision_margin=margin, nS=r["nS"], nT=r["nT"])
+    print("wrote torus_256x400_oracle.npz", r["nS"], r["nT"], len(dec))
 
 
 def main():
+    if "--torus" in sys.argv:
+        make_torus()
+        return
     V, F = read_obj(os.path.join(HERE, "icosphere.obj"))
     u0 = eigen_random_complex(F.shape[0])
     r = run_pipeline(V, F, ratio=0.03, u0=u0)

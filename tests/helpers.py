@@ -27,3 +27,19 @@ def phase_error(x, ref):
     c = np.vdot(ref, x)
     c = c / abs(c) if abs(c) > 0 else 1.0
     return np.linalg.norm(x - c * ref) / np.linalg.norm(ref)
+
+
+def torus_config():
+    """The 204,800-face torus of the FP32-vs-FP64 configuration: R = 1, r = 0.35 on a
+    256 x 400 grid, N(0, 0.002) jitter along the normals (seed 1); u0 from PCG64(1)."""
+    from metriko_amd.meshgen import torus
+    V, F = torus(256, 400, 1.0, 0.35, jitter=0.002, seed=1)
+    g = np.random.Generator(np.random.PCG64(1))
+    u0 = g.uniform(-1, 1, F.shape[0]) + 1j * g.uniform(-1, 1, F.shape[0])
+    return V, F, u0
+
+
+def cut_hash(idx):
+    """Order-sensitive checksum of the cut-mesh corner indices."""
+    w = (np.arange(idx.size, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(1)) & np.uint64((1 << 40) - 1)
+    return int((w * np.asarray(idx).reshape(-1).astype(np.uint64)).sum() & np.uint64((1 << 61) - 1))

@@ -46,3 +46,30 @@ def test_gpu_pipeline_reproduces_golden(gpu_ctx, golden):
     assert np.array_equal(np.nonzero(out["seam"])[0], golden["seam_edges"])
     assert sorted(map(tuple, out["decisions"])) == sorted(map(tuple, golden["decisions"].tolist()))
     assert np.abs(out["uv"] - golden["uv"]).max() / np.abs(golden["uv"]).max() < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [mk.PREC_MIXED_MG, mk.PREC_FP64])
+def test_gpu_pipeline_torus_204800(gpu_ctx, precision):
+    """The 204,800-face torus (R = 1, r = 0.35, 256 x 400, normal jitter 0.002) against the
+    oracle's fixture (tests/golden/make_golden.py --torus): singularities, seam, cut-mesh
+    indices (checksum) and rounding decisions bit-exact, the field within 1e-8 up to a global
+    phase and uv within 1e-8 relative on sampled faces / corners, in both precision modes."""
+    from tests.helpers import cut_hash, torus_config
+    g = np.load(os.path.join(GOLDEN, "torus_256x400_oracle.npz"), allow_pickle=False)
+    V, F, u0 = torus_config()
+    o = mk.opts(precision=precision)
+    out = gpu_ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=o, igm_opts=o)
+    print(precision, out["info"])
+    assert phase_error(out["compressed"][g["field_faces"]], g["field"]) < 1e-8
+    sing = np.nonzero(out["singular"])[0]
+    assert np.array_equal(sing, g["singular_v"]) and np.array_equal(out["singular"][sing], g["singular_idx"])
+    assert np.array_equal(np.nonzero(out["seam"])[0], g["seam_edges"])
+    assert out["info"]["cutV"] == int(g["cutV"]) and cut_hash(out["cut_idx"]) == int(g["cut_hash"])
+    assert (out["info"]["nS"], out["info"]["nT"]) == (int(g["nS"]), int(g["nT"]))
+    assert sorted(map(tuple, out["decisions"])) == sorted(map(tuple, g["decisions"].tolist()))
+    firm = g["decision_margin"] > 1e-6  # decisions after a solve keep their order
+    got = [tuple(d) for d in out["decisions"]]
+    assert [t for t, f in zip(got, firm) if f] == [tuple(d) for d, f in zip(g["decisions"].tolist(), firm) if f]
+    uv = np.asarray(out["uv"]).reshape(-1, 2)[g["uv_corners"]]
+    assert np.abs(uv - g["uv"]).max() / np.abs(g["uv"]).max() < 1e-8
