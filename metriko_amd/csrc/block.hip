@@ -172,13 +172,13 @@ template void spmm_launch<float>(mk_ctx*, const mk_op*, int, const float*, float
 
 // ---- batched PCG --------------------------------------------------------------------------
 
-// Per-column grand totals of K quantities (the thread's column is threadIdx.x % MW, blockDim
-// 256, grid <= kBlkParts).  Returns true in the last block to arrive, with tot[k][j] filled.
+// Per-column partial sums of K quantities of this block (the thread's column is
+// threadIdx.x % MW, blockDim 256, grid <= kBlkParts), stored at
+// partials[(k * 64 + j) * kBlkParts + block].  No fence or ticket: a one-block finishing kernel
+// adds them (colsum_total) after the kernel boundary, in a fixed order.
 template <int K, int MW>
-__device__ __forceinline__ bool colsum_last(double (&v)[K], double* partials, unsigned* counter,
-                                            double (*tot)[64]) {
+__device__ __forceinline__ void colsum_write(double (&v)[K], double* partials) {
     __shared__ double sh[K][4][64];
-    __shared__ int is_last;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -189,30 +189,23 @@ __device__ __forceinline__ bool colsum_last(double (&v)[K], double* partials, un
     }
     __syncthreads();
     if (threadIdx.x < MW) {
+        const int t = threadIdx.x;
 #pragma unroll
-        for (int k = 0; k < K; k++) {
-            const int t = threadIdx.x;
+        for (int k = 0; k < K; k++)
             partials[(size_t(k) * 64 + t) * kBlkParts + blockIdx.x] = sh[k][0][t] + sh[k][1][t] + sh[k][2][t] + sh[k][3][t];
-        }
-        __threadfence();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = (atomicAdd(counter, 1u) == gridDim.x - 1) ? 1 : 0;
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
-    const int j = threadIdx.x % MW;
-    double acc[K];
+}
+// Column totals over nparts blocks (one block of 256 threads): tot[k][j], j < MW, valid in every
+// thread after the call.
+template <int K, int MW>
+__device__ __forceinline__ void colsum_total(const double* __restrict__ partials, int nparts, double (*tot)[64]) {
+    __shared__ double sh[K][4][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, j = threadIdx.x % MW;
 #pragma unroll
     for (int k = 0; k < K; k++) {
-        acc[k] = 0.0;
-        const volatile double* pk = partials + (size_t(k) * 64 + j) * kBlkParts;
-        for (unsigned b = threadIdx.x / MW; b < gridDim.x; b += 256 / MW) acc[k] += pk[b];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double a = acc[k];
+        double a = 0.0;
+        const double* pk = partials + (size_t(k) * 64 + j) * kBlkParts;
+        for (int b = threadIdx.x / MW; b < nparts; b += 256 / MW) a += pk[b];
 #pragma unroll
         for (int off = MW; off < 64; off <<= 1) a += __shfl_xor(a, off, 64);
         if (lane < MW) sh[k][wid][lane] = a;
@@ -223,9 +216,7 @@ __device__ __forceinline__ bool colsum_last(double (&v)[K], double* partials, un
 #pragma unroll
         for (int k = 0; k < K; k++) tot[k][t] = sh[k][0][t] + sh[k][1][t] + sh[k][2][t] + sh[k][3][t];
     }
-    if (threadIdx.x == 0) *counter = 0u;
     __syncthreads();
-    return true;
 }
 
 __device__ __forceinline__ void bstate_all_done(BState* st, int mw) {
@@ -237,9 +228,7 @@ __device__ __forceinline__ void bstate_all_done(BState* st, int mw) {
 // X = 0 ; R = B ; R32 = float(B) ; bb_j = |B_j|^2
 template <int MW>
 __global__ void __launch_bounds__(256) bk_init0(size_t len, const double* __restrict__ B, double* __restrict__ X,
-                                              double* __restrict__ R, float* __restrict__ R32, BState* st,
-                                              double* partials, unsigned* counter) {
-    __shared__ double tot[1][64];
+                                              double* __restrict__ R, float* __restrict__ R32, double* partials) {
     double v[1] = {0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double b = B[e];
@@ -248,7 +237,13 @@ __global__ void __launch_bounds__(256) bk_init0(size_t len, const double* __rest
         R32[e] = float(b);
         v[0] += b * b;
     }
-    if (colsum_last<1, MW>(v, partials, counter, tot) && threadIdx.x < MW) {
+    colsum_write<1, MW>(v, partials);
+}
+template <int MW>
+__global__ void __launch_bounds__(256) bk_init0_fin(BState* st, const double* __restrict__ partials, int nparts) {
+    __shared__ double tot[1][64];
+    colsum_total<1, MW>(partials, nparts, tot);
+    if (threadIdx.x < MW) {
         const int j = threadIdx.x;
         st->bb[j] = tot[0][j];
         st->it[j] = 0;
@@ -259,9 +254,7 @@ __global__ void __launch_bounds__(256) bk_init0(size_t len, const double* __rest
 // P = Zold = Z32 ; rz_j = R.Z32 ; rr_j = R.R ; done flags
 template <int MW>
 __global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
-                                              double* __restrict__ P, double* __restrict__ Zold, BState* st,
-                                              double* partials, unsigned* counter) {
-    __shared__ double tot[2][64];
+                                              double* __restrict__ P, double* __restrict__ Zold, double* partials) {
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double z = double(Z32[e]), r = R[e];
@@ -270,7 +263,13 @@ __global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __rest
         v[0] += r * z;
         v[1] += r * r;
     }
-    if (colsum_last<2, MW>(v, partials, counter, tot)) {
+    colsum_write<2, MW>(v, partials);
+}
+template <int MW>
+__global__ void __launch_bounds__(256) bk_init1_fin(BState* st, const double* __restrict__ partials, int nparts) {
+    __shared__ double tot[2][64];
+    colsum_total<2, MW>(partials, nparts, tot);
+    {
         if (threadIdx.x < MW) {
             const int j = threadIdx.x;
             const double rz = tot[0][j], rr = tot[1][j];
@@ -289,12 +288,18 @@ __global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __rest
 // pq_j = P.Q ; alpha_j = rz_j / pq_j (0 for finished columns)
 template <int MW>
 __global__ void __launch_bounds__(256) bk_alpha(size_t len, const double* __restrict__ P, const double* __restrict__ Q,
-                                              BState* st, double* partials, unsigned* counter) {
+                                              const BState* st, double* partials) {
     if (st->all_done) return;
-    __shared__ double tot[1][64];
     double v[1] = {0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) v[0] += P[e] * Q[e];
-    if (colsum_last<1, MW>(v, partials, counter, tot) && threadIdx.x < MW) {
+    colsum_write<1, MW>(v, partials);
+}
+template <int MW>
+__global__ void __launch_bounds__(256) bk_alpha_fin(BState* st, const double* __restrict__ partials, int nparts) {
+    if (st->all_done) return;
+    __shared__ double tot[1][64];
+    colsum_total<1, MW>(partials, nparts, tot);
+    if (threadIdx.x < MW) {
         const int j = threadIdx.x;
         const double pq = tot[0][j];
         if (st->done[j]) { st->alpha[j] = 0.0; }
@@ -312,10 +317,8 @@ __global__ void __launch_bounds__(256) bk_alpha(size_t len, const double* __rest
 template <int MW>
 __global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict__ X, double* __restrict__ R,
                                                float* __restrict__ R32, const double* __restrict__ P,
-                                               const double* __restrict__ Q, BState* st, double* partials,
-                                               unsigned* counter) {
+                                               const double* __restrict__ Q, const BState* st, double* partials) {
     if (st->all_done) return;
-    __shared__ double tot[1][64];
     const double alpha = st->alpha[threadIdx.x % MW];
     double v[1] = {0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
@@ -328,7 +331,14 @@ __global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict_
         }
         v[0] += r * r;
     }
-    if (colsum_last<1, MW>(v, partials, counter, tot)) {
+    colsum_write<1, MW>(v, partials);
+}
+template <int MW>
+__global__ void __launch_bounds__(256) bk_update_fin(BState* st, const double* __restrict__ partials, int nparts) {
+    if (st->all_done) return;
+    __shared__ double tot[1][64];
+    colsum_total<1, MW>(partials, nparts, tot);
+    {
         if (threadIdx.x < MW) {
             const int j = threadIdx.x;
             const double rr = tot[0][j];
@@ -347,17 +357,22 @@ __global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict_
 // flexible beta_j = (R.Z32 - R.Zold) / rz_j ; rz_j = R.Z32
 template <int MW>
 __global__ void __launch_bounds__(256) bk_beta(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
-                                             const double* __restrict__ Zold, BState* st, double* partials,
-                                             unsigned* counter) {
+                                             const double* __restrict__ Zold, const BState* st, double* partials) {
     if (st->all_done) return;
-    __shared__ double tot[2][64];
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double r = R[e];
         v[0] += r * double(Z32[e]);
         v[1] += r * Zold[e];
     }
-    if (colsum_last<2, MW>(v, partials, counter, tot)) {
+    colsum_write<2, MW>(v, partials);
+}
+template <int MW>
+__global__ void __launch_bounds__(256) bk_beta_fin(BState* st, const double* __restrict__ partials, int nparts) {
+    if (st->all_done) return;
+    __shared__ double tot[2][64];
+    colsum_total<2, MW>(partials, nparts, tot);
+    {
         if (threadIdx.x < MW) {
             const int j = threadIdx.x;
             if (!st->done[j]) {
@@ -402,17 +417,19 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
     const size_t len = size_t(A->n) * MW;
     DevBuf<double> R(len), P(len), Q(len), Zold(len), part(size_t(2) * 64 * kBlkParts);
     DevBuf<float> R32(len), Z32(len);
-    DevBuf<unsigned> cnt(1);
     DevBuf<BState> st(1);
-    MK_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), c->stream));
     MK_HIP(hipMemsetAsync(st.p, 0, sizeof(BState), c->stream));
-    const unsigned g = unsigned(std::min<size_t>(kRedGrid, std::max<size_t>(1, (len + 255) / 256)));
+    const unsigned g = unsigned(std::min<size_t>(std::min<size_t>(size_t(fin_grid()), size_t(kBlkParts)),
+                                                 std::max<size_t>(1, (len + 255) / 256)));
+    const int ng = int(g);
     // the recurrence residual is driven to half the target so the true residual meets it
     hipLaunchKernelGGL(k_set_bstate, dim3(1), dim3(1), 0, c->stream, st.p, 0.25 * rel_tol * rel_tol);
-    hipLaunchKernelGGL((bk_init0<MW>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, R32.p, st.p, part.p, cnt.p);
+    hipLaunchKernelGGL((bk_init0<MW>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, R32.p, part.p);
+    hipLaunchKernelGGL((bk_init0_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
     mg_apply32_block(c, mg, MW, R32.p, Z32.p, nullptr);
     hipLaunchKernelGGL((bk_init1<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                       (const float*)Z32.p, P.p, Zold.p, st.p, part.p, cnt.p);
+                       (const float*)Z32.p, P.p, Zold.p, part.p);
+    hipLaunchKernelGGL((bk_init1_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
     MK_HIP(hipGetLastError());
     BState h{};
     int iters = 0;
@@ -425,12 +442,15 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
         for (int k = 0; k < batch; k++) {
             spmm_launch<double>(c, A, MW, P.p, Q.p, nullptr, SPMV_PLAIN, &st.p->all_done);
             hipLaunchKernelGGL((bk_alpha<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)P.p,
-                               (const double*)Q.p, st.p, part.p, cnt.p);
+                               (const double*)Q.p, (const BState*)st.p, part.p);
+            hipLaunchKernelGGL((bk_alpha_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
             hipLaunchKernelGGL((bk_update<MW>), dim3(g), dim3(256), 0, c->stream, len, X, R.p, R32.p,
-                               (const double*)P.p, (const double*)Q.p, st.p, part.p, cnt.p);
+                               (const double*)P.p, (const double*)Q.p, (const BState*)st.p, part.p);
+            hipLaunchKernelGGL((bk_update_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
             mg_apply32_block(c, mg, MW, R32.p, Z32.p, &st.p->all_done);
             hipLaunchKernelGGL((bk_beta<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                               (const float*)Z32.p, (const double*)Zold.p, st.p, part.p, cnt.p);
+                               (const float*)Z32.p, (const double*)Zold.p, (const BState*)st.p, part.p);
+            hipLaunchKernelGGL((bk_beta_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
             hipLaunchKernelGGL((bk_pupdate<MW>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const float*)Z32.p,
                                Zold.p, (const BState*)st.p);
         }

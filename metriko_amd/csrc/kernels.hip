@@ -44,10 +44,89 @@ void timing_flush(mk_ctx* c) {
 }
 
 
+// ---- second launch of the two-launch reductions ---------------------------------------------
+// k_finish<K, F>: one block sums the K partials of the nparts blocks of the previous kernel
+// (fixed order) and thread 0 applies F to the totals; F::guard skips it once the CG state is
+// done (the working kernel then left no partials), skip (if set) does the same.
+template <int K, typename F>
+__global__ void __launch_bounds__(1024) k_finish(CgState* st, const double* __restrict__ partials, int nparts,
+                                                 double* out, const int* skip) {
+    if (skip && *skip) return;
+    if (F::guard && st->done) return;
+    double tot[K];
+    partials_sum<K>(partials, nparts, tot);
+    if (threadIdx.x == 0) F::apply(st, tot, out);
+}
+struct FinOut {  // plain total
+    static constexpr bool guard = false;
+    __device__ static void apply(CgState*, const double* tot, double* out) { out[0] = tot[0]; }
+};
+struct FinCgInit {  // rz, rr of the first residual
+    static constexpr bool guard = false;
+    __device__ static void apply(CgState* st, const double* tot, double*) {
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it = 0;
+        st->status = MK_OK;
+        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
+        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
+    }
+};
+struct FinCgInitZ {  // as FinCgInit, z from an external preconditioner (must be positive)
+    static constexpr bool guard = false;
+    __device__ static void apply(CgState* st, const double* tot, double* o) {
+        FinCgInit::apply(st, tot, o);
+        if (!st->done && !(tot[0] > 0.0)) { st->status = MK_ERR_NOT_SPD; st->done = 1; }
+    }
+};
+struct FinCgAlpha {  // pq -> alpha
+    static constexpr bool guard = true;
+    __device__ static void apply(CgState* st, const double* tot, double*) {
+        st->pq = tot[0];
+        if (!(tot[0] > 0.0)) {
+            st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
+            st->done = 1;
+        } else {
+            st->alpha = st->rz / tot[0];
+        }
+    }
+};
+struct FinCgUpdate {  // rz, rr after the x / r / z update -> beta, convergence
+    static constexpr bool guard = true;
+    __device__ static void apply(CgState* st, const double* tot, double*) {
+        st->beta = tot[0] / st->rz;
+        st->rz = tot[0];
+        st->rr = tot[1];
+        st->it += 1;
+        if (!(tot[1] == tot[1]) || !(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (tot[1] <= st->tol2 * st->bb) st->done = 1;
+    }
+};
+struct FinCgRr {  // rr after the x / r update -> convergence
+    static constexpr bool guard = true;
+    __device__ static void apply(CgState* st, const double* tot, double*) {
+        st->rr = tot[0];
+        st->it += 1;
+        if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
+        else if (tot[0] <= st->tol2 * st->bb) st->done = 1;
+    }
+};
+struct FinCgBeta {  // rz of the preconditioned residual -> beta
+    static constexpr bool guard = true;
+    __device__ static void apply(CgState* st, const double* tot, double*) {
+        if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
+        else { st->beta = tot[0] / st->rz; st->rz = tot[0]; }
+    }
+};
+template <int K, typename F>
+static inline void finish(mk_ctx* c, unsigned nparts, double* out = nullptr, const int* skip = nullptr) {
+    hipLaunchKernelGGL((k_finish<K, F>), dim3(1), dim3(1024), 0, c->stream, c->state.p, (const double*)c->partials.p,
+                       int(nparts), out, skip);
+}
+
 template <typename T, bool CPLX, int MODE>
 __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restrict__ x, T* __restrict__ y,
-                                               const T* __restrict__ b, double* partials, unsigned* counter,
-                                               double* out, const int* skip) {
+                                               const T* __restrict__ b, double* partials, const int* skip) {
     if (skip && *skip) return;
     double contrib = 0.0;
     // reducing modes run a capped grid that strides over the virtual blocks
@@ -72,8 +151,8 @@ __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restri
         }
     }
     if constexpr (MODE == 1 || MODE == 2) {
-        double v[1] = {contrib}, tot[1];
-        if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+        double v[1] = {contrib};
+        partials_write<1>(v, partials);
     }
 }
 
@@ -83,8 +162,7 @@ __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restri
 // is per element (real).
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_init(int n, const T* __restrict__ r, const T* __restrict__ dinv,
-                                             T* __restrict__ z, T* __restrict__ p, CgState* st, double* partials,
-                                             unsigned* counter) {
+                                             T* __restrict__ z, T* __restrict__ p, double* partials) {
     double rz = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const T d = dinv ? dinv[i] : T(1);
@@ -101,21 +179,14 @@ __global__ void __launch_bounds__(256) cg_init(int n, const T* __restrict__ r, c
             rr += double(r0) * r0;
         }
     }
-    double v[2] = {rz, rr}, tot[2];
-    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->it = 0;
-        st->status = MK_OK;
-        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
-        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
-    }
+    double v[2] = {rz, rr};
+    partials_write<2>(v, partials);
 }
 
 // q = A p ; pq = p^H q ; alpha = rz / pq
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_spmv(SellDev<T> A, const T* __restrict__ p, T* __restrict__ q,
-                                             CgState* st, double* partials, unsigned* counter) {
+                                             const CgState* st, double* partials) {
     if (st->done) return;
     double contrib = 0.0;
     for (int vb = blockIdx.x; vb < A.nvb; vb += gridDim.x) {
@@ -131,24 +202,15 @@ __global__ void __launch_bounds__(256) cg_spmv(SellDev<T> A, const T* __restrict
             contrib += double(p[row]) * double(ar);
         }
     }
-    double v[1] = {contrib}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->pq = tot[0];
-        if (!(tot[0] > 0.0)) {
-            st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
-            st->done = 1;
-        } else {
-            st->alpha = st->rz / tot[0];
-        }
-    }
+    double v[1] = {contrib};
+    partials_write<1>(v, partials);
 }
 
 // x += alpha p ; r -= alpha q ; z = dinv r ; rz, rr ; beta
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
                                                const T* __restrict__ p, const T* __restrict__ q,
-                                               const T* __restrict__ dinv, CgState* st, double* partials,
-                                               unsigned* counter) {
+                                               const T* __restrict__ dinv, const CgState* st, double* partials) {
     if (st->done) return;
     const T alpha = T(st->alpha);
     double rz = 0.0, rr = 0.0;
@@ -176,21 +238,14 @@ __global__ void __launch_bounds__(256) cg_update(int n, T* __restrict__ x, T* __
             rr += double(rk) * rk;
         }
     }
-    double v[2] = {rz, rr}, tot[2];
-    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->beta = tot[0] / st->rz;
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->it += 1;
-        if (!(tot[1] == tot[1]) || !(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
-        else if (tot[1] <= st->tol2 * st->bb) st->done = 1;
-    }
+    double v[2] = {rz, rr};
+    partials_write<2>(v, partials);
 }
 
 // preconditioned variant (z from an external preconditioner): init, update, beta
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_init_z(int n, const T* __restrict__ r, const T* __restrict__ z,
-                                               T* __restrict__ p, CgState* st, double* partials, unsigned* counter) {
+                                               T* __restrict__ p, double* partials) {
     double rz = 0.0, rr = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if constexpr (CPLX) {
@@ -205,22 +260,14 @@ __global__ void __launch_bounds__(256) cg_init_z(int n, const T* __restrict__ r,
             rr += double(r0) * r0;
         }
     }
-    double v[2] = {rz, rr}, tot[2];
-    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->rz = tot[0];
-        st->rr = tot[1];
-        st->it = 0;
-        st->status = MK_OK;
-        st->done = (tot[1] <= st->tol2 * st->bb) ? 1 : 0;
-        if (!(tot[0] == tot[0]) || !(tot[1] == tot[1])) { st->status = MK_ERR_NAN; st->done = 1; }
-        else if (!st->done && !(tot[0] > 0.0)) { st->status = MK_ERR_NOT_SPD; st->done = 1; }
-    }
+    double v[2] = {rz, rr};
+    partials_write<2>(v, partials);
 }
 
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) cg_update_noz(int n, T* __restrict__ x, T* __restrict__ r,
-                                                   const T* __restrict__ p, const T* __restrict__ q, CgState* st,
-                                                   double* partials, unsigned* counter) {
+                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                   const CgState* st, double* partials) {
     if (st->done) return;
     const T alpha = T(st->alpha);
     double rr = 0.0;
@@ -235,29 +282,21 @@ __global__ void __launch_bounds__(256) cg_update_noz(int n, T* __restrict__ x, T
             rr += double(rk) * rk;
         }
     }
-    double v[1] = {rr}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
-        st->rr = tot[0];
-        st->it += 1;
-        if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
-        else if (tot[0] <= st->tol2 * st->bb) st->done = 1;
-    }
+    double v[1] = {rr};
+    partials_write<1>(v, partials);
 }
 
 template <typename T, bool CPLX>
-__global__ void __launch_bounds__(256) cg_beta(int n, const T* __restrict__ r, const T* __restrict__ z, CgState* st,
-                                             double* partials, unsigned* counter) {
+__global__ void __launch_bounds__(256) cg_beta(int n, const T* __restrict__ r, const T* __restrict__ z,
+                                             const CgState* st, double* partials) {
     if (st->done) return;
     double rz = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if constexpr (CPLX) rz += double(r[2 * i]) * z[2 * i] + double(r[2 * i + 1]) * z[2 * i + 1];
         else rz += double(r[i]) * z[i];
     }
-    double v[1] = {rz}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
-        if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
-        else { st->beta = tot[0] / st->rz; st->rz = tot[0]; }
-    }
+    double v[1] = {rz};
+    partials_write<1>(v, partials);
 }
 
 template <typename T>
@@ -278,8 +317,7 @@ __global__ void __launch_bounds__(256) cg_pupdate(size_t len, T* __restrict__ p,
 // x += alpha p ; r -= alpha q ; r32 = float(r) ; rr
 __global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict__ x, double* __restrict__ r,
                                                 float* __restrict__ r32, const double* __restrict__ p,
-                                                const double* __restrict__ q, CgState* st, double* partials,
-                                                unsigned* counter) {
+                                                const double* __restrict__ q, CgState* st, double* partials) {
     if (st->done) return;
     const double alpha = st->alpha;
     double rr = 0.0;
@@ -290,8 +328,14 @@ __global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict
         r32[i] = float(rk);
         rr += rk * rk;
     }
-    double v[1] = {rr}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double v[1] = {rr};
+    partials_write<1>(v, partials);
+}
+__global__ void __launch_bounds__(1024) cgm_update_fin(CgState* st, const double* __restrict__ partials, int nparts) {
+    if (st->done) return;
+    double tot[1];
+    partials_sum<1>(partials, nparts, tot);
+    if (threadIdx.x == 0) {
         st->rr = tot[0];
         st->it += 1;
         if (!(tot[0] == tot[0])) { st->status = MK_ERR_NAN; st->done = 1; }
@@ -303,15 +347,21 @@ __global__ void __launch_bounds__(256) cgm_update(size_t len, double* __restrict
 // iteration); rz = a
 __global__ void __launch_bounds__(256) cgm_beta2(size_t len, const double* __restrict__ r,
                                                const float* __restrict__ z32, const double* __restrict__ zold,
-                                               CgState* st, double* partials, unsigned* counter) {
+                                               CgState* st, double* partials) {
     if (st->done) return;
     double a = 0.0, m = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
         a += r[i] * double(z32[i]);
         m += r[i] * zold[i];
     }
-    double v[2] = {a, m}, tot[2];
-    if (partials_last<2>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double v[2] = {a, m};
+    partials_write<2>(v, partials);
+}
+__global__ void __launch_bounds__(1024) cgm_beta2_fin(CgState* st, const double* __restrict__ partials, int nparts) {
+    if (st->done) return;
+    double tot[2];
+    partials_sum<2>(partials, nparts, tot);
+    if (threadIdx.x == 0) {
         if (!(tot[0] > 0.0)) { st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN; st->done = 1; }
         else {
             st->beta = st->first ? 0.0 : (tot[0] - tot[1]) / st->rz;
@@ -334,8 +384,7 @@ struct Fp32Gather {
 template <bool CPLX>
 __global__ void __launch_bounds__(256) cgm_spmv(SellDev<double> A, const float* __restrict__ z32,
                                               double* __restrict__ p, double* __restrict__ q,
-                                              double* __restrict__ zold, CgState* st, double* partials,
-                                              unsigned* counter) {
+                                              double* __restrict__ zold, CgState* st, double* partials) {
     if (st->done) return;
     const double beta = st->beta;
     double contrib = 0.0;
@@ -360,8 +409,14 @@ __global__ void __launch_bounds__(256) cgm_spmv(SellDev<double> A, const float* 
             contrib += p0 * q0;
         }
     }
-    double v[1] = {contrib}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double v[1] = {contrib};
+    partials_write<1>(v, partials);
+}
+__global__ void __launch_bounds__(1024) cgm_spmv_fin(CgState* st, const double* __restrict__ partials, int nparts) {
+    if (st->done) return;
+    double tot[1];
+    partials_sum<1>(partials, nparts, tot);
+    if (threadIdx.x == 0) {
         st->pq = tot[0];
         if (!(tot[0] > 0.0)) {
             st->status = (tot[0] == tot[0]) ? MK_ERR_NOT_SPD : MK_ERR_NAN;
@@ -375,7 +430,7 @@ __global__ void __launch_bounds__(256) cgm_spmv(SellDev<double> A, const float* 
 // p = q = zold = 0 ; rr = r.r ; state for the first iteration
 __global__ void __launch_bounds__(256) cgm_init2(size_t len, const double* __restrict__ r, double* __restrict__ p,
                                                double* __restrict__ q, double* __restrict__ zold, CgState* st,
-                                               double* partials, unsigned* counter) {
+                                               double* partials) {
     double rr = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x) {
         p[i] = 0.0;
@@ -383,8 +438,13 @@ __global__ void __launch_bounds__(256) cgm_init2(size_t len, const double* __res
         zold[i] = 0.0;
         rr += r[i] * r[i];
     }
-    double v[1] = {rr}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) {
+    double v[1] = {rr};
+    partials_write<1>(v, partials);
+}
+__global__ void __launch_bounds__(1024) cgm_init2_fin(CgState* st, const double* __restrict__ partials, int nparts) {
+    double tot[1];
+    partials_sum<1>(partials, nparts, tot);
+    if (threadIdx.x == 0) {
         st->rr = tot[0];
         st->rz = 1.0;
         st->it = 0;
@@ -415,19 +475,18 @@ __global__ void k_mass_mul(int n, const double* mass, const double* u, double* b
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) { b[2 * i] = mass[i] * u[2 * i]; b[2 * i + 1] = mass[i] * u[2 * i + 1]; }
 }
-__global__ void k_mass_dot(int n, const double* mass, const double* x, double* partials, unsigned* counter,
-                           double* out) {
+__global__ void k_mass_dot(int n, const double* mass, const double* x, double* partials) {
     double c = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += mass[i] * (x[2 * i] * x[2 * i] + x[2 * i + 1] * x[2 * i + 1]);
-    double v[1] = {c}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+    double v[1] = {c};
+    partials_write<1>(v, partials);
 }
-__global__ void k_dot(size_t len, const double* x, const double* y, double* partials, unsigned* counter, double* out) {
+__global__ void k_dot(size_t len, const double* x, const double* y, double* partials) {
     double c = 0.0;
     for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += size_t(gridDim.x) * blockDim.x)
         c += x[i] * y[i];
-    double v[1] = {c}, tot[1];
-    if (partials_last<1>(v, partials, counter, tot) && threadIdx.x == 0) out[0] = tot[0];
+    double v[1] = {c};
+    partials_write<1>(v, partials);
 }
 __global__ void k_cvt_scaled(size_t len, const double* x, const double* s, float* y) {
     const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -622,11 +681,10 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
     static const char* kname[] = {"spmv", "spmv_dot", "spmv_resid", "spmv_resid_nored", "spmv_accum"};
     SpmvTimer tm(c, bytes, A, kname[int(mode)], sizeof(T) == 4);
-    const unsigned gr = std::min(g, kRedGrid);  // reducing modes: capped grid
+    const unsigned gr = std::max(1u, std::min(g, fin_grid()));  // reducing modes: capped grid
 #define MK_SPMV(CPLX, MODE)                                                                                \
     hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
-                       c->stream, D, x, y, b,                                                              \
-                       c->partials.p, c->counter.p, out, skip)
+                       c->stream, D, x, y, b, c->partials.p, skip)
 #define MK_SPMV_ALL(CPLX)                            \
     switch (mode) {                                  \
         case SPMV_PLAIN: MK_SPMV(CPLX, 0); break;    \
@@ -638,6 +696,7 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
     if (A->cplx) { MK_SPMV_ALL(true) } else { MK_SPMV_ALL(false) }
 #undef MK_SPMV_ALL
 #undef MK_SPMV
+    if (mode == SPMV_DOT || mode == SPMV_RESID) finish<1, FinOut>(c, gr, out, skip);
     MK_HIP(hipGetLastError());
 }
 // One Chebyshev step fused with the residual it needs, row-local:
@@ -833,9 +892,9 @@ void scale_dev64(mk_ctx* c, size_t len, const double* s_dev, int invert_sqrt, do
     MK_HIP(hipGetLastError());
 }
 void dot64(mk_ctx* c, size_t len, const double* x, const double* y, double* out_dev) {
-    unsigned g = rgrid(len);
-    g = std::max(g, 1u);
-    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, c->stream, len, x, y, c->partials.p, c->counter.p, out_dev);
+    const unsigned g = fgrid(len);
+    hipLaunchKernelGGL(k_dot, dim3(g), dim3(256), 0, c->stream, len, x, y, c->partials.p);
+    finish<1, FinOut>(c, g, out_dev);
     MK_HIP(hipGetLastError());
 }
 void mass_mul(mk_ctx* c, int n, const double* mass, const double* u, double* b) {
@@ -843,8 +902,9 @@ void mass_mul(mk_ctx* c, int n, const double* mass, const double* u, double* b) 
     MK_HIP(hipGetLastError());
 }
 void mass_dot(mk_ctx* c, int n, const double* mass, const double* x, double* out_dev) {
-    hipLaunchKernelGGL(k_mass_dot, dim3(rgrid(n)), dim3(256), 0, c->stream, n, mass, x, c->partials.p,
-                       c->counter.p, out_dev);
+    const unsigned g = fgrid(size_t(n));
+    hipLaunchKernelGGL(k_mass_dot, dim3(g), dim3(256), 0, c->stream, n, mass, x, c->partials.p);
+    finish<1, FinOut>(c, g, out_dev);
     MK_HIP(hipGetLastError());
 }
 void cvt64to32_scaled(mk_ctx* c, size_t len, const double* x, const double* s_dev, float* y) {
@@ -866,13 +926,17 @@ template <typename T>
 static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, T* q, double bytes) {
     unsigned g = 0;
     const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
-    SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
-    if (A->cplx)
-        hipLaunchKernelGGL((cg_spmv<T, true>), dim3(std::min(g, kRedGrid)), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
-                           c->partials.p, c->counter.p);
-    else
-        hipLaunchKernelGGL((cg_spmv<T, false>), dim3(std::min(g, kRedGrid)), dim3(A->block), 0, c->stream, D, p, q, c->state.p,
-                           c->partials.p, c->counter.p);
+    const unsigned gr = std::max(1u, std::min(g, fin_grid()));
+    {
+        SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
+        if (A->cplx)
+            hipLaunchKernelGGL((cg_spmv<T, true>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+                               (const CgState*)c->state.p, c->partials.p);
+        else
+            hipLaunchKernelGGL((cg_spmv<T, false>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+                               (const CgState*)c->state.p, c->partials.p);
+    }
+    finish<1, FinCgAlpha>(c, gr);
 }
 
 // Runs `batch` calls of body() on the context stream: eagerly, or -- when graphs are on, timing
@@ -932,20 +996,23 @@ static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T*
     const size_t len = size_t(n) * (A->cplx ? 2 : 1);
     const unsigned g = blocks_for(n);
     CgState* st = c->state.p;
+    const unsigned gn = fgrid(size_t(n));
     launch_cg_spmv<T>(c, A, val, p, q, bytes);
     if (A->cplx)
-        hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, p, q, st,
-                           c->partials.p, c->counter.p);
+        hipLaunchKernelGGL((cg_update_noz<T, true>), dim3(gn), dim3(256), 0, c->stream, n, x, r, p, q,
+                           (const CgState*)st, c->partials.p);
     else
-        hipLaunchKernelGGL((cg_update_noz<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, p, q, st,
-                           c->partials.p, c->counter.p);
+        hipLaunchKernelGGL((cg_update_noz<T, false>), dim3(gn), dim3(256), 0, c->stream, n, x, r, p, q,
+                           (const CgState*)st, c->partials.p);
+    finish<1, FinCgRr>(c, gn);
     mg_apply(c, mg, r, z, &st->done);
     if (A->cplx)
-        hipLaunchKernelGGL((cg_beta<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
-                           c->counter.p);
+        hipLaunchKernelGGL((cg_beta<T, true>), dim3(gn), dim3(256), 0, c->stream, n, r, z, (const CgState*)st,
+                           c->partials.p);
     else
-        hipLaunchKernelGGL((cg_beta<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, st, c->partials.p,
-                           c->counter.p);
+        hipLaunchKernelGGL((cg_beta<T, false>), dim3(gn), dim3(256), 0, c->stream, n, r, z, (const CgState*)st,
+                           c->partials.p);
+    finish<1, FinCgBeta>(c, gn);
     hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z, (const CgState*)st);
 }
 
@@ -959,20 +1026,21 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
     MK_CHECK(g <= kMaxParts, MK_ERR_ARG, "operator too large for the reduction buffer");
     CgState* st = c->state.p;
     hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, bb_dev, rel_tol * rel_tol);
+    const unsigned gn = fgrid(size_t(n));
     if (mg) {
         mg_apply(c, mg, r, z, nullptr);
         if (A->cplx)
-            hipLaunchKernelGGL((cg_init_z<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
-                               c->counter.p);
+            hipLaunchKernelGGL((cg_init_z<T, true>), dim3(gn), dim3(256), 0, c->stream, n, r, z, p, c->partials.p);
         else
-            hipLaunchKernelGGL((cg_init_z<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, z, p, st, c->partials.p,
-                               c->counter.p);
-    } else if (A->cplx)
-        hipLaunchKernelGGL((cg_init<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
-                           c->partials.p, c->counter.p);
-    else
-        hipLaunchKernelGGL((cg_init<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, r, dinv, z, p, st,
-                           c->partials.p, c->counter.p);
+            hipLaunchKernelGGL((cg_init_z<T, false>), dim3(gn), dim3(256), 0, c->stream, n, r, z, p, c->partials.p);
+        finish<2, FinCgInitZ>(c, gn);
+    } else {
+        if (A->cplx)
+            hipLaunchKernelGGL((cg_init<T, true>), dim3(gn), dim3(256), 0, c->stream, n, r, dinv, z, p, c->partials.p);
+        else
+            hipLaunchKernelGGL((cg_init<T, false>), dim3(gn), dim3(256), 0, c->stream, n, r, dinv, z, p, c->partials.p);
+        finish<2, FinCgInit>(c, gn);
+    }
     MK_HIP(hipGetLastError());
     if (mg) {
         CgState h{};
@@ -1002,11 +1070,12 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
         for (int b = 0; b < batch; b++) {
             launch_cg_spmv<T>(c, A, val, p, q, bytes);
             if (A->cplx)
-                hipLaunchKernelGGL((cg_update<T, true>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
-                                   st, c->partials.p, c->counter.p);
+                hipLaunchKernelGGL((cg_update<T, true>), dim3(gn), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                                   (const CgState*)st, c->partials.p);
             else
-                hipLaunchKernelGGL((cg_update<T, false>), dim3(rgrid(n)), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
-                                   st, c->partials.p, c->counter.p);
+                hipLaunchKernelGGL((cg_update<T, false>), dim3(gn), dim3(256), 0, c->stream, n, x, r, z, p, q, dinv,
+                                   (const CgState*)st, c->partials.p);
+            finish<2, FinCgUpdate>(c, gn);
             hipLaunchKernelGGL(cg_pupdate<T>, dim3(blocks_for(len)), dim3(256), 0, c->stream, len, p, z,
                                (const CgState*)st);
             launched++;
@@ -1062,28 +1131,33 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
     CgState* st = c->state.p;
     hipLaunchKernelGGL(k_set_state_dev, dim3(1), dim3(1), 0, c->stream, st, (const double*)bb, rel_tol * rel_tol);
     hipLaunchKernelGGL(k_to32, dim3(g), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.r32.p);
-    hipLaunchKernelGGL(cgm_init2, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.p.p, w.q.p,
-                       w.z.p, st, c->partials.p, c->counter.p);
+    const unsigned gl = fgrid(len);
+    hipLaunchKernelGGL(cgm_init2, dim3(gl), dim3(256), 0, c->stream, len, (const double*)w.r.p, w.p.p, w.q.p,
+                       w.z.p, st, c->partials.p);
+    hipLaunchKernelGGL(cgm_init2_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gl));
     mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
     MK_HIP(hipGetLastError());
     unsigned gs = 0;
     const SellDev<double> D = sell_dev<double>(A, A->val64.p, A->block, &gs);
-    const unsigned gsr = std::min(gs, kRedGrid);
+    const unsigned gsr = std::max(1u, std::min(gs, fin_grid()));
     // one iteration: beta, p/q update with A z (fused), x/r update, V-cycle on the new residual
     auto iteration = [&] {
-        hipLaunchKernelGGL(cgm_beta2, dim3(rgrid(len)), dim3(256), 0, c->stream, len, (const double*)w.r.p,
-                           (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p, c->counter.p);
+        hipLaunchKernelGGL(cgm_beta2, dim3(gl), dim3(256), 0, c->stream, len, (const double*)w.r.p,
+                           (const float*)w.z32.p, (const double*)w.z.p, st, c->partials.p);
+        hipLaunchKernelGGL(cgm_beta2_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gl));
         {
             SpmvTimer tm(c, A->bytes64 + double(len) * (4.0 + 8.0 * 5.0), A, "cg_spmv", false);
             if (A->cplx)
                 hipLaunchKernelGGL((cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
-                                   w.p.p, w.q.p, w.z.p, st, c->partials.p, c->counter.p);
+                                   w.p.p, w.q.p, w.z.p, st, c->partials.p);
             else
                 hipLaunchKernelGGL((cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
-                                   w.p.p, w.q.p, w.z.p, st, c->partials.p, c->counter.p);
+                                   w.p.p, w.q.p, w.z.p, st, c->partials.p);
         }
-        hipLaunchKernelGGL(cgm_update, dim3(rgrid(len)), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
-                           (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p, c->counter.p);
+        hipLaunchKernelGGL(cgm_spmv_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gsr));
+        hipLaunchKernelGGL(cgm_update, dim3(gl), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,
+                           (const double*)w.p.p, (const double*)w.q.p, st, c->partials.p);
+        hipLaunchKernelGGL(cgm_update_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gl));
         mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
     };
     CgState h{};
@@ -1104,8 +1178,12 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
 }
 
 // Iterative refinement: FP64 residual r = b - A x, FP32 PCG correction on r/||r||.
-SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o, Mg* pre) {
+SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o, Mg* pre,
+                      double target) {
     MK_CHECK(o.precision >= MK_PREC_FP64 && o.precision <= MK_PREC_MIXED_MG, MK_ERR_ARG, "unknown precision mode");
+    // goal: where the iteration stops; o.tol: what the result must meet (a goal below the FP64
+    // floor eps ||A|| ||x|| / ||b|| of the true residual is met as far as that floor allows)
+    const double goal = (target > 0.0 && target < o.tol) ? target : o.tol;
     const bool jac = o.precond == MK_PRECOND_JACOBI;
     Mg* mg = (o.precond == MK_PRECOND_AMG) ? (pre ? pre : A->mg) : nullptr;
     MK_CHECK(o.precond != MK_PRECOND_AMG || mg, MK_ERR_ARG, "AMG preconditioner requested but not built");
@@ -1117,8 +1195,8 @@ SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, con
         // the recurrences can drift from the true residual: it is recomputed in FP64 and the
         // solve restarts from x (at most twice) while it is above the tolerance
         for (int attempt = 0;; attempt++) {
-            const CgResult cr = mg32 ? cg64_mg32(c, A, b, x, o.tol, o.maxiter - res.iters, o.check_every, mg)
-                                     : cg64(c, A, b, x, o.tol, o.maxiter - res.iters, jac, o.check_every, false, mg);
+            const CgResult cr = mg32 ? cg64_mg32(c, A, b, x, goal, o.maxiter - res.iters, o.check_every, mg)
+                                     : cg64(c, A, b, x, goal, o.maxiter - res.iters, jac, o.check_every, false, mg);
             res.iters += cr.iters;
             Work& w = work(c);
             double h[2];
@@ -1165,13 +1243,13 @@ SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, con
         const double rel = std::sqrt(hrr / hbb);
         res.rel_residual = rel;
         if (!(rel == rel)) { res.status = MK_ERR_NAN; break; }
-        if (rel <= o.tol) break;
-        if (res.refinements >= o.max_refine) { res.status = MK_ERR_NO_CONVERGENCE; break; }
-        if (rel > 0.5 * prev) { if (++stagn >= 3) { res.status = MK_ERR_STAGNATION; break; } }
+        if (rel <= goal) break;
+        if (res.refinements >= o.max_refine) { res.status = rel <= o.tol ? MK_OK : MK_ERR_NO_CONVERGENCE; break; }
+        if (rel > 0.5 * prev) { if (++stagn >= 3) { res.status = rel <= o.tol ? MK_OK : MK_ERR_STAGNATION; break; } }
         else stagn = 0;
         prev = rel;
         cvt64to32_scaled(c, len, w.t.p, rr, rhs32.p);
-        const double itol = std::max(o.inner_tol, 0.5 * o.tol / rel);
+        const double itol = std::max(o.inner_tol, 0.5 * goal / rel);
         CgResult cr = cg32(c, A, rhs32.p, d32.p, std::min(itol, 0.5), o.maxiter - res.iters, jac, o.check_every, mg);
         res.iters += cr.iters;
         res.refinements++;

@@ -641,6 +641,7 @@ template <typename T>
 Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta, int coarse_max,
                const std::vector<T>* nullvec = nullptr, int nnull = 0) {
     HostHier<T> H = build_host<T>(A0, theta, coarse_max, nullvec, nnull);
+    const auto t_up = std::chrono::steady_clock::now();
     auto* g = new Mg();
     try {
         g->cplx = !std::is_same<T, double>::value;
@@ -675,6 +676,8 @@ Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta
             L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len); L.y64.alloc(len);
             L.b32.alloc(len); L.x32.alloc(len); L.r32.alloc(len); L.d32.alloc(len); L.y32.alloc(len);
         }
+        MK_LOG("[mg] hierarchy upload %.0f ms\n",
+               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count());
     } catch (...) {
         mg_free(g);
         throw;

@@ -129,7 +129,6 @@ struct mk_ctx {
     std::string last_error;
     // reduction scratch: partial sums [kMaxParts * kMaxK], arrival counter, CG state
     mk::DevBuf<double> partials;
-    mk::DevBuf<unsigned int> counter;
     mk::DevBuf<mk::CgState> state;
     mk::DevBuf<double> scal;   // general device scalars
     // SpMV timing (events around each SpMV launch when enabled), attributed per
@@ -260,14 +259,20 @@ CgResult cg64(mk_ctx* c, const mk_op* A, const double* b, double* x, double rel_
 CgResult cg32(mk_ctx* c, const mk_op* A, const float* b, float* x, double rel_tol, int maxiter, bool jacobi,
               int check_every, Mg* mg = nullptr);
 
-// refinement driver: x (device FP64) warm start, b device FP64; pre overrides A's own multigrid
+// refinement driver: x (device FP64) warm start, b device FP64; pre overrides A's own multigrid.
+// target (when below o.tol): the relative residual the iteration aims at; the result only has
+// to meet o.tol, so a target under the FP64 floor of the true residual costs no restarts.
 struct SolveResult { int status; int iters; int refinements; double rel_residual; };
 SolveResult solve_spd(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts& o,
-                      Mg* pre = nullptr);
+                      Mg* pre = nullptr, double target = 0.0);
 
 inline Work& work(mk_ctx* c) { return c->work; }
 
 void stats_reset(mk_ctx* c);
+// y = G^+ g for a Hermitian positive semi-definite m x m G (row-major); eigenvalues below
+// 1e-13 of the largest are dropped (Galerkin starts over nearly parallel windows).
+std::vector<std::complex<double>> herm_pinv_solve(int m, const std::vector<std::complex<double>>& G,
+                                                  const std::vector<std::complex<double>>& g);
 void stats_fill(mk_ctx* c, mk_stats* s);
 void timing_flush(mk_ctx* c);
 

@@ -39,67 +39,37 @@ __device__ __forceinline__ void block_sum(double (&v)[K]) {
     __syncthreads();
 }
 
-// Deterministic grid reduction with a two-level arrival ticket.  Each block writes its
-// partial sums; blocks arrive on the counter of their group of kGroup blocks, the last
-// block of a group sums the group's partials (fixed order) into a group slot and arrives on
-// the global counter, and the last group to finish sums the group slots (fixed order).  A
-// single global ticket would serialise every block of a large grid on one atomic address.
-// Returns true in the last block, with the grand totals in tot (all threads of that block).
-// Layout: partials[k*kMaxParts + block], group slots at partials[kMaxK*kMaxParts + k*kMaxGroups
-// + group]; counter[0] global, counter[1 + group] per group; counters are left at zero.
-static constexpr int kGroup = 32;
-static constexpr size_t kMaxGroups = kMaxParts / kGroup;
+// Fence-free grid reduction in two launches: the working kernel leaves one partial sum per
+// block (partials[k * kMaxParts + block]) and a one-block finishing kernel adds them in a fixed
+// order after the kernel boundary.  No device-scope fence or ticket per block: on a chip whose
+// L2 is private to each XCD a device-scope release writes back the block's L2, which costs far
+// more than the extra launch in kernels that stream large vectors.
 template <int K>
-__device__ __forceinline__ bool partials_last(double (&v)[K], double* partials, unsigned* counter, double (&tot)[K]) {
-    __shared__ int is_last;
-    __shared__ double shtot[K];
+__device__ __forceinline__ void partials_write(double (&v)[K], double* partials) {
     block_sum<K>(v);
-    const unsigned G = gridDim.x;
-    const unsigned grp = blockIdx.x / kGroup;
-    const unsigned ngrp = (G + kGroup - 1) / kGroup;
-    const unsigned gsize = (grp + 1 == ngrp) ? G - grp * kGroup : kGroup;
-    double* gslot = partials + size_t(kMaxK) * kMaxParts;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; k++) partials[k * kMaxParts + blockIdx.x] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(counter + 1 + grp, 1u);
-        is_last = 0;
-        if (t == gsize - 1) {  // last of its group: group sum, then the global ticket
-            __threadfence();
-            counter[1 + grp] = 0u;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const volatile double* pk = partials + k * kMaxParts + grp * kGroup;
-                double s = 0.0;
-                for (unsigned i = 0; i < gsize; i++) s += pk[i];
-                gslot[k * kMaxGroups + grp] = s;
-            }
-            __threadfence();
-            const unsigned t2 = atomicAdd(counter, 1u);
-            is_last = (t2 == ngrp - 1) ? 1 : 0;
-        }
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+}
+// Sum of nparts partials per value, fixed order; every thread of the (single) block gets tot.
+template <int K>
+__device__ __forceinline__ void partials_sum(const double* __restrict__ partials, int nparts, double (&tot)[K]) {
+    __shared__ double shtot[K];
     double acc[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         acc[k] = 0.0;
-        const volatile double* pk = gslot + k * kMaxGroups;
-        for (unsigned i = threadIdx.x; i < ngrp; i += blockDim.x) acc[k] += pk[i];
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) acc[k] += partials[k * kMaxParts + i];
     }
     block_sum<K>(acc);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < K; k++) shtot[k] = acc[k];
-        *counter = 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < K; k++) tot[k] = shtot[k];
-    return true;
 }
 
 template <typename T>
@@ -228,8 +198,27 @@ __device__ __forceinline__ bool spmv_row(const SellDev<T>& A, const T* __restric
 static inline unsigned blocks_for(size_t len, int bs = 256) { return unsigned((len + bs - 1) / bs); }
 // grid of the grid-stride reduction kernels: few enough blocks that the per-block arrival
 // (partial store, device fence, ticket) stays a small part of the kernel
-static constexpr unsigned kRedGrid = 512;
-static inline unsigned rgrid(size_t len) { return std::max(1u, std::min(blocks_for(len), kRedGrid)); }
+// Grid cap of the reducing kernels (grid-stride loops; partial sums per block): large enough
+// to keep every CU's memory pipeline busy, small enough for a short partial-sum pass.
+static inline unsigned red_grid() {
+    static const unsigned g = [] {
+        const char* s = std::getenv("METRIKO_RED_GRID");
+        const long v = s ? std::atol(s) : 0;
+        return v >= 64 && v <= 16384 ? unsigned(v) : 512u;
+    }();
+    return g;
+}
+static inline unsigned rgrid(size_t len) { return std::max(1u, std::min(blocks_for(len), red_grid())); }
+// Grid cap of the fence-free two-launch reductions (partials_write / partials_sum).
+static inline unsigned fin_grid() {
+    static const unsigned g = [] {
+        const char* s = std::getenv("METRIKO_FIN_GRID");
+        const long v = s ? std::atol(s) : 0;
+        return v >= 64 && v <= long(kMaxParts) ? unsigned(v) : 4096u;
+    }();
+    return g;
+}
+static inline unsigned fgrid(size_t len) { return std::max(1u, std::min(blocks_for(len), fin_grid())); }
 
 hipEvent_t next_event(mk_ctx* c);
 void timing_flush(mk_ctx* c);
