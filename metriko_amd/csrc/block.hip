@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     const V* __restrict__ Xv = reinterpret_cast<const V*>(X);
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR, j = lane - sub * LPR;
-    const int gw = int(blockIdx.x) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+    const int gw = xcd_block(int(blockIdx.x), int(gridDim.x)) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     V acc;
     vzero(acc);
     int row = -1;
@@ -109,7 +109,32 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             cols = A.long_col;
             vals = A.long_val;
         }
-        for (int64_t k = ptr[lw] + sub; k < ptr[lw + 1]; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
+        // a medium or long row is one wavefront's serial chain: eight independent gathers in
+        // flight per lane keep it off the kernel's critical path
+        {
+            V a1, a2, a3, a4, a5, a6, a7;
+            vzero(a1); vzero(a2); vzero(a3); vzero(a4); vzero(a5); vzero(a6); vzero(a7);
+            int64_t k = ptr[lw] + sub;
+            const int64_t e = ptr[lw + 1];
+            for (; k + 7 * R < e; k += 8 * R) {
+                const int c0 = cols[k], c1 = cols[k + R], c2 = cols[k + 2 * R], c3 = cols[k + 3 * R];
+                const int c4 = cols[k + 4 * R], c5 = cols[k + 5 * R], c6 = cols[k + 6 * R], c7 = cols[k + 7 * R];
+                const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
+                const V x2 = Xv[size_t(c2) * LPR + j], x3 = Xv[size_t(c3) * LPR + j];
+                const V x4 = Xv[size_t(c4) * LPR + j], x5 = Xv[size_t(c5) * LPR + j];
+                const V x6 = Xv[size_t(c6) * LPR + j], x7 = Xv[size_t(c7) * LPR + j];
+                vfma(acc, vals[k], x0);
+                vfma(a1, vals[k + R], x1);
+                vfma(a2, vals[k + 2 * R], x2);
+                vfma(a3, vals[k + 3 * R], x3);
+                vfma(a4, vals[k + 4 * R], x4);
+                vfma(a5, vals[k + 5 * R], x5);
+                vfma(a6, vals[k + 6 * R], x6);
+                vfma(a7, vals[k + 7 * R], x7);
+            }
+            for (; k < e; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
+            acc = vadd(vadd(vadd(acc, a1), vadd(a2, a3)), vadd(vadd(a4, a5), vadd(a6, a7)));
+        }
 #pragma unroll
         for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
         if (sub != 0) return;

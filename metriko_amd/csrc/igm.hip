@@ -13,6 +13,7 @@
 //    x* = xt + Z (wK^-2 I + U Z)^{-1} (r - U xt), xt = A0^{-1} b0, Z = A0^{-1} U^T
 //    (Sherman-Morrison-Woodbury; no 1e10-weighted quantity is ever formed).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -286,8 +287,77 @@ bool lm_solve(Ctx& K, size_t n, double* x0, double* x_out, int maxIter, double f
 // FullPivLU::kernel puts at each kernel column's free variable.  Returns n*k values
 // (column-major), or nothing when a step does not apply or the vectors are not null to
 // round-off; only the multigrid convergence rate depends on them.
-std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F, int n, const int* intIdx, int nint,
-                              const std::vector<int>& nonPart, const std::vector<int>& uniqueJ,
+// Non-owning CSR view (the caller's G2 arrays are used in place).
+struct CsrView {
+    int nr = 0, nc = 0;
+    const int64_t* ptr = nullptr;
+    const int* col = nullptr;
+    const double* val = nullptr;
+    int rows() const { return nr; }
+    int cols() const { return nc; }
+    static CsrView of(const SprsD& A) { return {A.rows(), A.cols(), A.ptr.data(), A.col.data(), A.val.data()}; }
+};
+
+// X2F = G2 * UFull from the caller's G2 arrays.  UFull rows hold one entry (unconstrained
+// DOFs) or a USmall row (the few constrained DOFs), so an X2F row is the merge of a handful of
+// products: expanded, sorted by column (stable), duplicates summed in order and exact zeros
+// dropped (the SpGEMM + pruned() result).  Two OpenMP passes (count, fill); a G2 column out
+// of range is reported as MK_ERR_ARG.
+SprsD x2f_product(const CsrView& G2, const SprsD& UF) {
+    const int rows = G2.nr;
+    SprsD X(rows, UF.cols());
+    X.ptr.assign(size_t(rows) + 1, 0);
+    std::atomic<int> bad{0};
+    auto expand = [&](int r, std::vector<std::pair<int, double>>& buf) {
+        buf.clear();
+        for (int64_t q = G2.ptr[r]; q < G2.ptr[r + 1]; q++) {
+            const int c = G2.col[q];
+            if (c < 0 || c >= G2.nc) { bad.store(1, std::memory_order_relaxed); continue; }
+            for (int64_t t = UF.ptr[c]; t < UF.ptr[c + 1]; t++) buf.emplace_back(UF.col[t], G2.val[q] * UF.val[t]);
+        }
+        for (size_t i = 1; i < buf.size(); i++) {  // stable insertion sort (a few entries)
+            const auto e = buf[i];
+            size_t j = i;
+            while (j > 0 && buf[j - 1].first > e.first) { buf[j] = buf[j - 1]; j--; }
+            buf[j] = e;
+        }
+        size_t w = 0;
+        for (size_t i = 0; i < buf.size();) {
+            const int cc = buf[i].first;
+            double sum = 0.0;
+            while (i < buf.size() && buf[i].first == cc) sum += buf[i++].second;
+            if (sum != 0.0) buf[w++] = {cc, sum};
+        }
+        buf.resize(w);
+    };
+#pragma omp parallel
+    {
+        std::vector<std::pair<int, double>> buf;
+#pragma omp for schedule(static, 4096)
+        for (int r = 0; r < rows; r++) {
+            expand(r, buf);
+            X.ptr[size_t(r) + 1] = int64_t(buf.size());
+        }
+    }
+    if (bad.load()) throw Error(MK_ERR_ARG, "G2 column out of range");
+    for (int r = 0; r < rows; r++) X.ptr[size_t(r) + 1] += X.ptr[size_t(r)];
+    X.col.resize(size_t(X.ptr[size_t(rows)]));
+    X.val.resize(size_t(X.ptr[size_t(rows)]));
+#pragma omp parallel
+    {
+        std::vector<std::pair<int, double>> buf;
+#pragma omp for schedule(static, 4096)
+        for (int r = 0; r < rows; r++) {
+            expand(r, buf);
+            int64_t w = X.ptr[size_t(r)];
+            for (const auto& e : buf) { X.col[size_t(w)] = e.first; X.val[size_t(w)] = e.second; w++; }
+        }
+    }
+    return X;
+}
+
+std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD& X2F, int n, const int* intIdx,
+                              int nint, const std::vector<int>& nonPart, const std::vector<int>& uniqueJ,
                               const metriko::MatXd& US) {
     const int nfull = G2.cols(), k = X2F.cols(), npn = int(nonPart.size());
     if (n < 1 || n > 4) return {};
@@ -305,16 +375,16 @@ std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F
     // S = [G2; Cfull]: only its rows that touch a transition DOF matter for the transition
     // solve; the normal matrix over the transition DOFs is small and dense
     if (nint > 4096) return {};
-    struct SRow { const SprsD* A; int row; };
+    struct SRow { const CsrView* A; int row; };
     std::vector<SRow> trows;
-    for (const SprsD* A : {&G2, &Cf})
+    for (const CsrView* A : {&G2, &Cf})
         for (int i = 0; i < A->rows(); i++)
             for (int64_t q = A->ptr[i]; q < A->ptr[i + 1]; q++)
                 if (tpos[A->col[q]] >= 0) { trows.push_back({A, i}); break; }
     const size_t nt = size_t(std::max(nint, 1));
     std::vector<double> Mt(nt * nt, 0.0);  // sum over those rows of the outer product of transition entries
     for (const SRow& t : trows) {
-        const SprsD& A = *t.A;
+        const CsrView& A = *t.A;
         for (int64_t p = A.ptr[t.row]; p < A.ptr[t.row + 1]; p++) {
             const int a = tpos[A.col[p]];
             if (a < 0) continue;
@@ -332,7 +402,7 @@ std::vector<double> near_null(const SprsD& G2, const SprsD& Cf, const SprsD& X2F
             // b = -GT^T (S y), over the rows that touch a transition DOF
             std::vector<double> b(static_cast<size_t>(nint), 0.0), x(static_cast<size_t>(nint), 0.0);
             for (const SRow& t : trows) {
-                const SprsD& A = *t.A;
+                const CsrView& A = *t.A;
                 double sy = 0.0;
                 for (int64_t q = A.ptr[t.row]; q < A.ptr[t.row + 1]; q++) sy += A.val[q] * y[A.col[q]];
                 for (int64_t q = A.ptr[t.row]; q < A.ptr[t.row + 1]; q++)
@@ -426,11 +496,9 @@ extern "C" mk_status mk_iterative_rounding(
         for (int i = 0; i < nsing; i++) MK_CHECK(singularIdcs[i] >= 0 && singularIdcs[i] < nfull, MK_ERR_ARG, "singularIdcs");
         for (int i = 0; i < nint; i++) MK_CHECK(integerIdcs[i] >= 0 && integerIdcs[i] < nfull, MK_ERR_ARG, "integerIdcs");
         // ---- host setup (iter_rounding_init.h:195-251) ----
-        SprsD G2(g_rows, g_cols);
-        G2.ptr.assign(g_ptr, g_ptr + g_rows + 1);
-        G2.col.assign(g_col, g_col + g_ptr[g_rows]);
-        G2.val.assign(g_val, g_val + g_ptr[g_rows]);
-        for (int64_t k = 0; k < G2.nonZeros(); k++) MK_CHECK(G2.col[k] >= 0 && G2.col[k] < g_cols, MK_ERR_ARG, "G2 column");
+        MK_CHECK(g_rows >= 0 && g_cols > 0 && g_ptr && g_ptr[0] == 0, MK_ERR_ARG, "G2");
+        for (int i = 0; i < g_rows; i++) MK_CHECK(g_ptr[i + 1] >= g_ptr[i], MK_ERR_ARG, "G2 row pointers");
+        const CsrView G2{g_rows, g_cols, g_ptr, g_col, g_val};  // the caller's arrays, in place
         SprsD Cf(c_rows, c_cols);
         if (c_rows > 0) {
             Cf.ptr.assign(c_ptr, c_ptr + c_rows + 1);
@@ -476,7 +544,7 @@ extern "C" mk_status mk_iterative_rounding(
             UF.setFromTriplets(T);
         }
         MK_LOG("[igm] UFull %d x %d nnz %lld, %.0f ms\n", nfull, k, (long long)UF.nonZeros(), now_ms() - t_all);
-        SprsD X2F = (G2 * UF).pruned();      // G2 * UFull (naive, without length)
+        SprsD X2F = x2f_product(G2, UF);     // G2 * UFull (naive, without length)
         MK_LOG("[igm] X2F nnz %lld, %.0f ms\n", (long long)X2F.nonZeros(), now_ms() - t_all);
         SprsD X2Ft = X2F.transpose();
         MK_LOG("[igm] X2F^T %.0f ms\n", now_ms() - t_all);
@@ -524,7 +592,7 @@ extern "C" mk_status mk_iterative_rounding(
             MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
             hA0.op->label = "igm_A0";
             if (o.precond == MK_PRECOND_AMG) {
-                hA0.op->nullvec = near_null(G2, Cf, X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
+                hA0.op->nullvec = near_null(G2, CsrView::of(Cf), X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
                 hA0.op->nnull = hA0.op->nullvec.empty() ? 0 : n;
                 S.null_vectors = hA0.op->nnull;
             }

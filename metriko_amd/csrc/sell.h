@@ -72,6 +72,16 @@ __device__ __forceinline__ void partials_sum(const double* __restrict__ partials
     for (int k = 0; k < K; k++) tot[k] = shtot[k];
 }
 
+// The blocks of a launch go round-robin over the 8 XCDs, each with its own L2.  xcd_block maps
+// the hardware block id b in [0, nb) to a logical block id so that every XCD works through one
+// contiguous range of logical blocks: neighbouring rows then find the x entries they gather
+// in their own XCD's L2.  A bijection for any nb.
+static constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % kXcd, q = nb / kXcd, r = nb % kXcd;
+    return x * q + (x < r ? x : r) + b / kXcd;
+}
+
 template <typename T>
 struct SellDev {
     int n_short, C, nb_sell, nlong;
