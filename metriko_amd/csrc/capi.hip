@@ -330,12 +330,19 @@ void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
     std::vector<int64_t> rp;
     std::vector<int> cc;
     std::vector<double> vv;
-    const double t0 = now_ms();
-    host_csr(c, A, rp, cc, vv);
-    MK_LOG("[mg] host CSR of the operator %.0f ms\n", now_ms() - t0);
-    A->mg = A->cplx ? mg_build_complex(c, A->n, rp.data(), cc.data(), vv.data(), o)
-                    : mg_build_real(c, A->n, rp.data(), cc.data(), vv.data(), o,
-                                    A->nnull ? &A->nullvec : nullptr, A->nnull);
+    const int64_t* pp = A->h_ptr.data();
+    const int* pc = A->h_col.data();
+    const double* pv = A->h_val.data();
+    if (A->h_ptr.empty()) {  // device-assembled operator: export it
+        const double t0 = now_ms();
+        host_csr(c, A, rp, cc, vv);
+        MK_LOG("[mg] host CSR of the operator %.0f ms\n", now_ms() - t0);
+        pp = rp.data();
+        pc = cc.data();
+        pv = vv.data();
+    }
+    A->mg = A->cplx ? mg_build_complex(c, A->n, pp, pc, pv, o)
+                    : mg_build_real(c, A->n, pp, pc, pv, o, A->nnull ? &A->nullvec : nullptr, A->nnull);
     mg_set_labels(A->mg, A->label);
 }
 
@@ -350,6 +357,36 @@ void stats_fill(mk_ctx* c, mk_stats* s) {
     s->spmv_ms = c->spmv_ms;
     s->spmv_calls = c->spmv_calls;
     s->spmv_bytes = c->spmv_bytes;
+}
+}  // namespace mk
+
+namespace mk {
+// A real operator that takes over a host CSR (no copy of its arrays); same layout as
+// mk_op_create_real.
+mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector<int>&& col,
+                       std::vector<double>&& val, const mk_solve_opts& o) {
+    MK_CHECK(n > 0 && ptr.size() == size_t(n) + 1 && col.size() == size_t(ptr[size_t(n)]) && val.size() == col.size(),
+             MK_ERR_ARG, "operator arrays");
+    MK_HIP(hipSetDevice(c->device));
+    auto* op = new mk_op();
+    try {
+        op->ctx = c;
+        op->n = n;
+        op->m = n;
+        op->cplx = false;
+        op->nnz = ptr[size_t(n)];
+        op->block = o.block_size;
+        MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 1024, MK_ERR_ARG, "block_size");
+        op->h_ptr = std::move(ptr);
+        op->h_col = std::move(col);
+        op->h_val = std::move(val);
+        const int C = o.format == MK_FMT_CSR ? 64 : o.slice_height;
+        build_sell(op, op->h_ptr.data(), op->h_col.data(), op->h_val.data(), C, o.format == MK_FMT_CSR ? 1 : o.sigma);
+    } catch (...) {
+        delete op;
+        throw;
+    }
+    return op;
 }
 }  // namespace mk
 
@@ -500,6 +537,7 @@ static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_
         *out = op;
     });
 }
+
 
 mk_status mk_op_create_real(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
                             const mk_solve_opts* o, mk_op** op) {

@@ -194,6 +194,25 @@ struct Ctx {
     }
 };
 
+// Values of s A + d I for a matrix whose rows all hold their diagonal entry (E = X2F^T X2F
+// of a variable set every column of X2F touches); OpenMP over rows.
+std::vector<double> scaled_shifted(const SprsD& A, double s, double d) {
+    const int n = A.rows();
+    std::vector<double> v(A.val.size());
+    std::atomic<int> missing{0};
+#pragma omp parallel for schedule(static, 4096)
+    for (int i = 0; i < n; i++) {
+        bool diag = false;
+        for (int64_t q = A.ptr[i]; q < A.ptr[i + 1]; q++) {
+            v[size_t(q)] = s * A.val[size_t(q)];
+            if (A.col[size_t(q)] == i) { v[size_t(q)] += d; diag = true; }
+        }
+        if (!diag) missing.store(1, std::memory_order_relaxed);
+    }
+    if (missing.load()) throw Error(MK_ERR_SINGULAR, "rounding operator: a variable no gradient row touches");
+    return v;
+}
+
 mk_op* make_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
     mk_op* op = nullptr;
     mk_status s = mk_op_create_real(c, A.rows(), A.ptr.data(), A.col.data(), A.val.data(), &o, &op);
@@ -585,10 +604,8 @@ extern "C" mk_status mk_iterative_rounding(
         const double wI = 1.0, wK = 10e4, wC = 0.01;
         {
             const double ts = now_ms();
-            SprsD A0 = E;
-            for (auto& v : A0.val) v *= length * wI * wI;
-            A0 = A0 + SprsD::identity(k, wC * wC);
-            hA0.op = make_op(c, A0, o);
+            hA0.op = mk::op_create_owned(c, k, std::vector<int64_t>(E.ptr), std::vector<int>(E.col),
+                                         scaled_shifted(E, length * wI * wI, wC * wC), o);
             MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
             hA0.op->label = "igm_A0";
             if (o.precond == MK_PRECOND_AMG) {

@@ -10,6 +10,8 @@
 // block to arrive (atomic ticket) sums the partials in a fixed order.  Reruns are
 // therefore bit-identical.
 #include <algorithm>
+#include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -516,20 +518,58 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     const int vw = op->cplx ? 2 : 1;
     MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
     MK_CHECK(sigma >= 1, MK_ERR_ARG, "sigma must be >= 1");
+    const auto t_start = std::chrono::steady_clock::now();
+    auto lap_ms = [&] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    };
     // long rows (> max(64, 4 x median length)) go to the wavefront-per-row part
     std::vector<int> lens(static_cast<size_t>(n));
+#pragma omp parallel for schedule(static, 65536) if (n > 100000)
     for (int i = 0; i < n; i++) lens[i] = int(rowptr[i + 1] - rowptr[i]);
+    // median row length (the element at sorted position n/2) from a length histogram
     int med = 0;
     if (n > 0) {
-        std::vector<int> s(lens);
-        std::nth_element(s.begin(), s.begin() + n / 2, s.end());
-        med = s[size_t(n / 2)];
+        constexpr int kHist = 4096;
+        std::vector<int64_t> hist(kHist + 1, 0);
+#pragma omp parallel if (n > 100000)
+        {
+            std::vector<int64_t> h(kHist + 1, 0);
+#pragma omp for schedule(static, 65536) nowait
+            for (int i = 0; i < n; i++) h[size_t(std::min(lens[i], kHist))]++;
+#pragma omp critical
+            for (int b = 0; b <= kHist; b++) hist[size_t(b)] += h[size_t(b)];
+        }
+        int64_t cum = 0;
+        for (med = 0; med < kHist; med++) {
+            cum += hist[size_t(med)];
+            if (cum > n / 2) break;
+        }
     }
     const int thr = std::max(64, 4 * med);
-    // medium rows (more than kMidLen entries) get a group of lanes each instead of one lane
+    // medium rows (more than kMidLen entries) get a group of lanes each instead of one lane;
+    // the three classes keep ascending row order (chunked counts, then a parallel fill)
     constexpr int kMidLen = 12;
-    std::vector<int> order, lrows, mrows;
-    for (int i = 0; i < n; i++) (lens[i] > thr ? lrows : lens[i] > kMidLen ? mrows : order).push_back(i);
+    auto cls = [&](int i) { return lens[i] > thr ? 2 : lens[i] > kMidLen ? 1 : 0; };
+    constexpr int kChunk = 65536;
+    const int nch = (n + kChunk - 1) / kChunk;
+    std::vector<int> ccount(size_t(nch) * 3, 0);
+#pragma omp parallel for schedule(static) if (n > 100000)
+    for (int ch = 0; ch < nch; ch++)
+        for (int i = ch * kChunk; i < std::min(n, (ch + 1) * kChunk); i++) ccount[size_t(ch) * 3 + cls(i)]++;
+    std::vector<int> coff(size_t(nch) * 3, 0);
+    int tot3[3] = {0, 0, 0};
+    for (int k3 = 0; k3 < 3; k3++)
+        for (int ch = 0; ch < nch; ch++) { coff[size_t(ch) * 3 + k3] = tot3[k3]; tot3[k3] += ccount[size_t(ch) * 3 + k3]; }
+    std::vector<int> order(static_cast<size_t>(tot3[0])), mrows(static_cast<size_t>(tot3[1])),
+        lrows(static_cast<size_t>(tot3[2]));
+#pragma omp parallel for schedule(static) if (n > 100000)
+    for (int ch = 0; ch < nch; ch++) {
+        int w[3] = {coff[size_t(ch) * 3], coff[size_t(ch) * 3 + 1], coff[size_t(ch) * 3 + 2]};
+        for (int i = ch * kChunk; i < std::min(n, (ch + 1) * kChunk); i++) {
+            const int k3 = cls(i);
+            (k3 == 0 ? order : k3 == 1 ? mrows : lrows)[size_t(w[k3]++)] = i;
+        }
+    }
     const int ns_rows = int(order.size());
     // remaining rows of uneven length: sort inside windows of 1024 rows
     if (sigma <= 1 && ns_rows > 0) {
@@ -553,16 +593,19 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     const int ns = (n_rows + C - 1) / C;
     std::vector<int64_t> sptr(size_t(ns) + 1, 0);
     std::vector<int> sw(size_t(ns), 0);
+#pragma omp parallel for schedule(static, 1024) if (ns > 10000)
     for (int s = 0; s < ns; s++) {
         int w = 0;
         for (int l = 0; l < C && s * C + l < n_rows; l++) w = std::max<int>(w, lens[order[s * C + l]]);
         sw[s] = w;
-        sptr[s + 1] = sptr[s] + int64_t(w) * C;
     }
+    for (int s = 0; s < ns; s++) sptr[s + 1] = sptr[s] + int64_t(sw[s]) * C;
     const int64_t stored = sptr[ns];
-    std::vector<int> hcol(static_cast<size_t>(stored));
-    std::vector<double> hval(size_t(stored) * vw, 0.0);
-    std::vector<float> hval32(size_t(stored) * vw, 0.0f);
+    const double t_layout = lap_ms();
+    // filled completely below (padding included): no zero-initialisation pass
+    std::unique_ptr<int[]> hcol(new int[size_t(std::max<int64_t>(stored, 1))]);
+    std::unique_ptr<double[]> hval(new double[size_t(std::max<int64_t>(stored, 1)) * vw]);
+    std::unique_ptr<float[]> hval32(new float[size_t(std::max<int64_t>(stored, 1)) * vw]);
 #pragma omp parallel for schedule(dynamic, 64) if (stored > 200000)
     for (int s = 0; s < ns; s++) {
         for (int l = 0; l < C; l++) {
@@ -580,6 +623,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
                     }
                 } else {
                     hcol[e] = (row >= 0 && row < op->m) ? row : 0;  // padding: valid column, zero value
+                    for (int c = 0; c < vw; c++) { hval[e * vw + c] = 0.0; hval32[e * vw + c] = 0.0f; }
                 }
             }
         }
@@ -633,8 +677,12 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     }
     // keep every uploaded array non-empty (an operator may consist of long rows only)
     if (sw.empty()) sw.push_back(0);
-    if (hcol.empty()) { hcol.push_back(0); hval.assign(size_t(vw), 0.0); hval32.assign(size_t(vw), 0.0f); }
+    if (stored == 0) {
+        hcol[0] = 0;
+        for (int c = 0; c < vw; c++) { hval[c] = 0.0; hval32[c] = 0.0f; }
+    }
     if (order.empty()) order.push_back(0);
+    const double t_fill = lap_ms();
     hipStream_t st = op->ctx->stream;
     op->nslices = ns;
     op->stored = stored;  // SELL entries only (the long part is long_ptr[nlong] entries)
@@ -642,9 +690,10 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->nlong = nlong;
     op->slice_ptr.upload(sptr.data(), sptr.size(), st);
     op->slice_w.upload(sw.data(), std::max<size_t>(sw.size(), 1), st);
-    op->col.upload(hcol.data(), std::max<size_t>(hcol.size(), 1), st);
-    op->val64.upload(hval.data(), std::max<size_t>(hval.size(), 1), st);
-    op->val32.upload(hval32.data(), std::max<size_t>(hval32.size(), 1), st);
+    const size_t nst = size_t(std::max<int64_t>(stored, 1));
+    op->col.upload(hcol.get(), nst, st);
+    op->val64.upload(hval.get(), nst * vw, st);
+    op->val32.upload(hval32.get(), nst * vw, st);
     op->dinv64.upload(dinv.data(), dinv.size(), st);
     op->dinv32.upload(dinv32.data(), dinv32.size(), st);
     if (permuted) op->perm.upload(order.data(), std::max<size_t>(order.size(), 1), st);
@@ -666,6 +715,9 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         op->long_val32.upload(lval32.data(), lval32.size(), st);
     }
     MK_HIP(hipStreamSynchronize(st));
+    if (n > 1000000)
+        MK_LOG("[sell] %s %d x %d, %lld stored: layout %.0f ms, fill %.0f ms, upload %.0f ms\n", op->label.c_str(), n,
+               op->m, (long long)stored, t_layout, t_fill - t_layout, lap_ms() - t_fill);
     const double sv64 = 8.0 * vw, sv32 = 4.0 * vw;
     const double meta = double(ns) * 12.0 + (permuted ? 4.0 * n : 0.0) + double(nlong + nmid) * 12.0;
     op->bytes64 = double(op->nnz) * (sv64 + 4.0) + meta + double(n) * sv64 * 2.0;

@@ -117,6 +117,8 @@ int mg_levels(const Mg* g);
 void mg_set_labels(Mg* g, const std::string& base);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
 void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
+mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector<int>&& col,
+                       std::vector<double>&& val, const mk_solve_opts& o);
 int64_t mg_nnz(const Mg* g);
 int mg_level_rows(const Mg* g, int l);
 
