@@ -548,7 +548,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     const int thr = std::max(64, 4 * med);
     // medium rows (more than kMidLen entries) get a group of lanes each instead of one lane;
     // the three classes keep ascending row order (chunked counts, then a parallel fill)
-    constexpr int kMidLen = 12;
+    constexpr int kMidLen = kSellMaxLen;
     auto cls = [&](int i) { return lens[i] > thr ? 2 : lens[i] > kMidLen ? 1 : 0; };
     constexpr int kChunk = 65536;
     const int nch = (n + kChunk - 1) / kChunk;

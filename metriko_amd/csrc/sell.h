@@ -10,6 +10,9 @@ namespace mk {
 static constexpr int kMaxWaves = 16;        // 1024 threads / 64
 static constexpr size_t kMaxParts = 1 << 17;  // partial-sum slots per reduced quantity
 static constexpr int kMaxK = 4;
+// Longest row kept in the SELL part; longer rows go to the medium (lane group per row) or
+// long (wavefront per row) parts at upload, so SELL slice widths never exceed it.
+static constexpr int kSellMaxLen = 12;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -133,17 +136,25 @@ __device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int 
         const int64_t base = A.slice_ptr[s] + lane;
         const int w = A.slice_w[s];
         row = A.perm ? A.perm[r] : r;
-        for (int j = 0; j < w; j++) {
-            const int64_t e = base + int64_t(j) * A.C;
-            const int c = A.col[e];
-            T xr, xi;
-            g(c, xr, xi);
-            if constexpr (CPLX) {
-                const T vr = A.val[2 * e], vi = A.val[2 * e + 1];
-                ar += vr * xr - vi * xi;
-                ai += vr * xi + vi * xr;
-            } else {
-                ar += A.val[e] * xr;
+        // every column index first (w <= kSellMaxLen), then the gathers: two memory latencies
+        // per row instead of two per entry
+        int ci[kSellMaxLen];
+#pragma unroll
+        for (int j = 0; j < kSellMaxLen; j++)
+            if (j < w) ci[j] = A.col[base + int64_t(j) * A.C];
+#pragma unroll
+        for (int j = 0; j < kSellMaxLen; j++) {
+            if (j < w) {
+                const int64_t e = base + int64_t(j) * A.C;
+                T xr, xi;
+                g(ci[j], xr, xi);
+                if constexpr (CPLX) {
+                    const T vr = A.val[2 * e], vi = A.val[2 * e + 1];
+                    ar += vr * xr - vi * xi;
+                    ai += vr * xi + vi * xr;
+                } else {
+                    ar += A.val[e] * xr;
+                }
             }
         }
         return true;
