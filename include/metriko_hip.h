@@ -49,7 +49,7 @@ typedef struct {
     int format;         /* MK_FMT_* (device operator layout), default SELL */
     int slice_height;   /* SELL slice height C (multiple of 64), default 64 */
     int sigma;          /* SELL sorting window (rows), 1 = unsorted, default 1 */
-    int block_size;     /* threads per block for SpMV, default 256 */
+    int block_size;     /* threads per block for SpMV: 64, 128 or 256, default 256 */
     int precond;        /* MK_PRECOND_*, default AMG */
     double inner_tol;   /* FP32 inner relative tolerance per refinement step, default 1e-5 */
     int max_refine;     /* max refinement steps, default 12 */
@@ -208,6 +208,37 @@ mk_status mk_curvature_dir(int nV, int nF, const double* V, const int32_t* F, do
 enum { MK_MATCH_PRINCIPAL = 0, MK_MATCH_CURL = 1 };
 mk_status mk_field_matching(int nV, int nF, const double* V, const int32_t* F, int N, const double* compressed,
                             int matching_type, int32_t* singular, int32_t* matching, double* effort);
+
+/* --- batched configuration (no reference counterpart as a whole) ---------------------------
+ * mk_autotune_spmv: times the FP64 SpMV of an n x n CSR matrix (complex when cplx, values
+ * interleaved) in each candidate device layout -- the CSR-like layout (slice height 64,
+ * unsorted) and SELL slice heights 64/128/256 x sorting windows 1/256 x block sizes 128/256 --
+ * and writes the fastest into *best (a copy of *base with format, slice_height, sigma and
+ * block_size set); *us receives its average launch time in microseconds. */
+mk_status mk_autotune_spmv(mk_ctx* ctx, int n, const int64_t* rowptr, const int32_t* col, const double* val,
+                           int cplx, const mk_solve_opts* base, mk_solve_opts* best, double* us);
+/* mk_batch_run: for each of nmesh closed meshes (V[m]: nV[m] x 3, F[m]: nF[m] x 3), the
+ * Smoothest N-RoSy field (n_iter inverse-iteration steps, face_rosy_field.h:17-44 with
+ * solveSmallestEig, sparse_linear_solver.h:29-54; u0[m] interleaved [2 nF[m]], or NULL for the
+ * std::rand stream) and the u/v Poisson integration of its first direction g_u (root 0 in the
+ * face basis) and g_v = n x g_u: min sum_f A_f |grad u - g_f|^2 with vertex 0 pinned to 0.
+ * All meshes share one block-diagonal operator per stage (one launch per kernel for the whole
+ * batch, field layout picked by mk_autotune_spmv on a row sample); every mesh's solves meet
+ * opts->tol relative to its own right-hand side.  Outputs (either may be NULL per mesh):
+ * compressed[m] [2 nF[m]] interleaved, uv[m] [2 nV[m]] = (u, v) per vertex. */
+typedef struct {
+    int meshes;
+    int64_t faces, verts;
+    int field_iterations, poisson_iterations;          /* PCG iterations over all solves */
+    double field_rel_residual, poisson_rel_residual;   /* bound on every mesh's own relative residual */
+    double ms_host_setup, ms_field, ms_poisson, ms_total;
+    int tuned_slice_height, tuned_sigma, tuned_block;  /* layout picked for the field operator */
+    double tuned_spmv_us;                              /* its SpMV time on the row sample */
+} mk_batch_stats;
+mk_status mk_batch_run(mk_ctx* ctx, int nmesh, const int32_t* nV, const int32_t* nF, const double* const* V,
+                       const int32_t* const* F, int N, int n_iter, const double* const* u0,
+                       const mk_solve_opts* opts, double* const* compressed, double* const* uv,
+                       mk_batch_stats* stats);
 
 /* Host-only multigrid PCG (no GPU; test and tuning aid): builds the same smoothed-aggregation
  * hierarchy the device path uses for the CSR matrix (complex when cplx != 0, values

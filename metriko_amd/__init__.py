@@ -23,7 +23,8 @@ EXPORTED = [
     "mk_op_create_real", "mk_op_create_complex", "mk_op_destroy", "mk_op_apply", "mk_spd_solve",
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
-    "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching",
+    "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
+    "mk_batch_run",
 ]
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
 MATCH_PRINCIPAL, MATCH_CURL = 0, 1
@@ -60,6 +61,15 @@ class PipelineInfo(C.Structure):
                [(n, C.c_double) for n in ("ms_hmesh", "ms_field_asm", "ms_field_solve", "ms_matching", "ms_seam",
                                           "ms_cut", "ms_setup", "ms_integ", "ms_total")] + \
                [("field", Stats), ("igm", IgmStats)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("meshes", C.c_int), ("faces", C.c_int64), ("verts", C.c_int64),
+                ("field_iterations", C.c_int), ("poisson_iterations", C.c_int),
+                ("field_rel_residual", C.c_double), ("poisson_rel_residual", C.c_double),
+                ("ms_host_setup", C.c_double), ("ms_field", C.c_double), ("ms_poisson", C.c_double),
+                ("ms_total", C.c_double), ("tuned_slice_height", C.c_int), ("tuned_sigma", C.c_int),
+                ("tuned_block", C.c_int), ("tuned_spmv_us", C.c_double)]
 
 
 def struct_dict(s):
@@ -119,6 +129,11 @@ def lib():
                                  C.POINTER(SolveOpts), _D, C.POINTER(Stats)]
     L.mk_curvature_dir.argtypes = [C.c_int, C.c_int, _D, _I32, _D]
     L.mk_field_matching.argtypes = [C.c_int, C.c_int, _D, _I32, C.c_int, _D, C.c_int, _I32, _I32, _D]
+    L.mk_autotune_spmv.argtypes = [_P, C.c_int, _I64, _I32, _D, C.c_int, C.POINTER(SolveOpts), C.POINTER(SolveOpts),
+                                   C.POINTER(C.c_double)]
+    L.mk_batch_run.argtypes = [_P, C.c_int, _I32, _I32, C.POINTER(_D), C.POINTER(_I32), C.c_int, C.c_int,
+                               C.POINTER(_D), C.POINTER(SolveOpts), C.POINTER(_D), C.POINTER(_D),
+                               C.POINTER(BatchStats)]
     L.mk_mg_host_pcg.argtypes = [C.c_int, _I64, _I32, _D, C.c_int, _D, C.c_int, _D, _D, C.c_double, C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     for name in EXPORTED:
@@ -401,6 +416,47 @@ class Context:
                                           None if u is None else _ptr(u, _D), n_iter, C.byref(o or opts()),
                                           _ptr(x, _D), C.byref(st)), "mk_field_solve")
         return x[0::2] + 1j * x[1::2], struct_dict(st)
+
+    def batch(self, meshes, N=4, n_iter=50, u0s=None, o=None):
+        """mk_batch_run: Smoothest field + u/v Poisson for every (V, F) in meshes in one batch.
+        Returns a list of {"compressed", "u", "v"} and the batch statistics."""
+        M = len(meshes)
+        Vs, Fs, comps, uvs, u0a = [], [], [], [], []
+        for i, (V, F) in enumerate(meshes):
+            V, F = _mesh_arrays(V, F)
+            Vs.append(V)
+            Fs.append(F)
+            comps.append(np.zeros(2 * F.shape[0]))
+            uvs.append(np.zeros(2 * V.shape[0]))
+            if u0s is not None:
+                u0a.append(_cplx_in(u0s[i]))
+        nV = (C.c_int32 * M)(*[V.shape[0] for V in Vs])
+        nF = (C.c_int32 * M)(*[F.shape[0] for F in Fs])
+        Vp = (_D * M)(*[_ptr(V, _D) for V in Vs])
+        Fp = (_I32 * M)(*[_ptr(F, _I32) for F in Fs])
+        Cp = (_D * M)(*[_ptr(x, _D) for x in comps])
+        Up = (_D * M)(*[_ptr(x, _D) for x in uvs])
+        U0 = (_D * M)(*[_ptr(x, _D) for x in u0a]) if u0s is not None else None
+        st = BatchStats()
+        self._check(self.L.mk_batch_run(self.h, M, nV, nF, Vp, Fp, N, n_iter, U0, C.byref(o or opts()), Cp, Up,
+                                        C.byref(st)), "mk_batch_run")
+        out = [{"compressed": x[0::2] + 1j * x[1::2], "u": y[0::2].copy(), "v": y[1::2].copy()}
+               for x, y in zip(comps, uvs)]
+        return out, struct_dict(st)
+
+    def autotune_spmv(self, A, o=None):
+        """mk_autotune_spmv on a scipy CSR matrix: the fastest layout's options and its time (us)."""
+        A = A.tocsr()
+        cplx = np.iscomplexobj(A.data)
+        rp = np.ascontiguousarray(A.indptr, np.int64)
+        ci = np.ascontiguousarray(A.indices, np.int32)
+        val = _cplx_in(A.data) if cplx else np.ascontiguousarray(A.data, np.float64)
+        best = SolveOpts()
+        us = C.c_double()
+        self._check(self.L.mk_autotune_spmv(self.h, A.shape[0], _ptr(rp, _I64), _ptr(ci, _I32), _ptr(val, _D),
+                                            int(cplx), C.byref(o or opts()), C.byref(best), C.byref(us)),
+                    "mk_autotune_spmv")
+        return best, us.value
 
     def pipeline(self, V, F, ratio=0.03, u0=None, n_iter=50, field_opts=None, igm_opts=None):
         """example_igm in one call (mk_pipeline_run); returns outputs and per-stage info."""

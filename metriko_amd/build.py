@@ -9,7 +9,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmetriko_hip.so")
-SOURCES = ["kernels.hip", "block.hip", "mg.hip", "capi.hip", "igm.hip", "pipeline.hip"]
+SOURCES = ["kernels.hip", "block.hip", "mg.hip", "capi.hip", "igm.hip", "pipeline.hip", "batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("METRIKO_ARCH", "gfx950")
 

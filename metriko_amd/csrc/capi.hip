@@ -376,7 +376,8 @@ mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector
         op->cplx = false;
         op->nnz = ptr[size_t(n)];
         op->block = o.block_size;
-        MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 1024, MK_ERR_ARG, "block_size");
+        MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 256, MK_ERR_ARG,
+                 "block_size must be 64, 128 or 256 (the SpMV kernels are built for at most 256 threads)");
         op->h_ptr = std::move(ptr);
         op->h_col = std::move(col);
         op->h_val = std::move(val);
@@ -518,7 +519,8 @@ static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_
             op->cplx = cplx;
             op->nnz = rowptr[n];
             op->block = o.block_size;
-            MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 1024, MK_ERR_ARG, "block_size");
+            MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 256, MK_ERR_ARG,
+                 "block_size must be 64, 128 or 256 (the SpMV kernels are built for at most 256 threads)");
             for (int i = 0; i < n; i++) MK_CHECK(rowptr[i + 1] >= rowptr[i], MK_ERR_ARG, "rowptr not monotone");
             int64_t bad = 0;
 #pragma omp parallel for reduction(+ : bad) schedule(static, 65536) if (op->nnz > 1000000)
@@ -538,6 +540,64 @@ static mk_status op_create(mk_ctx* c, int n, const int64_t* rowptr, const int32_
     });
 }
 
+
+mk_status mk_autotune_spmv(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val, int cplx,
+                           const mk_solve_opts* base, mk_solve_opts* best, double* us) {
+    if (!c || n <= 0 || !rowptr || !col || !val || !best) return MK_ERR_ARG;
+    return guard(c, [&] {
+        const mk_solve_opts b0 = resolve(base);
+        MK_HIP(hipSetDevice(c->device));
+        struct Cand { int format, C, sigma, block; };
+        std::vector<Cand> cand = {{MK_FMT_CSR, 64, 1, 256}};
+        for (int C : {64, 128, 256})
+            for (int sigma : {1, 256})
+                for (int block : {128, 256}) cand.push_back({MK_FMT_SELL, C, sigma, block});
+        const size_t len = size_t(n) * (cplx ? 2 : 1);
+        DevBuf<double> dx(len), dy(len);
+        {
+            std::vector<double> hx(len);
+            for (size_t i = 0; i < len; i++) hx[i] = 1.0 + 1e-3 * double(i % 97);
+            dx.upload(hx.data(), len, c->stream);
+        }
+        hipEvent_t e0, e1;
+        MK_HIP(hipEventCreate(&e0));
+        MK_HIP(hipEventCreate(&e1));
+        double best_ms = HUGE_VAL;
+        for (const Cand& k : cand) {
+            mk_solve_opts o = b0;
+            o.format = k.format;
+            o.slice_height = k.C;
+            o.sigma = k.sigma;
+            o.block_size = k.block;
+            o.precond = MK_PRECOND_NONE;
+            mk_op* op = nullptr;
+            const mk_status st = cplx ? mk_op_create_complex(c, n, rowptr, col, val, &o, &op)
+                                      : mk_op_create_real(c, n, rowptr, col, val, &o, &op);
+            if (st != MK_OK) continue;
+            for (int w = 0; w < 3; w++) mk::spmv64(c, op, dx.p, dy.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+            constexpr int reps = 20;
+            MK_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; r++) mk::spmv64(c, op, dx.p, dy.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+            MK_HIP(hipEventRecord(e1, c->stream));
+            MK_HIP(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            MK_HIP(hipEventElapsedTime(&ms, e0, e1));
+            mk_op_destroy(op);
+            if (double(ms) / reps < best_ms) {
+                best_ms = double(ms) / reps;
+                *best = b0;
+                best->format = k.format;
+                best->slice_height = k.C;
+                best->sigma = k.sigma;
+                best->block_size = k.block;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        MK_CHECK(best_ms < HUGE_VAL, MK_ERR_HIP, "autotune: no candidate layout could be built");
+        if (us) *us = best_ms * 1e3;
+    });
+}
 
 mk_status mk_op_create_real(mk_ctx* c, int n, const int64_t* rowptr, const int32_t* col, const double* val,
                             const mk_solve_opts* o, mk_op** op) {
@@ -707,6 +767,8 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
             op->C = C;
             op->sigma = 1;
             op->block = o.block_size;
+            MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 256, MK_ERR_ARG,
+                     "block_size must be 64, 128 or 256");
             const int ns = (nF + C - 1) / C;
             op->nslices = ns;
             op->stored = int64_t(ns) * 4 * C;

@@ -517,6 +517,9 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     const int n = op->n;
     const int vw = op->cplx ? 2 : 1;
     MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
+    // every SpMV kernel is compiled with __launch_bounds__(256)
+    MK_CHECK(op->block % 64 == 0 && op->block >= 64 && op->block <= 256, MK_ERR_ARG,
+             "block_size must be 64, 128 or 256");
     MK_CHECK(sigma >= 1, MK_ERR_ARG, "sigma must be >= 1");
     const auto t_start = std::chrono::steady_clock::now();
     auto lap_ms = [&] {

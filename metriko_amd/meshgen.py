@@ -124,3 +124,27 @@ def jittered(V, F, sigma, seed):
     vn /= np.linalg.norm(vn, axis=1)[:, None]
     g = np.random.Generator(np.random.PCG64(seed))
     return V + vn * (sigma * g.standard_normal(V.shape[0]))[:, None], F
+
+
+def batch_meshes(count=64, seed=3, max_torus_verts=160000, max_level=6):
+    """The batched set: ``count`` closed meshes of uneven size drawn from PCG64(seed).  Each
+    mesh is an icosphere of level 3..max_level (642 / 2,562 / 10,242 / 40,962 v) or a torus
+    (R = 1, r = 0.35) whose vertex count is log-uniform in [2,000, max_torus_verts] on an
+    n_ring x n_tube grid with n_ring / n_tube = 1.5625 (the 400 x 256 aspect); every mesh gets
+    N(0, 0.002) jitter along its normal (its own seed).  Returns a list of (V, F)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(count):
+        sub = 1000 * seed + i
+        if g.random() < 0.5:
+            level = int(g.integers(3, max_level + 1))
+            V, F = icosphere(level)
+            h = np.random.Generator(np.random.PCG64(sub))
+            V = V * (1.0 + 0.002 * h.standard_normal(V.shape[0]))[:, None]
+        else:
+            nv = float(np.exp(g.uniform(np.log(2000.0), np.log(max_torus_verts))))
+            n_tube = max(8, int(round((nv / 1.5625) ** 0.5)))
+            n_ring = max(8, int(round(1.5625 * n_tube)))
+            V, F = torus(n_tube, n_ring, 1.0, 0.35, jitter=0.002, seed=sub)
+        out.append((V, F))
+    return out
