@@ -26,6 +26,7 @@ namespace metriko {
     public:
         int N, n;
         SprsD vtrans2cut, constraint, uncompress;
+        SprsD fullx2Nfunc;  // vtrans2cut * uncompress (a local of integ() in the reference; kept for the uv map)
         VecXi he2matching, he2transidx, integerIdcs, singularIdcs, fixedIdcs;
         VecXd fixedVals;
         MatXi lreductor;
@@ -224,7 +225,7 @@ namespace metriko {
 
     // ---- integ (parameterization_integ.ipp:3-120) ------------------------------------------
     inline bool RosyParameterization::integ() {
-        SprsD fullx2Nfunc = vtrans2cut * uncompress;
+        fullx2Nfunc = vtrans2cut * uncompress;
         SprsD Cfull = constraint * uncompress;
         reduce_to_linearly_independent(Cfull);
         // G2: per cut face and corner, gradient of the hat function in the face frame

@@ -161,6 +161,27 @@ mk_status mk_iterative_rounding(mk_ctx* ctx, int nfixed, const int32_t* fixedIdc
                                 int max_dec, int32_t* dec_index, int32_t* dec_value, double* dec_margin, int* ndec,
                                 mk_igm_stats* stats);
 
+/* --- differentiation: the sizing hook (no reference counterpart) ---------------------------
+ * mk_keep_igm_state(ctx, 1) makes the next iterative rounding (mk_iterative_rounding or
+ * mk_pipeline_run) keep its last LM round's system
+ *   (A0 + wK^2 U^T U) x = length wI^2 X2F^T (sigma .* F2) + wC^2 xPrev + wK^2 U^T r
+ * (integer decisions, proximal point xPrev and Woodbury columns of that round held fixed);
+ * mk_igm_state_take hands it to the caller.  sigma is a per-face scale of the target field
+ * (sigma = exp(-mean log-size of the face's vertices); sigma = 1 reproduces the rounding's
+ * x).  mk_igm_sized_uv: fullx[nfull] and corner uv[6 nF] (as mk_pipeline_run's uv; needs a
+ * state from mk_pipeline_run) for a given sigma[nF].  mk_igm_sized_uv_grad: the adjoint
+ * solve H lambda = UFull^T P^T g_uv (Woodbury against A0, multigrid PCG) and
+ * g_scale[f] = d(g_uv . uv)/d sigma_f. */
+typedef struct mk_igm_state mk_igm_state;
+mk_status mk_keep_igm_state(mk_ctx* ctx, int on);
+mk_status mk_igm_state_take(mk_ctx* ctx, mk_igm_state** state);
+mk_status mk_igm_state_destroy(mk_igm_state* state);
+mk_status mk_igm_state_sizes(const mk_igm_state* state, int* nF, int* nfull, int* k, int* nfixed,
+                             int* has_corner_map);
+mk_status mk_igm_sized_uv(mk_ctx* ctx, mk_igm_state* state, const double* face_scale, double* fullx, double* uv);
+mk_status mk_igm_sized_uv_grad(mk_ctx* ctx, mk_igm_state* state, const double* face_scale, const double* g_uv,
+                               double* g_scale);
+
 /* --- driver ---------------------------------------------------------------------------
  * example/example_igm.cpp:14-36 in one call: Hmesh -> Smoothest field -> Curl matching ->
  * seam -> cut mesh -> combed field -> extrinsic field -> RosyParameterization(seamless,

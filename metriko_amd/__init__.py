@@ -24,7 +24,8 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run",
+    "mk_batch_run", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
+    "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
 MATCH_PRINCIPAL, MATCH_CURL = 0, 1
@@ -134,6 +135,12 @@ def lib():
     L.mk_batch_run.argtypes = [_P, C.c_int, _I32, _I32, C.POINTER(_D), C.POINTER(_I32), C.c_int, C.c_int,
                                C.POINTER(_D), C.POINTER(SolveOpts), C.POINTER(_D), C.POINTER(_D),
                                C.POINTER(BatchStats)]
+    L.mk_keep_igm_state.argtypes = [_P, C.c_int]
+    L.mk_igm_state_take.argtypes = [_P, C.POINTER(_P)]
+    L.mk_igm_state_destroy.argtypes = [_P]
+    L.mk_igm_state_sizes.argtypes = [_P] + [C.POINTER(C.c_int)] * 5
+    L.mk_igm_sized_uv.argtypes = [_P, _P, _D, _D, _D]
+    L.mk_igm_sized_uv_grad.argtypes = [_P, _P, _D, _D, _D]
     L.mk_mg_host_pcg.argtypes = [C.c_int, _I64, _I32, _D, C.c_int, _D, C.c_int, _D, _D, C.c_double, C.c_int,
                                  C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     for name in EXPORTED:
@@ -308,6 +315,46 @@ def field_matching(V, F, compressed, N=4, kind="curl"):
     return {"singular": sing, "matching": mat, "effort": eff}
 
 
+class SizingState:
+    """The last LM round's system kept from a pipeline run (mk_keep_igm_state): the sized solve
+    sigma -> corner uv and its adjoint (mk_igm_sized_uv / mk_igm_sized_uv_grad)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.h = ctx, h
+        vals = [C.c_int() for _ in range(5)]
+        ctx._check(ctx.L.mk_igm_state_sizes(h, *[C.byref(v) for v in vals]), "mk_igm_state_sizes")
+        self.nF, self.nfull, self.k, self.nfixed, self.has_corner_map = [v.value for v in vals]
+
+    def uv(self, sigma):
+        sig = np.ascontiguousarray(sigma, np.float64)
+        assert sig.shape == (self.nF,)
+        fx = np.zeros(self.nfull)
+        uv = np.zeros(6 * self.nF)
+        self.ctx._check(self.ctx.L.mk_igm_sized_uv(self.ctx.h, self.h, _ptr(sig, _D), _ptr(fx, _D), _ptr(uv, _D)),
+                        "mk_igm_sized_uv")
+        return fx, uv
+
+    def grad(self, sigma, g_uv):
+        sig = np.ascontiguousarray(sigma, np.float64)
+        g = np.ascontiguousarray(g_uv, np.float64)
+        assert sig.shape == (self.nF,) and g.shape == (6 * self.nF,)
+        out = np.zeros(self.nF)
+        self.ctx._check(self.ctx.L.mk_igm_sized_uv_grad(self.ctx.h, self.h, _ptr(sig, _D), _ptr(g, _D), _ptr(out, _D)),
+                        "mk_igm_sized_uv_grad")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.mk_igm_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One mk_ctx (HIP stream + scratch) on one device."""
 
@@ -443,6 +490,14 @@ class Context:
         out = [{"compressed": x[0::2] + 1j * x[1::2], "u": y[0::2].copy(), "v": y[1::2].copy()}
                for x, y in zip(comps, uvs)]
         return out, struct_dict(st)
+
+    def keep_igm_state(self, on=True):
+        self._check(self.L.mk_keep_igm_state(self.h, int(on)), "mk_keep_igm_state")
+
+    def take_igm_state(self):
+        h = C.c_void_p()
+        self._check(self.L.mk_igm_state_take(self.h, C.byref(h)), "mk_igm_state_take")
+        return SizingState(self, h)
 
     def autotune_spmv(self, A, o=None):
         """mk_autotune_spmv on a scipy CSR matrix: the fastest layout's options and its time (us)."""

@@ -456,6 +456,8 @@ mk_status mk_finalize(mk_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto e : c->ev_pool) (void)hipEventDestroy(e);
     c->ev_pool.clear();
+    if (c->igm_state) mk_igm_state_destroy(c->igm_state);
+    c->igm_state = nullptr;
     c->work = mk::Work();
     c->partials.release();
     c->state.release();

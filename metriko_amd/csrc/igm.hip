@@ -19,6 +19,7 @@
 #include <cstring>
 #include <numeric>
 #include "mk_internal.h"
+#include "igm_state.h"
 #include "metriko/core/common/sparse.h"
 #include "metriko/core/solver/matrix_ops.h"
 
@@ -813,7 +814,8 @@ extern "C" mk_status mk_iterative_rounding(
                 ddots.download(out.data(), out.size(), K.st());
                 K.sync();
             };
-            int lm_rounds = 0, rounds = 0;
+            int lm_rounds = 0, rounds = 0, last_r = 0;
+            DevBuf<double> dxprev_last;
             DevBuf<double> pv2, dd2, tr2;
             double tm_xt = 0.0, tm_lm = 0.0, tm_all0 = now_ms();
             int xt_iters = 0;
@@ -866,6 +868,12 @@ extern "C" mk_status mk_iterative_rounding(
                 rounded = true;
                 lm_rounds++;
                 const int r = int(fixedI.size());
+                if (c->keep_igm_state) {  // the last LM round's system is kept for the sized solves
+                    last_r = r;
+                    dxprev_last.ensure(static_cast<size_t>(k));
+                    MK_HIP(hipMemcpyAsync(dxprev_last.p, dxprev.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice,
+                                          K.st()));
+                }
                 std::vector<double> UZ(size_t(r) * size_t(r));
                 std::vector<int> zidx(static_cast<size_t>(r));
                 for (int a = 0; a < r; a++) {
@@ -936,6 +944,48 @@ extern "C" mk_status mk_iterative_rounding(
                 std::copy(x0.begin(), x0.end(), fullx);
             }
             S.rounding_ms = now_ms() - t0;
+            if (c->keep_igm_state && rounded && last_r > 0) {
+                // the last LM round's system (A0 + wK^2 U^T U) x = length wI^2 X2F^T F2 + wC^2 xPrev
+                // + wK^2 U^T r, with its Woodbury columns, for mk_igm_sized_uv / _grad
+                auto* st = new mk_igm_state();
+                st->ctx = c;
+                st->o = o;
+                st->k = k;
+                st->nfull = nfull;
+                st->l1 = int(l1);
+                st->nF = nF;
+                st->N = N;
+                st->r = last_r;
+                st->length = length;
+                st->wI = wI;
+                st->wK = wK;
+                st->wC = wC;
+                st->A0 = hA0.op;
+                hA0.op = nullptr;
+                st->X2F = hX2F.op;
+                hX2F.op = nullptr;
+                st->UF = std::move(UF);
+                st->X2Ft = std::move(X2Ft);
+                st->F2.assign(F2, F2 + l1);
+                st->fixedV.assign(fixedV.begin(), fixedV.begin() + last_r);
+                st->UZ.resize(size_t(last_r) * size_t(last_r));
+                st->Urows.resize(size_t(last_r));
+                st->Z.alloc(size_t(k) * size_t(last_r));
+                for (int a = 0; a < last_r; a++) {
+                    const int pa = candPos[fixedI[size_t(a)]];
+                    st->Urows[size_t(a)] = fixedI[size_t(a)];
+                    for (int bq = 0; bq < last_r; bq++)
+                        st->UZ[size_t(a) * last_r + bq] = UZall[size_t(pa) * nc_ + size_t(candPos[fixedI[size_t(bq)]])];
+                    MK_HIP(hipMemcpyAsync(st->Z.p + size_t(a) * size_t(k), Zall.p + size_t(pa) * size_t(k),
+                                          size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+                }
+                st->xprev.alloc(static_cast<size_t>(k));
+                MK_HIP(hipMemcpyAsync(st->xprev.p, dxprev_last.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice,
+                                      K.st()));
+                K.sync();
+                if (c->igm_state) mk_igm_state_destroy(c->igm_state);
+                c->igm_state = st;
+            }
         }
         if (ndec) *ndec = nd;
         K.sync();
@@ -952,3 +1002,175 @@ extern "C" mk_status mk_iterative_rounding(
         return MK_ERR_ARG;
     }
 }
+
+namespace {
+// U y for the kept fixed rows (device), to the host
+std::vector<double> state_rowdots(mk_igm_state* st, const double* y) {
+    mk_ctx* c = st->ctx;
+    if (!st->uptr.p) {
+        std::vector<int64_t> hp{0};
+        std::vector<int> hc;
+        std::vector<double> hv;
+        for (int row : st->Urows) {
+            for (int64_t q = st->UF.ptr[row]; q < st->UF.ptr[row + 1]; q++) {
+                hc.push_back(st->UF.col[q]);
+                hv.push_back(st->UF.val[q]);
+            }
+            hp.push_back(int64_t(hc.size()));
+        }
+        st->uptr.upload(hp.data(), hp.size(), c->stream);
+        st->ucol.upload(hc.data(), std::max<size_t>(hc.size(), 1), c->stream);
+        st->uval.upload(hv.data(), std::max<size_t>(hv.size(), 1), c->stream);
+    }
+    DevBuf<double> out(static_cast<size_t>(st->r));
+    hipLaunchKernelGGL(k_rowset_dot, dim3(st->r), dim3(256), 0, c->stream, st->r, (const int64_t*)st->uptr.p,
+                       (const int*)st->ucol.p, (const double*)st->uval.p, y, out.p);
+    std::vector<double> h(static_cast<size_t>(st->r));
+    out.download(h.data(), h.size(), c->stream);
+    MK_HIP(hipStreamSynchronize(c->stream));
+    return h;
+}
+// M^{-1} v with M = U Z + I / wK^2 (r x r)
+std::vector<double> state_msolve(const mk_igm_state* st, const std::vector<double>& v) {
+    const int r = st->r;
+    std::vector<double> M(st->UZ);
+    for (int i = 0; i < r; i++) M[size_t(i) * r + i] += 1.0 / (st->wK * st->wK);
+    return dense_solve(M, v, r);
+}
+// x = y + Z c (device)
+void state_combine(mk_igm_state* st, const double* y, const std::vector<double>& cf, double* x) {
+    mk_ctx* c = st->ctx;
+    std::vector<int> idx(static_cast<size_t>(st->r));
+    std::iota(idx.begin(), idx.end(), 0);
+    DevBuf<int> didx;
+    DevBuf<double> dc;
+    didx.upload(idx.data(), idx.size(), c->stream);
+    dc.upload(cf.data(), cf.size(), c->stream);
+    hipLaunchKernelGGL(k_combine, dim3(nblk(size_t(st->k))), dim3(256), 0, c->stream, size_t(st->k), st->r, y,
+                       (const double*)st->Z.p, (const int*)didx.p, (const double*)dc.p, x);
+    MK_HIP(hipStreamSynchronize(c->stream));
+}
+void check_scale(const mk_igm_state* st, const double* face_scale) {
+    for (int f = 0; f < st->nF; f++)
+        MK_CHECK(face_scale[f] == face_scale[f] && face_scale[f] > 0.0, MK_ERR_ARG, "face_scale must be positive");
+}
+template <typename Fn>
+mk_status state_guard(mk_ctx* c, Fn&& fn) {
+    try {
+        fn();
+        return MK_OK;
+    } catch (const Error& e) {
+        if (c) c->last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        if (c) c->last_error = e.what();
+        return MK_ERR_ARG;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+mk_status mk_keep_igm_state(mk_ctx* c, int on) {
+    if (!c) return MK_ERR_ARG;
+    c->keep_igm_state = on != 0;
+    return MK_OK;
+}
+
+mk_status mk_igm_state_take(mk_ctx* c, mk_igm_state** st) {
+    if (!c || !st) return MK_ERR_ARG;
+    *st = c->igm_state;
+    c->igm_state = nullptr;
+    return *st ? MK_OK : MK_ERR_ARG;
+}
+
+mk_status mk_igm_state_destroy(mk_igm_state* st) {
+    if (!st) return MK_ERR_ARG;
+    (void)hipSetDevice(st->ctx->device);
+    delete st;
+    return MK_OK;
+}
+
+mk_status mk_igm_state_sizes(const mk_igm_state* st, int* nF, int* nfull, int* k, int* nfixed, int* has_corner_map) {
+    if (!st) return MK_ERR_ARG;
+    if (nF) *nF = st->nF;
+    if (nfull) *nfull = st->nfull;
+    if (k) *k = st->k;
+    if (nfixed) *nfixed = st->r;
+    if (has_corner_map) *has_corner_map = st->has_P ? 1 : 0;
+    return MK_OK;
+}
+
+mk_status mk_igm_sized_uv(mk_ctx* c, mk_igm_state* st, const double* face_scale, double* fullx, double* uv) {
+    if (!c || !st || st->ctx != c || !face_scale) return MK_ERR_ARG;
+    return state_guard(c, [&] {
+        MK_HIP(hipSetDevice(c->device));
+        check_scale(st, face_scale);
+        MK_CHECK(!uv || st->has_P, MK_ERR_ARG, "no corner map: the state did not come from mk_pipeline_run");
+        const int k = st->k;
+        // b = length wI^2 X2F^T (sigma .* F2) + wC^2 xPrev
+        std::vector<double> sf(static_cast<size_t>(st->l1)), b(static_cast<size_t>(k));
+        const int per = 2 * st->N;
+        for (int f = 0; f < st->nF; f++)
+            for (int j = 0; j < per; j++) sf[size_t(f) * per + j] = face_scale[f] * st->F2[size_t(f) * per + j];
+        st->X2Ft.multiply(sf.data(), b.data());
+        for (auto& v : b) v *= st->length * st->wI * st->wI;
+        DevBuf<double> db, dxt(static_cast<size_t>(k)), dx(static_cast<size_t>(k));
+        db.upload(b.data(), b.size(), c->stream);
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(size_t(k))), dim3(256), 0, c->stream, size_t(k), 1.0, (const double*)db.p,
+                           st->wC * st->wC, (const double*)st->xprev.p, db.p);
+        MK_HIP(hipMemcpyAsync(dxt.p, st->xprev.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        const mk::SolveResult sr = mk::solve_spd(c, st->A0, db.p, dxt.p, st->o);
+        MK_CHECK(sr.status == MK_OK, sr.status, "sized solve: " + std::string(mk_status_str(sr.status)));
+        // x = xt + Z M^{-1} (r - U xt)
+        const std::vector<double> ux = state_rowdots(st, dxt.p);
+        std::vector<double> rhs(static_cast<size_t>(st->r));
+        for (int i = 0; i < st->r; i++) rhs[size_t(i)] = st->fixedV[size_t(i)] - ux[size_t(i)];
+        state_combine(st, dxt.p, state_msolve(st, rhs), dx.p);
+        std::vector<double> xs(static_cast<size_t>(k)), fx(static_cast<size_t>(st->nfull));
+        dx.download(xs.data(), xs.size(), c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        st->UF.multiply(xs.data(), fx.data());
+        if (fullx) std::copy(fx.begin(), fx.end(), fullx);
+        if (uv) st->P.multiply(fx.data(), uv);
+    });
+}
+
+mk_status mk_igm_sized_uv_grad(mk_ctx* c, mk_igm_state* st, const double* face_scale, const double* g_uv,
+                               double* g_scale) {
+    if (!c || !st || st->ctx != c || !face_scale || !g_uv || !g_scale) return MK_ERR_ARG;
+    return state_guard(c, [&] {
+        MK_HIP(hipSetDevice(c->device));
+        check_scale(st, face_scale);
+        MK_CHECK(st->has_P, MK_ERR_ARG, "no corner map: the state did not come from mk_pipeline_run");
+        const int k = st->k;
+        // g = UFull^T P^T g_uv (host, transposed products)
+        std::vector<double> gf(static_cast<size_t>(st->nfull), 0.0), g(static_cast<size_t>(k), 0.0);
+        for (int i = 0; i < st->P.rows(); i++)
+            for (int64_t q = st->P.ptr[i]; q < st->P.ptr[i + 1]; q++) gf[size_t(st->P.col[q])] += st->P.val[q] * g_uv[i];
+        for (int i = 0; i < st->UF.rows(); i++)
+            for (int64_t q = st->UF.ptr[i]; q < st->UF.ptr[i + 1]; q++) g[size_t(st->UF.col[q])] += st->UF.val[q] * gf[size_t(i)];
+        // adjoint: lambda = H^{-1} g = A0^{-1} g - Z M^{-1} U A0^{-1} g
+        DevBuf<double> dg, dy(static_cast<size_t>(k)), dl(static_cast<size_t>(k)), dw(static_cast<size_t>(st->l1));
+        dg.upload(g.data(), g.size(), c->stream);
+        MK_HIP(hipMemsetAsync(dy.p, 0, size_t(k) * sizeof(double), c->stream));
+        const mk::SolveResult sr = mk::solve_spd(c, st->A0, dg.p, dy.p, st->o);
+        MK_CHECK(sr.status == MK_OK, sr.status, "adjoint solve: " + std::string(mk_status_str(sr.status)));
+        std::vector<double> d = state_msolve(st, state_rowdots(st, dy.p));
+        for (auto& v : d) v = -v;
+        state_combine(st, dy.p, d, dl.p);
+        // d/d sigma_f of lambda^T b = wI^2 (length X2F lambda)_(rows of f) . F2_(rows of f)
+        mk::spmv64(c, st->X2F, dl.p, dw.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        std::vector<double> w(static_cast<size_t>(st->l1));
+        dw.download(w.data(), w.size(), c->stream);
+        MK_HIP(hipStreamSynchronize(c->stream));
+        const int per = 2 * st->N;
+        for (int f = 0; f < st->nF; f++) {
+            double a = 0.0;
+            for (int j = 0; j < per; j++) a += w[size_t(f) * per + j] * st->F2[size_t(f) * per + j];
+            g_scale[f] = st->wI * st->wI * a;
+        }
+    });
+}
+
+}  // extern "C"

@@ -146,6 +146,9 @@ struct mk_ctx {
     double spmv_bytes = 0.0;
     bool use_graphs = true;  // replay captured PCG iteration batches
     int64_t graph_replays = 0, graph_captures = 0;
+    // the next iterative rounding keeps its last LM round's system (mk_keep_igm_state)
+    bool keep_igm_state = false;
+    struct mk_igm_state* igm_state = nullptr;
 };
 
 namespace mk { struct Mg; void mg_free(Mg*); }

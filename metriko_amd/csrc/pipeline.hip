@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cstring>
 #include "mk_internal.h"
+#include "igm_state.h"
 #include "metriko/core/vectorfield/face_rosy_field.h"
 #include "metriko/core/igm/parameterization.h"
 
@@ -267,6 +268,24 @@ mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int3
         t = now_ms();
         const bool ok = rp.integ();
         info->ms_integ = now_ms() - t;
+        if (c->keep_igm_state && c->igm_state && !c->igm_state->has_P) {
+            // corner map: uv of face i, corner j = rows N cut.idx(i, j) + {0, 1} of fullx2Nfunc
+            mk_igm_state* st = c->igm_state;
+            metriko::SprsD& P = st->P;
+            P = metriko::SprsD(6 * nF, rp.fullx2Nfunc.cols());
+            P.ptr.assign(size_t(6 * nF) + 1, 0);
+            for (int i = 0; i < nF; i++)
+                for (int j = 0; j < 3; j++)
+                    for (int kk = 0; kk < 2; kk++) {
+                        const int src = N * H.cut->idx(i, j) + kk, dst = 6 * i + 2 * j + kk;
+                        for (int64_t q = rp.fullx2Nfunc.ptr[src]; q < rp.fullx2Nfunc.ptr[src + 1]; q++) {
+                            P.col.push_back(rp.fullx2Nfunc.col[q]);
+                            P.val.push_back(rp.fullx2Nfunc.val[q]);
+                        }
+                        P.ptr[size_t(dst) + 1] = int64_t(P.col.size());
+                    }
+            st->has_P = true;
+        }
         info->igm = rt.last_igm;
         info->success = ok ? 1 : 0;
         info->cutV = H.cut->nV;

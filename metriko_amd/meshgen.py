@@ -148,3 +148,106 @@ def batch_meshes(count=64, seed=3, max_torus_verts=160000, max_level=6):
             V, F = torus(n_tube, n_ring, 1.0, 0.35, jitter=0.002, seed=sub)
         out.append((V, F))
     return out
+
+
+def _triple_torus_field(P, seed):
+    """Signed field of the three-handle blob: a smooth union (soft-min, k = 16) of three tori
+    (R = 0.75, r = 0.3, centres x = -1.4, 0, 1.4 in the z = 0 plane) plus a small random smooth
+    bump field drawn from PCG64(seed)."""
+    g = np.random.Generator(np.random.PCG64(seed))
+    d = []
+    for cx in (-1.4, 0.0, 1.4):
+        q = np.sqrt((P[..., 0] - cx) ** 2 + P[..., 1] ** 2) - 0.75
+        d.append(np.sqrt(q * q + P[..., 2] ** 2) - 0.3)
+    k = 16.0
+    m = np.minimum(np.minimum(d[0], d[1]), d[2])
+    f = m - np.log(sum(np.exp(-k * (di - m)) for di in d)) / k
+    for _ in range(6):
+        a = 0.012 * g.standard_normal()
+        w = 2.5 * g.standard_normal(3)
+        ph = g.uniform(0, 2 * np.pi)
+        f = f + a * np.sin(P @ w + ph)
+    return f
+
+
+def genus3_blob(h=0.028, seed=4, smooth=20):
+    """Closed genus-3 surface: the zero set of the three-handle blob field sampled on a grid of
+    spacing h and triangulated by marching tetrahedra (six tetrahedra per cell around the main
+    diagonal, so neighbouring cells agree; no ambiguous cases).  Surface points interpolate the
+    field linearly along grid edges, clamped to [0.05, 0.95] of the edge; shared edge points
+    are welded; triangles face the outside (positive field); then ``smooth`` Taubin iterations.
+    h = 0.028 gives about 250k faces."""
+    lo = np.array([-2.65, -1.45, -0.70])
+    hi = np.array([2.65, 1.45, 0.70])
+    n = np.ceil((hi - lo) / h).astype(int) + 1
+    xs = [lo[a] + h * np.arange(n[a]) for a in range(3)]
+    P = np.stack(np.meshgrid(*xs, indexing="ij"), -1)
+    f = _triple_torus_field(P, seed)
+    f[f == 0.0] = 1e-12
+    nx, ny, nz = n
+    gid = lambda i, j, k: (i * ny + j) * nz + k  # noqa: E731
+    i, j, k = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    corner = [gid(i, j, k), gid(i + 1, j, k), gid(i + 1, j + 1, k), gid(i, j + 1, k),
+              gid(i, j, k + 1), gid(i + 1, j, k + 1), gid(i + 1, j + 1, k + 1), gid(i, j + 1, k + 1)]
+    tets = [(0, 1, 2, 6), (0, 2, 3, 6), (0, 3, 7, 6), (0, 7, 4, 6), (0, 4, 5, 6), (0, 5, 1, 6)]
+    fv = f.ravel()
+    Pv = P.reshape(-1, 3)
+    tris, refs = [], []
+    for t in tets:
+        T = np.stack([corner[c] for c in t], 1)
+        inside = fv[T] < 0
+        cnt = inside.sum(1)
+        for case in (1, 3, 2):
+            sel = cnt == case
+            if not sel.any():
+                continue
+            Ts, ins = T[sel], inside[sel]
+            if case == 3:
+                ins = ~ins  # one outside vertex: same triangle, orientation fixed below
+            order = np.argsort(~ins, axis=1, kind="stable")  # marked vertices first
+            V4 = np.take_along_axis(Ts, order, 1)
+            if case in (1, 3):
+                a = V4[:, 0]
+                e = [(a, V4[:, 1]), (a, V4[:, 2]), (a, V4[:, 3])]
+                tris.append(np.stack([np.stack(x, 1) for x in e], 1))
+                refs.append(np.stack([V4[:, 1], a] if case == 1 else [a, V4[:, 1]], 1))
+            else:
+                a, b, c, d = V4[:, 0], V4[:, 1], V4[:, 2], V4[:, 3]
+                q = [(a, c), (a, d), (b, d), (b, c)]
+                Q = np.stack([np.stack(x, 1) for x in q], 1)
+                tris.append(Q[:, [0, 1, 2]])
+                tris.append(Q[:, [0, 2, 3]])
+                r = np.stack([c, a], 1)
+                refs.append(r)
+                refs.append(r)
+    E = np.concatenate(tris, 0)                # (m, 3, 2) grid-edge endpoints per triangle corner
+    R = np.concatenate(refs, 0)                # (m, 2) (outside, inside) grid vertices
+    Es = np.sort(E, axis=2).reshape(-1, 2)
+    key = Es[:, 0] * fv.size + Es[:, 1]
+    uk, inv = np.unique(key, return_inverse=True)
+    a, b = uk // fv.size, uk % fv.size
+    t = np.clip(fv[a] / (fv[a] - fv[b]), 0.05, 0.95)
+    V = Pv[a] + t[:, None] * (Pv[b] - Pv[a])
+    F = inv.reshape(-1, 3).astype(np.int64)
+    nrm = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    out = (nrm * (Pv[R[:, 0]] - Pv[R[:, 1]])).sum(1) < 0
+    F[out] = F[out][:, [0, 2, 1]]
+    return taubin_smooth(V, F, iterations=smooth), F
+
+
+def taubin_smooth(V, F, iterations=20, lam=0.5, mu=-0.53):
+    """Taubin lambda|mu smoothing with uniform neighbour weights (no shrinkage to first
+    order): removes the grid-scale ripple of marching tetrahedra, keeps the topology."""
+    if iterations <= 0:
+        return V
+    n = V.shape[0]
+    e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    import scipy.sparse as sp
+    A = sp.csr_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n))
+    A = ((A + A.T) > 0).astype(np.float64)
+    deg = np.asarray(A.sum(axis=1)).ravel()
+    for _ in range(iterations):
+        for w in (lam, mu):
+            V = V + w * (A @ V / deg[:, None] - V)
+    return V

@@ -468,6 +468,9 @@ class IterativeRounding:
             Msmall = np.eye(len(fixedI)) / wK ** 2 + Uf @ Z
             xt = A0lu.solve(b_int + wC ** 2 * xPrev)
             xstar = xt + Z @ np.linalg.solve(Msmall, r - Uf @ xt)
+            # the last LM round's system, for the sizing hook (oracle/sizing.py)
+            self.last_round = {"xPrev": xPrev.copy(), "Uf": Uf, "r": r.copy(), "Z": Z, "M": Msmall, "A0lu": A0lu,
+                               "X2F": X2F, "wI": wI, "wK": wK, "wC": wC}
 
             def dir_s(xs, Js=Js, obj_s=obj_s, xstar=xstar):
                 if self.literal:
