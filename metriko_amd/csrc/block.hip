@@ -56,13 +56,24 @@ __device__ __forceinline__ float4 vadd(const float4& b, const float4& a) {
 }
 __device__ __forceinline__ double2 vadd(const double2& b, const double2& a) { return make_double2(b.x + a.x, b.y + a.y); }
 
-// Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  A lane carries
-// one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront covers 64 / LPR
-// rows of the SELL part at a time; medium and long rows take a whole wavefront whose row
-// groups split the entries.
+__device__ __forceinline__ float4 vaxpy(float a, const float4& x, float b, const float4& y) {  // a x + b y
+    return make_float4(a * x.x + b * y.x, a * x.y + b * y.y, a * x.z + b * y.z, a * x.w + b * y.w);
+}
+__device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, const double2& y) {
+    return make_double2(a * x.x + b * y.x, a * x.y + b * y.y);
+}
+
+// Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  MODE 5 / 6: one
+// Chebyshev smoothing step fused with the residual it needs, row-local (Y = x_out, X = x_in,
+// never the same buffer): r = B - A X ; d = cs D^-1 r (5) or ca d + cs D^-1 r (6) ; Y = X + d.
+// A lane carries one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront
+// covers 64 / LPR rows of the SELL part at a time; medium and long rows take a whole
+// wavefront whose row groups split the entries.
 template <typename T, int MW, int MODE>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
-                                               T* __restrict__ Y, const T* __restrict__ B, const int* skip) {
+                                               T* __restrict__ Y, const T* __restrict__ B, const int* skip,
+                                               T* __restrict__ D = nullptr, const T* __restrict__ dinv = nullptr,
+                                               T ca = T(0), T cs = T(0)) {
     if (skip && *skip) return;
     using V = typename Vec16<T>::type;
     constexpr int N = Vec16<T>::N;
@@ -145,6 +156,16 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     if constexpr (MODE == 0) Yv[o] = acc;
     if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
     if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
+    if constexpr (MODE == 5 || MODE == 6) {
+        V* Dv = reinterpret_cast<V*>(D);
+        const V r = vsub(reinterpret_cast<const V*>(B)[o], acc);
+        const T sd = cs * dinv[row];
+        V d;
+        if constexpr (MODE == 5) d = vaxpy(sd, r, T(0), r);
+        else d = vaxpy(sd, r, ca, Dv[o]);
+        Dv[o] = d;
+        Yv[o] = vadd(Xv[o], d);
+    }
 }
 
 template <typename T, int MW>
@@ -168,6 +189,43 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
             break;
         default: throw Error(MK_ERR_ARG, "spmm: unsupported mode");
     }
+}
+
+template <typename T, int MW>
+static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, T* D,
+                         const T* dinv, T ca, T cs, bool first, const int* skip) {
+    unsigned g0 = 0;
+    const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
+    constexpr int R = 64 / (MW / Vec16<T>::N);
+    const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
+    const int64_t waves = int64_t(nw_sell) + A->nmid + A->nlong;
+    const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
+    if (first)
+        hipLaunchKernelGGL((sell_spmm<T, MW, 5>), dim3(g), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D, dinv,
+                           ca, cs);
+    else
+        hipLaunchKernelGGL((sell_spmm<T, MW, 6>), dim3(g), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D, dinv,
+                           ca, cs);
+}
+
+// One fused Chebyshev step on a block of mw right-hand sides (x_out != x_in).
+void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const float* x_in, float* x_out, const float* b, float* d,
+                      float ca, float cs, bool first, const int* skip) {
+    MK_CHECK(!A->cplx && x_in != x_out, MK_ERR_ARG, "fused block Chebyshev: real operator, x_out != x_in");
+    const double vb = 4.0 * mw;
+    const double bytes = A->bytes32 - double(A->n) * 2.0 * 4.0 + double(A->m) * vb +
+                         double(A->n) * vb * (first ? 4.0 : 5.0) + double(A->n) * 4.0;
+    SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64_cheb" : mw == 32 ? "spmm32_cheb" : mw == 16 ? "spmm16_cheb" :
+                 "spmm8_cheb", true);
+    const float* dinv = A->dinv32.p;
+    switch (mw) {
+        case 8: spmm_cheb_mw<float, 8>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
+        case 16: spmm_cheb_mw<float, 16>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
+        case 32: spmm_cheb_mw<float, 32>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
+        case 64: spmm_cheb_mw<float, 64>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
+        default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
+    }
+    MK_HIP(hipGetLastError());
 }
 
 template <typename T>

@@ -35,7 +35,7 @@ struct MgLevel {
     int n = 0;
     DevBuf<double> b64, x64, r64, d64, y64;  // y: second iterate buffer of the fused smoother
     DevBuf<float> b32, x32, r32, d32, y32;
-    DevBuf<float> bb32, bx32, br32, bd32;  // n x mw blocks for the batched V-cycle
+    DevBuf<float> bb32, bx32, br32, bd32, by32;  // n x mw blocks for the batched V-cycle
 };
 
 struct Mg {
@@ -882,22 +882,31 @@ static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, 
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
     const size_t len = size_t(L.n) * mw;
-    float* r = L.br32.p;
     float* d = L.bd32.p;
-    const float* src = b;
-    if (!zero_start) {
-        spmm_launch<float>(c, L.A, mw, x, r, b, SPMV_RESID_NORED, skip);
-        src = r;
+    float* y = L.by32.p;
+    // Every step after a zero start is one fused kernel (residual + update, x_in -> x_out);
+    // the iterate ping-pongs between x and y, and the first buffer is chosen so that the
+    // last step lands in x.
+    float* cur = x;
+    int fused = g->deg;
+    if (zero_start) {
+        cur = ((g->deg - 1) % 2 == 0) ? x : y;
+        hipLaunchKernelGGL(k_bcheb_first, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p,
+                           b, d, cur, float(1.0 / theta), 1, skip);
+        fused = g->deg - 1;
     }
-    hipLaunchKernelGGL(k_bcheb_first, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p, src,
-                       d, x, float(1.0 / theta), zero_start ? 1 : 0, skip);
-    for (int k = 1; k < g->deg; k++) {
-        const double rho1 = 1.0 / (2.0 * sigma - rho0);
-        spmm_launch<float>(c, L.A, mw, x, r, b, SPMV_RESID_NORED, skip);
-        hipLaunchKernelGGL(k_bcheb_next, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p,
-                           (const float*)r, d, x, float(rho1 * rho0), float(2.0 * rho1 / delta), skip);
-        rho0 = rho1;
+    for (int k = 0; k < fused; k++) {
+        float* out = (cur == x) ? y : x;
+        if (!zero_start && k == 0) {
+            spmm_cheb_launch(c, L.A, mw, cur, out, b, d, 0.0f, float(1.0 / theta), true, skip);
+        } else {
+            const double rho1 = 1.0 / (2.0 * sigma - rho0);
+            spmm_cheb_launch(c, L.A, mw, cur, out, b, d, float(rho1 * rho0), float(2.0 * rho1 / delta), false, skip);
+            rho0 = rho1;
+        }
+        cur = out;
     }
+    if (cur != x) copy_launch<float>(c, len, cur, x, skip);
 }
 
 static void vcycle_block(mk_ctx* c, Mg* g, int l, int mw, int sh, const float* b, float* x, const int* skip) {
@@ -1079,7 +1088,7 @@ void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const 
         MgLevel& L = g->lv[l];
         const size_t len = size_t(L.n) * mw;
         if (l > 0) { L.bb32.ensure(len); L.bx32.ensure(len); }
-        if (l + 1 < g->lv.size()) { L.br32.ensure(len); L.bd32.ensure(len); }
+        if (l + 1 < g->lv.size()) { L.br32.ensure(len); L.bd32.ensure(len); L.by32.ensure(len); }
     }
     vcycle_block(c, g, 0, mw, sh, b, x, skip);
 }

@@ -232,6 +232,10 @@ template <typename T>
 void copy_launch(mk_ctx* c, size_t len, const T* x, T* y, const int* skip);
 template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip);
+// one Chebyshev smoothing step fused with its residual on a block of mw right-hand sides:
+// r = b - A x_in ; d = cs D^-1 r (first) or ca d + cs D^-1 r ; x_out = x_in + d
+void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const float* x_in, float* x_out, const float* b, float* d,
+                      float ca, float cs, bool first, const int* skip);
 // A X = B for mw right-hand sides (B, X: n x mw row-major, device); X starts from zero.
 BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
                       int maxiter, int check_every);
