@@ -263,7 +263,11 @@ class RosyParameterization:
         self.ir = IterativeRounding(self.fixedIdcs, self.fixedVals, self.singularIdcs, self.integerIdcs,
                                     self.length, Cfull, self.G2full, N, n, cut.nF, self.seamless,
                                     self.roundSeams, self.rawF2Vec, literal=literal)
-        self.success = self.ir.run()
+        if self.localInjectivity:
+            from .injective import run_local_injectivity
+            self.success = run_local_injectivity(self.ir)
+        else:
+            self.success = self.ir.run()
         self.fullx = self.ir.fullx
         NF = self.fullx2Nfunc @ self.fullx
         self.nfn = NF.reshape(cut.nV, N)

@@ -13,7 +13,7 @@ from .seam import compute_combed_field, compute_cut_mesh, compute_extrinsic_fiel
 
 
 def run_pipeline(V, F, N=4, ratio=0.03, u0=None, n_iter=50, literal=False, seamless=True,
-                 round_seams=False):
+                 round_seams=False, local_injectivity=False):
     """Hmesh -> Smoothest field -> Curl matching -> seam -> cut -> comb -> IGM.
 
     Flags as ``example_igm.cpp:24-27`` (seamless, no injectivity, roundSeams=false);
@@ -36,7 +36,7 @@ def run_pipeline(V, F, N=4, ratio=0.03, u0=None, n_iter=50, literal=False, seaml
     t["combinatorics"] = time.perf_counter() - t0
     rp = RosyParameterization(mesh, cutm, ext, rawf.singular, cmatch, seam, N, ratio)
     rp.seamless = seamless
-    rp.localInjectivity = False
+    rp.localInjectivity = local_injectivity
     rp.roundSeams = round_seams
     t0 = time.perf_counter()
     rp.setup()
