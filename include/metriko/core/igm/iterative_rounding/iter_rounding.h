@@ -4,8 +4,8 @@
 // (NaiveIntegration KKT, naive LM pass, SeamlessIntegration rounding loop).  The whole
 // loop runs behind mk_iterative_rounding: the constraint kernel basis and operator
 // assembly on the host, the Poisson KKT, the LM directions, the energies and the FP64
-// rounding decisions on the GPU.  Local injectivity (the barrier LM) is not built in
-// this tier: requesting it raises.
+// rounding decisions on the GPU; with localInjectivity the damped barrier LM passes
+// (injective_barrier.h, levenberg_marquardt.h DiagonalDamping) run there too.
 #ifndef METRIKO_ITER_ROUNDING_H
 #define METRIKO_ITER_ROUNDING_H
 #include <functional>
@@ -19,9 +19,8 @@ namespace metriko {
                                    const bool localInjectivity, const bool verbose, const VecXd& intrinsicField,
                                    VecXd& fullx, std::vector<int>* decIndex = nullptr,
                                    std::vector<int>* decValue = nullptr) {
-        if (localInjectivity) throw std::runtime_error("iterative_rounding: localInjectivity is not built in this tier");
         auto& rt = hip::runtime();
-        mk_igm_flags fl{seamless ? 1 : 0, roundSeams ? 1 : 0, 0, verbose ? 1 : 0};
+        mk_igm_flags fl{seamless ? 1 : 0, roundSeams ? 1 : 0, localInjectivity ? 1 : 0, verbose ? 1 : 0};
         fullx = VecXd(G2.cols());
         const int cap = singularIdcs.size() + integerIdcs.size() + 1;
         std::vector<int32_t> di(static_cast<size_t>(cap)), dv(static_cast<size_t>(cap));
@@ -40,7 +39,10 @@ namespace metriko {
         return success != 0;
     }
 
-    // Reference signature (iter_rounding.h:33-51); the callback only acts with local injectivity.
+    // Reference signature (iter_rounding.h:33-51).  The reference calls iter_cb with the
+    // current fullx after every naive LM iteration and ends that pass when it returns true
+    // (iter_rounding_init.h:28); the device loop makes no per-iteration host callbacks, so
+    // iter_cb is not called here.
     inline bool iterative_rounding(const VecXi& fixedIdcs, const VecXd& fixedVals, const VecXi& singularIdcs,
                                    const VecXi& integerIdcs, const double length, const SprsD& Cfull, const SprsD& G2,
                                    const int N, const int n, const int nF, const bool seamless, const bool roundSeams,

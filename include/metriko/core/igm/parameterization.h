@@ -270,9 +270,19 @@ namespace metriko {
         }
         const double length = std::sqrt(bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]) * gridscale / norm;
         SprsD G2full = G2 * fullx2Nfunc;
+        if (localInjectivity) {   // the naive pass's iter_cb (parameterization_integ.ipp:45-63)
+            std::vector<int32_t> ci(size_t(3) * cut.nF);
+            for (int f = 0; f < cut.nF; f++)
+                for (int j = 0; j < 3; j++) ci[size_t(3) * f + j] = cut.idx(f, j);
+            hip::check(mk_igm_injectivity_check(hip::runtime().get(), N, cut.nF, ci.data(), fullx2Nfunc.rows(),
+                                                fullx2Nfunc.cols(), fullx2Nfunc.ptr.data(), fullx2Nfunc.col.data(),
+                                                fullx2Nfunc.val.data()),
+                       "mk_igm_injectivity_check");
+        }
         bool success = iterative_rounding(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2full, N, n,
                                           cut.nF, seamless, roundSeams, localInjectivity, verbose, F2, fullx,
                                           &decisionIndex, &decisionValue);
+        if (localInjectivity) mk_igm_injectivity_check(hip::runtime().get(), N, 0, nullptr, 0, 0, nullptr, nullptr, nullptr);
         nfn.resize(cut.nV, N);
         cfn.resize(raw.nF, N * 3);
         VecXd NF = fullx2Nfunc * fullx;

@@ -134,9 +134,11 @@ mk_status mk_field_smallest_eig(mk_ctx* ctx, const mk_op* L, const double* mass,
                                 const mk_solve_opts* opts, double* x_out, mk_stats* stats);
 
 /* --- IGM (a12-a16) ---------------------------------------------------------------------
- * Whole iterative_rounding (igm/iterative_rounding/iter_rounding.h:33-113) with the
- * seamless / no-injectivity flag set: NaiveIntegration KKT (iter_rounding_init.h:74-232),
- * naive LM pass, SeamlessIntegration rounding loop (iter_rounding_loop.h).  Inputs mirror the
+ * Whole iterative_rounding (igm/iterative_rounding/iter_rounding.h:33-113), every flag
+ * combination: NaiveIntegration KKT (iter_rounding_init.h:74-232), naive LM pass,
+ * SeamlessIntegration rounding loop (iter_rounding_loop.h).  flags->local_injectivity runs
+ * both LM passes with DiagonalDamping (lambda from 0.01) and the InjectiveBarrier rows
+ * (injective_barrier.h:10-110), also when !seamless (then the naive pass alone).  Inputs mirror the
  * reference's 17 parameters: G2 (= G2_cut * fullx2Nfunc, CSR rows 2N*nF x cols n*nR),
  * Cfull (CSR, already reduced to independent rows), intrinsic field F2[2N*nF].
  * Outputs: fullx[n*nR]; decisions (index, value) in rounding order (ndec on return).
@@ -160,6 +162,16 @@ mk_status mk_iterative_rounding(mk_ctx* ctx, int nfixed, const int32_t* fixedIdc
                                 const mk_solve_opts* opts, double* fullx, int* success,
                                 int max_dec, int32_t* dec_index, int32_t* dec_value, double* dec_margin, int* ndec,
                                 mk_igm_stats* stats);
+
+/* locally_injectivity_check (parameterization_integ.ipp:45-63), the iter_cb RosyParameterization
+ * hands iterative_rounding (iter_rounding.h:49, called per naive LM iteration by
+ * NaiveIntegration::post_iteration, iter_rounding_init.h:28): after this call, every
+ * mk_iterative_rounding with local_injectivity whose G2 has f_cols columns ends its naive LM pass
+ * once every cut face f keeps a non-negative orientation in (u, v) = functions 0 and 1 of
+ * fullx2Nfunc * fullx at its corners cut_idx[3f .. 3f+2].  fullx2Nfunc: CSR, f_rows = N * cut
+ * vertices, f_cols = fullx entries.  nF = 0 removes the check. */
+mk_status mk_igm_injectivity_check(mk_ctx* ctx, int N, int nF, const int32_t* cut_idx, int f_rows, int f_cols,
+                                   const int64_t* f_ptr, const int32_t* f_col, const double* f_val);
 
 /* --- differentiation: the sizing hook (no reference counterpart) ---------------------------
  * mk_keep_igm_state(ctx, 1) makes the next iterative rounding (mk_iterative_rounding or

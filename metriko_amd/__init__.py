@@ -24,7 +24,7 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
@@ -135,6 +135,7 @@ def lib():
     L.mk_batch_run.argtypes = [_P, C.c_int, _I32, _I32, C.POINTER(_D), C.POINTER(_I32), C.c_int, C.c_int,
                                C.POINTER(_D), C.POINTER(SolveOpts), C.POINTER(_D), C.POINTER(_D),
                                C.POINTER(BatchStats)]
+    L.mk_igm_injectivity_check.argtypes = [_P, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I64, _I32, _D]
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
     L.mk_igm_state_take.argtypes = [_P, C.POINTER(_P)]
     L.mk_igm_state_destroy.argtypes = [_P]
@@ -512,6 +513,60 @@ class Context:
                                             int(cplx), C.byref(o or opts()), C.byref(best), C.byref(us)),
                     "mk_autotune_spmv")
         return best, us.value
+
+    def iterative_rounding(self, fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n, nF,
+                           F2, seamless=True, round_seams=False, local_injectivity=False, o=None,
+                           cut_idx=None, fullx2Nfunc=None):
+        """iterative_rounding (iter_rounding.h:33-51) through mk_iterative_rounding: Cfull and
+        G2 as scipy CSR; returns fullx, success, decisions [(index, value, margin)], stats.
+        cut_idx (cut faces x 3) and fullx2Nfunc (scipy CSR) give the naive pass the
+        locally_injectivity_check RosyParameterization passes as iter_cb."""
+        if cut_idx is not None:
+            ci = np.ascontiguousarray(cut_idx, np.int32).reshape(-1)
+            f2n = fullx2Nfunc.tocsr()
+            fp, fc, fv_ = (np.ascontiguousarray(f2n.indptr, np.int64), np.ascontiguousarray(f2n.indices, np.int32),
+                           np.ascontiguousarray(f2n.data, np.float64))
+            self._check(self.L.mk_igm_injectivity_check(self.h, N, len(ci) // 3, _ptr(ci, _I32), f2n.shape[0],
+                                                         f2n.shape[1], _ptr(fp, _I64), _ptr(fc, _I32), _ptr(fv_, _D)),
+                        "mk_igm_injectivity_check")
+        try:
+            return self._iterative_rounding(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n,
+                                            nF, F2, seamless, round_seams, local_injectivity, o)
+        finally:
+            if cut_idx is not None:
+                self.L.mk_igm_injectivity_check(self.h, N, 0, None, 0, 0, None, None, None)
+
+    def _iterative_rounding(self, fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n, nF,
+                            F2, seamless, round_seams, local_injectivity, o):
+        fi = np.ascontiguousarray(fixedIdcs, np.int32)
+        fv = np.ascontiguousarray(fixedVals, np.float64)
+        si = np.ascontiguousarray(singularIdcs, np.int32)
+        ii = np.ascontiguousarray(integerIdcs, np.int32)
+        Cf = Cfull.tocsr()
+        G = G2.tocsr()
+        cp, cc, cv = (np.ascontiguousarray(Cf.indptr, np.int64), np.ascontiguousarray(Cf.indices, np.int32),
+                      np.ascontiguousarray(Cf.data, np.float64))
+        gp, gc, gv = (np.ascontiguousarray(G.indptr, np.int64), np.ascontiguousarray(G.indices, np.int32),
+                      np.ascontiguousarray(G.data, np.float64))
+        f2 = np.ascontiguousarray(F2, np.float64)
+        fl = IgmFlags(int(seamless), int(round_seams), int(local_injectivity), 0)
+        fullx = np.zeros(G.shape[1])
+        cap = len(si) + len(ii) + 1
+        di = np.zeros(cap, np.int32)
+        dv = np.zeros(cap, np.int32)
+        dm = np.zeros(cap)
+        nd = C.c_int()
+        ok = C.c_int()
+        st = IgmStats()
+        self._check(self.L.mk_iterative_rounding(
+            self.h, len(fi), _ptr(fi, _I32), _ptr(fv, _D), len(si), _ptr(si, _I32), len(ii), _ptr(ii, _I32),
+            float(length), Cf.shape[0], Cf.shape[1], _ptr(cp, _I64), _ptr(cc, _I32), _ptr(cv, _D), G.shape[0],
+            G.shape[1], _ptr(gp, _I64), _ptr(gc, _I32), _ptr(gv, _D), N, n, nF, C.byref(fl), _ptr(f2, _D),
+            C.byref(o or opts()), _ptr(fullx, _D), C.byref(ok), cap, _ptr(di, _I32), _ptr(dv, _I32), _ptr(dm, _D),
+            C.byref(nd), C.byref(st)), "mk_iterative_rounding")
+        m = min(nd.value, cap)
+        dec = list(zip(di[:m].tolist(), dv[:m].tolist(), dm[:m].tolist()))
+        return fullx, bool(ok.value), dec, struct_dict(st)
 
     def pipeline(self, V, F, ratio=0.03, u0=None, n_iter=50, field_opts=None, igm_opts=None):
         """example_igm in one call (mk_pipeline_run); returns outputs and per-stage info."""

@@ -491,6 +491,794 @@ std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD&
     return out;
 }
 
+
+// ---- local injectivity (iter_rounding.h:33-113 with localInjectivity = true) ---------------
+// Both LM passes run LMSolver with DiagonalDamping (levenberg_marquardt.h:13-83,140-231,
+// lambda from 0.01) on the reference's objectives with the InjectiveBarrier rows
+// (injective_barrier.h:10-110, iter_rounding_common.h:60-94).  Each direction solves the
+// damped normal equations (J^T J + diag(lambda colsum J.^2)) d = -J^T E exactly as the
+// reference's dampJ^T dampJ does, in this form:
+//  * the 1e8 / 1e10 weighted rows (fixed values and their damping entries) are a low-rank
+//    term W^T diag(omega) W handled by Sherman-Morrison-Woodbury;
+//  * the rest, Hs, is assembled on the host from E = length X2F^T X2F plus the active barrier
+//    rows and solved by multigrid PCG on the device (one hierarchy per direction);
+//  * the naive pass eliminates f face by face (its f-block is block diagonal, 2N x 2N per
+//    face) and solves the reduced system in x.
+// Barrier values, derivatives, the gradients -J^T E and the energies are device kernels.
+
+// barrier rows per (face i, pair j): r = (v_j x v_{j+1}) / V_ij, s = 0.5 (injective_barrier.h:53-103)
+__global__ void k_barrier(int nF, int N, const double* field, const double* V, double* fB, double* der, double* S) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nF * N) return;
+    const int i = t / N, j = t % N, j1 = (j + 1) % N;
+    const double* a = field + size_t(2 * N) * i + 2 * j;
+    const double* b = field + size_t(2 * N) * i + 2 * j1;
+    const double v = V[t], s = 0.5;
+    const double r = (a[0] * b[1] - a[1] * b[0]) / v;
+    const double q = r / s;
+    const double bar = q * q * q - 3.0 * q * q + 3.0 * q;
+    double b2 = 1.0 / bar - 1.0;
+    if (r <= 0.0) b2 = INFINITY;
+    if (r >= s) b2 = 0.0;
+    fB[t] = b2;
+    if (!der) return;
+    double d = 3.0 * (r * r / (s * s * s)) - 6.0 * (r / (s * s)) + 3.0 / s;
+    if (r <= 0.0) d = INFINITY;
+    if (r >= s) d = 0.0;
+    double g = -d / (bar * bar);
+    if (fabs(b2) < 10e-9) g = 0.0;
+    else if (b2 == INFINITY) g = INFINITY;
+    der[t] = g;
+    S[4 * size_t(t) + 0] = b[1] / v;
+    S[4 * size_t(t) + 1] = -b[0] / v;
+    S[4 * size_t(t) + 2] = -a[1] / v;
+    S[4 * size_t(t) + 3] = a[0] / v;
+}
+// out[c] = (S^T diag(der) e)[c] over the 2N nF field entries: column (face i, vector j,
+// component k) is slot k of pair (i, j) and slot 2 + k of pair (i, j - 1)
+__global__ void k_barrier_tmul(int nF, int N, const double* S, const double* der, const double* e, double* out) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= size_t(2 * N) * nF) return;
+    const int i = int(c / (2 * N)), w = int(c % (2 * N)), j = w >> 1, k = w & 1;
+    const size_t q0 = size_t(i) * N + j, q1 = size_t(i) * N + (j + N - 1) % N;
+    out[c] = S[4 * q0 + k] * der[q0] * e[q0] + S[4 * q1 + 2 + k] * der[q1] * e[q1];
+}
+// per-face 2N x 2N operator: out = blocks[slot[face]] * in, or s * in for faces without a block
+__global__ void k_face_apply(size_t l1, int N2, double s, const int* slot, const double* blocks, const double* in,
+                             double* out) {
+    const size_t c = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= l1) return;
+    const size_t f = c / N2;
+    const int w = int(c % N2), sl = slot[f];
+    if (sl < 0) { out[c] = s * in[c]; return; }
+    const double* B = blocks + size_t(sl) * N2 * N2 + size_t(w) * N2;
+    double a = 0.0;
+    for (int b = 0; b < N2; b++) a += B[b] * in[f * N2 + b];
+    out[c] = a;
+}
+// y = a x + b z + c w
+__global__ void k_lin3(size_t n, double a, const double* x, double b, const double* z, double c, const double* w,
+                       double* y) {
+    const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * z[i] + c * w[i];
+}
+// flag[f] = 1 where cut face f turns negative in (u, v) = functions 0, 1 of NF
+__global__ void k_flips(int nF, int N, const int* idx, const double* NF, double* flag) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const double* a = NF + size_t(N) * idx[3 * f];
+    const double* b = NF + size_t(N) * idx[3 * f + 1];
+    const double* c = NF + size_t(N) * idx[3 * f + 2];
+    const double d1x = b[0] - a[0], d1y = b[1] - a[1], d2x = c[0] - a[0], d2y = c[1] - a[1];
+    flag[f] = (d1x * d2y - d1y * d2x < 0.0) ? 1.0 : 0.0;
+}
+// y[idx[t]] += val[t] (distinct indices)
+__global__ void k_add_sparse(int m, const int* idx, const double* val, double* y) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) y[idx[t]] += val[t];
+}
+
+// small dense SPD inverse (Gauss-Jordan with partial pivoting), row-major n x n
+std::vector<double> dense_inverse(std::vector<double> A, int n) {
+    std::vector<double> I(size_t(n) * n, 0.0);
+    for (int i = 0; i < n; i++) I[size_t(i) * n + i] = 1.0;
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++) if (std::abs(A[size_t(i) * n + k]) > std::abs(A[size_t(p) * n + k])) p = i;
+        if (A[size_t(p) * n + k] == 0.0) throw Error(MK_ERR_SINGULAR, "injectivity: singular face block");
+        if (p != k)
+            for (int j = 0; j < n; j++) {
+                std::swap(A[size_t(k) * n + j], A[size_t(p) * n + j]);
+                std::swap(I[size_t(k) * n + j], I[size_t(p) * n + j]);
+            }
+        const double piv = A[size_t(k) * n + k];
+        for (int j = 0; j < n; j++) { A[size_t(k) * n + j] /= piv; I[size_t(k) * n + j] /= piv; }
+        for (int i = 0; i < n; i++) {
+            if (i == k) continue;
+            const double l = A[size_t(i) * n + k];
+            if (l == 0.0) continue;
+            for (int j = 0; j < n; j++) { A[size_t(i) * n + j] -= l * A[size_t(k) * n + j]; I[size_t(i) * n + j] -= l * I[size_t(k) * n + j]; }
+        }
+    }
+    return I;
+}
+
+// H + diag(dv) for a matrix whose rows all hold their diagonal entry
+void add_diagonal(SprsD& H, const std::vector<double>& dv) {
+    for (int i = 0; i < H.rows(); i++) {
+        bool found = false;
+        for (int64_t q = H.ptr[i]; q < H.ptr[i + 1]; q++)
+            if (H.col[q] == i) { H.val[q] += dv[size_t(i)]; found = true; break; }
+        if (!found) throw Error(MK_ERR_SINGULAR, "injectivity: a variable no gradient row touches");
+    }
+}
+
+// sparse sum of rows of M (row index list rows[], coefficients coef[]) as sorted (col, val)
+void merge_rows(const SprsD& M, const std::vector<int>& rows, const std::vector<double>& coef,
+                std::vector<std::pair<int, double>>& out) {
+    out.clear();
+    for (size_t a = 0; a < rows.size(); a++)
+        for (int64_t q = M.ptr[rows[a]]; q < M.ptr[rows[a] + 1]; q++) out.emplace_back(M.col[q], coef[a] * M.val[q]);
+    std::sort(out.begin(), out.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    size_t u = 0;
+    for (size_t t = 0; t < out.size(); t++) {
+        if (u && out[u - 1].first == out[t].first) out[u - 1].second += out[t].second;
+        else out[u++] = out[t];
+    }
+    out.resize(u);
+}
+
+struct InjSolver {
+    Ctx& K;
+    const mk_op* A0;   // near-null vectors of the x-space operators
+    int k;
+    DevBuf<double> y, Z, drow, dd, dw;
+    DevBuf<int> didx;
+    InjSolver(Ctx& K_, const mk_op* A0_, int k_) : K(K_), A0(A0_), k(k_), y(size_t(k_)), drow(size_t(k_)) {}
+
+    // g += sum_i coef_i R_i (rows of R)
+    void add_rows(const SprsD& R, const std::vector<double>& coef, double* g) {
+        std::vector<int> rows(static_cast<size_t>(R.rows()));
+        std::iota(rows.begin(), rows.end(), 0);
+        std::vector<std::pair<int, double>> m;
+        merge_rows(R, rows, coef, m);
+        if (m.empty()) return;
+        std::vector<int> ix(m.size());
+        std::vector<double> vv(m.size());
+        for (size_t t = 0; t < m.size(); t++) { ix[t] = m[t].first; vv[t] = m[t].second; }
+        didx.upload(ix.data(), ix.size(), K.st());
+        dw.upload(vv.data(), vv.size(), K.st());
+        hipLaunchKernelGGL(k_add_sparse, dim3(nblk(m.size())), dim3(256), 0, K.st(), int(m.size()), didx.p, dw.p, g);
+    }
+    // R x on the device -> host
+    std::vector<double> rowdots(const SprsD& R, const double* x) {
+        std::vector<double> out(static_cast<size_t>(R.rows()), 0.0);
+        if (!R.rows()) return out;
+        RowSet rs;
+        for (int i = 0; i < R.rows(); i++) rs.push(R, i);
+        rs.sync(K.st());
+        dd.ensure(size_t(R.rows()));
+        K.rowdots(rs, x, dd.p);
+        dd.download(out.data(), out.size(), K.st());
+        K.sync();
+        return out;
+    }
+    // multigrid PCG to the solve tolerance, at most 1000 iterations; an unconverged or broken-
+    // down result is kept (an LM direction only has to lower the energy, which the caller
+    // tests; a non-finite one makes the step a rejected one)
+    void pcg(const mk_op* A, const double* b, double* x) {
+        mk_solve_opts o2 = K.o;
+        o2.maxiter = std::min(o2.maxiter, 1000);
+        const mk::SolveResult r = mk::solve_spd(K.c, A, b, x, o2);
+        K.acc.iterations += r.iters;
+        K.acc.refinements += r.refinements;
+        K.acc.solves += 1;
+        if (r.status == MK_ERR_ARG || r.status == MK_ERR_HIP || r.status == MK_ERR_ALLOC ||
+            r.status == MK_ERR_UNSUPPORTED)
+            throw Error(r.status, "IGM solve failed: " + std::string(mk_status_str(r.status)));
+    }
+    // (Hs + W^T diag(omega) W) d = g; Hs SPD (host CSR, k x k), W few sparse rows
+    void solve(const SprsD& Hs, const SprsD& W, const std::vector<double>& omega, const double* g, double* d) {
+        OpHolder h;
+        h.op = mk::op_create_owned(K.c, k, std::vector<int64_t>(Hs.ptr), std::vector<int>(Hs.col),
+                                   std::vector<double>(Hs.val), K.o);
+        h.op->label = "igm_inj";
+        h.op->nullvec = A0->nullvec;
+        h.op->nnull = A0->nnull;
+        mk::ensure_mg(K.c, h.op, K.o);
+        hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), y.p);
+        pcg(h.op, g, y.p);
+        const int m = W.rows();
+        if (m == 0) {
+            MK_HIP(hipMemcpyAsync(d, y.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+            return;
+        }
+        Z.ensure(size_t(k) * size_t(m));
+        RowSet ws;
+        for (int i = 0; i < m; i++) ws.push(W, i);
+        ws.sync(K.st());
+        std::vector<double> M(static_cast<size_t>(m) * m), col(static_cast<size_t>(m));
+        dd.ensure(size_t(m));
+        for (int j = 0; j < m; j++) {
+            double* z = Z.p + size_t(j) * k;
+            hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), drow.p);
+            const int64_t b = ws.hptr[size_t(j)], e = ws.hptr[size_t(j) + 1];
+            if (e > b)
+                hipLaunchKernelGGL(k_scatter_row, dim3(nblk(size_t(e - b))), dim3(256), 0, K.st(), b, e, ws.col.p,
+                                   ws.val.p, drow.p);
+            hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), z);
+            pcg(h.op, drow.p, z);
+            K.rowdots(ws, z, dd.p);
+            dd.download(col.data(), col.size(), K.st());
+            K.sync();
+            for (int i = 0; i < m; i++) M[size_t(i) * m + j] = col[size_t(i)];
+        }
+        for (int i = 0; i < m; i++) M[size_t(i) * m + i] += 1.0 / omega[size_t(i)];
+        K.rowdots(ws, y.p, dd.p);
+        dd.download(col.data(), col.size(), K.st());
+        K.sync();
+        std::vector<double> cc = dense_solve(M, col, m);
+        for (auto& v : cc) v = -v;
+        std::vector<int> idx(static_cast<size_t>(m));
+        std::iota(idx.begin(), idx.end(), 0);
+        didx.upload(idx.data(), idx.size(), K.st());
+        dw.upload(cc.data(), cc.size(), K.st());
+        hipLaunchKernelGGL(k_combine, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), m, y.p, Z.p,
+                           (const int*)didx.p, dw.p, d);
+    }
+};
+
+// LMSolver::solve with DiagonalDamping (lambda from 0.01).  energy(x) -> ||E(x)||^2;
+// jac(x) evaluates the Jacobian state at x; dir(lambda, d) -> false when that Jacobian is not
+// finite (the reference's direction is then NaN and no later step is taken); post(x) -> true
+// ends the solve after an iteration (ST->post_iteration).
+template <typename Energy, typename Jac, typename Dir, typename Post>
+bool lm_damped(Ctx& K, size_t n, double* x, int maxIter, double funcTol, Energy&& energy, Jac&& jac, Dir&& dir,
+               Post&& post, DevBuf<double>& d, DevBuf<double>& trial, int& iters) {
+    d.ensure(n);
+    trial.ensure(n);
+    double lam = 0.01;
+    jac(x);
+    for (int it = 0;;) {
+        iters++;
+        if (!dir(lam, d.p)) return false;
+        const double dn = std::sqrt(K.sqdiff(n, 1.0, d.p, nullptr));
+        if (dn < funcTol) return true;
+        const double e_prev = energy(x);
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(n)), dim3(256), 0, K.st(), n, 1.0, x, 1.0, d.p, trial.p);
+        const double e_new = energy(trial.p);
+        const bool acc = e_prev > e_new;
+        if (acc) {
+            MK_HIP(hipMemcpyAsync(x, trial.p, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+            if (std::abs(e_prev - e_new) < funcTol) return false;
+            jac(x);
+        }
+        lam = (acc && e_new != INFINITY) ? lam / 10.0 : lam * 10.0;
+        if (post(x)) return true;
+        it++;
+        if (!(it <= maxIter)) return false;
+    }
+}
+
+struct InjInputs {
+    int k, nfull, l1, N, nF;
+    double length;
+    const SprsD* UF;
+    const SprsD* X2F;    // G2 UFull (without length)
+    const SprsD* E;      // length X2F^T X2F
+    const SprsD* Cm;     // UFull rows at fixedIdcs
+    const double* fixedVals;
+    const double* F2;
+    const std::vector<double>* XSmall;
+    const mk_op* A0;
+    mk_op* B;            // length X2F (device)
+    mk_op* Bt;           // length X2F^T (device)
+};
+
+// barrier state of one field (device), with host copies of der and S after a Jacobian update
+struct BarrierState {
+    DevBuf<double> V, fB, der, S, tmul;
+    std::vector<double> hder, hS;
+    int N, nF;
+    void init(Ctx& K, const double* F2, int N_, int nF_) {
+        N = N_;
+        nF = nF_;
+        std::vector<double> v(size_t(N) * nF);
+        for (int i = 0; i < nF; i++)
+            for (int j = 0; j < N; j++) {
+                const double* a = F2 + size_t(2 * N) * i + 2 * j;
+                const double* b = F2 + size_t(2 * N) * i + 2 * ((j + 1) % N);
+                v[size_t(i) * N + j] = std::sqrt(a[0] * a[0] + a[1] * a[1]) * std::sqrt(b[0] * b[0] + b[1] * b[1]);
+            }
+        V.upload(v.data(), v.size(), K.st());
+        fB.alloc(v.size());
+        der.alloc(v.size());
+        S.alloc(4 * v.size());
+        tmul.alloc(size_t(2 * N) * nF);
+    }
+    // ||fB(field)||^2
+    double energy(Ctx& K, const double* field) {
+        const size_t m = size_t(N) * nF;
+        hipLaunchKernelGGL(k_barrier, dim3(nblk(m)), dim3(256), 0, K.st(), nF, N, field, V.p, fB.p, nullptr, nullptr);
+        return K.sqdiff(m, 1.0, fB.p, nullptr);
+    }
+    // fB, der, S at field; tmul = S^T diag(der) fB; false when a derivative is not finite
+    bool update(Ctx& K, const double* field) {
+        const size_t m = size_t(N) * nF;
+        hipLaunchKernelGGL(k_barrier, dim3(nblk(m)), dim3(256), 0, K.st(), nF, N, field, V.p, fB.p, der.p, S.p);
+        hipLaunchKernelGGL(k_barrier_tmul, dim3(nblk(2 * m)), dim3(256), 0, K.st(), nF, N, S.p, der.p, fB.p, tmul.p);
+        hder.resize(m);
+        hS.resize(4 * m);
+        der.download(hder.data(), m, K.st());
+        S.download(hS.data(), 4 * m, K.st());
+        K.sync();
+        for (double v : hder) if (!std::isfinite(v)) return false;
+        return true;
+    }
+};
+
+// Hs for the seamless pass: wI^2 length E + wC^2 I + sum_q wB^2 v_q^T v_q plus the damping
+// diagonal lambda (wI^2 length diag E + wC^2 + wB^2 sum_q v_q.^2); v_q = der_q S_q X2F'.
+// Barrier rows with wB^2 |v_q|^2 above tau (near-flipped pairs: der grows like 1/r^2) go to
+// `stiff` instead, for the low-rank part of the solve.
+using SparseRow = std::vector<std::pair<int, double>>;
+SprsD seamless_soft(const InjInputs& in, const BarrierState& bs, double lam, double wI, double wC, double wB,
+                    std::vector<SparseRow>& stiff) {
+    const int k = in.k;
+    SprsD H = *in.E;   // X2F'^T X2F' = length E
+    for (auto& v : H.val) v *= wI * wI * in.length;
+    std::vector<double> dv(static_cast<size_t>(k), 0.0);
+    double dmax = 0.0;
+    for (int i = 0; i < k; i++)
+        for (int64_t q = in.E->ptr[i]; q < in.E->ptr[i + 1]; q++)
+            if (in.E->col[q] == i) {
+                dv[size_t(i)] = lam * wI * wI * in.length * in.E->val[q];
+                dmax = std::max(dmax, wI * wI * in.length * in.E->val[q]);
+            }
+    for (auto& v : dv) v += wC * wC * (1.0 + lam);
+    const double tau = 1e4 * (dmax + wC * wC);
+    stiff.clear();
+    std::vector<TripD> T;
+    SparseRow vq;
+    const int N = bs.N;
+    for (int i = 0; i < bs.nF; i++)
+        for (int j = 0; j < N; j++) {
+            const size_t q = size_t(i) * N + j;
+            const double g = bs.hder[q];
+            if (g == 0.0) continue;
+            const int j1 = (j + 1) % N;
+            const int r0 = 2 * N * i;
+            std::vector<int> rows{r0 + 2 * j, r0 + 2 * j + 1, r0 + 2 * j1, r0 + 2 * j1 + 1};
+            std::vector<double> cf(4);
+            for (int s = 0; s < 4; s++) cf[size_t(s)] = g * bs.hS[4 * q + s] * in.length;
+            merge_rows(*in.X2F, rows, cf, vq);
+            double n2 = 0.0;
+            for (auto& a : vq) {
+                dv[size_t(a.first)] += lam * wB * wB * a.second * a.second;
+                n2 += a.second * a.second;
+            }
+            if (wB * wB * n2 > tau) { stiff.push_back(vq); continue; }
+            for (auto& a : vq)
+                for (auto& b : vq) T.emplace_back(a.first, b.first, wB * wB * a.second * b.second);
+        }
+    if (!T.empty()) {
+        SprsD Bar(k, k);
+        Bar.setFromTriplets(T);
+        H = H + Bar;
+    }
+    add_diagonal(H, dv);
+    return H;
+}
+
+// W with the rows `extra` appended (weight w each)
+void append_rows(SprsD& W, std::vector<double>& omega, const std::vector<SparseRow>& extra, double w) {
+    if (extra.empty()) return;
+    std::vector<TripD> T;
+    for (int i = 0; i < W.rows(); i++)
+        for (int64_t q = W.ptr[i]; q < W.ptr[i + 1]; q++) T.emplace_back(i, W.col[q], W.val[q]);
+    int r = W.rows();
+    for (const auto& row : extra) {
+        for (const auto& e : row) T.emplace_back(r, e.first, e.second);
+        omega.push_back(w);
+        r++;
+    }
+    SprsD W2(r, W.cols());
+    W2.setFromTriplets(T);
+    W = std::move(W2);
+}
+
+// the low-rank stiff rows: R (weight wR each) and unit rows on the columns R touches
+// (weight lambda wD colsum(R.^2), the damping entries of rows weighted wD)
+void stiff_rows(const SprsD& R, double wR, double wD, double lam, int k, SprsD& W, std::vector<double>& omega) {
+    std::vector<double> cs(size_t(k), 0.0);
+    std::vector<int> cols;
+    for (int64_t q = 0; q < R.nonZeros(); q++) {
+        if (cs[size_t(R.col[q])] == 0.0 && R.val[q] != 0.0) cols.push_back(R.col[q]);
+        cs[size_t(R.col[q])] += R.val[q] * R.val[q];
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    std::vector<TripD> T;
+    omega.clear();
+    for (int i = 0; i < R.rows(); i++) {
+        for (int64_t q = R.ptr[i]; q < R.ptr[i + 1]; q++) T.emplace_back(i, R.col[q], R.val[q]);
+        omega.push_back(wR);
+    }
+    int r = R.rows();
+    for (int c : cols) {
+        const double om = lam * wD * cs[size_t(c)];
+        if (!(om > 0.0)) continue;
+        T.emplace_back(r++, c, 1.0);
+        omega.push_back(om);
+    }
+    W = SprsD(r, k);
+    W.setFromTriplets(T);
+}
+
+// LMSolver over z = (x, f) for the naive pass (iter_rounding_init.h:60-72, weights
+// iter_rounding_init.h:184-191: wInteg = wConst = 1e4, wClose = 1, wBarrier = 1e-4).
+// J = [wI (-B | I); wC (0 | I); wK (Cm | 0); wI wC D S (0 | I)], B = length X2F.  Its normal
+// matrix has the face-block-diagonal f-block Hff = a I + K (a = (1 + lambda)(wI^2 + wC^2),
+// K = (wI wC)^2 (S^T D^2 S + lambda diag colsum (DS).^2)); with P = Hff^-1 and
+// Q = P ((a - wI^2) I + K) = I - wI^2 P the reduced system is
+//   (wI^2 B^T Q B + wK^2 Cm^T Cm + lambda diag_x) dx = wI^2 B^T (Q res + P gb) - wK^2 Cm^T cres,
+//   df = P (-wI^2 (f - B (x + dx)) + gb),
+// res = f - B x, gb = -wC^2 (f - F2) - wI wC wB S^T D fB, cres = Cm x - fixedVals (Q is formed
+// without the cancellation of I - wI^2 P).
+struct NaivePass {
+    Ctx& K;
+    const InjInputs& in;
+    InjSolver& sol;
+    BarrierState& bs;
+    const double wI = 10e3, wK = 10e3, wC = 1.0, wB = 1e-4;
+    DevBuf<double> dF2, Bx, res, gb, t1, t2, tk, gx, dP, dQ;
+    DevBuf<int> dslot;
+    std::vector<double> cres;
+    const double* zcur = nullptr;
+    bool finite = true;
+    NaivePass(Ctx& K_, const InjInputs& in_, InjSolver& sol_, BarrierState& bs_) : K(K_), in(in_), sol(sol_), bs(bs_) {
+        const size_t l1 = size_t(in.l1), k = size_t(in.k);
+        dF2.upload(in.F2, l1, K.st());
+        Bx.alloc(l1); res.alloc(l1); gb.alloc(l1); t1.alloc(l1); t2.alloc(l1);
+        tk.alloc(k); gx.alloc(k);
+        dslot.alloc(size_t(in.nF));
+    }
+    std::vector<double> cm_res(const double* x) {
+        std::vector<double> r = sol.rowdots(*in.Cm, x);
+        for (int i = 0; i < in.Cm->rows(); i++) r[size_t(i)] -= in.fixedVals[i];
+        return r;
+    }
+    double energy(const double* z) {
+        const size_t l1 = size_t(in.l1);
+        const double* f = z + in.k;
+        mk::spmv64(K.c, in.B, z, Bx.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        const double a = K.sqdiff(l1, 1.0, f, Bx.p);
+        const double b = K.sqdiff(l1, 1.0, f, dF2.p);
+        double c = 0.0;
+        for (double v : cm_res(z)) c += v * v;
+        const double d = bs.energy(K, f);
+        return wI * wI * a + wC * wC * b + wK * wK * c + wB * wB * d;
+    }
+    void jac(const double* z) {
+        const size_t l1 = size_t(in.l1);
+        const double* f = z + in.k;
+        zcur = z;
+        mk::spmv64(K.c, in.B, z, Bx.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        finite = bs.update(K, f);
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, 1.0, f, -1.0, Bx.p, res.p);
+        hipLaunchKernelGGL(k_lin3, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, -wC * wC, f, wC * wC, dF2.p,
+                           -wI * wC * wB, bs.tmul.p, gb.p);
+        cres = cm_res(z);
+    }
+    bool dir(double lam, double* dz) {
+        if (!finite) return false;
+        const int N = in.N, N2 = 2 * N, nF = in.nF, k = in.k;
+        const size_t l1 = size_t(in.l1);
+        const double a = (1.0 + lam) * (wI * wI + wC * wC);
+        const double am = lam * wI * wI + (1.0 + lam) * wC * wC;   // a - wI^2
+        const double p = 1.0 / a, q = am / a;
+        const double wf = wI * wC;
+        std::vector<int> slot(size_t(nF), -1);
+        std::vector<double> Pb, Qb;
+        std::vector<int> act;
+        for (int i = 0; i < nF; i++) {
+            bool any = false;
+            for (int j = 0; j < N; j++) any |= bs.hder[size_t(i) * N + j] != 0.0;
+            if (!any) continue;
+            slot[size_t(i)] = int(act.size());
+            act.push_back(i);
+            std::vector<double> Kb(size_t(N2) * N2, 0.0), cs(size_t(N2), 0.0);
+            for (int j = 0; j < N; j++) {
+                const size_t qq = size_t(i) * N + j;
+                const double g = bs.hder[qq];
+                if (g == 0.0) continue;
+                const int cl[4] = {2 * j, 2 * j + 1, 2 * ((j + 1) % N), 2 * ((j + 1) % N) + 1};
+                double sv[4];
+                for (int s = 0; s < 4; s++) sv[s] = g * bs.hS[4 * qq + s];
+                for (int s = 0; s < 4; s++) {
+                    cs[size_t(cl[s])] += sv[s] * sv[s];
+                    for (int t = 0; t < 4; t++) Kb[size_t(cl[s]) * N2 + cl[t]] += sv[s] * sv[t];
+                }
+            }
+            for (int c = 0; c < N2; c++) Kb[size_t(c) * N2 + c] += lam * cs[size_t(c)];
+            for (auto& v : Kb) v *= wf * wf;
+            std::vector<double> H(Kb);
+            for (int c = 0; c < N2; c++) H[size_t(c) * N2 + c] += a;
+            std::vector<double> P = dense_inverse(H, N2);
+            std::vector<double> Qi(size_t(N2) * N2, 0.0);   // P (am I + K)
+            for (int r = 0; r < N2; r++)
+                for (int c = 0; c < N2; c++) {
+                    double s = am * P[size_t(r) * N2 + c];
+                    for (int t = 0; t < N2; t++) s += P[size_t(r) * N2 + t] * Kb[size_t(t) * N2 + c];
+                    Qi[size_t(r) * N2 + c] = s;
+                }
+            Pb.insert(Pb.end(), P.begin(), P.end());
+            Qb.insert(Qb.end(), Qi.begin(), Qi.end());
+        }
+        dslot.upload(slot.data(), slot.size(), K.st());
+        if (!act.empty()) {
+            dP.upload(Pb.data(), Pb.size(), K.st());
+            dQ.upload(Qb.data(), Qb.size(), K.st());
+        }
+        const double* Pp = act.empty() ? nullptr : dP.p;
+        const double* Qp = act.empty() ? nullptr : dQ.p;
+        // reduced right-hand side
+        hipLaunchKernelGGL(k_face_apply, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, N2, q, dslot.p, Qp, res.p, t1.p);
+        hipLaunchKernelGGL(k_face_apply, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, N2, p, dslot.p, Pp, gb.p, t2.p);
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, wI * wI, t1.p, wI * wI, t2.p, t1.p);
+        mk::spmv64(K.c, in.Bt, t1.p, gx.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        {
+            std::vector<double> cf(cres.size());
+            for (size_t i = 0; i < cres.size(); i++) cf[i] = -wK * wK * cres[i];
+            sol.add_rows(*in.Cm, cf, gx.p);
+        }
+        // reduced operator: wI^2 q length E + lambda wI^2 length diag E + active face terms + pin
+        SprsD H = *in.E;
+        const double L = in.length;
+        for (auto& v : H.val) v *= wI * wI * q * L;
+        std::vector<double> dv(static_cast<size_t>(k), 0.0);
+        for (int i = 0; i < k; i++)
+            for (int64_t t = in.E->ptr[i]; t < in.E->ptr[i + 1]; t++)
+                if (in.E->col[t] == i) dv[size_t(i)] = lam * wI * wI * L * in.E->val[t];
+        if (!act.empty()) {
+            std::vector<TripD> T;
+            const SprsD& X = *in.X2F;
+            for (size_t s = 0; s < act.size(); s++) {
+                const int i = act[s];
+                const double* Qi = Qb.data() + s * size_t(N2) * N2;
+                for (int ra = 0; ra < N2; ra++)
+                    for (int rb = 0; rb < N2; rb++) {
+                        const double cab = wI * wI * (Qi[size_t(ra) * N2 + rb] - (ra == rb ? q : 0.0)) * L * L;
+                        if (cab == 0.0) continue;
+                        const int A = N2 * i + ra, Bq = N2 * i + rb;
+                        for (int64_t u = X.ptr[A]; u < X.ptr[A + 1]; u++)
+                            for (int64_t v = X.ptr[Bq]; v < X.ptr[Bq + 1]; v++)
+                                T.emplace_back(X.col[u], X.col[v], cab * X.val[u] * X.val[v]);
+                    }
+            }
+            SprsD Bar(k, k);
+            Bar.setFromTriplets(T);
+            H = H + Bar;
+        }
+        add_diagonal(H, dv);
+        double dmean = 0.0;
+        for (int i = 0; i < k; i++)
+            for (int64_t t = H.ptr[i]; t < H.ptr[i + 1]; t++) if (H.col[t] == i) dmean += H.val[t];
+        dmean = std::min(k ? dmean / k : 1.0, 0.5 * wK * wK);
+        if (!(dmean > 0.0)) dmean = 1.0;
+        if (in.Cm->rows()) H = H + (in.Cm->transpose() * (*in.Cm)).scaled(dmean);
+        SprsD W;
+        std::vector<double> omega;
+        stiff_rows(*in.Cm, wK * wK - dmean, wK * wK, lam, k, W, omega);
+        sol.solve(H, W, omega, gx.p, dz);
+        // df = P (-wI^2 (f - B (x + dx)) + gb)
+        hipLaunchKernelGGL(k_lin2, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), 1.0, zcur, 1.0, dz, tk.p);
+        mk::spmv64(K.c, in.B, tk.p, t2.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        hipLaunchKernelGGL(k_lin3, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, -wI * wI, zcur + k, wI * wI, t2.p, 1.0,
+                           gb.p, t2.p);
+        hipLaunchKernelGGL(k_face_apply, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, N2, p, dslot.p, Pp, t2.p, dz + k);
+        return true;
+    }
+};
+
+// LMSolver over x for one seamless round (iter_rounding_loop.h:77-86, weights
+// iter_rounding_loop.h:109: wInteg = 1, wConst = 1e5, wClose = 0.01, wBarrier = 1e-4):
+// J = [wI X2F'; wC I; wK U; wB D S X2F'], X2F' = length X2F.
+struct SeamlessPass {
+    Ctx& K;
+    const InjInputs& in;
+    InjSolver& sol;
+    BarrierState& bs;
+    const double wI = 1.0, wK = 10e4, wC = 0.01, wB = 1e-4;
+    const double* F2_;     // device, l1
+    const double* xPrev;   // device, k
+    SprsD U;
+    std::vector<double> r;
+    DevBuf<double> field, t1, g;
+    std::vector<double> ures;
+    bool finite = true;
+    SeamlessPass(Ctx& K_, const InjInputs& in_, InjSolver& sol_, BarrierState& bs_, const double* F2u,
+                 const double* xp)
+        : K(K_), in(in_), sol(sol_), bs(bs_), F2_(F2u), xPrev(xp) {
+        field.alloc(size_t(in.l1));
+        t1.alloc(size_t(in.l1));
+        g.alloc(size_t(in.k));
+    }
+    std::vector<double> u_res(const double* x) {
+        std::vector<double> v = sol.rowdots(U, x);
+        for (size_t i = 0; i < v.size(); i++) v[i] -= r[i];
+        return v;
+    }
+    double energy(const double* x) {
+        const size_t l1 = size_t(in.l1), k = size_t(in.k);
+        mk::spmv64(K.c, in.B, x, field.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        const double a = K.sqdiff(l1, 1.0, field.p, F2_);
+        const double b = K.sqdiff(k, 1.0, x, xPrev);
+        double c = 0.0;
+        for (double v : u_res(x)) c += v * v;
+        const double d = bs.energy(K, field.p);
+        return wI * wI * a + wC * wC * b + wK * wK * c + wB * wB * d;
+    }
+    void jac(const double* x) {
+        const size_t l1 = size_t(in.l1), k = size_t(in.k);
+        mk::spmv64(K.c, in.B, x, field.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        finite = bs.update(K, field.p);
+        // -J^T E = X2F'^T (wI^2 (F2_ - field) - wB^2 S^T D fB) - wC^2 (x - xPrev) - wK^2 U^T (U x - r)
+        hipLaunchKernelGGL(k_lin3, dim3(nblk(l1)), dim3(256), 0, K.st(), l1, wI * wI, F2_, -wI * wI, field.p,
+                           -wB * wB, bs.tmul.p, t1.p);
+        mk::spmv64(K.c, in.Bt, t1.p, g.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+        hipLaunchKernelGGL(k_lin3, dim3(nblk(k)), dim3(256), 0, K.st(), k, 1.0, g.p, -wC * wC, x, wC * wC, xPrev, g.p);
+        ures = u_res(x);
+        std::vector<double> cf(ures.size());
+        for (size_t i = 0; i < ures.size(); i++) cf[i] = -wK * wK * ures[i];
+        sol.add_rows(U, cf, g.p);
+    }
+    bool dir(double lam, double* d) {
+        if (!finite) return false;
+        std::vector<SparseRow> stiff;
+        SprsD H = seamless_soft(in, bs, lam, wI, wC, wB, stiff);
+        SprsD W;
+        std::vector<double> omega;
+        stiff_rows(U, wK * wK, wK * wK, lam, in.k, W, omega);
+        append_rows(W, omega, stiff, wB * wB);
+        sol.solve(H, W, omega, g.p, d);
+        return true;
+    }
+};
+
+// iterative_rounding with localInjectivity = true: the naive LM pass, then (seamless) the
+// rounding loop of SeamlessIntegration with a damped barrier LM per round
+// (iter_rounding.h:82-112, iter_rounding_loop.h:29-100).
+template <typename Record>
+void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, const int32_t* singularIdcs,
+                        int nsing, const int32_t* integerIdcs, int nint, double* fullx, int* success, Record&& record,
+                        mk_igm_stats& S) {
+    const int k = in.k;
+    const size_t l1 = size_t(in.l1);
+    InjSolver sol(K, in.A0, k);
+    BarrierState bs;
+    bs.init(K, in.F2, in.N, in.nF);
+    DevBuf<double> z(size_t(k) + l1), dd, trial;
+    z.upload(in.XSmall->data(), size_t(k), K.st());
+    MK_HIP(hipMemcpyAsync(z.p + k, in.F2, l1 * sizeof(double), hipMemcpyHostToDevice, K.st()));
+    double t0 = now_ms();
+    int lm_iters = 0;
+    {
+        // the naive pass's iter_cb: locally_injectivity_check (parameterization_integ.ipp:45-63)
+        // when the caller set one (mk_igm_injectivity_check)
+        const mk_inj_check_data& ck = K.c->inj_check;
+        const bool checked = ck.nF > 0 && ck.cols == in.nfull;
+        OpHolder hN;
+        DevBuf<int> didx;
+        DevBuf<double> dNF, dflag;
+        if (checked) {
+            SprsD f2n(ck.rows, ck.cols);
+            f2n.ptr = ck.ptr;
+            f2n.col = ck.col;
+            f2n.val = ck.val;
+            const SprsD Nf = f2n * (*in.UF);
+            hN.op = make_rect_op(K.c, Nf, K.o);
+            hN.op->label = "igm_inj_check";
+            didx.upload(ck.idx.data(), ck.idx.size(), K.st());
+            dNF.alloc(size_t(ck.rows));
+            dflag.alloc(size_t(ck.nF));
+        }
+        auto post = [&](const double* v) {
+            if (!checked) return false;
+            mk::spmv64(K.c, hN.op, v, dNF.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+            hipLaunchKernelGGL(k_flips, dim3(nblk(size_t(ck.nF))), dim3(256), 0, K.st(), ck.nF, ck.N, didx.p, dNF.p,
+                               dflag.p);
+            return K.sqdiff(size_t(ck.nF), 1.0, dflag.p, nullptr) == 0.0;
+        };
+        NaivePass np(K, in, sol, bs);
+        lm_damped(K, size_t(k) + l1, z.p, 100, 10e-10, [&](const double* v) { return np.energy(v); },
+                  [&](const double* v) { np.jac(v); }, [&](double lam, double* d) { return np.dir(lam, d); }, post,
+                  dd, trial, lm_iters);
+    }
+    S.naive_ms = now_ms() - t0;
+    MK_LOG("[igm] injective naive LM %.0f ms, %d iterations\n", S.naive_ms, lm_iters);
+    std::vector<double> xs(static_cast<size_t>(k));
+    z.download(xs.data(), xs.size(), K.st());
+    K.sync();
+    *success = 1;
+    if (!flags->seamless) {
+        in.UF->multiply(xs.data(), fullx);
+        return;
+    }
+    t0 = now_ms();
+    const SprsD& UF = *in.UF;
+    DevBuf<double> xcur(static_cast<size_t>(k)), xprev(static_cast<size_t>(k)), dcand;
+    MK_HIP(hipMemcpyAsync(xcur.p, z.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+    const double* F2u = z.p + k;   // F2_ = f of the naive pass
+    std::vector<int> singular(singularIdcs, singularIdcs + nsing), integer(integerIdcs, integerIdcs + nint);
+    RowSet cand;
+    std::vector<int> candPos(size_t(in.nfull), -1), candIdx;
+    for (int i : singular) if (candPos[i] < 0) { candPos[i] = int(candIdx.size()); candIdx.push_back(i); cand.push(UF, i); }
+    for (int i : integer) if (candPos[i] < 0) { candPos[i] = int(candIdx.size()); candIdx.push_back(i); cand.push(UF, i); }
+    cand.sync(K.st());
+    dcand.alloc(std::max<size_t>(candIdx.size(), 1));
+    std::vector<int> left = flags->round_seams ? integer : singular;
+    std::vector<int> fixedI;
+    std::vector<double> fixedV;
+    std::vector<double> Xc(candIdx.size());
+    bool roundedSingulars = false, rounded = false;
+    int rounds = 0, lm_rounds = 0;
+    while (!left.empty()) {
+        MK_HIP(hipMemcpyAsync(xprev.p, xcur.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
+        K.rowdots(cand, xcur.p, dcand.p);
+        dcand.download(Xc.data(), Xc.size(), K.st());
+        K.sync();
+        double minDif = 3276700.0;
+        int minIdx = -1;
+        for (size_t i = 0; i < left.size(); i++) {
+            const double v = Xc[size_t(candPos[left[i]])];
+            const double dif = std::fabs(v - std::round(v));
+            if (dif < minDif) { minIdx = int(i); minDif = dif; }
+        }
+        const int pick = left[size_t(minIdx)];
+        const double Xp = Xc[size_t(candPos[pick])];
+        fixedI.push_back(pick);
+        fixedV.push_back(double(int(std::round(Xp))));
+        record(pick, int(std::round(Xp)), minDif);
+        left.erase(left.begin() + minIdx);
+        if (left.empty() && !flags->round_seams && !roundedSingulars) {
+            left = integer;
+            roundedSingulars = true;
+        }
+        rounds++;
+        if (!(minDif > 10e-7)) continue;
+        rounded = true;
+        lm_rounds++;
+        SeamlessPass sp(K, in, sol, bs, F2u, xprev.p);
+        sp.U = UF.rows_of(fixedI);
+        sp.r = fixedV;
+        int it = 0;
+        lm_damped(K, size_t(k), xcur.p, 100, 1e-7, [&](const double* v) { return sp.energy(v); },
+                  [&](const double* v) { sp.jac(v); }, [&](double lam, double* d) { return sp.dir(lam, d); },
+                  [](const double*) { return false; }, dd, trial, it);
+        lm_iters += it;
+        if (lm_rounds % 5 == 0) MK_LOG("[igm] injective rounding: %d rounds, %d LM iterations\n", rounds, lm_iters);
+        // post_checking (iter_rounding_loop.h:89-100)
+        std::vector<double> ux = sol.rowdots(sp.U, xcur.p);
+        double mx = -1.0;
+        for (double v : ux) mx = std::max(mx, std::abs(v - std::round(v)));
+        if (!(mx <= 10e-7)) { *success = 0; break; }
+    }
+    S.rounds = rounds;
+    S.lm_rounds = lm_rounds;
+    S.n_fixed = int(fixedI.size());
+    MK_LOG("[igm] injective rounding %.0f ms: %d rounds, %d with LM, %d LM iterations\n", now_ms() - t0, rounds,
+           lm_rounds, lm_iters);
+    if (rounded) {
+        xcur.download(xs.data(), xs.size(), K.st());
+        K.sync();
+        UF.multiply(xs.data(), fullx);
+    } else {
+        std::copy(in.XSmall->begin(), in.XSmall->end(), xs.begin());
+        UF.multiply(xs.data(), fullx);
+    }
+    S.rounding_ms = now_ms() - t0;
+}
 }  // namespace
 
 extern "C" mk_status mk_iterative_rounding(
@@ -501,7 +1289,6 @@ extern "C" mk_status mk_iterative_rounding(
     double* fullx, int* success, int max_dec, int32_t* dec_index, int32_t* dec_value, double* dec_margin, int* ndec,
     mk_igm_stats* stats) {
     if (!c || !flags || !F2 || !fullx || !success || !g_ptr || g_rows != 2 * N * nF || c_cols != g_cols) return MK_ERR_ARG;
-    if (flags->local_injectivity) return MK_ERR_UNSUPPORTED;
     try {
         mk_solve_opts o;
         mk_default_opts(&o);
@@ -661,7 +1448,17 @@ extern "C" mk_status mk_iterative_rounding(
             }
             nd++;
         };
-        if (!flags->seamless) {
+        if (flags->local_injectivity) {
+            // damped barrier LM passes (iter_rounding.h:54-113 with localInjectivity = true)
+            OpHolder hB, hBt;
+            hB.op = make_rect_op(c, X2F, o, length);
+            hB.op->label = "igm_inj_B";
+            hBt.op = make_rect_op(c, X2Ft, o, length);
+            hBt.op->label = "igm_inj_Bt";
+            const InjInputs in{k, nfull, g_rows, N, nF, length, &UF, &X2F, &E, &Cm, fixedVals, F2, &XSmall,
+                               hA0.op, hB.op, hBt.op};
+            injective_rounding(K, in, flags, singularIdcs, nsing, integerIdcs, nint, fullx, success, record, S);
+        } else if (!flags->seamless) {
             std::copy(x0.begin(), x0.end(), fullx);
         } else {
             // ---- naive LM over (x, f) (iter_rounding.h:82-86), weights iter_rounding_init.h:184 ----
@@ -1070,6 +1867,33 @@ mk_status state_guard(mk_ctx* c, Fn&& fn) {
 }  // namespace
 
 extern "C" {
+
+mk_status mk_igm_injectivity_check(mk_ctx* c, int N, int nF, const int32_t* cut_idx, int f_rows, int f_cols,
+                                   const int64_t* f_ptr, const int32_t* f_col, const double* f_val) {
+    if (!c) return MK_ERR_ARG;
+    if (nF <= 0) {
+        c->inj_check = mk_inj_check_data{};
+        return MK_OK;
+    }
+    if (N < 2 || !cut_idx || f_rows <= 0 || f_cols <= 0 || f_rows % N || !f_ptr || f_ptr[0] != 0) return MK_ERR_ARG;
+    const int nV = f_rows / N;
+    for (int64_t i = 0; i < 3 * int64_t(nF); i++)
+        if (cut_idx[i] < 0 || cut_idx[i] >= nV) { c->last_error = "cut_idx out of range"; return MK_ERR_ARG; }
+    for (int i = 0; i < f_rows; i++) if (f_ptr[i + 1] < f_ptr[i]) return MK_ERR_ARG;
+    for (int64_t q = 0; q < f_ptr[f_rows]; q++)
+        if (f_col[q] < 0 || f_col[q] >= f_cols) { c->last_error = "fullx2Nfunc column out of range"; return MK_ERR_ARG; }
+    mk_inj_check_data d;
+    d.N = N;
+    d.nF = nF;
+    d.rows = f_rows;
+    d.cols = f_cols;
+    d.idx.assign(cut_idx, cut_idx + 3 * int64_t(nF));
+    d.ptr.assign(f_ptr, f_ptr + f_rows + 1);
+    d.col.assign(f_col, f_col + f_ptr[f_rows]);
+    d.val.assign(f_val, f_val + f_ptr[f_rows]);
+    c->inj_check = std::move(d);
+    return MK_OK;
+}
 
 mk_status mk_keep_igm_state(mk_ctx* c, int on) {
     if (!c) return MK_ERR_ARG;

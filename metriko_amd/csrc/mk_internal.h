@@ -124,6 +124,16 @@ int mg_level_rows(const Mg* g, int l);
 
 }  // namespace mk
 
+// locally_injectivity_check data (mk_igm_injectivity_check): corners of the cut faces and
+// fullx2Nfunc (CSR, N functions per cut vertex); nF = 0: no check
+struct mk_inj_check_data {
+    int N = 0, nF = 0, rows = 0, cols = 0;
+    std::vector<int32_t> idx;
+    std::vector<int64_t> ptr;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+};
+
 struct mk_ctx {
     mk::Work work;
     int device = 0;
@@ -149,6 +159,7 @@ struct mk_ctx {
     // the next iterative rounding keeps its last LM round's system (mk_keep_igm_state)
     bool keep_igm_state = false;
     struct mk_igm_state* igm_state = nullptr;
+    mk_inj_check_data inj_check;
 };
 
 namespace mk { struct Mg; void mg_free(Mg*); }

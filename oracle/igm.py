@@ -264,8 +264,9 @@ class RosyParameterization:
                                     self.length, Cfull, self.G2full, N, n, cut.nF, self.seamless,
                                     self.roundSeams, self.rawF2Vec, literal=literal)
         if self.localInjectivity:
-            from .injective import run_local_injectivity
-            self.success = run_local_injectivity(self.ir)
+            from .injective import locally_injective, run_local_injectivity
+            f2n, cidx = self.fullx2Nfunc, np.asarray(self.cut.idx)
+            self.success = run_local_injectivity(self.ir, lambda X: locally_injective(f2n, cidx, N, X))
         else:
             self.success = self.ir.run()
         self.fullx = self.ir.fullx

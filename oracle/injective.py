@@ -69,9 +69,10 @@ def _damp_vector(J, lam):
     return lam * np.asarray(J.multiply(J).sum(axis=0)).ravel()
 
 
-def lm_damped(x0, objective, solve, maxIter, funcTol, fooTol, lam0, stats):
+def lm_damped(x0, objective, solve, maxIter, funcTol, fooTol, lam0, stats, post=None):
     """LMSolver::solve with DiagonalDamping (levenberg_marquardt.h:13-83,140-231).
-    objective(x, needJ) -> (E, J); solve(J, damp, rhs) -> d with (J^T J + diag(damp)) d = rhs."""
+    objective(x, needJ) -> (E, J); solve(J, damp, rhs) -> d with (J^T J + diag(damp)) d = rhs;
+    post(x) -> True ends the solve after an iteration (ST->post_iteration)."""
     lam = lam0
     prevx = x0.copy()
     E, J = objective(prevx, True)
@@ -103,6 +104,8 @@ def lm_damped(x0, objective, solve, maxIter, funcTol, fooTol, lam0, stats):
         else:
             lam *= 10.0
         damp = _damp_vector(J, lam)
+        if post is not None and post(x):
+            return x, True
         it += 1
         prevx = x
         if not it <= maxIter:
@@ -127,10 +130,20 @@ def _woodbury_solve(J, damp, rhs, c0, nC, wK):
     return y - Z @ np.linalg.solve(M, U @ y)
 
 
-def run_local_injectivity(ir):
+def locally_injective(fullx2Nfunc, cut_idx, N, X):
+    """locally_injectivity_check (parameterization_integ.ipp:45-63): every cut face keeps a
+    non-negative orientation in the (u, v) = functions 0, 1 of fullx2Nfunc X."""
+    NF = (fullx2Nfunc @ X).reshape(-1, N)
+    a, b, c = NF[cut_idx[:, 0], :2], NF[cut_idx[:, 1], :2], NF[cut_idx[:, 2], :2]
+    d1, d2 = b - a, c - a
+    return not np.any(d1[:, 0] * d2[:, 1] - d1[:, 1] * d2[:, 0] < 0)
+
+
+def run_local_injectivity(ir, check=None):
     """iterative_rounding (iter_rounding.h:33-113) with localInjectivity = true on an oracle
     IterativeRounding ``ir`` (its KKT set-up done); fills ir.fullx / ir.decisions and returns
-    success."""
+    success.  check(fullx) is the naive pass's iter_cb (NaiveIntegration::post_iteration,
+    iter_rounding_init.h:28): True ends that pass."""
     ir.stats = {"solves": 0, "rounds": 0, "lm_rounds": 0}
     N, nF, k = ir.N, ir.nF, ir.k
     F2 = np.asarray(ir.F2)
@@ -162,7 +175,8 @@ def run_local_injectivity(ir):
         return E, sp.vstack([gInteg, gClose, gConst, gB]).tocsr()
 
     XF = np.concatenate([ir.XSmall, F2])
-    xn, _ = lm_damped(XF, obj_n, _direct_solve, 100, 10e-10, 10e-10, 0.01, ir.stats)
+    post = (lambda xf_: check(UF @ xf_[:k])) if check is not None else None
+    xn, _ = lm_damped(XF, obj_n, _direct_solve, 100, 10e-10, 10e-10, 0.01, ir.stats, post)
     ir.x_naive = xn
     ir.fullx = UF @ xn[:k]
     if not ir.seamless:

@@ -130,3 +130,18 @@ def test_active_barrier_case():
     assert max(seen) > 0
     assert a["success"]
     np.testing.assert_allclose(a["fullx"], b["fullx"], atol=1e-6)
+
+
+def test_local_injectivity_check():
+    """locally_injectivity_check on two triangles with fullx2Nfunc = identity (N = 2): the
+    identity map is injective; moving one corner across the shared edge flips a face."""
+    idx = np.array([[0, 1, 2], [0, 2, 3]])
+    X = np.array([0.0, 0.0, 1.0, 0.0, 1.0, 1.0, 0.0, 1.0])
+    f2n = sp.identity(8, format="csr")
+    assert inj.locally_injective(f2n, idx, 2, X)
+    Y = X.copy()
+    Y[6:8] = 2.0, 0.5          # vertex 3 past the diagonal 0-2: face (0, 2, 3) turns clockwise
+    assert not inj.locally_injective(f2n, idx, 2, Y)
+    Z = X.copy()
+    Z[6:8] = 2.0, 2.0          # vertex 3 on the line 0-2: zero area counts as injective
+    assert inj.locally_injective(f2n, idx, 2, Z)
