@@ -632,8 +632,10 @@ struct InjSolver {
     Ctx& K;
     const mk_op* A0;   // near-null vectors of the x-space operators
     int k;
-    DevBuf<double> y, Z, drow, dd, dw;
-    DevBuf<int> didx;
+    DevBuf<double> y, Z, drow, dd, dw, Bp, Xp, dwz;
+    DevBuf<int> didx, dsel;
+    bool jacobi = false;
+    int jacobi_solves = 0;
     InjSolver(Ctx& K_, const mk_op* A0_, int k_) : K(K_), A0(A0_), k(k_), y(size_t(k_)), drow(size_t(k_)) {}
 
     // g += sum_i coef_i R_i (rows of R)
@@ -668,7 +670,8 @@ struct InjSolver {
     // tests; a non-finite one makes the step a rejected one)
     void pcg(const mk_op* A, const double* b, double* x) {
         mk_solve_opts o2 = K.o;
-        o2.maxiter = std::min(o2.maxiter, 1000);
+        o2.maxiter = std::min(o2.maxiter, jacobi ? 5000 : 1000);
+        if (jacobi) o2.precond = MK_PRECOND_JACOBI;
         const mk::SolveResult r = mk::solve_spd(K.c, A, b, x, o2);
         K.acc.iterations += r.iters;
         K.acc.refinements += r.refinements;
@@ -677,15 +680,28 @@ struct InjSolver {
             r.status == MK_ERR_UNSUPPORTED)
             throw Error(r.status, "IGM solve failed: " + std::string(mk_status_str(r.status)));
     }
-    // (Hs + W^T diag(omega) W) d = g; Hs SPD (host CSR, k x k), W few sparse rows
-    void solve(const SprsD& Hs, const SprsD& W, const std::vector<double>& omega, const double* g, double* d) {
+    // (Hs + W^T diag(omega) W) d = g; Hs SPD (host CSR, k x k), W few sparse rows.
+    // diag_dominant: the damping makes Hs well conditioned under Jacobi scaling (Hs holds
+    // lambda diag(J^T J) on top of J^T J), so Jacobi PCG needs no hierarchy set-up; otherwise
+    // a multigrid hierarchy of its own (Jacobi when aggregation cannot coarsen Hs).
+    void solve(const SprsD& Hs, const SprsD& W, const std::vector<double>& omega, const double* g, double* d,
+               bool diag_dominant) {
         OpHolder h;
         h.op = mk::op_create_owned(K.c, k, std::vector<int64_t>(Hs.ptr), std::vector<int>(Hs.col),
                                    std::vector<double>(Hs.val), K.o);
         h.op->label = "igm_inj";
         h.op->nullvec = A0->nullvec;
         h.op->nnull = A0->nnull;
-        mk::ensure_mg(K.c, h.op, K.o);
+        jacobi = diag_dominant || K.o.precond != MK_PRECOND_AMG;
+        if (!jacobi) {
+            try {
+                mk::ensure_mg(K.c, h.op, K.o);
+            } catch (const Error& e) {
+                if (e.code != MK_ERR_UNSUPPORTED) throw;
+                jacobi = true;
+            }
+        }
+        if (jacobi && K.o.precond == MK_PRECOND_AMG) jacobi_solves++;
         hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), y.p);
         pcg(h.op, g, y.p);
         const int m = W.rows();
@@ -699,20 +715,49 @@ struct InjSolver {
         ws.sync(K.st());
         std::vector<double> M(static_cast<size_t>(m) * m), col(static_cast<size_t>(m));
         dd.ensure(size_t(m));
-        for (int j = 0; j < m; j++) {
-            double* z = Z.p + size_t(j) * k;
-            hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), drow.p);
-            const int64_t b = ws.hptr[size_t(j)], e = ws.hptr[size_t(j) + 1];
-            if (e > b)
-                hipLaunchKernelGGL(k_scatter_row, dim3(nblk(size_t(e - b))), dim3(256), 0, K.st(), b, e, ws.col.p,
-                                   ws.val.p, drow.p);
-            hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), z);
-            pcg(h.op, drow.p, z);
-            K.rowdots(ws, z, dd.p);
-            dd.download(col.data(), col.size(), K.st());
-            K.sync();
-            for (int i = 0; i < m; i++) M[size_t(i) * m + j] = col[size_t(i)];
+        const bool blk = !jacobi && K.o.precision != MK_PREC_FP64 && h.op->mg;
+        if (blk) {
+            // the Woodbury columns as blocks of up to 64 right-hand sides (one matrix read per step)
+            const int mwmax = m > 32 ? 64 : m > 16 ? 32 : m > 8 ? 16 : 8;
+            Bp.ensure(size_t(k) * mwmax);
+            Xp.ensure(size_t(k) * mwmax);
+            for (int b0 = 0, cnt = 0; b0 < m; b0 += cnt) {
+                cnt = std::min(mwmax, m - b0);
+                const int mw = cnt > 32 ? 64 : cnt > 16 ? 32 : cnt > 8 ? 16 : 8;
+                std::vector<int> sel(static_cast<size_t>(mw), -1);
+                for (int j = 0; j < cnt; j++) sel[size_t(j)] = b0 + j;
+                dsel.upload(sel.data(), sel.size(), K.st());
+                hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k) * mw)), dim3(256), 0, K.st(), size_t(k) * mw, Bp.p);
+                hipLaunchKernelGGL(k_scatter_rows_packed, dim3(unsigned(mw)), dim3(64), 0, K.st(), mw, dsel.p, ws.ptr.p,
+                                   ws.col.p, ws.val.p, Bp.p);
+                const mk::BlockResult br = mk::block_pcg(K.c, h.op, h.op->mg, mw, Bp.p, Xp.p, K.o.tol,
+                                                         std::min(K.o.maxiter, 1000), K.o.check_every);
+                K.acc.iterations += br.iters;
+                K.acc.solves += cnt;
+                if (br.status == MK_ERR_ARG || br.status == MK_ERR_HIP || br.status == MK_ERR_ALLOC ||
+                    br.status == MK_ERR_UNSUPPORTED)
+                    throw Error(br.status, "IGM block solve failed: " + std::string(mk_status_str(br.status)));
+                hipLaunchKernelGGL(k_unpack_cols, dim3(nblk(size_t(k) * mw)), dim3(256), 0, K.st(), size_t(k), mw,
+                                   dsel.p, (const double*)Xp.p, Z.p);
+            }
+        } else {
+            for (int j = 0; j < m; j++) {
+                double* z = Z.p + size_t(j) * k;
+                hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), drow.p);
+                const int64_t b = ws.hptr[size_t(j)], e = ws.hptr[size_t(j) + 1];
+                if (e > b)
+                    hipLaunchKernelGGL(k_scatter_row, dim3(nblk(size_t(e - b))), dim3(256), 0, K.st(), b, e, ws.col.p,
+                                       ws.val.p, drow.p);
+                hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), z);
+                pcg(h.op, drow.p, z);
+            }
         }
+        // W Z (m x m), all pairs in one launch: WZ[i * m + j] = row i of W . Z[:, j]
+        dwz.ensure(size_t(m) * m);
+        hipLaunchKernelGGL(k_uz, dim3(nblk(size_t(m) * m)), dim3(256), 0, K.st(), m, size_t(k), ws.ptr.p, ws.col.p,
+                           ws.val.p, (const double*)Z.p, dwz.p);
+        dwz.download(M.data(), M.size(), K.st());
+        K.sync();
         for (int i = 0; i < m; i++) M[size_t(i) * m + i] += 1.0 / omega[size_t(i)];
         K.rowdots(ws, y.p, dd.p);
         dd.download(col.data(), col.size(), K.st());
@@ -1070,7 +1115,9 @@ struct NaivePass {
         SprsD W;
         std::vector<double> omega;
         stiff_rows(*in.Cm, wK * wK - dmean, wK * wK, lam, k, W, omega);
-        sol.solve(H, W, omega, gx.p, dz);
+        // Hs = lambda wI^2 length (E / (1 + lambda) + diag E) + ...: Jacobi-conditioned for
+        // every lambda well above wC^2 / wI^2 = 1e-8
+        sol.solve(H, W, omega, gx.p, dz, lam >= 1e-6);
         // df = P (-wI^2 (f - B (x + dx)) + gb)
         hipLaunchKernelGGL(k_lin2, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), 1.0, zcur, 1.0, dz, tk.p);
         mk::spmv64(K.c, in.B, tk.p, t2.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
@@ -1141,7 +1188,10 @@ struct SeamlessPass {
         std::vector<double> omega;
         stiff_rows(U, wK * wK, wK * wK, lam, in.k, W, omega);
         append_rows(W, omega, stiff, wB * wB);
-        sol.solve(H, W, omega, g.p, d);
+        // Hs = A0 + lambda diag(A0) + ...: Jacobi condition about 2 / lambda.  Below
+        // lambda = 0.1 multigrid with block PCG for the Woodbury columns; above it (rejected
+        // steps) Jacobi, where aggregation of the diagonal-heavy Hs stalls
+        sol.solve(H, W, omega, g.p, d, lam >= 0.1);
         return true;
     }
 };
@@ -1267,8 +1317,8 @@ void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, 
     S.rounds = rounds;
     S.lm_rounds = lm_rounds;
     S.n_fixed = int(fixedI.size());
-    MK_LOG("[igm] injective rounding %.0f ms: %d rounds, %d with LM, %d LM iterations\n", now_ms() - t0, rounds,
-           lm_rounds, lm_iters);
+    MK_LOG("[igm] injective rounding %.0f ms: %d rounds, %d with LM, %d LM iterations (%d directions by Jacobi "
+           "PCG)\n", now_ms() - t0, rounds, lm_rounds, lm_iters, sol.jacobi_solves);
     if (rounded) {
         xcur.download(xs.data(), xs.size(), K.st());
         K.sync();
