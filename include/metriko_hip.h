@@ -173,6 +173,25 @@ mk_status mk_iterative_rounding(mk_ctx* ctx, int nfixed, const int32_t* fixedIdc
 mk_status mk_igm_injectivity_check(mk_ctx* ctx, int N, int nF, const int32_t* cut_idx, int f_rows, int f_cols,
                                    const int64_t* f_ptr, const int32_t* f_col, const double* f_val);
 
+/* --- QEx quad extraction (host) -----------------------------------------------------------
+ * example_qex.cpp:84-140 on the corner uv of an injective seamless parameterization:
+ * qex::sanitization (qex/sanitization.h), generate_q_vert, generate_vqvert_qport,
+ * generate_eqvert_qport, generate_fqvert_qport (qex/gen_q_port.h), generate_q_edge
+ * (qex/gen_q_edge.h), generate_q_faces (qex/gen_q_face.h).  Closed meshes, N = 4.
+ * matching[nE] (combed, edges in Hmesh order), singular[nV], uv[6 nF] (corner uv, corner
+ * 3 f + j at F[f, j]).  Outputs fill up to their capacity; counts are always set:
+ * uv_out[6 nF] the sanitized uv; q-vertices (vertex ones, then edge, then face) q_uv[2],
+ * q_pos[3], q_sid[1] each; ports p_ids[6] = (vid, eid, fid, prev, next, connected),
+ * p_uvdir[4] = (u, v, dir u, dir v); edges e_ports[2] (port indices); faces f_halfs[4]
+ * (half 2 i = edge i from its first port, 2 i + 1 reversed).  ctx may be NULL (then no
+ * error text). */
+typedef struct { int n_vq, n_eq, n_fq, n_ports, n_edges, n_faces; } mk_qex_counts;
+mk_status mk_qex_extract(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, int N,
+                         const int32_t* matching, const int32_t* singular, const double* uv, double* uv_out,
+                         int cap_q, double* q_uv, double* q_pos, int32_t* q_sid, int cap_p, int32_t* p_ids,
+                         double* p_uvdir, int cap_e, int32_t* e_ports, int cap_f, int32_t* f_halfs,
+                         mk_qex_counts* counts);
+
 /* --- differentiation: the sizing hook (no reference counterpart) ---------------------------
  * mk_keep_igm_state(ctx, 1) makes the next iterative rounding (mk_iterative_rounding or
  * mk_pipeline_run) keep its last LM round's system

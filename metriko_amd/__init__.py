@@ -24,7 +24,7 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_igm_injectivity_check", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
@@ -55,6 +55,10 @@ class IgmStats(C.Structure):
                 ("batch_iterations", C.c_int), ("kkt_rel_residual", C.c_double),
                 ("setup_ms", C.c_double),
                 ("kkt_ms", C.c_double), ("naive_ms", C.c_double), ("rounding_ms", C.c_double), ("solve", Stats)]
+
+
+class QexCounts(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("n_vq", "n_eq", "n_fq", "n_ports", "n_edges", "n_faces")]
 
 
 class PipelineInfo(C.Structure):
@@ -137,6 +141,8 @@ def lib():
                                C.POINTER(BatchStats)]
     L.mk_igm_injectivity_check.argtypes = [_P, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I64, _I32, _D]
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
+    L.mk_qex_extract.argtypes = [_P, C.c_int, C.c_int, _D, _I32, C.c_int, _I32, _I32, _D, _D, C.c_int, _D, _D, _I32,
+                                 C.c_int, _I32, _D, C.c_int, _I32, C.c_int, _I32, C.POINTER(QexCounts)]
     L.mk_igm_state_take.argtypes = [_P, C.POINTER(_P)]
     L.mk_igm_state_destroy.argtypes = [_P]
     L.mk_igm_state_sizes.argtypes = [_P] + [C.POINTER(C.c_int)] * 5
@@ -256,6 +262,35 @@ def mg_host_pcg(A, b, tol=1e-10, maxit=500, nullvec=None):
     if cplx:
         x = x[0::2] + 1j * x[1::2]
     return x, it.value, lv.value, rr.value
+
+
+def qex_extract(V, F, matching, singular, uv, N=4):
+    """QEx quad extraction (mk_qex_extract, host code; no GPU needed) on corner uv [3 nF, 2]:
+    sanitized uv, q-vertices (kind 0/1/2 = on a vertex / edge / face), ports, edges, quads."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    mt = np.ascontiguousarray(matching, np.int32)
+    sg = np.ascontiguousarray(singular, np.int32)
+    u = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+    cnt = QexCounts()
+    args = (None, nV, nF, _ptr(V, _D), _ptr(F, _I32), N, _ptr(mt, _I32), _ptr(sg, _I32), _ptr(u, _D))
+    s = L.mk_qex_extract(*args, None, 0, None, None, None, 0, None, None, 0, None, 0, None, C.byref(cnt))
+    if s:
+        raise MetrikoError(s, "mk_qex_extract")
+    nq = cnt.n_vq + cnt.n_eq + cnt.n_fq
+    uvo = np.zeros((3 * nF, 2))
+    qu, qp, qs = np.zeros((nq, 2)), np.zeros((nq, 3)), np.zeros(nq, np.int32)
+    pid, puv = np.zeros((cnt.n_ports, 6), np.int32), np.zeros((cnt.n_ports, 4))
+    ep, fh = np.zeros((cnt.n_edges, 2), np.int32), np.zeros((cnt.n_faces, 4), np.int32)
+    s = L.mk_qex_extract(*args, _ptr(uvo, _D), nq, _ptr(qu, _D), _ptr(qp, _D), _ptr(qs, _I32), cnt.n_ports,
+                         _ptr(pid, _I32), _ptr(puv, _D), cnt.n_edges, _ptr(ep, _I32), cnt.n_faces, _ptr(fh, _I32),
+                         C.byref(cnt))
+    if s:
+        raise MetrikoError(s, "mk_qex_extract")
+    kind = np.repeat(np.arange(3, dtype=np.int32), [cnt.n_vq, cnt.n_eq, cnt.n_fq])
+    return {"uv": uvo, "q_uv": qu, "q_pos": qp, "q_sid": qs, "q_kind": kind, "port_ids": pid, "port_uvdir": puv,
+            "edges": ep, "faces": fh}
 
 
 def combinatorics(V, F, compressed, N=4):

@@ -6,6 +6,11 @@
 #include "igm_state.h"
 #include "metriko/core/vectorfield/face_rosy_field.h"
 #include "metriko/core/igm/parameterization.h"
+#include "metriko/core/qex/sanitization.h"
+#include "metriko/core/qex/gen_q_vert.h"
+#include "metriko/core/qex/gen_q_port.h"
+#include "metriko/core/qex/gen_q_edge.h"
+#include "metriko/core/qex/gen_q_face.h"
 
 namespace {
 double now_ms() {
@@ -317,3 +322,59 @@ mk_status mk_pipeline_run(mk_ctx* c, int nV, int nF, const double* V, const int3
 }
 
 }  // extern "C"
+
+extern "C" mk_status mk_qex_extract(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, int N,
+                                    const int32_t* matching, const int32_t* singular, const double* uv,
+                                    double* uv_out, int cap_q, double* q_uv, double* q_pos, int32_t* q_sid,
+                                    int cap_p, int32_t* p_ids, double* p_uvdir, int cap_e, int32_t* e_ports,
+                                    int cap_f, int32_t* f_halfs, mk_qex_counts* counts) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !matching || !singular || !uv || !counts || N != 4) return MK_ERR_ARG;
+    return guarded(ctx, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        VecXi mt(mesh.nE), sg(nV);
+        for (int e = 0; e < mesh.nE; e++) mt[e] = matching[e];
+        for (int v = 0; v < nV; v++) sg[v] = singular[v];
+        VecXc cfn(3 * nF);
+        for (int c = 0; c < 3 * nF; c++) cfn[c] = complex(uv[2 * c], uv[2 * c + 1]);
+        // example_qex.cpp:84-140: sanitize, q-verts, ports (vertex, edge, face), edges, faces
+        qex::sanitization(mesh, mt, sg, N, cfn);
+        std::vector<qex::Qvert> vq, eq, fq;
+        qex::generate_q_vert(mesh, cfn, vq, eq, fq);
+        std::vector<qex::Qport> ports;
+        qex::generate_vqvert_qport(mesh, cfn, vq, ports);
+        qex::generate_eqvert_qport(mesh, cfn, eq, ports);
+        qex::generate_fqvert_qport(mesh, fq, ports);
+        const std::vector<qex::Qedge> edges = qex::generate_q_edge(mesh, cfn, mt, ports);
+        const std::vector<qex::Qface> faces = qex::generate_q_faces(ports, edges);
+        *counts = mk_qex_counts{int(vq.size()), int(eq.size()), int(fq.size()), int(ports.size()), int(edges.size()),
+                                int(faces.size())};
+        if (uv_out)
+            for (int c = 0; c < 3 * nF; c++) { uv_out[2 * c] = cfn[c].real(); uv_out[2 * c + 1] = cfn[c].imag(); }
+        int k = 0;
+        for (const auto* L : {&vq, &eq, &fq})
+            for (const qex::Qvert& q : *L) {
+                if (k < cap_q) {
+                    if (q_uv) { q_uv[2 * k] = q.uv.real(); q_uv[2 * k + 1] = q.uv.imag(); }
+                    if (q_pos) for (int j = 0; j < 3; j++) q_pos[3 * k + j] = q.pos[j];
+                    if (q_sid) q_sid[k] = q.sid;
+                }
+                k++;
+            }
+        for (int i = 0; i < int(ports.size()) && i < cap_p; i++) {
+            const qex::Qport& p = ports[size_t(i)];
+            if (p_ids) {
+                int32_t* o = p_ids + 6 * i;
+                o[0] = p.vid; o[1] = p.eid; o[2] = p.fid; o[3] = p.prev_id; o[4] = p.next_id; o[5] = p.isConnected;
+            }
+            if (p_uvdir) {
+                double* o = p_uvdir + 4 * i;
+                o[0] = p.uv.real(); o[1] = p.uv.imag(); o[2] = p.dir.real(); o[3] = p.dir.imag();
+            }
+        }
+        for (int i = 0; i < int(edges.size()) && i < cap_e; i++)
+            if (e_ports) { e_ports[2 * i] = edges[size_t(i)].port1.idx; e_ports[2 * i + 1] = edges[size_t(i)].port2.idx; }
+        for (int i = 0; i < int(faces.size()) && i < cap_f; i++)
+            if (f_halfs) for (int j = 0; j < 4; j++) f_halfs[4 * i + j] = faces[size_t(i)].qhalfs[size_t(j)].idx;
+    });
+}

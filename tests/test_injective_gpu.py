@@ -66,3 +66,34 @@ def test_injective_rounding_matches_oracle(gpu_ctx, case):
     if seamless:
         assert sorted(d[:2] for d in dec) == sorted(tuple(d[:2]) for d in ref["decisions"])
     np.testing.assert_allclose(fullx, ref["fullx"], atol=TOL.get(case, 1e-8), rtol=0)
+
+
+def test_gpu_uv_feeds_qex(gpu_ctx):
+    """example_qex's flow with the GPU rounding: injective uv from mk_iterative_rounding, corner
+    uv as RosyParameterization::integ lays it out, then the host QEx extraction; the same quad
+    mesh as the oracle's QEx on the oracle's uv."""
+    import metriko_amd as mk
+    from oracle import qex
+    V, F = icosphere(2)
+    ref = run_pipeline(V, F, ratio=0.1, local_injectivity=True)
+    rp = ref["rp"]
+    ir = rp.ir
+    fullx, ok, dec, st = gpu_ctx.iterative_rounding(
+        ir.fixedIdcs, ir.fixedVals, ir.singularIdcs, ir.integerIdcs, ir.length, ir.Cfull, ir.G2, ir.N, ir.n,
+        ir.nF, ir.F2, seamless=True, round_seams=False, local_injectivity=True,
+        cut_idx=np.asarray(rp.cut.idx), fullx2Nfunc=rp.fullx2Nfunc)
+    assert ok
+    NF = (rp.fullx2Nfunc @ fullx).reshape(-1, ir.N)
+    cut = np.asarray(rp.cut.idx)
+    uv = np.zeros((3 * len(F), 2))
+    for j in range(3):
+        uv[j::3] = NF[cut[:, j], :2]
+    np.testing.assert_allclose(uv, ref["uv"], atol=1e-8, rtol=0)
+    mt = np.asarray(ref["combed_matching"], np.int32)
+    sg = np.asarray(ref["singular"], np.int32)
+    got = mk.qex_extract(V, F, mt, sg, uv)
+    want = qex.extract(ref["mesh"], ref["uv"][:, 0] + 1j * ref["uv"][:, 1], mt, sg)
+    print(f"[injective] GPU uv -> QEx: {len(got['faces'])} quads (oracle {len(want['faces'])})", flush=True)
+    assert np.array_equal(got["edges"], np.array(want["edges"], np.int32).reshape(-1, 2))
+    assert np.array_equal(got["faces"], np.array(want["faces"], np.int32).reshape(-1, 4))
+    assert len(got["q_sid"]) - len(got["edges"]) + len(got["faces"]) == 2
