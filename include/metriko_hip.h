@@ -192,6 +192,14 @@ mk_status mk_qex_extract(mk_ctx* ctx, int nV, int nF, const double* V, const int
                          double* p_uvdir, int cap_e, int32_t* e_ports, int cap_f, int32_t* f_halfs,
                          mk_qex_counts* counts);
 
+/* Files (host): mk_read_obj = the triangle OBJ reader example_igm / example_qex use
+ * (igl::readOBJ, `v` and `f a b c` lines; call with capacities 0 for nV / nF first);
+ * mk_cfn_write_json / mk_cfn_read_json = the corner-uv dump {"cfn": [re, im, ...]} of
+ * example_qex.cpp:14-43 (n complex values, interleaved). */
+mk_status mk_read_obj(const char* path, int cap_v, int cap_f, double* V, int32_t* F, int* nV, int* nF);
+mk_status mk_cfn_write_json(const char* path, int n, const double* cfn);
+mk_status mk_cfn_read_json(const char* path, int cap, double* cfn, int* n);
+
 /* --- differentiation: the sizing hook (no reference counterpart) ---------------------------
  * mk_keep_igm_state(ctx, 1) makes the next iterative rounding (mk_iterative_rounding or
  * mk_pipeline_run) keep its last LM round's system

@@ -24,7 +24,8 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_read_obj", "mk_cfn_write_json",
+    "mk_cfn_read_json", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
@@ -141,6 +142,9 @@ def lib():
                                C.POINTER(BatchStats)]
     L.mk_igm_injectivity_check.argtypes = [_P, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I64, _I32, _D]
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
+    L.mk_read_obj.argtypes = [C.c_char_p, C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mk_cfn_write_json.argtypes = [C.c_char_p, C.c_int, _D]
+    L.mk_cfn_read_json.argtypes = [C.c_char_p, C.c_int, _D, C.POINTER(C.c_int)]
     L.mk_qex_extract.argtypes = [_P, C.c_int, C.c_int, _D, _I32, C.c_int, _I32, _I32, _D, _D, C.c_int, _D, _D, _I32,
                                  C.c_int, _I32, _D, C.c_int, _I32, C.c_int, _I32, C.POINTER(QexCounts)]
     L.mk_igm_state_take.argtypes = [_P, C.POINTER(_P)]
@@ -262,6 +266,46 @@ def mg_host_pcg(A, b, tol=1e-10, maxit=500, nullvec=None):
     if cplx:
         x = x[0::2] + 1j * x[1::2]
     return x, it.value, lv.value, rr.value
+
+
+def read_obj(path):
+    """Triangle OBJ through the library's reader (mk_read_obj): V [nV, 3], F [nF, 3]."""
+    L = lib()
+    nV, nF = C.c_int(), C.c_int()
+    p = str(path).encode()
+    s = L.mk_read_obj(p, 0, 0, None, None, C.byref(nV), C.byref(nF))
+    if s:
+        raise MetrikoError(s, "mk_read_obj")
+    V = np.zeros((nV.value, 3))
+    F = np.zeros((nF.value, 3), np.int32)
+    s = L.mk_read_obj(p, nV.value, nF.value, _ptr(V, _D), _ptr(F, _I32), C.byref(nV), C.byref(nF))
+    if s:
+        raise MetrikoError(s, "mk_read_obj")
+    return V, F
+
+
+def write_cfn_json(path, cfn):
+    """Corner uv (complex [3 nF] or real [3 nF, 2]) to the {"cfn": [...]} dump (mk_cfn_write_json)."""
+    c = np.asarray(cfn)
+    a = np.ascontiguousarray(np.stack([c.real, c.imag], 1) if np.iscomplexobj(c) else c, np.float64).reshape(-1)
+    s = lib().mk_cfn_write_json(str(path).encode(), a.size // 2, _ptr(a, _D))
+    if s:
+        raise MetrikoError(s, "mk_cfn_write_json")
+
+
+def read_cfn_json(path):
+    """The {"cfn": [...]} dump as complex corner uv (mk_cfn_read_json)."""
+    L = lib()
+    n = C.c_int()
+    p = str(path).encode()
+    s = L.mk_cfn_read_json(p, 0, None, C.byref(n))
+    if s:
+        raise MetrikoError(s, "mk_cfn_read_json")
+    a = np.zeros(2 * n.value)
+    s = L.mk_cfn_read_json(p, n.value, _ptr(a, _D), C.byref(n))
+    if s:
+        raise MetrikoError(s, "mk_cfn_read_json")
+    return a[0::2] + 1j * a[1::2]
 
 
 def qex_extract(V, F, matching, singular, uv, N=4):

@@ -6,6 +6,7 @@
 #include "igm_state.h"
 #include "metriko/core/vectorfield/face_rosy_field.h"
 #include "metriko/core/igm/parameterization.h"
+#include "metriko/core/common/io.h"
 #include "metriko/core/qex/sanitization.h"
 #include "metriko/core/qex/gen_q_vert.h"
 #include "metriko/core/qex/gen_q_port.h"
@@ -376,5 +377,39 @@ extern "C" mk_status mk_qex_extract(mk_ctx* ctx, int nV, int nF, const double* V
             if (e_ports) { e_ports[2 * i] = edges[size_t(i)].port1.idx; e_ports[2 * i + 1] = edges[size_t(i)].port2.idx; }
         for (int i = 0; i < int(faces.size()) && i < cap_f; i++)
             if (f_halfs) for (int j = 0; j < 4; j++) f_halfs[4 * i + j] = faces[size_t(i)].qhalfs[size_t(j)].idx;
+    });
+}
+
+extern "C" mk_status mk_read_obj(const char* path, int cap_v, int cap_f, double* V, int32_t* F, int* nV, int* nF) {
+    if (!path || !nV || !nF) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        metriko::MatXd Vm;
+        metriko::MatXi Fm;
+        if (!metriko::read_obj(path, Vm, Fm)) throw mk::Error(MK_ERR_ARG, "cannot open OBJ file");
+        *nV = Vm.rows();
+        *nF = Fm.rows();
+        if (V && Vm.rows() <= cap_v) for (size_t i = 0; i < Vm.d.size(); i++) V[i] = Vm.d[i];
+        if (F && Fm.rows() <= cap_f) for (size_t i = 0; i < Fm.d.size(); i++) F[i] = Fm.d[i];
+    });
+}
+
+extern "C" mk_status mk_cfn_write_json(const char* path, int n, const double* cfn) {
+    if (!path || n < 0 || (n && !cfn)) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        metriko::VecXc c(n);
+        for (int i = 0; i < n; i++) c[i] = metriko::complex(cfn[2 * i], cfn[2 * i + 1]);
+        metriko::write_cfn_json(path, c);
+    });
+}
+
+extern "C" mk_status mk_cfn_read_json(const char* path, int cap, double* cfn, int* n) {
+    if (!path || !n) return MK_ERR_ARG;
+    return guarded(nullptr, [&] {
+        metriko::MatXd uv1;
+        metriko::VecXc uv2;
+        if (!metriko::read_cfn_json(path, uv1, uv2)) throw mk::Error(MK_ERR_ARG, "cannot open cfn file");
+        *n = uv2.size();
+        if (cfn && uv2.size() <= cap)
+            for (int i = 0; i < uv2.size(); i++) { cfn[2 * i] = uv2[i].real(); cfn[2 * i + 1] = uv2[i].imag(); }
     });
 }
