@@ -68,8 +68,10 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
 // never the same buffer): r = B - A X ; d = cs D^-1 r (5) or ca d + cs D^-1 r (6) ; Y = X + d.
 // A lane carries one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront
 // covers 64 / LPR rows of the SELL part at a time; medium and long rows take a whole
-// wavefront whose row groups split the entries.
-template <typename T, int MW, int MODE>
+// wavefront whose row groups split the entries.  PART 1 / 2: only the SELL rows / only the
+// medium and long rows (launched with nw_sell = 0), so the SELL launch does not carry the
+// registers of the eight-gather chains and runs at full occupancy.
+template <typename T, int MW, int MODE, int PART = 0>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
                                                T* __restrict__ Y, const T* __restrict__ B, const int* skip,
                                                T* __restrict__ D = nullptr, const T* __restrict__ dinv = nullptr,
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     V acc;
     vzero(acc);
     int row = -1;
-    if (gw < nw_sell) {
+    if (PART != 2 && gw < nw_sell) {
         const int r = gw * R + sub;
         if (r >= A.n_short) return;
         const int s = r / A.C, lr = r - s * A.C;
@@ -107,6 +109,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             vfma(acc, A.val[e], Xv[size_t(A.col[e]) * LPR + j]);
         }
     } else {
+        if constexpr (PART == 1) return;
         int lw = gw - nw_sell;
         const int* rows = A.mid_rows;
         const int64_t* ptr = A.mid_ptr;
@@ -175,20 +178,20 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
     const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
     const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
-    const int64_t waves = int64_t(nw_sell) + A->nmid + A->nlong;
-    const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
+    const int64_t wl = int64_t(A->nmid) + A->nlong;
+    const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
+    const unsigned g2 = unsigned((wl + 3) / 4);
+    // SELL rows, then (when there are any) the medium and long rows
+#define MK_SPMM_PARTS(M)                                                                                        \
+    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip); \
+    if (g2) hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip);
     switch (mode) {
-        case SPMV_PLAIN:
-            hipLaunchKernelGGL((sell_spmm<T, MW, 0>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
-            break;
-        case SPMV_RESID_NORED:
-            hipLaunchKernelGGL((sell_spmm<T, MW, 3>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
-            break;
-        case SPMV_ACCUM:
-            hipLaunchKernelGGL((sell_spmm<T, MW, 4>), dim3(g), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip);
-            break;
+        case SPMV_PLAIN: MK_SPMM_PARTS(0) break;
+        case SPMV_RESID_NORED: MK_SPMM_PARTS(3) break;
+        case SPMV_ACCUM: MK_SPMM_PARTS(4) break;
         default: throw Error(MK_ERR_ARG, "spmm: unsupported mode");
     }
+#undef MK_SPMM_PARTS
 }
 
 template <typename T, int MW>
@@ -198,14 +201,17 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
     const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
     const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
-    const int64_t waves = int64_t(nw_sell) + A->nmid + A->nlong;
-    const unsigned g = unsigned(std::max<int64_t>(1, (waves + 3) / 4));
-    if (first)
-        hipLaunchKernelGGL((sell_spmm<T, MW, 5>), dim3(g), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D, dinv,
+    const int64_t wl = int64_t(A->nmid) + A->nlong;
+    const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
+    const unsigned g2 = unsigned((wl + 3) / 4);
+#define MK_CHEB_PARTS(M)                                                                                            \
+    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D,  \
+                       dinv, ca, cs);                                                                               \
+    if (g2)                                                                                                         \
+        hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip, D, dinv, \
                            ca, cs);
-    else
-        hipLaunchKernelGGL((sell_spmm<T, MW, 6>), dim3(g), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D, dinv,
-                           ca, cs);
+    if (first) { MK_CHEB_PARTS(5) } else { MK_CHEB_PARTS(6) }
+#undef MK_CHEB_PARTS
 }
 
 // One fused Chebyshev step on a block of mw right-hand sides (x_out != x_in).
