@@ -13,6 +13,8 @@
 //    x* = xt + Z (wK^-2 I + U Z)^{-1} (r - U xt), xt = A0^{-1} b0, Z = A0^{-1} U^T
 //    (Sherman-Morrison-Woodbury; no 1e10-weighted quantity is ever formed).
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -680,27 +682,52 @@ struct InjSolver {
             r.status == MK_ERR_UNSUPPORTED)
             throw Error(r.status, "IGM solve failed: " + std::string(mk_status_str(r.status)));
     }
-    // (Hs + W^T diag(omega) W) d = g; Hs SPD (host CSR, k x k), W few sparse rows.
+    // (Hs + W^T diag(omega) W) d = g; Hs SPD (host CSR, k x k, from build()), W few sparse rows.
     // diag_dominant: the damping makes Hs well conditioned under Jacobi scaling (Hs holds
     // lambda diag(J^T J) on top of J^T J), so Jacobi PCG needs no hierarchy set-up; otherwise
-    // a multigrid hierarchy of its own (Jacobi when aggregation cannot coarsen Hs).
-    void solve(const SprsD& Hs, const SprsD& W, const std::vector<double>& omega, const double* g, double* d,
-               bool diag_dominant) {
-        OpHolder h;
-        h.op = mk::op_create_owned(K.c, k, std::vector<int64_t>(Hs.ptr), std::vector<int>(Hs.col),
-                                   std::vector<double>(Hs.val), K.o);
-        h.op->label = "igm_inj";
-        h.op->nullvec = A0->nullvec;
-        h.op->nnull = A0->nnull;
-        jacobi = diag_dominant || K.o.precond != MK_PRECOND_AMG;
-        if (!jacobi) {
-            try {
-                mk::ensure_mg(K.c, h.op, K.o);
-            } catch (const Error& e) {
-                if (e.code != MK_ERR_UNSUPPORTED) throw;
-                jacobi = true;
-            }
+    // a multigrid hierarchy of its own (Jacobi when aggregation cannot coarsen Hs).  key != 0:
+    // Hs depends on key alone (lambda and the pass, no barrier row active), and the operator
+    // with its hierarchy is kept for the next direction with the same key (up to 16).
+    struct CachedOp {
+        mk_op* op = nullptr;
+        bool jacobi = false;
+        ~CachedOp() { if (op) mk_op_destroy(op); }
+    };
+    std::map<uint64_t, std::unique_ptr<CachedOp>> cache;
+    int cache_hits = 0;
+    template <typename Build>
+    void solve(Build&& build, uint64_t key, const SprsD& W, const std::vector<double>& omega, const double* g,
+               double* d, bool diag_dominant) {
+        std::unique_ptr<CachedOp> local;
+        CachedOp* hc = nullptr;
+        if (key) {
+            const auto it = cache.find(key);
+            if (it != cache.end()) { hc = it->second.get(); cache_hits++; }
         }
+        if (!hc) {
+            auto co = std::make_unique<CachedOp>();
+            {
+                SprsD Hs = build();
+                co->op = mk::op_create_owned(K.c, k, std::move(Hs.ptr), std::move(Hs.col), std::move(Hs.val), K.o);
+            }
+            co->op->label = "igm_inj";
+            co->op->nullvec = A0->nullvec;
+            co->op->nnull = A0->nnull;
+            co->jacobi = diag_dominant || K.o.precond != MK_PRECOND_AMG;
+            if (!co->jacobi) {
+                try {
+                    mk::ensure_mg(K.c, co->op, K.o);
+                } catch (const Error& e) {
+                    if (e.code != MK_ERR_UNSUPPORTED) throw;
+                    co->jacobi = true;
+                }
+            }
+            hc = co.get();
+            if (key && cache.size() < 16) cache.emplace(key, std::move(co));
+            else local = std::move(co);
+        }
+        struct { mk_op* op; } h{hc->op};
+        jacobi = hc->jacobi;
         if (jacobi && K.o.precond == MK_PRECOND_AMG) jacobi_solves++;
         hipLaunchKernelGGL(k_zero, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), y.p);
         pcg(h.op, g, y.p);
@@ -981,12 +1008,17 @@ struct NaivePass {
     std::vector<double> cres;
     const double* zcur = nullptr;
     bool finite = true;
+    double mean_ediag = 0.0;   // mean diagonal entry of E
     NaivePass(Ctx& K_, const InjInputs& in_, InjSolver& sol_, BarrierState& bs_) : K(K_), in(in_), sol(sol_), bs(bs_) {
         const size_t l1 = size_t(in.l1), k = size_t(in.k);
         dF2.upload(in.F2, l1, K.st());
         Bx.alloc(l1); res.alloc(l1); gb.alloc(l1); t1.alloc(l1); t2.alloc(l1);
         tk.alloc(k); gx.alloc(k);
         dslot.alloc(size_t(in.nF));
+        for (int i = 0; i < in.k; i++)
+            for (int64_t t = in.E->ptr[i]; t < in.E->ptr[i + 1]; t++)
+                if (in.E->col[t] == i) mean_ediag += in.E->val[t];
+        mean_ediag /= std::max(1, in.k);
     }
     std::vector<double> cm_res(const double* x) {
         std::vector<double> r = sol.rowdots(*in.Cm, x);
@@ -1078,46 +1110,51 @@ struct NaivePass {
             sol.add_rows(*in.Cm, cf, gx.p);
         }
         // reduced operator: wI^2 q length E + lambda wI^2 length diag E + active face terms + pin
-        SprsD H = *in.E;
+        // pin weight of the fixed rows inside Hs (the rest, wK^2 - pin, goes to the Woodbury
+        // rows): the mean diagonal of the Poisson part, a function of lambda alone
         const double L = in.length;
-        for (auto& v : H.val) v *= wI * wI * q * L;
-        std::vector<double> dv(static_cast<size_t>(k), 0.0);
-        for (int i = 0; i < k; i++)
-            for (int64_t t = in.E->ptr[i]; t < in.E->ptr[i + 1]; t++)
-                if (in.E->col[t] == i) dv[size_t(i)] = lam * wI * wI * L * in.E->val[t];
-        if (!act.empty()) {
-            std::vector<TripD> T;
-            const SprsD& X = *in.X2F;
-            for (size_t s = 0; s < act.size(); s++) {
-                const int i = act[s];
-                const double* Qi = Qb.data() + s * size_t(N2) * N2;
-                for (int ra = 0; ra < N2; ra++)
-                    for (int rb = 0; rb < N2; rb++) {
-                        const double cab = wI * wI * (Qi[size_t(ra) * N2 + rb] - (ra == rb ? q : 0.0)) * L * L;
-                        if (cab == 0.0) continue;
-                        const int A = N2 * i + ra, Bq = N2 * i + rb;
-                        for (int64_t u = X.ptr[A]; u < X.ptr[A + 1]; u++)
-                            for (int64_t v = X.ptr[Bq]; v < X.ptr[Bq + 1]; v++)
-                                T.emplace_back(X.col[u], X.col[v], cab * X.val[u] * X.val[v]);
-                    }
-            }
-            SprsD Bar(k, k);
-            Bar.setFromTriplets(T);
-            H = H + Bar;
-        }
-        add_diagonal(H, dv);
-        double dmean = 0.0;
-        for (int i = 0; i < k; i++)
-            for (int64_t t = H.ptr[i]; t < H.ptr[i + 1]; t++) if (H.col[t] == i) dmean += H.val[t];
-        dmean = std::min(k ? dmean / k : 1.0, 0.5 * wK * wK);
+        double dmean = std::min(wI * wI * L * (q + lam) * mean_ediag, 0.5 * wK * wK);
         if (!(dmean > 0.0)) dmean = 1.0;
-        if (in.Cm->rows()) H = H + (in.Cm->transpose() * (*in.Cm)).scaled(dmean);
+        auto build = [&]() {
+            SprsD H = *in.E;
+            for (auto& v : H.val) v *= wI * wI * q * L;
+            std::vector<double> dv(static_cast<size_t>(k), 0.0);
+            for (int i = 0; i < k; i++)
+                for (int64_t t = in.E->ptr[i]; t < in.E->ptr[i + 1]; t++)
+                    if (in.E->col[t] == i) dv[size_t(i)] = lam * wI * wI * L * in.E->val[t];
+            if (!act.empty()) {
+                std::vector<TripD> T;
+                const SprsD& X = *in.X2F;
+                for (size_t s = 0; s < act.size(); s++) {
+                    const int i = act[s];
+                    const double* Qi = Qb.data() + s * size_t(N2) * N2;
+                    for (int ra = 0; ra < N2; ra++)
+                        for (int rb = 0; rb < N2; rb++) {
+                            const double cab = wI * wI * (Qi[size_t(ra) * N2 + rb] - (ra == rb ? q : 0.0)) * L * L;
+                            if (cab == 0.0) continue;
+                            const int A = N2 * i + ra, Bq = N2 * i + rb;
+                            for (int64_t u = X.ptr[A]; u < X.ptr[A + 1]; u++)
+                                for (int64_t v = X.ptr[Bq]; v < X.ptr[Bq + 1]; v++)
+                                    T.emplace_back(X.col[u], X.col[v], cab * X.val[u] * X.val[v]);
+                        }
+                }
+                SprsD Bar(k, k);
+                Bar.setFromTriplets(T);
+                H = H + Bar;
+            }
+            add_diagonal(H, dv);
+            if (in.Cm->rows()) H = H + (in.Cm->transpose() * (*in.Cm)).scaled(dmean);
+            return H;
+        };
         SprsD W;
         std::vector<double> omega;
         stiff_rows(*in.Cm, wK * wK - dmean, wK * wK, lam, k, W, omega);
         // Hs = lambda wI^2 length (E / (1 + lambda) + diag E) + ...: Jacobi-conditioned for
-        // every lambda well above wC^2 / wI^2 = 1e-8
-        sol.solve(H, W, omega, gx.p, dz, lam >= 1e-6);
+        // every lambda well above wC^2 / wI^2 = 1e-8; without active faces it depends on lambda
+        // alone and is cached
+        uint64_t key = 0;
+        if (act.empty()) { std::memcpy(&key, &lam, sizeof(key)); key ^= 0x5a5a5a5a00000000ull; }
+        sol.solve(build, key, W, omega, gx.p, dz, lam >= 1e-6);
         // df = P (-wI^2 (f - B (x + dx)) + gb)
         hipLaunchKernelGGL(k_lin2, dim3(nblk(size_t(k))), dim3(256), 0, K.st(), size_t(k), 1.0, zcur, 1.0, dz, tk.p);
         mk::spmv64(K.c, in.B, tk.p, t2.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
@@ -1182,16 +1219,27 @@ struct SeamlessPass {
     }
     bool dir(double lam, double* d) {
         if (!finite) return false;
+        bool active = false;
+        for (double v : bs.hder) active |= v != 0.0;
+        // with barrier rows active Hs (and the stiff barrier rows) are assembled now; without,
+        // Hs depends on lambda alone and is assembled only when no cached operator has it
         std::vector<SparseRow> stiff;
-        SprsD H = seamless_soft(in, bs, lam, wI, wC, wB, stiff);
+        SprsD H;
+        if (active) H = seamless_soft(in, bs, lam, wI, wC, wB, stiff);
         SprsD W;
         std::vector<double> omega;
         stiff_rows(U, wK * wK, wK * wK, lam, in.k, W, omega);
         append_rows(W, omega, stiff, wB * wB);
+        uint64_t key = 0;
+        if (!active) std::memcpy(&key, &lam, sizeof(key));
         // Hs = A0 + lambda diag(A0) + ...: Jacobi condition about 2 / lambda.  Below
         // lambda = 0.1 multigrid with block PCG for the Woodbury columns; above it (rejected
         // steps) Jacobi, where aggregation of the diagonal-heavy Hs stalls
-        sol.solve(H, W, omega, g.p, d, lam >= 0.1);
+        sol.solve([&]() {
+            if (active) return std::move(H);
+            std::vector<SparseRow> none;
+            return seamless_soft(in, bs, lam, wI, wC, wB, none);
+        }, key, W, omega, g.p, d, lam >= 0.1);
         return true;
     }
 };
@@ -1318,7 +1366,8 @@ void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, 
     S.lm_rounds = lm_rounds;
     S.n_fixed = int(fixedI.size());
     MK_LOG("[igm] injective rounding %.0f ms: %d rounds, %d with LM, %d LM iterations (%d directions by Jacobi "
-           "PCG)\n", now_ms() - t0, rounds, lm_rounds, lm_iters, sol.jacobi_solves);
+           "PCG, %d on a cached operator)\n", now_ms() - t0, rounds, lm_rounds, lm_iters, sol.jacobi_solves,
+           sol.cache_hits);
     if (rounded) {
         xcur.download(xs.data(), xs.size(), K.st());
         K.sync();
