@@ -1295,6 +1295,7 @@ void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, 
     }
     S.naive_ms = now_ms() - t0;
     MK_LOG("[igm] injective naive LM %.0f ms, %d iterations\n", S.naive_ms, lm_iters);
+    sol.cache.clear();   // the kept operators are the naive pass's; the rounding passes key their own
     std::vector<double> xs(static_cast<size_t>(k));
     z.download(xs.data(), xs.size(), K.st());
     K.sync();
