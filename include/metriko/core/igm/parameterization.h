@@ -270,7 +270,13 @@ namespace metriko {
         }
         const double length = std::sqrt(bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]) * gridscale / norm;
         SprsD G2full = G2 * fullx2Nfunc;
-        if (localInjectivity) {   // the naive pass's iter_cb (parameterization_integ.ipp:45-63)
+        // the naive pass's iter_cb (parameterization_integ.ipp:45-63), registered for this call
+        // only (removed again on every exit path)
+        struct CheckScope {
+            int N = 0;
+            ~CheckScope() { if (N) mk_igm_injectivity_check(hip::runtime().get(), N, 0, nullptr, 0, 0, nullptr, nullptr, nullptr); }
+        } check_scope;
+        if (localInjectivity) {
             std::vector<int32_t> ci(size_t(3) * cut.nF);
             for (int f = 0; f < cut.nF; f++)
                 for (int j = 0; j < 3; j++) ci[size_t(3) * f + j] = cut.idx(f, j);
@@ -278,11 +284,11 @@ namespace metriko {
                                                 fullx2Nfunc.cols(), fullx2Nfunc.ptr.data(), fullx2Nfunc.col.data(),
                                                 fullx2Nfunc.val.data()),
                        "mk_igm_injectivity_check");
+            check_scope.N = N;
         }
         bool success = iterative_rounding(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2full, N, n,
                                           cut.nF, seamless, roundSeams, localInjectivity, verbose, F2, fullx,
                                           &decisionIndex, &decisionValue);
-        if (localInjectivity) mk_igm_injectivity_check(hip::runtime().get(), N, 0, nullptr, 0, 0, nullptr, nullptr, nullptr);
         nfn.resize(cut.nV, N);
         cfn.resize(raw.nF, N * 3);
         VecXd NF = fullx2Nfunc * fullx;
