@@ -74,6 +74,51 @@ __global__ void k_seg_rayleigh(const int64_t* __restrict__ beg, const int64_t* _
     if (threadIdx.x == 0)
         for (int k = 0; k < 3; k++) part[static_cast<size_t>(blockIdx.x) * 3 + k] = v[k];
 }
+// Galerkin start of each mesh's inverse iteration on the window of its last kBWin iterates
+// (the single-mesh field's start, per segment): per block, w_j^H a and w_j^H b (complex) for
+// the nw window columns (column j at W + len j), 4 doubles per column.
+constexpr int kBWin = 4;
+constexpr int kSegMaxK = 4 * kBWin;
+__global__ void k_seg_window_dots(const int64_t* __restrict__ beg, const int64_t* __restrict__ end, int nw,
+                                  size_t len, const double* __restrict__ W, const double* __restrict__ a,
+                                  const double* __restrict__ b, double* __restrict__ part) {
+    double acc[kSegMaxK];
+#pragma unroll
+    for (int k = 0; k < kSegMaxK; k++) acc[k] = 0.0;
+    for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x) {
+        const double ar = a[2 * i], ai = a[2 * i + 1], br = b[2 * i], bi = b[2 * i + 1];
+#pragma unroll
+        for (int j = 0; j < kBWin; j++) {
+            if (j < nw) {
+                const double wr = W[size_t(j) * len + 2 * i], wi = W[size_t(j) * len + 2 * i + 1];
+                acc[4 * j] += wr * ar + wi * ai;
+                acc[4 * j + 1] += wr * ai - wi * ar;
+                acc[4 * j + 2] += wr * br + wi * bi;
+                acc[4 * j + 3] += wr * bi - wi * br;
+            }
+        }
+    }
+    mk::block_sum<kSegMaxK>(acc);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < kSegMaxK; k++) part[static_cast<size_t>(blockIdx.x) * kSegMaxK + k] = acc[k];
+}
+// x = sum_j y[seg][j] w_j over each block's elements (complex coefficients, 2 kBWin per segment)
+__global__ void k_seg_window_combine(const int* __restrict__ seg, const int64_t* __restrict__ beg,
+                                     const int64_t* __restrict__ end, int nw, size_t len,
+                                     const double* __restrict__ W, const double* __restrict__ Y,
+                                     double* __restrict__ x) {
+    const double* y = Y + static_cast<size_t>(seg[blockIdx.x]) * 2 * kBWin;
+    for (int64_t i = beg[blockIdx.x] + threadIdx.x; i < end[blockIdx.x]; i += blockDim.x) {
+        double xr = 0.0, xi = 0.0;
+        for (int j = 0; j < nw; j++) {
+            const double wr = W[size_t(j) * len + 2 * i], wi = W[size_t(j) * len + 2 * i + 1];
+            xr += y[2 * j] * wr - y[2 * j + 1] * wi;
+            xi += y[2 * j] * wi + y[2 * j + 1] * wr;
+        }
+        x[2 * i] = xr;
+        x[2 * i + 1] = xi;
+    }
+}
 // q0 = x^H M x (complex elements)
 __global__ void k_seg_massnorm(const int64_t* __restrict__ beg, const int64_t* __restrict__ end,
                                const double* __restrict__ x, const double* __restrict__ mass,
@@ -116,8 +161,8 @@ struct Seg {
     DevBuf<double> part, out, s;
     std::vector<double> h;
     Seg(mk_ctx* c_, SegTable& T_) : c(c_), T(T_) {
-        part.alloc(static_cast<size_t>(T.nblk) * 3);
-        out.alloc(static_cast<size_t>(T.nseg) * 3);
+        part.alloc(static_cast<size_t>(T.nblk) * kSegMaxK);
+        out.alloc(static_cast<size_t>(T.nseg) * kSegMaxK);
         s.alloc(static_cast<size_t>(T.nseg));
     }
     // K totals per segment to the host
@@ -381,22 +426,57 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
         mk_solve_opts og = of;
         og.tol = gtol;
         std::vector<double> f1(static_cast<size_t>(nmesh)), f2(static_cast<size_t>(nmesh));
+        // per mesh: Galerkin matrix G = W^H L W of its window (slot order) and W^H b
+        DevBuf<double> dW(lenF * kBWin), dY(static_cast<size_t>(nmesh) * 2 * kBWin);
+        std::vector<std::complex<double>> G(static_cast<size_t>(nmesh) * kBWin * kBWin),
+            gw(static_cast<size_t>(nmesh) * kBWin);
+        std::vector<double> hY(static_cast<size_t>(nmesh) * 2 * kBWin, 0.0);
         for (int it = 0; it < n_iter; it++) {
             mk::mass_mul(c, int(nFt), dm.p, du.p, db.p);
             mk::spmv64(c, Lop, du.p, dLu.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
             hipLaunchKernelGGL(k_seg_rayleigh, dim3(TF.nblk), dim3(256), 0, c->stream, (const int64_t*)TF.beg.p,
                                (const int64_t*)TF.end.p, (const double*)du.p, (const double*)dLu.p,
                                (const double*)db.p, (const double*)dm.p, SF.part.p);
-            const std::vector<double>& h = SF.finish(3);
+            const std::vector<double> h = SF.finish(3);
+            // the iterate enters its mesh's window (slot s); the start is the Galerkin solution on
+            // the window, x0 = W (W^H L W)^+ W^H b per mesh (with the window {u} alone: the
+            // Rayleigh-quotient start u (u^H M u) / (u^H L u)), in the scaled system
+            const int slot = it % kBWin, nw = std::min(it + 1, kBWin);
+            MK_HIP(hipMemcpyAsync(dW.p + lenF * size_t(slot), du.p, lenF * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c->stream));
+            hipLaunchKernelGGL(k_seg_window_dots, dim3(TF.nblk), dim3(256), 0, c->stream, (const int64_t*)TF.beg.p,
+                               (const int64_t*)TF.end.p, nw, lenF, (const double*)dW.p, (const double*)dLu.p,
+                               (const double*)db.p, SF.part.p);
+            const std::vector<double>& hd = SF.finish(kSegMaxK);
             for (int m = 0; m < nmesh; m++) {
                 const double num = h[static_cast<size_t>(3 * m)], den = h[static_cast<size_t>(3 * m + 1)], bb = h[static_cast<size_t>(3 * m + 2)];
                 MK_CHECK(num == num && num > 0.0 && den > 0.0 && bb > 0.0, MK_ERR_NAN,
                          "mesh " + std::to_string(m) + ": Rayleigh quotient not positive");
                 const double sb = 1.0 / std::sqrt(bb);
-                f1[static_cast<size_t>(m)] = sb * den / num;  // start u / theta, in the scaled system
                 f2[static_cast<size_t>(m)] = sb;
+                const double* q = hd.data() + static_cast<size_t>(m) * kSegMaxK;
+                std::complex<double>* Gm = G.data() + static_cast<size_t>(m) * kBWin * kBWin;
+                std::complex<double>* gm = gw.data() + static_cast<size_t>(m) * kBWin;
+                for (int j = 0; j < nw; j++) {
+                    const std::complex<double> gjs(q[4 * j], q[4 * j + 1]);   // w_j^H L w_s
+                    Gm[j * kBWin + slot] = gjs;
+                    Gm[slot * kBWin + j] = std::conj(gjs);
+                    gm[j] = {q[4 * j + 2], q[4 * j + 3]};                      // w_j^H b
+                }
+                Gm[slot * kBWin + slot] = q[4 * slot];
+                std::vector<std::complex<double>> Gn(static_cast<size_t>(nw) * nw), gn(gm, gm + nw);
+                for (int i = 0; i < nw; i++)
+                    for (int j = 0; j < nw; j++) Gn[static_cast<size_t>(i) * nw + j] = Gm[i * kBWin + j];
+                const std::vector<std::complex<double>> y = mk::herm_pinv_solve(nw, Gn, gn);
+                for (int j = 0; j < nw; j++) {
+                    hY[static_cast<size_t>(m) * 2 * kBWin + 2 * j] = sb * y[static_cast<size_t>(j)].real();
+                    hY[static_cast<size_t>(m) * 2 * kBWin + 2 * j + 1] = sb * y[static_cast<size_t>(j)].imag();
+                }
             }
-            SF.scale(2, f1, du.p, dx.p);
+            dY.upload(hY.data(), hY.size(), c->stream);
+            hipLaunchKernelGGL(k_seg_window_combine, dim3(TF.nblk), dim3(256), 0, c->stream, (const int*)TF.seg.p,
+                               (const int64_t*)TF.beg.p, (const int64_t*)TF.end.p, nw, lenF, (const double*)dW.p,
+                               (const double*)dY.p, dx.p);
             SF.scale(2, f2, db.p, db.p);
             // each mesh's right-hand side has unit norm: the global tol / sqrt(#meshes) bounds
             // every mesh's own relative residual by tol (aiming at a tenth of it, as the single
