@@ -192,6 +192,17 @@ mk_status mk_qex_extract(mk_ctx* ctx, int nV, int nF, const double* V, const int
                          double* p_uvdir, int cap_e, int32_t* e_ports, int cap_f, int32_t* f_halfs,
                          mk_qex_counts* counts);
 
+/* QGP motorcycle graph (host): MotorcycleGraph(mesh, uv, matching, singular)
+ * (tmesh/motorcycle.h, example_qgp.cpp:110) on corner uv[6 nF] (as mk_qex_extract); fills up
+ * to capacity, n_ports / n_segs always set: ports port_ids[5] = (id, vertex, face, prev,
+ * next), port_uvdir[4] = (u, v, dir u, dir v); segments in curve order then segment order,
+ * seg_ids[6] = (curve, face, from type, to type, from crash curve, to crash curve; types
+ * 0 None, 1 HitL, 2 HitR, 3 HitB; -1 no curve), seg_uv[4] = (from u, v, to u, v). */
+mk_status mk_motorcycle_graph(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F,
+                              const int32_t* matching, const int32_t* singular, const double* uv, int cap_p,
+                              int32_t* port_ids, double* port_uvdir, int cap_s, int32_t* seg_ids, double* seg_uv,
+                              int* n_ports, int* n_segs);
+
 /* Files (host): mk_read_obj = the triangle OBJ reader example_igm / example_qex use
  * (igl::readOBJ, `v` and `f a b c` lines; call with capacities 0 for nV / nF first);
  * mk_cfn_write_json / mk_cfn_read_json = the corner-uv dump {"cfn": [re, im, ...]} of

@@ -24,7 +24,7 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_read_obj", "mk_cfn_write_json",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_read_obj", "mk_cfn_write_json",
     "mk_cfn_read_json", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
@@ -142,6 +142,8 @@ def lib():
                                C.POINTER(BatchStats)]
     L.mk_igm_injectivity_check.argtypes = [_P, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I64, _I32, _D]
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
+    L.mk_motorcycle_graph.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _D, C.c_int,
+                                      _I32, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_read_obj.argtypes = [C.c_char_p, C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_cfn_write_json.argtypes = [C.c_char_p, C.c_int, _D]
     L.mk_cfn_read_json.argtypes = [C.c_char_p, C.c_int, _D, C.POINTER(C.c_int)]
@@ -266,6 +268,30 @@ def mg_host_pcg(A, b, tol=1e-10, maxit=500, nullvec=None):
     if cplx:
         x = x[0::2] + 1j * x[1::2]
     return x, it.value, lv.value, rr.value
+
+
+def motorcycle_graph(V, F, matching, singular, uv):
+    """QGP motorcycle graph (mk_motorcycle_graph, host code) on corner uv [3 nF, 2]: ports
+    (id, vertex, face, prev, next) with their uv / direction, and every curve's segments
+    (curve, face, from/to type, from/to crash curve) with their end points."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    mt = np.ascontiguousarray(matching, np.int32)
+    sg = np.ascontiguousarray(singular, np.int32)
+    u = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+    npo, nse = C.c_int(), C.c_int()
+    args = (None, nV, nF, _ptr(V, _D), _ptr(F, _I32), _ptr(mt, _I32), _ptr(sg, _I32), _ptr(u, _D))
+    s = L.mk_motorcycle_graph(*args, 0, None, None, 0, None, None, C.byref(npo), C.byref(nse))
+    if s:
+        raise MetrikoError(s, "mk_motorcycle_graph")
+    pid, puv = np.zeros((npo.value, 5), np.int32), np.zeros((npo.value, 4))
+    sid, suv = np.zeros((nse.value, 6), np.int32), np.zeros((nse.value, 4))
+    s = L.mk_motorcycle_graph(*args, npo.value, _ptr(pid, _I32), _ptr(puv, _D), nse.value, _ptr(sid, _I32),
+                              _ptr(suv, _D), C.byref(npo), C.byref(nse))
+    if s:
+        raise MetrikoError(s, "mk_motorcycle_graph")
+    return {"port_ids": pid, "port_uvdir": puv, "seg_ids": sid, "seg_uv": suv}
 
 
 def read_obj(path):

@@ -8,6 +8,7 @@
 #include "metriko/core/igm/parameterization.h"
 #include "metriko/core/common/io.h"
 #include "metriko/core/qex/sanitization.h"
+#include "metriko/core/tmesh/motorcycle.h"
 #include "metriko/core/qex/gen_q_vert.h"
 #include "metriko/core/qex/gen_q_port.h"
 #include "metriko/core/qex/gen_q_edge.h"
@@ -411,5 +412,54 @@ extern "C" mk_status mk_cfn_read_json(const char* path, int cap, double* cfn, in
         *n = uv2.size();
         if (cfn && uv2.size() <= cap)
             for (int i = 0; i < uv2.size(); i++) { cfn[2 * i] = uv2[i].real(); cfn[2 * i + 1] = uv2[i].imag(); }
+    });
+}
+
+extern "C" mk_status mk_motorcycle_graph(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F,
+                                         const int32_t* matching, const int32_t* singular, const double* uv,
+                                         int cap_p, int32_t* port_ids, double* port_uvdir, int cap_s,
+                                         int32_t* seg_ids, double* seg_uv, int* n_ports, int* n_segs) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !matching || !singular || !uv || !n_ports || !n_segs) return MK_ERR_ARG;
+    return guarded(ctx, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        VecXi mt(mesh.nE), sg(nV);
+        for (int e = 0; e < mesh.nE; e++) mt[e] = matching[e];
+        for (int v = 0; v < nV; v++) sg[v] = singular[v];
+        VecXc cfn(3 * nF);
+        for (int c = 0; c < 3 * nF; c++) cfn[c] = complex(uv[2 * c], uv[2 * c + 1]);
+        const MotorcycleGraph g(mesh, cfn, mt, sg);
+        *n_ports = int(g.mports.size());
+        int ns = 0;
+        for (const Mcurv& c : g.mcurvs) ns += int(c.sgmts.size());
+        *n_segs = ns;
+        for (int i = 0; i < int(g.mports.size()) && i < cap_p; i++) {
+            const Mport& p = g.mports[size_t(i)];
+            if (port_ids) {
+                int32_t* o = port_ids + 5 * i;
+                o[0] = p.id; o[1] = p.vert.id; o[2] = p.face.id; o[3] = p.prev; o[4] = p.next;
+            }
+            if (port_uvdir) {
+                double* o = port_uvdir + 4 * i;
+                o[0] = p.uv.real(); o[1] = p.uv.imag(); o[2] = p.dir.real(); o[3] = p.dir.imag();
+            }
+        }
+        auto cid = [&](const Mcurv* c) { return c ? int32_t(c - g.mcurvs.data()) : int32_t(-1); };
+        int k = 0;
+        for (size_t ci = 0; ci < g.mcurvs.size(); ci++)
+            for (const Msgmt& s : g.mcurvs[ci].sgmts) {
+                if (k < cap_s) {
+                    if (seg_ids) {
+                        int32_t* o = seg_ids + 6 * k;
+                        o[0] = int32_t(ci); o[1] = s.face.id; o[2] = s.fr.type; o[3] = s.to.type;
+                        o[4] = cid(s.fr.crash); o[5] = cid(s.to.crash);
+                    }
+                    if (seg_uv) {
+                        double* o = seg_uv + 4 * k;
+                        o[0] = s.fr.uv.real(); o[1] = s.fr.uv.imag(); o[2] = s.to.uv.real(); o[3] = s.to.uv.imag();
+                    }
+                }
+                k++;
+            }
     });
 }
