@@ -203,6 +203,17 @@ mk_status mk_motorcycle_graph(mk_ctx* ctx, int nV, int nF, const double* V, cons
                               int32_t* port_ids, double* port_uvdir, int cap_s, int32_t* seg_ids, double* seg_uv,
                               int* n_ports, int* n_segs);
 
+/* QGP T-mesh (host): Tmesh(MotorcycleGraph(...).mcurvs) (tmesh/tmesh.h, example_qgp.cpp:111).
+ * T-edge i: tedges[3 i] = (curve, first segment, last segment) (segment indices within the
+ * curve, as mk_motorcycle_graph lists them), half-edges 2 i (along the curve) and 2 i + 1;
+ * th2sing[2 nTE] (singular vertex a half-edge leaves, -1 otherwise); T-quad q: half-edges
+ * quad_th[quad_ptr[q] .. quad_ptr[q + 1]) with their sides quad_side (0..3).  Filled up to
+ * capacity (cap_q = T-quads, cap_t = entries); the three counts are always set. */
+mk_status mk_tmesh_build(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, const int32_t* matching,
+                         const int32_t* singular, const double* uv, int cap_e, int32_t* tedges, int32_t* th2sing,
+                         int cap_q, int32_t* quad_ptr, int cap_t, int32_t* quad_th, int32_t* quad_side,
+                         int* n_tedges, int* n_tquads, int* n_entries);
+
 /* Files (host): mk_read_obj = the triangle OBJ reader example_igm / example_qex use
  * (igl::readOBJ, `v` and `f a b c` lines; call with capacities 0 for nV / nF first);
  * mk_cfn_write_json / mk_cfn_read_json = the corner-uv dump {"cfn": [re, im, ...]} of

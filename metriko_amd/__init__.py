@@ -24,7 +24,7 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_read_obj", "mk_cfn_write_json",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_tmesh_build", "mk_read_obj", "mk_cfn_write_json",
     "mk_cfn_read_json", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
@@ -144,6 +144,8 @@ def lib():
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
     L.mk_motorcycle_graph.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _D, C.c_int,
                                       _I32, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mk_tmesh_build.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, C.c_int, _I32,
+                                 C.c_int, _I32, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_read_obj.argtypes = [C.c_char_p, C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_cfn_write_json.argtypes = [C.c_char_p, C.c_int, _D]
     L.mk_cfn_read_json.argtypes = [C.c_char_p, C.c_int, _D, C.POINTER(C.c_int)]
@@ -292,6 +294,30 @@ def motorcycle_graph(V, F, matching, singular, uv):
     if s:
         raise MetrikoError(s, "mk_motorcycle_graph")
     return {"port_ids": pid, "port_uvdir": puv, "seg_ids": sid, "seg_uv": suv}
+
+
+def tmesh_build(V, F, matching, singular, uv):
+    """QGP T-mesh (mk_tmesh_build, host code): T-edges (curve, first, last segment),
+    th2sing, and the T-quads as lists of (half-edge, side)."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    mt = np.ascontiguousarray(matching, np.int32)
+    sg = np.ascontiguousarray(singular, np.int32)
+    u = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+    ne, nq, nt = C.c_int(), C.c_int(), C.c_int()
+    args = (None, nV, nF, _ptr(V, _D), _ptr(F, _I32), _ptr(mt, _I32), _ptr(sg, _I32), _ptr(u, _D))
+    s = L.mk_tmesh_build(*args, 0, None, None, 0, None, 0, None, None, C.byref(ne), C.byref(nq), C.byref(nt))
+    if s:
+        raise MetrikoError(s, "mk_tmesh_build")
+    te, ts = np.zeros((ne.value, 3), np.int32), np.zeros(2 * ne.value, np.int32)
+    qp, qt, qs = np.zeros(nq.value + 1, np.int32), np.zeros(nt.value, np.int32), np.zeros(nt.value, np.int32)
+    s = L.mk_tmesh_build(*args, ne.value, _ptr(te, _I32), _ptr(ts, _I32), nq.value, _ptr(qp, _I32), nt.value,
+                         _ptr(qt, _I32), _ptr(qs, _I32), C.byref(ne), C.byref(nq), C.byref(nt))
+    if s:
+        raise MetrikoError(s, "mk_tmesh_build")
+    quads = [(qt[qp[q]:qp[q + 1]].tolist(), qs[qp[q]:qp[q + 1]].tolist()) for q in range(nq.value)]
+    return {"tedges": te, "th2sing": ts, "tquads": quads}
 
 
 def read_obj(path):

@@ -9,6 +9,7 @@
 #include "metriko/core/common/io.h"
 #include "metriko/core/qex/sanitization.h"
 #include "metriko/core/tmesh/motorcycle.h"
+#include "metriko/core/tmesh/tmesh.h"
 #include "metriko/core/qex/gen_q_vert.h"
 #include "metriko/core/qex/gen_q_port.h"
 #include "metriko/core/qex/gen_q_edge.h"
@@ -461,5 +462,51 @@ extern "C" mk_status mk_motorcycle_graph(mk_ctx* ctx, int nV, int nF, const doub
                 }
                 k++;
             }
+    });
+}
+
+extern "C" mk_status mk_tmesh_build(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F,
+                                    const int32_t* matching, const int32_t* singular, const double* uv, int cap_e,
+                                    int32_t* tedges, int32_t* th2sing, int cap_q, int32_t* quad_ptr, int cap_t,
+                                    int32_t* quad_th, int32_t* quad_side, int* n_tedges, int* n_tquads,
+                                    int* n_entries) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !matching || !singular || !uv || !n_tedges || !n_tquads || !n_entries)
+        return MK_ERR_ARG;
+    return guarded(ctx, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        VecXi mt(mesh.nE), sg(nV);
+        for (int e = 0; e < mesh.nE; e++) mt[e] = matching[e];
+        for (int v = 0; v < nV; v++) sg[v] = singular[v];
+        VecXc cfn(3 * nF);
+        for (int c = 0; c < 3 * nF; c++) cfn[c] = complex(uv[2 * c], uv[2 * c + 1]);
+        const MotorcycleGraph g(mesh, cfn, mt, sg);
+        const Tmesh t(g.mcurvs);
+        *n_tedges = int(t.nTE);
+        *n_tquads = int(t.nTQ);
+        int tot = 0;
+        for (const Tquad& q : t.tquads) tot += int(q.thids.size());
+        *n_entries = tot;
+        for (int i = 0; i < int(t.nTE) && i < cap_e; i++) {
+            const Tedge& e = t.tedges[size_t(i)];
+            if (tedges) {
+                tedges[3 * i] = int32_t(e.seg_fr.curv - g.mcurvs.data());
+                tedges[3 * i + 1] = e.seg_fr.id;
+                tedges[3 * i + 2] = e.seg_to.id;
+            }
+            if (th2sing) { th2sing[2 * i] = t.th2sing[2 * i]; th2sing[2 * i + 1] = t.th2sing[2 * i + 1]; }
+        }
+        if (quad_ptr && int(t.nTQ) <= cap_q) {
+            int k = 0;
+            quad_ptr[0] = 0;
+            for (size_t q = 0; q < t.nTQ; q++) {
+                for (size_t j = 0; j < t.tquads[q].thids.size(); j++, k++)
+                    if (k < cap_t) {
+                        if (quad_th) quad_th[k] = t.tquads[q].thids[j];
+                        if (quad_side) quad_side[k] = t.tquads[q].sides[j];
+                    }
+                quad_ptr[q + 1] = k;
+            }
+        }
     });
 }
