@@ -214,6 +214,14 @@ mk_status mk_tmesh_build(mk_ctx* ctx, int nV, int nF, const double* V, const int
                          int cap_q, int32_t* quad_ptr, int cap_t, int32_t* quad_th, int32_t* quad_side,
                          int* n_tedges, int* n_tquads, int* n_entries);
 
+/* QGP quantization (host): compute_quantization(Tmesh, R) (quantization/quantization.h) on
+ * the T-mesh of mk_tmesh_build, R = each T-edge's uv length (example_qgp.cpp:112-123):
+ * ideal[nTE] = R, lengths[nTE] = the integer T-edge lengths X, valid = opposite T-quad
+ * sides equal and compute_validation passed.  n_tedges always set. */
+mk_status mk_qgp_quantize(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F, const int32_t* matching,
+                          const int32_t* singular, const double* uv, int cap, double* ideal, double* lengths,
+                          int* n_tedges, int* valid);
+
 /* Files (host): mk_read_obj = the triangle OBJ reader example_igm / example_qex use
  * (igl::readOBJ, `v` and `f a b c` lines; call with capacities 0 for nV / nF first);
  * mk_cfn_write_json / mk_cfn_read_json = the corner-uv dump {"cfn": [re, im, ...]} of

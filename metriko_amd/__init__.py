@@ -24,7 +24,7 @@ EXPORTED = [
     "mk_field_laplacian", "mk_op_export_csr", "mk_field_smallest_eig", "mk_iterative_rounding",
     "mk_pipeline_run", "mk_combinatorics", "mk_hmesh_export", "mk_hmesh_halfedges", "mk_mg_host_pcg",
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
-    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_tmesh_build", "mk_read_obj", "mk_cfn_write_json",
+    "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_tmesh_build", "mk_qgp_quantize", "mk_read_obj", "mk_cfn_write_json",
     "mk_cfn_read_json", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
     "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
 ]
@@ -146,6 +146,8 @@ def lib():
                                       _I32, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_tmesh_build.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, C.c_int, _I32,
                                  C.c_int, _I32, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mk_qgp_quantize.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _D, _D,
+                                  C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_read_obj.argtypes = [C.c_char_p, C.c_int, C.c_int, _D, _I32, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_cfn_write_json.argtypes = [C.c_char_p, C.c_int, _D]
     L.mk_cfn_read_json.argtypes = [C.c_char_p, C.c_int, _D, C.POINTER(C.c_int)]
@@ -318,6 +320,27 @@ def tmesh_build(V, F, matching, singular, uv):
         raise MetrikoError(s, "mk_tmesh_build")
     quads = [(qt[qp[q]:qp[q + 1]].tolist(), qs[qp[q]:qp[q + 1]].tolist()) for q in range(nq.value)]
     return {"tedges": te, "th2sing": ts, "tquads": quads}
+
+
+def qgp_quantize(V, F, matching, singular, uv):
+    """QGP quantization (mk_qgp_quantize, host code): ideal T-edge lengths R, integer lengths X
+    and whether X passed the reference's checks."""
+    L = lib()
+    V, F = _mesh_arrays(V, F)
+    nV, nF = V.shape[0], F.shape[0]
+    mt = np.ascontiguousarray(matching, np.int32)
+    sg = np.ascontiguousarray(singular, np.int32)
+    u = np.ascontiguousarray(np.asarray(uv, np.float64).reshape(-1))
+    ne, ok = C.c_int(), C.c_int()
+    args = (None, nV, nF, _ptr(V, _D), _ptr(F, _I32), _ptr(mt, _I32), _ptr(sg, _I32), _ptr(u, _D))
+    s = L.mk_qgp_quantize(*args, 0, None, None, C.byref(ne), None)
+    if s:
+        raise MetrikoError(s, "mk_qgp_quantize")
+    R, X = np.zeros(ne.value), np.zeros(ne.value)
+    s = L.mk_qgp_quantize(*args, ne.value, _ptr(R, _D), _ptr(X, _D), C.byref(ne), C.byref(ok))
+    if s:
+        raise MetrikoError(s, "mk_qgp_quantize")
+    return {"ideal": R, "lengths": X, "valid": bool(ok.value)}
 
 
 def read_obj(path):

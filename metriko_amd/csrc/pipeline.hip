@@ -10,6 +10,7 @@
 #include "metriko/core/qex/sanitization.h"
 #include "metriko/core/tmesh/motorcycle.h"
 #include "metriko/core/tmesh/tmesh.h"
+#include "metriko/core/quantization/quantization.h"
 #include "metriko/core/qex/gen_q_vert.h"
 #include "metriko/core/qex/gen_q_port.h"
 #include "metriko/core/qex/gen_q_edge.h"
@@ -507,6 +508,34 @@ extern "C" mk_status mk_tmesh_build(mk_ctx* ctx, int nV, int nF, const double* V
                     }
                 quad_ptr[q + 1] = k;
             }
+        }
+    });
+}
+
+extern "C" mk_status mk_qgp_quantize(mk_ctx* ctx, int nV, int nF, const double* V, const int32_t* F,
+                                     const int32_t* matching, const int32_t* singular, const double* uv, int cap,
+                                     double* ideal, double* lengths, int* n_tedges, int* valid) {
+    if (nV <= 0 || nF <= 0 || !V || !F || !matching || !singular || !uv || !n_tedges) return MK_ERR_ARG;
+    return guarded(ctx, [&] {
+        using namespace metriko;
+        Hmesh mesh(toV(nV, V), toF(nV, nF, F));
+        VecXi mt(mesh.nE), sg(nV);
+        for (int e = 0; e < mesh.nE; e++) mt[e] = matching[e];
+        for (int v = 0; v < nV; v++) sg[v] = singular[v];
+        VecXc cfn(3 * nF);
+        for (int c = 0; c < 3 * nF; c++) cfn[c] = complex(uv[2 * c], uv[2 * c + 1]);
+        const MotorcycleGraph g(mesh, cfn, mt, sg);
+        const Tmesh t(g.mcurvs);
+        // ideal lengths: the uv length of each T-edge's segments (example_qgp.cpp:112-123)
+        VecXd R = VecXd::Zero(int(t.nTE));
+        for (int i = 0; i < int(t.nTE); i++)
+            for (const Msgmt& s : t.tedges[size_t(i)].segments()) R[i] += std::abs(s.diff());
+        const VecXd X = compute_quantization(t, R);
+        *n_tedges = int(t.nTE);
+        if (valid) *valid = (validate_quantization(t, X) && compute_validation(t, X)) ? 1 : 0;
+        for (int i = 0; i < int(t.nTE) && i < cap; i++) {
+            if (ideal) ideal[i] = R[i];
+            if (lengths) lengths[i] = X[i];
         }
     });
 }
