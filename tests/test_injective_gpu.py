@@ -97,3 +97,38 @@ def test_gpu_uv_feeds_qex(gpu_ctx):
     assert np.array_equal(got["edges"], np.array(want["edges"], np.int32).reshape(-1, 2))
     assert np.array_equal(got["faces"], np.array(want["faces"], np.int32).reshape(-1, 4))
     assert len(got["q_sid"]) - len(got["edges"]) + len(got["faces"]) == 2
+
+
+def test_gpu_uv_feeds_qgp(gpu_ctx):
+    """example_qgp's flow with the GPU rounding: the naive injective pass (seamless = false) on
+    the GPU, corner uv, then the host motorcycle graph, T-mesh and quantization; the same
+    T-mesh and integer lengths as the oracle's chain on the oracle's uv."""
+    import metriko_amd as mk
+    from oracle.motorcycle import MotorcycleGraph
+    from oracle.quantization import Quantization
+    from oracle.tmesh import Tmesh
+    V, F = icosphere(2)
+    ref = run_pipeline(V, F, ratio=0.1, seamless=False, local_injectivity=True)
+    rp = ref["rp"]
+    ir = rp.ir
+    fullx, ok, dec, st = gpu_ctx.iterative_rounding(
+        ir.fixedIdcs, ir.fixedVals, ir.singularIdcs, ir.integerIdcs, ir.length, ir.Cfull, ir.G2, ir.N, ir.n,
+        ir.nF, ir.F2, seamless=False, round_seams=False, local_injectivity=True,
+        cut_idx=np.asarray(rp.cut.idx), fullx2Nfunc=rp.fullx2Nfunc)
+    assert ok
+    NF = (rp.fullx2Nfunc @ fullx).reshape(-1, ir.N)
+    cut = np.asarray(rp.cut.idx)
+    uv = np.zeros((3 * len(F), 2))
+    for j in range(3):
+        uv[j::3] = NF[cut[:, j], :2]
+    np.testing.assert_allclose(uv, ref["uv"], atol=1e-8, rtol=0)
+    mt = np.asarray(ref["combed_matching"], np.int32)
+    sg = np.asarray(ref["singular"], np.int32)
+    got_t = mk.tmesh_build(V, F, mt, sg, uv)
+    got_q = mk.qgp_quantize(V, F, mt, sg, uv)
+    tm = Tmesh(MotorcycleGraph(ref["mesh"], ref["uv"][:, 0] + 1j * ref["uv"][:, 1], mt, sg))
+    want = Quantization(tm).quantize()
+    print(f"[injective] GPU uv -> QGP: {len(got_t['tquads'])} T-quads, lengths {got_q['lengths'].astype(int).tolist()[:12]}",
+          flush=True)
+    assert got_t["tquads"] == [(list(t), list(s)) for t, s in tm.tquads]
+    assert got_q["valid"] and np.array_equal(got_q["lengths"], want)
