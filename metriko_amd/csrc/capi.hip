@@ -333,7 +333,11 @@ void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
     const int64_t* pp = A->h_ptr.data();
     const int* pc = A->h_col.data();
     const double* pv = A->h_val.data();
-    if (A->h_ptr.empty()) {  // device-assembled operator: export it
+    if (A->h_ptr.empty() && !A->mg_ptr.empty()) {  // device-assembled with a host copy for the set-up
+        pp = A->mg_ptr.data();
+        pc = A->mg_col.data();
+        pv = A->mg_val.data();
+    } else if (A->h_ptr.empty()) {  // device-assembled operator: export it
         const double t0 = now_ms();
         host_csr(c, A, rp, cc, vv);
         MK_LOG("[mg] host CSR of the operator %.0f ms\n", now_ms() - t0);
@@ -344,6 +348,9 @@ void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
     A->mg = A->cplx ? mg_build_complex(c, A->n, pp, pc, pv, o)
                     : mg_build_real(c, A->n, pp, pc, pv, o, A->nnull ? &A->nullvec : nullptr, A->nnull);
     mg_set_labels(A->mg, A->label);
+    std::vector<int64_t>().swap(A->mg_ptr);
+    std::vector<int>().swap(A->mg_col);
+    std::vector<double>().swap(A->mg_val);
 }
 
 void stats_reset(mk_ctx* c) {
@@ -801,6 +808,55 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
             hipLaunchKernelGGL(k_dinv_from_diag, dim3(nblk(nF)), dim3(256), 0, c->stream, nF, C, op->col.p,
                                op->val64.p, op->dinv64.p, op->dinv32.p);
             MK_HIP(hipGetLastError());
+            // the same rows on the host for the multigrid set-up, while the device assembles
+            // (spares the set-up an export of the device operator)
+            if (o.precond == MK_PRECOND_AMG) {
+                std::vector<int64_t>& rp = op->mg_ptr;
+                rp.assign(size_t(nF) + 1, 0);
+#pragma omp parallel for schedule(static, 4096)
+                for (int f = 0; f < nF; f++) {
+                    int m = 1;
+                    for (int k = 0; k < 3; k++) m += half_face[twin[face_halfs[3 * f + k]]] >= 0 ? 1 : 0;
+                    rp[size_t(f) + 1] = m;
+                }
+                for (int f = 0; f < nF; f++) rp[size_t(f) + 1] += rp[size_t(f)];
+                op->mg_col.resize(size_t(rp[size_t(nF)]));
+                op->mg_val.resize(size_t(rp[size_t(nF)]) * 2);
+                const double sgn = (N % 2 == 0) ? -1.0 : 1.0;
+#pragma omp parallel for schedule(static, 4096)
+                for (int f = 0; f < nF; f++) {
+                    int cc[4];
+                    double vr[4], vi[4];
+                    int m = 0;
+                    double diag = 0.0;
+                    for (int k = 0; k < 3; k++) {
+                        const int h = face_halfs[3 * f + k], tw = twin[h], g = half_face[tw];
+                        if (g < 0) continue;
+                        diag += 1.0;
+                        const double a = double(N) * (farg[h] - farg[tw]);
+                        cc[m] = g;
+                        vr[m] = sgn * std::cos(a);
+                        vi[m] = sgn * std::sin(a);
+                        m++;
+                    }
+                    cc[m] = f;
+                    vr[m] = diag + 1e-9;
+                    vi[m] = 0.0;
+                    m++;
+                    for (int i = 1; i < m; i++)
+                        for (int j = i; j > 0 && cc[j - 1] > cc[j]; j--) {
+                            std::swap(cc[j - 1], cc[j]);
+                            std::swap(vr[j - 1], vr[j]);
+                            std::swap(vi[j - 1], vi[j]);
+                        }
+                    const int64_t b = rp[size_t(f)];
+                    for (int j = 0; j < m; j++) {
+                        op->mg_col[size_t(b + j)] = cc[j];
+                        op->mg_val[2 * size_t(b + j)] = vr[j];
+                        op->mg_val[2 * size_t(b + j) + 1] = vi[j];
+                    }
+                }
+            }
             std::vector<int> hc(static_cast<size_t>(nF));
             cnt.download(hc.data(), hc.size(), c->stream);
             MK_HIP(hipStreamSynchronize(c->stream));

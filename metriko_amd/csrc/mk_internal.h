@@ -221,6 +221,11 @@ struct mk_op {
     std::vector<int64_t> h_ptr;
     std::vector<int> h_col;
     std::vector<double> h_val;
+    // host CSR used only by the multigrid set-up when the operator was assembled on the
+    // device (its values may differ from the device's in the last bit; released after use)
+    std::vector<int64_t> mg_ptr;
+    std::vector<int> mg_col;
+    std::vector<double> mg_val;
 };
 
 namespace mk {
