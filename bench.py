@@ -198,7 +198,10 @@ def main():
                 "uv_rel_err": float(np.abs(a["uv"] - b["uv"]).max() / np.abs(b["uv"]).max()),
                 "singular_equal": bool(np.array_equal(a["singular"], b["singular"])),
                 "seam_equal": bool(np.array_equal(a["seam"], b["seam"])),
-                "decisions_equal": a["decisions"] == b["decisions"]},
+                # the same (index, value) decisions; the order in which candidates whose
+                # distances to the grid agree to round-off are fixed may differ
+                "decisions_equal": sorted(a["decisions"]) == sorted(b["decisions"]),
+                "decisions_same_order": a["decisions"] == b["decisions"]},
         }
         result["speedup_mixed_vs_fp64"] = round(result["fp64"]["solve_ms"] / result["value"], 3)
     if not args.no_as_specified and args.level == LEVEL:
