@@ -2,8 +2,8 @@
 
 This package is a plain FP64 restatement of komietty/metriko's numerical core
 (halfedge mesh, Knoeppel smoothest 4-RoSy field, seam/cut/comb, IGM setup,
-Poisson KKT, Gauss-Newton/LM passes and iterative rounding) written with
-numpy/scipy.  It is the checker for the HIP implementation in ``metriko_amd``:
+Poisson KKT, Gauss-Newton/LM passes and iterative rounding, the local-injectivity
+barrier LM, and the QEx / QGP consumers of the uv) written with numpy/scipy.  It is the checker for the HIP implementation in ``metriko_amd``:
 
 * only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
   ``bench.py`` may import it, and only to check or to time a CPU baseline;
@@ -37,4 +37,14 @@ PINNING = {
     "fullpiv_kernel": "C @ K == 0, rank; brute force vs numpy on random ints",
     "kkt": "scipy spsolve on the assembled KKT; dense lstsq on tiny meshes",
     "lm/rounding": "integrality, seamlessness, literal dense normal-equation LM on tiny meshes",
+    "injective (barrier, damped LM)": "barrier closed form and finite differences, Rosenbrock "
+                                      "minimiser, Woodbury vs direct solve, equal to the plain "
+                                      "rounding where the barrier stays inactive",
+    "qex": "torus known answers (s^2 n m quads), Euler characteristic 2 on spheres, all ports "
+           "connected, transition closed form on the torus seams",
+    "motorcycle/tmesh": "one crash and one split per curve, T-quads going round four sides, "
+                        "Euler characteristic 2",
+    "quantization": "opposite T-quad sides equal, validation, weight closed form, heap order "
+                    "vs the comparator; SparseQR row choice and std::sort ties: parity unpinned",
+    "poisson/sizing": "see tests/test_batch_oracle.py and tests/test_sizing_oracle.py",
 }
