@@ -87,6 +87,11 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     const int gw = xcd_block(int(blockIdx.x), int(gridDim.x)) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     V acc;
     vzero(acc);
+    // modes 5 / 6 need the row's own x_in: taken from the gather of the diagonal entry, so the
+    // SELL rows read X once (medium and long rows, a few hundred, read it again at the end)
+    V xd;
+    vzero(xd);
+    bool have_xd = false;
     int row = -1;
     if (PART != 2 && gw < nw_sell) {
         const int r = gw * R + sub;
@@ -103,10 +108,19 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
             vfma(acc, v0, x0);
             vfma(acc, v1, x1);
+            if constexpr (MODE == 5 || MODE == 6) {
+                if (c0 == row) { xd = x0; have_xd = true; }
+                if (c1 == row) { xd = x1; have_xd = true; }
+            }
         }
         if (k < w) {
             const int64_t e = base + int64_t(k) * A.C;
-            vfma(acc, A.val[e], Xv[size_t(A.col[e]) * LPR + j]);
+            const int c0 = A.col[e];
+            const V x0 = Xv[size_t(c0) * LPR + j];
+            vfma(acc, A.val[e], x0);
+            if constexpr (MODE == 5 || MODE == 6) {
+                if (c0 == row) { xd = x0; have_xd = true; }
+            }
         }
     } else {
         if constexpr (PART == 1) return;
@@ -167,7 +181,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
         if constexpr (MODE == 5) d = vaxpy(sd, r, T(0), r);
         else d = vaxpy(sd, r, ca, Dv[o]);
         Dv[o] = d;
-        Yv[o] = vadd(Xv[o], d);
+        Yv[o] = vadd(have_xd ? xd : Xv[o], d);
     }
 }
 
@@ -218,9 +232,12 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
 void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const float* x_in, float* x_out, const float* b, float* d,
                       float ca, float cs, bool first, const int* skip) {
     MK_CHECK(!A->cplx && x_in != x_out, MK_ERR_ARG, "fused block Chebyshev: real operator, x_out != x_in");
+    // algorithmic bytes: the matrix (values, columns, slice / row metadata), the x_in block
+    // once (A->m rows: every column is gathered, the row's own entry among them), B read,
+    // d read (not on the first step), d and x_out written, the inverse diagonal
     const double vb = 4.0 * mw;
     const double bytes = A->bytes32 - double(A->n) * 2.0 * 4.0 + double(A->m) * vb +
-                         double(A->n) * vb * (first ? 4.0 : 5.0) + double(A->n) * 4.0;
+                         double(A->n) * vb * (first ? 3.0 : 4.0) + double(A->n) * 4.0;
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64_cheb" : mw == 32 ? "spmm32_cheb" : mw == 16 ? "spmm16_cheb" :
                  "spmm8_cheb", true);
     const float* dinv = A->dinv32.p;

@@ -624,7 +624,7 @@ mk_status mk_op_destroy(mk_op* op) {
 }
 
 mk_status mk_op_apply(mk_ctx* c, const mk_op* A, const double* x, double* y) {
-    if (!c || !A || !x || !y) return MK_ERR_ARG;
+    if (!c || !A || !x || !y || A->ctx != c) return MK_ERR_ARG;
     return guard(c, [&] {
         MK_HIP(hipSetDevice(c->device));
         const size_t len = size_t(A->n) * (A->cplx ? 2 : 1);
@@ -638,7 +638,7 @@ mk_status mk_op_apply(mk_ctx* c, const mk_op* A, const double* x, double* y) {
 
 mk_status mk_spd_solve(mk_ctx* c, const mk_op* A, const double* b, double* x, const mk_solve_opts* o_,
                        mk_stats* st) {
-    if (!c || !A || !b || !x) return MK_ERR_ARG;
+    if (!c || !A || !b || !x || A->ctx != c) return MK_ERR_ARG;
     mk_status inner = MK_OK;
     mk_status s = guard(c, [&] {
         const mk_solve_opts o = resolve(o_);
@@ -688,7 +688,7 @@ __global__ void k_unpack_block(size_t n, int mw, int cnt, const double* src, dou
 
 mk_status mk_spd_solve_many(mk_ctx* c, const mk_op* A, int m, const double* B, double* X, const mk_solve_opts* o_,
                             mk_stats* st) {
-    if (!c || !A || m < 0 || (m > 0 && (!B || !X))) return MK_ERR_ARG;
+    if (!c || !A || m < 0 || (m > 0 && (!B || !X)) || A->ctx != c) return MK_ERR_ARG;
     mk_status inner = MK_OK;
     mk_status s = guard(c, [&] {
         const mk_solve_opts o = resolve(o_);
@@ -874,7 +874,7 @@ mk_status mk_field_laplacian(mk_ctx* c, int nF, int nH, const int32_t* face_half
 }
 
 mk_status mk_op_export_csr(mk_ctx* c, const mk_op* A, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val) {
-    if (!c || !A || !nnz) return MK_ERR_ARG;
+    if (!c || !A || !nnz || A->ctx != c) return MK_ERR_ARG;
     return guard(c, [&] {
         std::vector<int64_t> rp;
         std::vector<int> cc;
@@ -889,7 +889,7 @@ mk_status mk_op_export_csr(mk_ctx* c, const mk_op* A, int64_t* nnz, int64_t* row
 
 mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, const double* u0, int n_iter,
                                 const mk_solve_opts* o_, double* x_out, mk_stats* st) {
-    if (!c || !L || !mass || !u0 || !x_out || n_iter < 0 || !L->cplx) return MK_ERR_ARG;
+    if (!c || !L || !mass || !u0 || !x_out || n_iter < 0 || !L->cplx || L->ctx != c) return MK_ERR_ARG;
     mk_status inner = MK_OK;
     mk_status s = guard(c, [&] {
         const mk_solve_opts o = resolve(o_);

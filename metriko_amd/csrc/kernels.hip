@@ -996,16 +996,29 @@ static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, 
 
 // Runs `batch` calls of body() on the context stream: eagerly, or -- when graphs are on, timing
 // is off and batch is the operator's usual check interval -- by replaying a hipGraph captured
-// from the same calls the first time this key (buffers, mode, batch) is seen.  body() must
-// only launch kernels on c->stream with arguments fixed by the key.
+// from the same calls the first time this key is seen.  body() must only launch kernels on
+// c->stream whose arguments are fixed by the key: the context (its id and its partials, state
+// and scalar buffers), the hierarchy (its id: its level buffers live as long as it does) and
+// the solve's vectors and value arrays (`buffers`).  The graphs are cached on the operator, so
+// an operator is only ever run on the context it was created on.
 template <typename F>
-static void run_batch(mk_ctx* c, const mk_op* A_, std::vector<uintptr_t> key, int batch, bool graphable, F&& body) {
+static void run_batch(mk_ctx* c, const mk_op* A_, const Mg* mg, std::vector<uintptr_t> buffers, int batch,
+                      bool graphable, F&& body) {
     mk_op* A = const_cast<mk_op*>(A_);
+    MK_CHECK(A->ctx == c, MK_ERR_ARG, "operator used with a context other than the one it was created on");
     if (!c->use_graphs || c->time_spmv || !graphable) {
         for (int b = 0; b < batch; b++) body();
         return;
     }
-    key.push_back(uintptr_t(batch));
+    GraphKeyParts kp;
+    kp.ctx_uid = c->uid;
+    kp.mg_uid = mg_uid(mg);
+    kp.state = uintptr_t(c->state.p);
+    kp.partials = uintptr_t(c->partials.p);
+    kp.scal = uintptr_t(c->scal.p);
+    kp.buffers = std::move(buffers);
+    kp.batch = batch;
+    const std::vector<uintptr_t> key = make_graph_key(kp);
     mk_op::Graph* G = nullptr;
     for (auto& g : A->graphs) if (g.key == key) { G = &g; break; }
     if (!G) {
@@ -1105,8 +1118,8 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
             timing_flush(c);
             if (h.done || h.it >= maxiter) break;
             const int batch = std::min(check_every, maxiter - h.it);
-            run_batch(c, A, {uintptr_t(sizeof(T)), uintptr_t(mg), uintptr_t(val), uintptr_t(x), uintptr_t(r),
-                             uintptr_t(z), uintptr_t(p), uintptr_t(q)},
+            run_batch(c, A, mg, {uintptr_t(sizeof(T)), uintptr_t(val), uintptr_t(x), uintptr_t(r),
+                                 uintptr_t(z), uintptr_t(p), uintptr_t(q)},
                       batch, batch == check_every, [&] { cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes); });
             MK_HIP(hipGetLastError());
         }
@@ -1222,8 +1235,9 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         timing_flush(c);
         if (h.done || h.it >= maxiter) break;
         const int batch = std::min(check_every, maxiter - h.it);
-        run_batch(c, A, {uintptr_t(4), uintptr_t(mg), uintptr_t(x), uintptr_t(w.r.p), uintptr_t(w.r32.p),
-                         uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p), uintptr_t(w.q.p)},
+        run_batch(c, A, mg, {uintptr_t(4), uintptr_t(A->val64.p), uintptr_t(x), uintptr_t(w.r.p),
+                             uintptr_t(w.r32.p), uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p),
+                             uintptr_t(w.q.p)},
                   batch, batch == check_every, iteration);
         MK_HIP(hipGetLastError());
     }

@@ -39,6 +39,7 @@ struct MgLevel {
 };
 
 struct Mg {
+    const uint64_t uid = next_uid();  // keys the PCG graphs captured through this hierarchy
     bool cplx = false;
     std::vector<MgLevel> lv;
     int nc = 0;
@@ -1093,6 +1094,7 @@ void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const 
     vcycle_block(c, g, 0, mw, sh, b, x, skip);
 }
 int mg_levels(const Mg* g) { return int(g->lv.size()); }
+uint64_t mg_uid(const Mg* g) { return g ? g->uid : 0; }
 void mg_set_labels(Mg* g, const std::string& base) {
     for (size_t l = 0; l < g->lv.size(); l++) {
         const std::string s = base + "_mg" + std::to_string(l);

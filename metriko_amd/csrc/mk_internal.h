@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include "metriko_hip.h"
+#include "graph_key.h"
 
 namespace mk {
 
@@ -114,6 +115,8 @@ void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
 // V-cycle on an n x mw row-major block (FP32)
 void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const int* skip);
 int mg_levels(const Mg* g);
+// process-unique id of a hierarchy (never reused after it is freed; keys captured graphs)
+uint64_t mg_uid(const Mg* g);
 void mg_set_labels(Mg* g, const std::string& base);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
 void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
@@ -135,6 +138,8 @@ struct mk_inj_check_data {
 };
 
 struct mk_ctx {
+    // process-unique id (never reused after mk_finalize; keys captured graphs)
+    const uint64_t uid = mk::next_uid();
     mk::Work work;
     int device = 0;
     hipStream_t stream = nullptr;

@@ -162,3 +162,26 @@ def test_graph_replay_matches_eager(gpu_ctx, precision):
     finally:
         gpu_ctx.set_graphs(True)
     assert np.linalg.norm(A @ eager[0] - rhs[0]) / np.linalg.norm(rhs[0]) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_operator_rejects_foreign_context(gpu_ctx):
+    """Captured graphs are cached on the operator and hold its context's buffers, so an operator
+    is only run on the context it was created on: another context gets MK_ERR_ARG, and the
+    owning context still solves (with graphs on) afterwards."""
+    nx = 40
+    T = sp.diags([-np.ones(nx - 1), 2.0 * np.ones(nx), -np.ones(nx - 1)], [-1, 0, 1])
+    A = (sp.kron(T, sp.identity(nx)) + sp.kron(sp.identity(nx), T) + 1e-3 * sp.identity(nx * nx)).tocsr()
+    op = gpu_ctx.op_real(A)
+    b = np.random.default_rng(1).standard_normal(A.shape[0])
+    x0, _ = op.solve(b)
+    with mk.Context(0) as other:
+        foreign = mk.Op(other, op.h, op.n, op.cplx)
+        try:
+            with pytest.raises(mk.MetrikoError) as ei:
+                foreign.solve(b)
+            assert ei.value.status == 1
+        finally:
+            foreign.h = None  # the handle belongs to gpu_ctx's operator
+    x1, _ = op.solve(b)
+    assert np.array_equal(x0, x1)
