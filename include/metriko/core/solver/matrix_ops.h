@@ -2,55 +2,114 @@
 //
 // Drop-in for the reference's core/solver/matrix_ops.h plus the Eigen FullPivLU
 // kernel that iter_rounding_init.h:129 calls.
+//
+// Restated from komietty/metriko include/metriko/core/solver/matrix_ops.h
+// (Copyright (C) 2021 Amir Vaxman, Copyright (C) 2025 Saki Komikado), which is subject to
+// the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_SOLVER_COMMON_H
 #define METRIKO_SOLVER_COMMON_H
 #include <algorithm>
 #include <cmath>
 #include <limits>
 #include "metriko/core/common/sparse.h"
+#include "metriko/core/solver/colamd.h"
 
 namespace metriko {
-    // Deterministic maximal linearly independent row subset, kept in row order: row i
-    // stays when its residual against the kept rows (two Gram-Schmidt passes) exceeds
-    // eps*1e3*ncols*||row||.  Stands in for reduce_to_linearly_independent
-    // (matrix_ops.h:12-28), whose row choice follows Eigen SparseQR<COLAMD>'s column
-    // permutation; that ordering is not restated (the kept subset spans the same row
-    // space, which is all the downstream kernel basis depends on up to its pivoting).
+    // reduce_to_linearly_independent (matrix_ops.h:12-28): the reference factorises C^T with
+    // Eigen::SparseQR<SprsD, COLAMDOrdering<int>> and keeps the rows of C named by the first
+    // rank entries of colsPermutation(), in that order.  SparseQR visits the columns of C^T
+    // (the rows of C) in COLAMD's order; a left-looking Householder step whose norm |beta| --
+    // the distance of the column to the span of the columns accepted before it -- is below
+    // 20 (m + n) max_j ||C^T(:,j)|| eps moves the column behind the accepted ones.  So the kept
+    // rows are the greedy independent subset of the COLAMD order, in that order.  Here the
+    // distances are computed per connected component of the constraint graph (rows sharing a
+    // column; disjoint supports cannot be dependent), by Householder steps on the component's
+    // dense block, with the global threshold.  Explicit zeros (from the structural product that
+    // built C) stay in the pattern: COLAMD sees them and so do the kept rows.
     inline void reduce_to_linearly_independent(SprsD& mat) {
         if (mat.rows() == 0) return;
-        std::vector<int> cols;
-        for (int64_t k = 0; k < mat.nonZeros(); k++) cols.push_back(mat.col[k]);
-        std::sort(cols.begin(), cols.end());
-        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-        const int nc = int(cols.size());
-        std::vector<int> cmap(size_t(mat.cols()), -1);
-        for (int j = 0; j < nc; j++) cmap[cols[j]] = j;
-        const double tol = std::numeric_limits<double>::epsilon() * 1e3 * std::max(nc, 1);
-        std::vector<std::vector<double>> Q;
-        std::vector<int> keep;
-        std::vector<double> r(static_cast<size_t>(nc));
-        for (int i = 0; i < mat.rows(); i++) {
-            std::fill(r.begin(), r.end(), 0.0);
-            for (int64_t k = mat.ptr[i]; k < mat.ptr[i + 1]; k++) r[cmap[mat.col[k]]] = mat.val[k];
-            double nrm = 0;
-            for (double x : r) nrm += x * x;
-            nrm = std::sqrt(nrm);
-            if (nrm == 0.0) continue;
-            for (int pass = 0; pass < 2; pass++)
-                for (auto& q : Q) {
-                    double d = 0;
-                    for (int j = 0; j < nc; j++) d += q[j] * r[j];
-                    for (int j = 0; j < nc; j++) r[j] -= d * q[j];
-                }
-            double rn = 0;
-            for (double x : r) rn += x * x;
-            rn = std::sqrt(rn);
-            if (rn > tol * nrm) {
-                for (double& x : r) x /= rn;
-                Q.push_back(r);
-                keep.push_back(i);
-            }
+        const int nr = mat.rows(), nc = mat.cols();
+        // C^T in compressed-column form is C in compressed-row form
+        const std::vector<int> p = colamd::order(nc, nr, mat.ptr, mat.col);
+        double max2 = 0.0;
+        for (int i = 0; i < nr; i++) {
+            double s2 = 0.0;
+            for (int64_t k = mat.ptr[i]; k < mat.ptr[i + 1]; k++) s2 += mat.val[k] * mat.val[k];
+            max2 = std::max(max2, std::sqrt(s2));
         }
+        if (max2 == 0.0) max2 = 1.0;
+        const double thr = 20.0 * (double(nc) + double(nr)) * max2 * std::numeric_limits<double>::epsilon();
+        // components: union-find over the columns each row touches
+        std::vector<int> uf(static_cast<size_t>(nc));
+        for (int j = 0; j < nc; j++) uf[j] = j;
+        auto find = [&](int x) { while (uf[x] != x) { uf[x] = uf[uf[x]]; x = uf[x]; } return x; };
+        for (int i = 0; i < nr; i++)
+            for (int64_t k = mat.ptr[i] + 1; k < mat.ptr[i + 1]; k++) {
+                const int a = find(mat.col[mat.ptr[i]]), b = find(mat.col[k]);
+                if (a != b) uf[std::max(a, b)] = std::min(a, b);
+            }
+        // per component: its rows in COLAMD order, its columns ascending
+        std::vector<int> comp_of_row(static_cast<size_t>(nr), -1), comp_id(static_cast<size_t>(nc), -1);
+        int ncomp = 0;
+        std::vector<std::vector<int>> crows, ccols;
+        for (int t = 0; t < nr; t++) {
+            const int i = p[t];
+            if (mat.ptr[i] == mat.ptr[i + 1]) continue;  // an empty row is never kept
+            const int root = find(mat.col[mat.ptr[i]]);
+            if (comp_id[root] < 0) { comp_id[root] = ncomp++; crows.emplace_back(); ccols.emplace_back(); }
+            comp_of_row[i] = comp_id[root];
+            crows[comp_id[root]].push_back(i);
+        }
+        for (int j = 0; j < nc; j++) {
+            const int r = find(j);
+            if (comp_id[r] >= 0) ccols[comp_id[r]].push_back(j);
+        }
+        std::vector<char> accepted(static_cast<size_t>(nr), 0);
+        std::vector<int> local(static_cast<size_t>(nc), -1);
+        for (int g = 0; g < ncomp; g++) {
+            const std::vector<int>& cols = ccols[g];
+            for (size_t a = 0; a < cols.size(); a++) local[cols[a]] = int(a);
+            const int m = int(cols.size());
+            std::vector<std::vector<double>> H;  // accepted Householder vectors (unit entry at its step)
+            std::vector<double> tau;
+            std::vector<double> v(static_cast<size_t>(m));
+            for (int i : crows[g]) {
+                std::fill(v.begin(), v.end(), 0.0);
+                for (int64_t k = mat.ptr[i]; k < mat.ptr[i + 1]; k++) v[local[mat.col[k]]] = mat.val[k];
+                for (size_t h = 0; h < H.size(); h++) {
+                    double d = 0.0;
+                    for (int a = 0; a < m; a++) d += H[h][a] * v[a];
+                    d *= tau[h];
+                    for (int a = 0; a < m; a++) v[a] -= d * H[h][a];
+                }
+                const int nz = int(H.size());
+                if (nz >= m) continue;
+                const double c0 = v[nz];
+                double sq = 0.0;
+                for (int a = nz + 1; a < m; a++) sq += v[a] * v[a];
+                double beta, t;
+                std::vector<double> hv(static_cast<size_t>(m), 0.0);
+                hv[nz] = 1.0;
+                if (sq == 0.0) {
+                    beta = c0;
+                    t = 0.0;
+                } else {
+                    beta = std::sqrt(c0 * c0 + sq);
+                    if (c0 >= 0.0) beta = -beta;
+                    for (int a = nz + 1; a < m; a++) hv[a] = v[a] / (c0 - beta);
+                    t = (beta - c0) / beta;
+                }
+                if (std::abs(beta) >= thr) {
+                    H.push_back(std::move(hv));
+                    tau.push_back(t);
+                    accepted[i] = 1;
+                }
+            }
+            for (int j : cols) local[j] = -1;
+        }
+        std::vector<int> keep;
+        for (int t = 0; t < nr; t++)
+            if (accepted[p[t]]) keep.push_back(p[t]);
         mat = mat.rows_of(keep);
     }
 

@@ -1411,7 +1411,8 @@ extern "C" mk_status mk_iterative_rounding(
             Cf.ptr.assign(c_ptr, c_ptr + c_rows + 1);
             Cf.col.assign(c_col, c_col + c_ptr[c_rows]);
             Cf.val.assign(c_val, c_val + c_ptr[c_rows]);
-            Cf.prune();  // structural zeros from the products carry no constraint
+            // explicit zeros stay: the reference collects the participating columns from every
+            // stored entry of Cfull (iter_rounding_init.h:98-110), zeros of its products included
         }
         std::vector<int> uniqueJ;
         for (int64_t k = 0; k < Cf.nonZeros(); k++) uniqueJ.push_back(Cf.col[k]);

@@ -32,8 +32,12 @@ PINNING = {
     "compute_seam": "cut-graph invariants (disk topology, singulars on seam); "
                     "exact edge set vs libigl: parity unpinned",
     "cut/comb/setup": "transition consistency, constraint rows vanish on x0",
-    "reduce_to_linearly_independent": "rank == numpy matrix_rank; row choice vs "
-                                      "Eigen SparseQR<COLAMD>: parity unpinned",
+    "reduce_to_linearly_independent": "COLAMD (oracle/colamd.py) == SuperLU's COLAMD through "
+                                      "scipy splu's column permutation on random matrices, dense "
+                                      "rows under SuperLU's threshold rule, hand-worked orderings; "
+                                      "SparseQR rank decisions == brute-force rank growth and the "
+                                      "|beta| threshold; Eigen's 0.5 dense fractions stated from "
+                                      "Eigen 3.4's Eigen_Colamd.h (tests/test_colamd.py)",
     "fullpiv_kernel": "C @ K == 0, rank; brute force vs numpy on random ints",
     "kkt": "scipy spsolve on the assembled KKT; dense lstsq on tiny meshes",
     "lm/rounding": "integrality, seamlessness, literal dense normal-equation LM on tiny meshes",
@@ -45,6 +49,7 @@ PINNING = {
     "motorcycle/tmesh": "one crash and one split per curve, T-quads going round four sides, "
                         "Euler characteristic 2",
     "quantization": "opposite T-quad sides equal, validation, weight closed form, heap order "
-                    "vs the comparator; SparseQR row choice and std::sort ties: parity unpinned",
+                    "vs the comparator; generator rows by the SparseQR<COLAMD> rule above; "
+                    "std::sort ties: parity unpinned",
     "poisson/sizing": "see tests/test_batch_oracle.py and tests/test_sizing_oracle.py",
 }

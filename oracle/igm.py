@@ -30,7 +30,8 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
-from .linalg import fullpiv_lu_kernel, independent_rows
+from .colamd import reduce_to_linearly_independent as _reduce_colamd, structural_product
+from .linalg import fullpiv_lu_kernel
 
 
 def _prune_zeros(A):
@@ -244,7 +245,7 @@ class RosyParameterization:
         """integ.ipp:3-120 (with the fixed example_igm flags)."""
         raw, cut, N, n = self.raw, self.cut, self.N, self.n
         self.fullx2Nfunc = (self.vtrans2cut @ self.uncompress).tocsr()
-        Cfull = (self.constraint @ self.uncompress).tocsr()
+        Cfull = structural_product(self.constraint, self.uncompress)
         Cfull = reduce_to_linearly_independent(Cfull)
         self.Cfull = Cfull
         G2, G3 = self.gradient_operators()
@@ -279,14 +280,11 @@ class RosyParameterization:
 
 
 def reduce_to_linearly_independent(C):
-    """``reduce_to_linearly_independent`` (matrix_ops.h:12-28): keep a maximal independent
-    row subset (see ``linalg.independent_rows`` for the documented row choice)."""
+    """``reduce_to_linearly_independent`` (matrix_ops.h:12-28): the rows SparseQR<COLAMD> on
+    C^T keeps, in its order (``oracle.colamd``); explicit zeros stay in the pattern."""
     if C.shape[0] == 0:
         return C
-    cols = np.unique(C.indices)
-    Cd = C[:, cols].toarray()
-    keep = independent_rows(Cd)
-    return C[keep, :].tocsr()
+    return _reduce_colamd(C)[1]
 
 
 class IterativeRounding:
@@ -307,8 +305,7 @@ class IterativeRounding:
 
     # -- iter_rounding_init.h:74-232 ---------------------------------------------------
     def _naive_setup(self):
-        C = self.Cfull.tocsr()
-        C.eliminate_zeros()
+        C = self.Cfull.tocsr()   # explicit zeros stay: their columns take part (iter_rounding_init.h:98-110)
         ncols = C.shape[1]
         uniqueJ = np.unique(C.indices) if C.nnz else np.zeros(0, dtype=np.int64)
         nonPart = np.setdiff1d(np.arange(ncols), uniqueJ)

@@ -79,35 +79,3 @@ def fullpiv_lu_kernel(A):
     for k in range(dimker):
         K[colq[rank + k], k] = 1.0
     return K, rank
-
-
-def independent_rows(Cd, tol_scale=1e3):
-    """Deterministic maximal linearly independent row subset, in row order.
-
-    Stands in for ``reduce_to_linearly_independent`` (solver/matrix_ops.h:12-28),
-    which keeps the first ``rank`` columns of Eigen ``SparseQR<COLAMD>`` on C^T
-    in that permutation's order.  COLAMD's ordering is not restated: this
-    greedy Gram-Schmidt (keep row i if its residual against the kept rows
-    exceeds tol*||row||, tol = eps*tol_scale*ncols) is the build's documented
-    choice and is shared bit-for-bit with the HIP path's host code.  Parity of
-    the selected subset against Eigen: unpinned.
-    """
-    Cd = np.asarray(Cd, dtype=np.float64)
-    nr, nc = Cd.shape
-    tol = np.finfo(np.float64).eps * tol_scale * max(nc, 1)
-    Q = []
-    keep = []
-    for i in range(nr):
-        r = Cd[i].copy()
-        nrm = np.sqrt(r @ r)
-        if nrm == 0.0:
-            continue
-        for q in Q:
-            r -= (q @ r) * q
-        for q in Q:  # second Gram-Schmidt pass for stability
-            r -= (q @ r) * q
-        rn = np.sqrt(r @ r)
-        if rn > tol * nrm:
-            Q.append(r / rn)
-            keep.append(i)
-    return keep

@@ -9,8 +9,8 @@ Test infrastructure only (see ``oracle/__init__.py``).  Restates the reference's
   ``std::priority_queue``; its pop order among entries the comparator does not separate is
   libstdc++'s heap order, so the queue here is a restatement of libstdc++'s
   ``push_heap`` / ``pop_heap`` (sift-up; hole to a leaf along the larger child, then sift-up);
-* ``construct_generating_vectors`` (quantization_evaluation.h) with the build's documented
-  row choice for ``reduce_to_linearly_independent`` (``oracle.linalg.independent_rows``);
+* ``construct_generating_vectors`` (quantization_evaluation.h) with the reference's row
+  choice for ``reduce_to_linearly_independent`` (SparseQR<COLAMD>, ``oracle.colamd``);
 * ``compute_quantization`` (quantization.h:28-122), with ties of the weight sort kept in
   (edge, sign) order, and the ideal lengths R of example_qgp.cpp:112-123.
 """
@@ -18,7 +18,9 @@ import math
 
 import numpy as np
 
-from .linalg import independent_rows
+import scipy.sparse as sp
+
+from .colamd import reduce_to_linearly_independent
 from .motorcycle import HITB, HITL, HITR, NONE
 
 
@@ -195,8 +197,7 @@ class Quantization:
                 r[u // 2] += 1.0
             rows.append(r)
         G = np.array(rows)
-        cols = np.unique(np.nonzero(G)[1])          # reduce_to_linearly_independent on the used columns
-        keep = independent_rows(G[:, cols])
+        keep, _ = reduce_to_linearly_independent(sp.csr_matrix(G))   # quantization_evaluation.h:86
         return G[keep].T            # generators as columns
 
     def quantize(self):

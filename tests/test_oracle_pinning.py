@@ -16,7 +16,7 @@ import scipy.sparse.linalg as spla
 from oracle.field import (FaceRosyField, GlibcRand, compute_connection, connection_laplacian, curl_matching,
                           eigen_random_complex, field_roots, solve_smallest_eig, compute_singular)
 from oracle.hmesh import Hmesh, edge_topology
-from oracle.linalg import fullpiv_lu_kernel, independent_rows
+from oracle.linalg import fullpiv_lu_kernel
 from oracle.pipeline import run_pipeline
 from oracle.seam import compute_cut_mesh, compute_seam
 from tests.helpers import mesh
@@ -261,15 +261,6 @@ def test_fullpiv_kernel(seed):
     assert np.linalg.matrix_rank(K) == c - rk
     # identity rows on the free columns
     assert np.sum(np.isclose(np.abs(K).max(axis=1), 1.0) & (np.count_nonzero(K, axis=1) == 1)) >= c - rk
-
-
-def test_independent_rows():
-    g = np.random.default_rng(3)
-    B = g.integers(-1, 2, (5, 9)).astype(float)
-    A = np.vstack([B, B[0] + B[1], 2 * B[3], B[2] - B[4]])
-    keep = independent_rows(A)
-    assert len(keep) == np.linalg.matrix_rank(A)
-    assert keep == list(range(np.linalg.matrix_rank(B)))
 
 
 # ---- IGM -------------------------------------------------------------------------------------
