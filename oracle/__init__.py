@@ -29,8 +29,10 @@ PINNING = {
                           "dense power iteration on tiny meshes",
     "field_roots": "closed form",
     "matching/singular": "Poincare-Hopf (sum = N*chi), rotation invariance",
-    "compute_seam": "cut-graph invariants (disk topology, singulars on seam); "
-                    "exact edge set vs libigl: parity unpinned",
+    "compute_seam": "cut-graph invariants (disk topology, singulars on seam); hand-worked "
+                    "octahedron cuts; MeshCutter == 2-core of the networkx BFS dual co-tree; "
+                    "Dijkstra == smallest (distance, id) target with smallest-id predecessors "
+                    "(tests/test_seam_pins.py)",
     "cut/comb/setup": "transition consistency, constraint rows vanish on x0",
     "reduce_to_linearly_independent": "COLAMD (oracle/colamd.py) == SuperLU's COLAMD through "
                                       "scipy splu's column permutation on random matrices, dense "
@@ -38,7 +40,10 @@ PINNING = {
                                       "SparseQR rank decisions == brute-force rank growth and the "
                                       "|beta| threshold; Eigen's 0.5 dense fractions stated from "
                                       "Eigen 3.4's Eigen_Colamd.h (tests/test_colamd.py)",
-    "fullpiv_kernel": "C @ K == 0, rank; brute force vs numpy on random ints",
+    "fullpiv_kernel": "pivot sequence and factors == LAPACK dgetc2 (complete pivoting) on "
+                      "tie-free matrices; hand-worked kernels with forced ties; reduced basis "
+                      "(identity on the free columns, direct solve on the pivot columns); "
+                      "C @ K == 0, rank (tests/test_fullpiv.py)",
     "kkt": "scipy spsolve on the assembled KKT; dense lstsq on tiny meshes",
     "lm/rounding": "integrality, seamlessness, literal dense normal-equation LM on tiny meshes",
     "injective (barrier, damped LM)": "barrier closed form and finite differences, Rosenbrock "

@@ -17,11 +17,22 @@ def fullpiv_lu_kernel(A):
     an identity on the free columns.  A zero column is returned when the
     kernel is trivial (Eigen returns max(1, dimker) columns).
     """
+    f = fullpiv_lu(A)
+    return _kernel_from_lu(f)
+
+
+def fullpiv_lu(A):
+    """Eigen 3.4 ``FullPivLU::computeInPlace``: returns the packed LU, the row and column
+    transpositions of every step (k <-> row_t[k], k <-> col_t[k]; identity after the step at
+    which the trailing block is exactly zero), the number of nonzero pivots, the largest
+    pivot magnitude and Q (Q[i] = the original column at position i)."""
     A = np.array(A, dtype=np.float64, copy=True)
     rows, cols = A.shape
     size = min(rows, cols)
     rowp = np.arange(rows)
     colq = np.arange(cols)
+    row_t = np.arange(size)
+    col_t = np.arange(size)
     nonzero_pivots = size
     max_pivot = 0.0
     lu = A
@@ -38,6 +49,7 @@ def fullpiv_lu_kernel(A):
             break
         if biggest > max_pivot:
             max_pivot = biggest
+        row_t[k], col_t[k] = row_big, col_big
         if k != row_big:
             lu[[k, row_big], :] = lu[[row_big, k], :]
             rowp[[k, row_big]] = rowp[[row_big, k]]
@@ -48,6 +60,15 @@ def fullpiv_lu_kernel(A):
             lu[k + 1:, k] /= lu[k, k]
         if k < size - 1:
             lu[k + 1:, k + 1:] -= np.outer(lu[k + 1:, k], lu[k, k + 1:])
+    return dict(lu=lu, row_t=row_t, col_t=col_t, nonzero_pivots=nonzero_pivots, max_pivot=max_pivot, q=colq)
+
+
+def _kernel_from_lu(f):
+    """Eigen 3.4 ``FullPivLU::kernel()`` from the factorisation of ``fullpiv_lu``."""
+    lu, colq = f["lu"], f["q"]
+    nonzero_pivots, max_pivot = f["nonzero_pivots"], f["max_pivot"]
+    rows, cols = lu.shape
+    size = min(rows, cols)
     threshold = np.finfo(np.float64).eps * size
     pre = max_pivot * threshold
     piv = [i for i in range(nonzero_pivots) if abs(lu[i, i]) > pre]

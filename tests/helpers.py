@@ -43,3 +43,28 @@ def cut_hash(idx):
     """Order-sensitive checksum of the cut-mesh corner indices."""
     w = (np.arange(idx.size, dtype=np.uint64) * np.uint64(2654435761) + np.uint64(1)) & np.uint64((1 << 40) - 1)
     return int((w * np.asarray(idx).reshape(-1).astype(np.uint64)).sum() & np.uint64((1 << 61) - 1))
+
+
+def assert_decisions_in_order(dec_gpu, dec_ref, tie=1e-6):
+    """Rounding decisions (index, value[, margin]) in the reference's order.
+
+    A decision whose reference margin exceeds `tie` is followed by a solve: it must sit at the
+    same position with the same (index, value).  Decisions within `tie` of an integer need no
+    solve and tie with each other at round-off level; each maximal run of them must hold the
+    same (index, value) pairs in the same positions of the sequence, in any order inside the run.
+    """
+    got = [tuple(int(x) for x in d[:2]) for d in dec_gpu]
+    ref = [tuple(int(x) for x in d[:2]) for d in dec_ref]
+    margin = [float(d[2]) for d in dec_ref]
+    assert len(got) == len(ref), (len(got), len(ref))
+    i = 0
+    while i < len(ref):
+        if margin[i] > tie:
+            assert got[i] == ref[i], (i, got[i], ref[i], margin[i])
+            i += 1
+            continue
+        j = i
+        while j < len(ref) and margin[j] <= tie:
+            j += 1
+        assert sorted(got[i:j]) == sorted(ref[i:j]), (i, j, got[i:j], ref[i:j])
+        i = j

@@ -8,7 +8,7 @@ import pytest
 import metriko_amd as mk
 from oracle.field import eigen_random_complex
 from oracle.pipeline import run_pipeline
-from tests.helpers import GOLDEN, mesh, phase_error
+from tests.helpers import GOLDEN, assert_decisions_in_order, mesh, phase_error
 
 
 @pytest.fixture(scope="module")
@@ -44,7 +44,8 @@ def test_gpu_pipeline_reproduces_golden(gpu_ctx, golden):
     assert phase_error(out["compressed"], golden["compressed"]) < 1e-8
     assert np.array_equal(out["singular"], golden["singular"])
     assert np.array_equal(np.nonzero(out["seam"])[0], golden["seam_edges"])
-    assert sorted(map(tuple, out["decisions"])) == sorted(map(tuple, golden["decisions"].tolist()))
+    ref = [(i, v, m) for (i, v), m in zip(golden["decisions"].tolist(), golden["decision_margin"].tolist())]
+    assert_decisions_in_order(out["decisions"], ref)
     assert np.abs(out["uv"] - golden["uv"]).max() / np.abs(golden["uv"]).max() < 1e-8
 
 
@@ -67,9 +68,7 @@ def test_gpu_pipeline_torus_204800(gpu_ctx, precision):
     assert np.array_equal(np.nonzero(out["seam"])[0], g["seam_edges"])
     assert out["info"]["cutV"] == int(g["cutV"]) and cut_hash(out["cut_idx"]) == int(g["cut_hash"])
     assert (out["info"]["nS"], out["info"]["nT"]) == (int(g["nS"]), int(g["nT"]))
-    assert sorted(map(tuple, out["decisions"])) == sorted(map(tuple, g["decisions"].tolist()))
-    firm = g["decision_margin"] > 1e-6  # decisions after a solve keep their order
-    got = [tuple(d) for d in out["decisions"]]
-    assert [t for t, f in zip(got, firm) if f] == [tuple(d) for d, f in zip(g["decisions"].tolist(), firm) if f]
+    ref = [(i, v, m) for (i, v), m in zip(g["decisions"].tolist(), g["decision_margin"].tolist())]
+    assert_decisions_in_order(out["decisions"], ref)
     uv = np.asarray(out["uv"]).reshape(-1, 2)[g["uv_corners"]]
     assert np.abs(uv - g["uv"]).max() / np.abs(g["uv"]).max() < 1e-8

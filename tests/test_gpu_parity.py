@@ -14,7 +14,7 @@ import metriko_amd as mk
 from oracle.field import connection_laplacian, eigen_random_complex, solve_smallest_eig
 from oracle.hmesh import Hmesh
 from oracle.pipeline import run_pipeline
-from tests.helpers import mesh, phase_error
+from tests.helpers import assert_decisions_in_order, mesh, phase_error
 
 pytestmark = pytest.mark.gpu
 
@@ -34,6 +34,22 @@ def test_field_laplacian_device_assembly(gpu_ctx, name):
     assert np.abs(d.data).max() < 1e-13
     # Hermitian, |offdiag| = 1
     assert abs(Lg - Lg.getH()).max() < 1e-14
+
+
+@pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED, mk.PREC_MIXED_MG])
+@pytest.mark.parametrize("name", ["ico2", "icosphere_obj", "torus_small"])
+def test_smallest_eig_matches_oracle(gpu_ctx, name, precision):
+    V, F = mesh(name)
+    m = Hmesh(V, F)
+    he = mk.hmesh_halfedges(V, F)
+    Lop = gpu_ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
+    u0 = eigen_random_complex(m.nF)
+    x, st = gpu_ctx.smallest_eig(Lop, he["face_area"], u0, 50, mk.opts(precision=precision))
+    ref = solve_smallest_eig(connection_laplacian(m, 4), m.faceArea, u0, 50)
+    err = phase_error(x, ref)
+    print(name, precision, "field err", err, st)
+    assert err < FIELD_TOL
+    assert st["rel_residual"] <= 1e-10
 
 
 @pytest.mark.parametrize("precision", [mk.PREC_FP64, mk.PREC_MIXED, mk.PREC_MIXED_MG])
@@ -78,9 +94,7 @@ def test_pipeline_matches_oracle(gpu_ctx, name, precision):
     assert np.array_equal(out["matching"], ref["combed_matching"])
     assert np.array_equal(out["cut_idx"], ref["cut"].idx)
     assert info["nS"] == ref["nS"] and info["nT"] == ref["nT"]
-    assert sorted(out["decisions"]) == sorted((i, v) for (i, v, _) in ref["decisions"])
-    a, b = _ordered_real_decisions(out["decisions"], ref["decisions"])
-    assert a == b
+    assert_decisions_in_order(out["decisions"], ref["decisions"])
     assert info["success"] == int(ref["success"])
     uv_err = np.abs(out["uv"] - ref["uv"]).max() / np.abs(ref["uv"]).max()
     print("uv rel err", uv_err)

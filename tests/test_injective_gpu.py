@@ -17,6 +17,7 @@ import pytest
 
 from metriko_amd.meshgen import icosphere, bumpy_icosphere
 from oracle.pipeline import run_pipeline
+from tests.helpers import assert_decisions_in_order
 
 pytestmark = pytest.mark.gpu
 
@@ -42,7 +43,7 @@ def test_injective_without_callback_matches_oracle(gpu_ctx):
         ir.nF, ir.F2, seamless=True, round_seams=False, local_injectivity=True)
     print(f"[injective] no callback: max |dx| {np.abs(fullx - ir.fullx).max():.3e}", flush=True)
     assert ok == ok_ref
-    assert sorted(d[:2] for d in dec) == sorted(tuple(d[:2]) for d in ir.decisions)
+    assert_decisions_in_order(dec, ir.decisions)
     np.testing.assert_allclose(fullx, ir.fullx, atol=1e-8, rtol=0)
 
 
@@ -64,7 +65,7 @@ def test_injective_rounding_matches_oracle(gpu_ctx, case):
           f"max |dx| {np.abs(fullx - ref['fullx']).max():.3e}", flush=True)
     assert ok == ref["success"]
     if seamless:
-        assert sorted(d[:2] for d in dec) == sorted(tuple(d[:2]) for d in ref["decisions"])
+        assert_decisions_in_order(dec, ref["decisions"])
     np.testing.assert_allclose(fullx, ref["fullx"], atol=TOL.get(case, 1e-8), rtol=0)
 
 
