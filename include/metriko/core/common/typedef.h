@@ -4,6 +4,9 @@
 // MatXi, VecXd, VecXc, SprsD, SprsC, ...) with small owning containers so the
 // headers build without Eigen.  Dense matrices are row-major; element access
 // is m(i, j) as in Eigen.
+//
+// Restated from komietty/metriko include/metriko/core/common/typedef.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_TYPEDEF_H
 #define METRIKO_TYPEDEF_H
 #include <cassert>

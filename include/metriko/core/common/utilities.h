@@ -5,6 +5,9 @@
 // calc_coefficient, find_extended_intersection, find_strict_intersection).  The 2x2
 // solves are written as Eigen's 2x2 inverse evaluates them (cofactors times 1/det, then
 // the matrix-vector product), so the FP64 results match an Eigen build term by term.
+//
+// Restated from komietty/metriko include/metriko/core/common/utilities.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_COMMON_UTILITIES_H
 #define METRIKO_COMMON_UTILITIES_H
 #include <algorithm>

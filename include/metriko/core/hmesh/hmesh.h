@@ -12,6 +12,9 @@
 //  * face2half = side F(f,0) -> F(f,1).
 // The element handles (Half, Vert, Edge, Face, ...) keep the reference API;
 // the arrays are plain std::vector so the HIP side can upload them directly.
+//
+// Restated from komietty/metriko include/metriko/core/hmesh/hmesh.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_HMESH_H
 #define METRIKO_HMESH_H
 #include <algorithm>

@@ -3,6 +3,9 @@
 // Drop-in for conversion_2d_3d of the reference's core/hmesh/utilities.h: the surface
 // point of a face at a uv, by the face's corner uv (corner of face.half() first, then of
 // its next and prev halfedges) and the corners' vertex positions.
+//
+// Restated from komietty/metriko include/metriko/core/hmesh/utilities.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_HMESH_UTILITIES_H
 #define METRIKO_HMESH_UTILITIES_H
 #include "../common/utilities.h"

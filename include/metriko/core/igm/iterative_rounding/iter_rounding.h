@@ -6,13 +6,46 @@
 // assembly on the host, the Poisson KKT, the LM directions, the energies and the FP64
 // rounding decisions on the GPU; with localInjectivity the damped barrier LM passes
 // (injective_barrier.h, levenberg_marquardt.h DiagonalDamping) run there too.
+//
+// Restated from komietty/metriko include/metriko/core/igm/iterative_rounding/iter_rounding.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_ITER_ROUNDING_H
 #define METRIKO_ITER_ROUNDING_H
 #include <functional>
+#include <memory>
 #include "metriko/core/common/sparse.h"
+#include "metriko/core/solver/suitesparse/suitesparse_pdefinite.h"
 #include "metriko/hip/runtime.h"
 
 namespace metriko {
+    // CholeskyWrapper (iter_rounding.h:14-31): the LinearSolver of LMSolver.  factorize()
+    // uploads the SPD normal matrix to the device, solve() runs multigrid PCG there (FP64 relative
+    // residual 1e-10 by default); failures are reported as false, which LMSolver turns into
+    // "Solver Failed to factorize!" (levenberg_marquardt.h:184-187).
+    class CholeskyWrapper {
+    public:
+        std::unique_ptr<suitesparse::PositiveDefiniteSolver<double>> llt;
+
+        bool factorize(const SprsD& A) {
+            try {
+                llt = std::make_unique<suitesparse::PositiveDefiniteSolver<double>>(A);
+                return true;
+            } catch (const std::exception&) {
+                llt.reset();
+                return false;
+            }
+        }
+        bool solve(const VecXd& rhs, VecXd& x) const {
+            if (!llt) return false;
+            try {
+                llt->solve(x, rhs);
+                return true;
+            } catch (const std::exception&) {
+                return false;
+            }
+        }
+    };
+
     inline bool iterative_rounding(const VecXi& fixedIdcs, const VecXd& fixedVals, const VecXi& singularIdcs,
                                    const VecXi& integerIdcs, const double length, const SprsD& Cfull, const SprsD& G2,
                                    const int N, const int n, const int nF, const bool seamless, const bool roundSeams,
@@ -40,15 +73,32 @@ namespace metriko {
     }
 
     // Reference signature (iter_rounding.h:33-51).  The reference calls iter_cb with the
-    // current fullx after every naive LM iteration and ends that pass when it returns true
-    // (iter_rounding_init.h:28); the device loop makes no per-iteration host callbacks, so
-    // iter_cb is not called here.
+    // current fullx (UFull x) after every naive LM iteration and ends that pass when it returns
+    // true (iter_rounding_init.h:28, levenberg_marquardt.h:221-224): a non-empty iter_cb is
+    // installed as the context's naive-pass callback for this call and cleared afterwards.
     inline bool iterative_rounding(const VecXi& fixedIdcs, const VecXd& fixedVals, const VecXi& singularIdcs,
                                    const VecXi& integerIdcs, const double length, const SprsD& Cfull, const SprsD& G2,
                                    const int N, const int n, const int nF, const bool seamless, const bool roundSeams,
                                    const bool localInjectivity, const bool verbose, const VecXd& intrinsicField,
                                    const std::function<bool(const VecXd& x)>& iter_cb, VecXd& fullx) {
-        (void)iter_cb;
+        if (!iter_cb)
+            return iterative_rounding(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n, nF,
+                                      seamless, roundSeams, localInjectivity, verbose, intrinsicField, fullx);
+        auto& rt = hip::runtime();
+        struct Call {
+            const std::function<bool(const VecXd&)>* cb;
+            static int trampoline(void* user, const double* x, int nx) {
+                const auto* self = static_cast<const Call*>(user);
+                VecXd v(nx);
+                for (int i = 0; i < nx; i++) v[i] = x[i];
+                return (*self->cb)(v) ? 1 : 0;
+            }
+        } call{&iter_cb};
+        struct Clear {
+            mk_ctx* c;
+            ~Clear() { mk_set_naive_iter_callback(c, nullptr, nullptr); }
+        } clear{rt.get()};
+        hip::check(mk_set_naive_iter_callback(rt.get(), &Call::trampoline, &call), "mk_set_naive_iter_callback");
         return iterative_rounding(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n, nF,
                                   seamless, roundSeams, localInjectivity, verbose, intrinsicField, fullx);
     }

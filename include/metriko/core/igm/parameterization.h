@@ -4,6 +4,9 @@
 // parameterization_integ.ipp).  setup() builds the transition/constraint matrices on the
 // host with the reference's numbering; integ() assembles the gradient operators and
 // hands the whole iterative_rounding to the GPU through the C ABI.
+//
+// Restated from komietty/metriko include/metriko/core/igm/parameterization.h, parameterization_setup.ipp, parameterization_integ.ipp
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_PARAMETERIZATION_H
 #define METRIKO_PARAMETERIZATION_H
 #include <memory>

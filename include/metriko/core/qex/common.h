@@ -5,6 +5,9 @@
 // transition from the face of h to the face of its twin, qex/common.h:73-94) and the
 // orientation / grid predicates with EPS = 1e-9.  The extraction runs on the host, on the
 // corner uv the GPU parameterization produced (cfn, corner 3 f + j at vertex F(f, j)).
+//
+// Restated from komietty/metriko include/metriko/core/qex/common.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_COMMON_H
 #define METRIKO_QEX_COMMON_H
 #include <cmath>

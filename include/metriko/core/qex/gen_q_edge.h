@@ -8,6 +8,9 @@
 // transition (heR, heT).  The reference scans the whole port list for each test; here the
 // candidates come from per-face / per-edge / per-vertex lists in port order, which gives the
 // same first match.  A trace longer than 4 nF + 16 faces throws (the reference would loop).
+//
+// Restated from komietty/metriko include/metriko/core/qex/gen_q_edge.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_GEN_Q_EDGE_H
 #define METRIKO_QEX_GEN_Q_EDGE_H
 #include <string>

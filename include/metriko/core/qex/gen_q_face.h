@@ -2,6 +2,9 @@
 // q-edge i from port1 to port2, half 2 i + 1 back; from the first unvisited half, three
 // times take the first half leaving the port before the current end port (prev_id); four
 // halves make a quad.
+//
+// Restated from komietty/metriko include/metriko/core/qex/gen_q_face.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_GEN_Q_FACE_H
 #define METRIKO_QEX_GEN_Q_FACE_H
 #include <unordered_map>

@@ -4,6 +4,9 @@
 // corner (or along its first edge); edge q-verts: the four directions across the two faces
 // of the edge, deduplicated by the normalised surface point one unit away; face q-verts:
 // the four axis directions.
+//
+// Restated from komietty/metriko include/metriko/core/qex/gen_q_port.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_GEN_Q_PORT_H
 #define METRIKO_QEX_GEN_Q_PORT_H
 #include "common.h"

@@ -2,6 +2,9 @@
 // uv points on mesh vertices (the first corner's uv within EPS of the grid; its uv is stored
 // as the fractional parts), strictly inside mesh edges (collinear, 0 < t < 1 by EPS) and
 // strictly inside faces, with their surface positions.
+//
+// Restated from komietty/metriko include/metriko/core/qex/gen_q_vert.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_GEN_Q_VERT_H
 #define METRIKO_QEX_GEN_Q_VERT_H
 #include "common.h"

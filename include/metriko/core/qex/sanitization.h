@@ -5,6 +5,9 @@
 // two of the largest |u|, |v| around the vertex, which rounds every corner of the vertex to
 // the same binary grid -- and the other corners are rebuilt from it through the transitions
 // of the halfedges crossed, so corners of one vertex agree exactly across seams.
+//
+// Restated from komietty/metriko include/metriko/core/qex/sanitization.h
+// (the komietty/metriko authors), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QEX_SANITIZATION_H
 #define METRIKO_QEX_SANITIZATION_H
 #include "common.h"

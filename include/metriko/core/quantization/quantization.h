@@ -6,6 +6,9 @@
 // of +-generator moves ordered by compute_weight, each kept when it lowers
 // |X ./ R - 1| and stays valid; then one pass over a second basis (shortest loops first).
 // |.| is the Euclidean norm summed in index order.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_H
 #define METRIKO_QUANTIZATION_H
 #include <cmath>

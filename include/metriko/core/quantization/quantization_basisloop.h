@@ -3,6 +3,9 @@
 // (std::priority_queue ordered by `compare`) crosses T-quads from a side to the opposite
 // side (over the twin of every half-edge there) until it returns to bgn; the loop is read
 // back through the predecessor map.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization_basisloop.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_BASISLOOP_H
 #define METRIKO_QUANTIZATION_BASISLOOP_H
 #include <algorithm>

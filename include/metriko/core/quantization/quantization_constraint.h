@@ -1,6 +1,9 @@
 // metriko (MI355X build) -- QGP quantization: opposite sides of every T-quad have equal
 // integer length (reference core/quantization/quantization_constraint.h): row 2 q holds
 // +1 on side 0 and -1 on side 2 of T-quad q, row 2 q + 1 the same for sides 1 and 3.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization_constraint.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_CONSTRAINT_H
 #define METRIKO_QUANTIZATION_CONSTRAINT_H
 #include <stdexcept>

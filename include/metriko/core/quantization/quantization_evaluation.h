@@ -2,6 +2,9 @@
 // (reference core/quantization/quantization_evaluation.h).  One basis loop per canonical
 // T-half-edge; row i counts the T-edges of loop i; a maximal linearly independent subset of
 // the rows (reduce_to_linearly_independent) transposed gives the generators as columns.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization_evaluation.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_EVALUATION_H
 #define METRIKO_QUANTIZATION_EVALUATION_H
 #include "quantization_basisloop.h"

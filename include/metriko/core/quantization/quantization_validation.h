@@ -2,6 +2,9 @@
 // (reference core/quantization/quantization_validation.h): every length positive, and from
 // each half-edge that leaves a singular vertex, the zero-length T-edges reachable through
 // adj_thalfs never lead back against the start of a curve.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization_validation.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_VALIDATION_H
 #define METRIKO_QUANTIZATION_VALIDATION_H
 #include <queue>

@@ -2,6 +2,9 @@
 // (reference core/quantization/quantization_weight.h).  d = ideal - current (negated for a
 // decrease): 1 / (d + 1) once a T-edge is a unit or more short, n / (d + 1) when within one
 // unit, n^2 / (1 - d) when already too long; decreasing a zero-length edge is 1e9.
+//
+// Restated from komietty/metriko include/metriko/core/quantization/quantization_weight.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_QUANTIZATION_WEIGHT_H
 #define METRIKO_QUANTIZATION_WEIGHT_H
 namespace metriko {

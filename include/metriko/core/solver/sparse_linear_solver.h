@@ -6,10 +6,14 @@
 // assembles L on the GPU and runs nIter warm-started PCG solves there
 // (FP32 inner + FP64 refinement by default), with the normalisation and
 // Rayleigh steps in FP64.
+//
+// Restated from komietty/metriko include/metriko/core/solver/sparse_linear_solver.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_SPARSE_LINEAR_SOLVER_H
 #define METRIKO_SPARSE_LINEAR_SOLVER_H
 #include <cstdlib>
 #include "metriko/core/common/sparse.h"
+#include "metriko/core/solver/suitesparse/suitesparse_square.h"
 #include "metriko/hip/runtime.h"
 
 namespace metriko {
@@ -68,15 +72,42 @@ namespace metriko {
         return solveSmallestEig(d, mass, u, nIter);
     }
 
-    // solveSquare (sparse_linear_solver.h:16-27) for the Hermitian positive definite lhs the
-    // reference's callers pass (CurvatureAligned: L - 0*M): PCG on the device.
+    // solveSquare (sparse_linear_solver.h:16-27): SquareSolver on the device -- PCG for the
+    // Hermitian positive definite lhs the reference's callers pass (CurvatureAligned: L - 0*M),
+    // the normal equations for any other nonsingular lhs.
     inline VecXc solveSquare(SprsC& lhs, VecXc& rhs) {
+        suitesparse::SquareSolver<complex> solver(lhs);
+        VecXc x;
+        solver.solve(x, rhs);
+        hip::runtime().last_field = solver.last;
+        return x;
+    }
+
+    // Eigen 3.4 VectorXd::Random(n): -1 + 2*rand()/RAND_MAX per entry (std::rand).
+    inline VecXd random_real_vector(int n) {
+        VecXd u(n);
+        for (int i = 0; i < n; i++) u[i] = -1.0 + (2.0 * double(std::rand())) / double(RAND_MAX);
+        return u;
+    }
+
+    // Real solveSmallestEig (sparse_linear_solver.h:56-70): the reference factorises L once
+    // (SimplicialLDLT) and runs nIter steps x = L^-1 (M u), u = x / sqrt(x^T M x), returning the
+    // last x (not normalised).  Here L and M live on the device; each step is a PCG solve
+    // (starting from the previous x) and two SpMVs.  The reference's per-step "iter i" trace on
+    // stdout is not reproduced.
+    inline VecXd solveSmallestEig(const SprsD& L, const SprsD& M, int nIter = 50) {
         auto& rt = hip::runtime();
-        DeviceOp d = upload_complex(lhs);
-        VecXc x = VecXc::Zero(lhs.rows());
-        hip::check(mk_spd_solve(rt.get(), d.op, reinterpret_cast<const double*>(rhs.data()),
-                                reinterpret_cast<double*>(x.data()), &rt.field_opts, &rt.last_field),
-                   "mk_spd_solve");
+        const suitesparse::detail::DeviceMatrix<double> dL(L, rt.igm_opts), dM(M, rt.igm_opts);
+        VecXd u = random_real_vector(L.rows());
+        VecXd x = u;
+        for (int i = 0; i < nIter; i++) {
+            dL.solve(x, dM.apply(u), rt.igm_opts, &rt.last_field);
+            const VecXd Mx = dM.apply(x);
+            double s = 0.0;
+            for (int k = 0; k < x.size(); k++) s += x[k] * Mx[k];
+            s = std::sqrt(s);
+            for (int k = 0; k < x.size(); k++) u[k] = x[k] / s;
+        }
         return x;
     }
 }

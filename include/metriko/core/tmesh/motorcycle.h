@@ -13,6 +13,9 @@
 //    from) and the curve ends in HitB.  Crossing an edge turns the direction by
 //    polar(1, pi/2 m), m the edge's matching seen from the crossing halfedge.
 // A curve that has not crashed after 4 nF + 16 sweeps throws (the reference would loop).
+//
+// Restated from komietty/metriko include/metriko/core/tmesh/motorcycle.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_MOTORCYCLE_H
 #define METRIKO_MOTORCYCLE_H
 #include <algorithm>

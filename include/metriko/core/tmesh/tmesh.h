@@ -11,6 +11,9 @@
 //    singular vertex).
 // Segments are matched as Msgmt::operator== does (face and both end points).  A quad walk
 // longer than 4 T-edges + 8 throws (the reference would loop).
+//
+// Restated from komietty/metriko include/metriko/core/tmesh/tmesh.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_TMESH_H
 #define METRIKO_TMESH_H
 #include <algorithm>

@@ -1,4 +1,7 @@
 // metriko (MI355X build) -- drop-in for the reference's core/vectorfield/base_field.h:10-23.
+//
+// Restated from komietty/metriko include/metriko/core/vectorfield/base_field.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_BASE_FIELD_H
 #define METRIKO_BASE_FIELD_H
 #include "metriko/core/hmesh/hmesh.h"

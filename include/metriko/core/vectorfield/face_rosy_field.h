@@ -7,6 +7,9 @@
 // The seam restates the libigl v2.5.0 routines the reference calls
 // (cut_mesh_from_singularities with a zero matching, dijkstra with unit weights,
 // adjacency_list) with linear-time equivalents that produce the same edge set.
+//
+// Restated from komietty/metriko include/metriko/core/vectorfield/face_rosy_field.h
+// (Copyright (C) 2025 Saki Komikado), which is subject to the terms of the Mozilla Public License v.2.0.
 #ifndef METRIKO_FACE_ROSY_FIELD_H
 #define METRIKO_FACE_ROSY_FIELD_H
 #include <algorithm>

@@ -104,9 +104,33 @@ mk_status mk_op_apply(mk_ctx* ctx, const mk_op* op, const double* x, double* y);
 mk_status mk_spd_solve(mk_ctx* ctx, const mk_op* op, const double* b, double* x, const mk_solve_opts* opts,
                        mk_stats* stats);
 
+/* Host callback of the naive LM pass: iterative_rounding's iter_cb (igm/iterative_rounding/
+ * iter_rounding.h:49, called through NaiveIntegration::post_iteration, iter_rounding_init.h:28,
+ * after every iteration of LMSolver::solve, levenberg_marquardt.h:221-224).  It receives
+ * fullx = UFull x of the current iterate (nfull values, host memory valid during the call);
+ * a nonzero return ends the naive pass there, as the reference's iter_cb returning true does.
+ * Set (cb != NULL) or cleared (cb == NULL) per context; while set it takes the place of the
+ * device check of mk_igm_injectivity_check. */
+typedef int (*mk_naive_iter_cb)(void* user, const double* fullx, int nfull);
+mk_status mk_set_naive_iter_callback(mk_ctx* ctx, mk_naive_iter_cb cb, void* user);
+
+/* Allocation-free variant of mk_spd_solve.  No counterpart in the reference: for callers that
+ * cannot allocate device memory during a solve (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  b and x are DEVICE arrays (complex interleaved;
+ * x is the warm start on input); FP64 PCG with opts->precond MK_PRECOND_JACOBI or _NONE
+ * (other modes: MK_ERR_UNSUPPORTED).  Every work vector, reduction partial and device scalar
+ * comes from the library's static per-device scratch (24 MiB, held until the process ends):
+ * at most 524,288 scalar unknowns (n real, 2 n complex), larger operators get MK_ERR_ARG.
+ * The call makes no device allocation and frees nothing; the operator itself is created
+ * beforehand (mk_op_create_*).  All calls on one GPU share its scratch: calls that overlap
+ * on one GPU (two host threads, two contexts) race. */
+mk_status mk_spd_solve_noalloc(mk_ctx* ctx, const mk_op* op, const double* b_dev, double* x_dev,
+                               const mk_solve_opts* opts, mk_stats* stats);
+
 /* A X = B for m right-hand sides (B, X: n x m column-major host arrays; X starts from zero).
  * Real operators with the multigrid preconditioner: batched PCG, up to 64 columns per block
- * (one matrix read serves every column), FP64 Krylov vectors and FP32 V-cycle.  Other
+ * (one matrix read serves every column), FP64 Krylov vectors and an FP32 V-cycle (FP64 with
+ * MK_PREC_FP64).  Other
  * configurations solve the columns one by one with mk_spd_solve's algorithm.  Replaces
  * repeated CholeskyWrapper::solve calls (iter_rounding.h:19). */
 mk_status mk_spd_solve_many(mk_ctx* ctx, const mk_op* op, int m, const double* B, double* X,

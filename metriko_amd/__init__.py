@@ -26,8 +26,10 @@ EXPORTED = [
     "mk_spd_solve_many", "mk_field_solve", "mk_curvature_dir", "mk_field_matching", "mk_autotune_spmv",
     "mk_batch_run", "mk_igm_injectivity_check", "mk_qex_extract", "mk_motorcycle_graph", "mk_tmesh_build", "mk_qgp_quantize", "mk_read_obj", "mk_cfn_write_json",
     "mk_cfn_read_json", "mk_keep_igm_state", "mk_igm_state_take", "mk_igm_state_destroy", "mk_igm_state_sizes",
-    "mk_igm_sized_uv", "mk_igm_sized_uv_grad",
+    "mk_igm_sized_uv", "mk_igm_sized_uv_grad", "mk_spd_solve_noalloc", "mk_set_naive_iter_callback",
 ]
+# iterative_rounding's iter_cb across the C ABI: (user, fullx, nfull) -> nonzero to stop
+NAIVE_ITER_CB = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int)
 FIELD_SMOOTHEST, FIELD_CURVATURE_ALIGNED = 0, 1
 MATCH_PRINCIPAL, MATCH_CURL = 0, 1
 
@@ -115,6 +117,7 @@ def lib():
     L.mk_op_destroy.argtypes = [_P]
     L.mk_op_apply.argtypes = [_P, _P, _D, _D]
     L.mk_spd_solve.argtypes = [_P, _P, _D, _D, C.POINTER(SolveOpts), C.POINTER(Stats)]
+    L.mk_spd_solve_noalloc.argtypes = [_P, _P, C.c_void_p, C.c_void_p, C.POINTER(SolveOpts), C.POINTER(Stats)]
     L.mk_spd_solve_many.argtypes = [_P, _P, C.c_int, _D, _D, C.POINTER(SolveOpts), C.POINTER(Stats)]
     L.mk_field_laplacian.argtypes = [_P, C.c_int, C.c_int, _I32, _I32, _I32, _D, C.c_int, C.POINTER(SolveOpts),
                                      C.POINTER(_P)]
@@ -142,6 +145,7 @@ def lib():
                                C.POINTER(BatchStats)]
     L.mk_igm_injectivity_check.argtypes = [_P, C.c_int, C.c_int, _I32, C.c_int, C.c_int, _I64, _I32, _D]
     L.mk_keep_igm_state.argtypes = [_P, C.c_int]
+    L.mk_set_naive_iter_callback.argtypes = [_P, NAIVE_ITER_CB, C.c_void_p]
     L.mk_motorcycle_graph.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _D, C.c_int,
                                       _I32, _D, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.mk_tmesh_build.argtypes = [_P, C.c_int, C.c_int, _D, _I32, _I32, _I32, _D, C.c_int, _I32, _I32, C.c_int, _I32,
@@ -670,11 +674,28 @@ class Context:
 
     def iterative_rounding(self, fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n, nF,
                            F2, seamless=True, round_seams=False, local_injectivity=False, o=None,
-                           cut_idx=None, fullx2Nfunc=None):
+                           cut_idx=None, fullx2Nfunc=None, iter_cb=None):
         """iterative_rounding (iter_rounding.h:33-51) through mk_iterative_rounding: Cfull and
         G2 as scipy CSR; returns fullx, success, decisions [(index, value, margin)], stats.
         cut_idx (cut faces x 3) and fullx2Nfunc (scipy CSR) give the naive pass the
-        locally_injectivity_check RosyParameterization passes as iter_cb."""
+        locally_injectivity_check RosyParameterization passes as iter_cb; iter_cb(fullx) -> bool
+        is a host callback in its place (mk_set_naive_iter_callback)."""
+        cb_ref = None
+        if iter_cb is not None:
+            def _tramp(user, x, nx):
+                return 1 if iter_cb(np.ctypeslib.as_array(x, shape=(nx,)).copy()) else 0
+            cb_ref = NAIVE_ITER_CB(_tramp)
+            self._check(self.L.mk_set_naive_iter_callback(self.h, cb_ref, None), "mk_set_naive_iter_callback")
+        try:
+            return self._iterative_rounding_checked(fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull,
+                                                    G2, N, n, nF, F2, seamless, round_seams, local_injectivity, o,
+                                                    cut_idx, fullx2Nfunc)
+        finally:
+            if cb_ref is not None:
+                self.L.mk_set_naive_iter_callback(self.h, NAIVE_ITER_CB(), None)
+
+    def _iterative_rounding_checked(self, fixedIdcs, fixedVals, singularIdcs, integerIdcs, length, Cfull, G2, N, n,
+                                    nF, F2, seamless, round_seams, local_injectivity, o, cut_idx, fullx2Nfunc):
         if cut_idx is not None:
             ci = np.ascontiguousarray(cut_idx, np.int32).reshape(-1)
             f2n = fullx2Nfunc.tocsr()

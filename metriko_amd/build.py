@@ -9,7 +9,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmetriko_hip.so")
-SOURCES = ["kernels.hip", "block.hip", "mg.hip", "capi.hip", "igm.hip", "pipeline.hip", "batch.hip"]
+SOURCES = ["kernels.hip", "block.hip", "mg.hip", "capi.hip", "igm.hip", "pipeline.hip", "batch.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("METRIKO_ARCH", "gfx950")
 
@@ -45,7 +45,27 @@ def build(verbose=False, jobs=4):
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_seams_test(verbose)
     return LIB
+
+
+SEAMS_SRC = os.path.join(ROOT, "tests", "native", "seams_test.cpp")
+SEAMS_BIN = os.path.join(ROOT, "tests", "native", "bin", "seams_test")
+
+
+def build_seams_test(verbose=False):
+    """The C++ test of the drop-in solver seams, linked with the library (run by a GPU test)."""
+    if not os.path.exists(SEAMS_SRC):
+        return None
+    os.makedirs(os.path.dirname(SEAMS_BIN), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), SEAMS_SRC, "-L", HERE, "-lmetriko_hip",
+           "-Wl,-rpath,$ORIGIN/../../../metriko_amd", "-o", SEAMS_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on seams_test.cpp:\n{r.stdout}\n{r.stderr}")
+    return SEAMS_BIN
 
 
 if __name__ == "__main__":

@@ -63,9 +63,12 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
     return make_double2(a * x.x + b * y.x, a * x.y + b * y.y);
 }
 
-// Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  MODE 5 / 6: one
-// Chebyshev smoothing step fused with the residual it needs, row-local (Y = x_out, X = x_in,
-// never the same buffer): r = B - A X ; d = cs D^-1 r (5) or ca d + cs D^-1 r (6) ; Y = X + d.
+// Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  MODES 5-7: one
+// Chebyshev smoothing step fused with the residual it needs, in the three-term form (the
+// step d_{k-1} of the two-term form is x_k - x_{k-1}), row-local, X = x_k (gathered) and
+// Y != X:  r = B - A X;  5: Y = X + cs D^-1 r (first step from a given x);
+// 6: Y = X + ca (X - Y) + cs D^-1 r (Y holds x_{k-1} on entry and x_{k+1} on exit);
+// 7: Y = (1 + ca) X + cs D^-1 r (the step after x_1 from a zero start, x_0 = 0).
 // A lane carries one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront
 // covers 64 / LPR rows of the SELL part at a time; medium and long rows take a whole
 // wavefront whose row groups split the entries.  PART 1 / 2: only the SELL rows / only the
@@ -74,8 +77,7 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
 template <typename T, int MW, int MODE, int PART = 0>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
                                                T* __restrict__ Y, const T* __restrict__ B, const int* skip,
-                                               T* __restrict__ D = nullptr, const T* __restrict__ dinv = nullptr,
-                                               T ca = T(0), T cs = T(0)) {
+                                               const T* __restrict__ dinv = nullptr, T ca = T(0), T cs = T(0)) {
     if (skip && *skip) return;
     using V = typename Vec16<T>::type;
     constexpr int N = Vec16<T>::N;
@@ -87,7 +89,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     const int gw = xcd_block(int(blockIdx.x), int(gridDim.x)) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     V acc;
     vzero(acc);
-    // modes 5 / 6 need the row's own x_in: taken from the gather of the diagonal entry, so the
+    // modes 5-7 need the row's own x_k: taken from the gather of the diagonal entry, so the
     // SELL rows read X once (medium and long rows, a few hundred, read it again at the end)
     V xd;
     vzero(xd);
@@ -108,7 +110,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
             vfma(acc, v0, x0);
             vfma(acc, v1, x1);
-            if constexpr (MODE == 5 || MODE == 6) {
+            if constexpr (MODE >= 5) {
                 if (c0 == row) { xd = x0; have_xd = true; }
                 if (c1 == row) { xd = x1; have_xd = true; }
             }
@@ -118,7 +120,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             const int c0 = A.col[e];
             const V x0 = Xv[size_t(c0) * LPR + j];
             vfma(acc, A.val[e], x0);
-            if constexpr (MODE == 5 || MODE == 6) {
+            if constexpr (MODE >= 5) {
                 if (c0 == row) { xd = x0; have_xd = true; }
             }
         }
@@ -173,15 +175,13 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     if constexpr (MODE == 0) Yv[o] = acc;
     if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
     if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
-    if constexpr (MODE == 5 || MODE == 6) {
-        V* Dv = reinterpret_cast<V*>(D);
+    if constexpr (MODE >= 5) {
+        const V xo = have_xd ? xd : Xv[o];
         const V r = vsub(reinterpret_cast<const V*>(B)[o], acc);
         const T sd = cs * dinv[row];
-        V d;
-        if constexpr (MODE == 5) d = vaxpy(sd, r, T(0), r);
-        else d = vaxpy(sd, r, ca, Dv[o]);
-        Dv[o] = d;
-        Yv[o] = vadd(have_xd ? xd : Xv[o], d);
+        if constexpr (MODE == 5) Yv[o] = vaxpy(T(1), xo, sd, r);
+        if constexpr (MODE == 7) Yv[o] = vaxpy(T(1) + ca, xo, sd, r);
+        if constexpr (MODE == 6) Yv[o] = vaxpy(T(1), vaxpy(T(1) + ca, xo, -ca, Yv[o]), sd, r);
     }
 }
 
@@ -209,8 +209,8 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
 }
 
 template <typename T, int MW>
-static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, T* D,
-                         const T* dinv, T ca, T cs, bool first, const int* skip) {
+static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, const T* dinv, T ca,
+                         T cs, int mode, const int* skip) {
     unsigned g0 = 0;
     const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
@@ -219,37 +219,51 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
     const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
     const unsigned g2 = unsigned((wl + 3) / 4);
 #define MK_CHEB_PARTS(M)                                                                                            \
-    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip, D,  \
+    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip,     \
                        dinv, ca, cs);                                                                               \
     if (g2)                                                                                                         \
-        hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip, D, dinv, \
+        hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip, dinv, \
                            ca, cs);
-    if (first) { MK_CHEB_PARTS(5) } else { MK_CHEB_PARTS(6) }
+    switch (mode) {
+        case 5: MK_CHEB_PARTS(5) break;
+        case 6: MK_CHEB_PARTS(6) break;
+        case 7: MK_CHEB_PARTS(7) break;
+        default: throw Error(MK_ERR_ARG, "block Chebyshev: mode 5, 6 or 7");
+    }
 #undef MK_CHEB_PARTS
 }
 
-// One fused Chebyshev step on a block of mw right-hand sides (x_out != x_in).
-void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const float* x_in, float* x_out, const float* b, float* d,
-                      float ca, float cs, bool first, const int* skip) {
+// One fused three-term Chebyshev step on a block of mw right-hand sides (modes 5-7 of
+// sell_spmm; x_out != x_in, x_out holds x_{k-1} on entry in mode 6).
+template <typename T>
+void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out, const T* b, T ca, T cs, int mode,
+                      const int* skip) {
     MK_CHECK(!A->cplx && x_in != x_out, MK_ERR_ARG, "fused block Chebyshev: real operator, x_out != x_in");
     // algorithmic bytes: the matrix (values, columns, slice / row metadata), the x_in block
     // once (A->m rows: every column is gathered, the row's own entry among them), B read,
-    // d read (not on the first step), d and x_out written, the inverse diagonal
-    const double vb = 4.0 * mw;
-    const double bytes = A->bytes32 - double(A->n) * 2.0 * 4.0 + double(A->m) * vb +
-                         double(A->n) * vb * (first ? 3.0 : 4.0) + double(A->n) * 4.0;
+    // x_out written (and read first in mode 6), the inverse diagonal
+    const double s = double(sizeof(T)), vb = s * mw;
+    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * s + double(A->m) * vb +
+                         double(A->n) * vb * (mode == 6 ? 3.0 : 2.0) + double(A->n) * s;
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64_cheb" : mw == 32 ? "spmm32_cheb" : mw == 16 ? "spmm16_cheb" :
-                 "spmm8_cheb", true);
-    const float* dinv = A->dinv32.p;
+                 "spmm8_cheb", sizeof(T) == 4);
+    const T* val;
+    const T* dinv;
+    if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
+    else { val = A->val32.p; dinv = A->dinv32.p; }
     switch (mw) {
-        case 8: spmm_cheb_mw<float, 8>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
-        case 16: spmm_cheb_mw<float, 16>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
-        case 32: spmm_cheb_mw<float, 32>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
-        case 64: spmm_cheb_mw<float, 64>(c, A, A->val32.p, x_in, x_out, b, d, dinv, ca, cs, first, skip); break;
+        case 8: spmm_cheb_mw<T, 8>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
+        case 16: spmm_cheb_mw<T, 16>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
+        case 32: spmm_cheb_mw<T, 32>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
+        case 64: spmm_cheb_mw<T, 64>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
         default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
     }
     MK_HIP(hipGetLastError());
 }
+template void spmm_cheb_launch<float>(mk_ctx*, const mk_op*, int, const float*, float*, const float*, float, float,
+                                      int, const int*);
+template void spmm_cheb_launch<double>(mk_ctx*, const mk_op*, int, const double*, double*, const double*, double,
+                                       double, int, const int*);
 
 template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip) {
@@ -331,16 +345,16 @@ __device__ __forceinline__ void bstate_all_done(BState* st, int mw) {
     st->all_done = all;
 }
 
-// X = 0 ; R = B ; R32 = float(B) ; bb_j = |B_j|^2
-template <int MW>
+// X = 0 ; R = B ; Rz = ZT(B) (the V-cycle's input copy; none for an FP64 V-cycle) ; bb_j = |B_j|^2
+template <int MW, typename ZT>
 __global__ void __launch_bounds__(256) bk_init0(size_t len, const double* __restrict__ B, double* __restrict__ X,
-                                              double* __restrict__ R, float* __restrict__ R32, double* partials) {
+                                              double* __restrict__ R, ZT* __restrict__ Rz, double* partials) {
     double v[1] = {0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double b = B[e];
         X[e] = 0.0;
         R[e] = b;
-        R32[e] = float(b);
+        if (Rz) Rz[e] = ZT(b);
         v[0] += b * b;
     }
     colsum_write<1, MW>(v, partials);
@@ -357,13 +371,13 @@ __global__ void __launch_bounds__(256) bk_init0_fin(BState* st, const double* __
     }
 }
 
-// P = Zold = Z32 ; rz_j = R.Z32 ; rr_j = R.R ; done flags
-template <int MW>
-__global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
+// P = Zold = Z ; rz_j = R.Z ; rr_j = R.R ; done flags
+template <int MW, typename ZT>
+__global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __restrict__ R, const ZT* __restrict__ Z,
                                               double* __restrict__ P, double* __restrict__ Zold, double* partials) {
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
-        const double z = double(Z32[e]), r = R[e];
+        const double z = double(Z[e]), r = R[e];
         P[e] = z;
         Zold[e] = z;
         v[0] += r * z;
@@ -419,10 +433,10 @@ __global__ void __launch_bounds__(256) bk_alpha_fin(BState* st, const double* __
     }
 }
 
-// X += alpha P ; R -= alpha Q ; R32 = float(R) ; rr_j ; convergence
-template <int MW>
+// X += alpha P ; R -= alpha Q ; Rz = ZT(R) (none for an FP64 V-cycle) ; rr_j ; convergence
+template <int MW, typename ZT>
 __global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict__ X, double* __restrict__ R,
-                                               float* __restrict__ R32, const double* __restrict__ P,
+                                               ZT* __restrict__ Rz, const double* __restrict__ P,
                                                const double* __restrict__ Q, const BState* st, double* partials) {
     if (st->all_done) return;
     const double alpha = st->alpha[threadIdx.x % MW];
@@ -433,7 +447,7 @@ __global__ void __launch_bounds__(256) bk_update(size_t len, double* __restrict_
             X[e] += alpha * P[e];
             r -= alpha * Q[e];
             R[e] = r;
-            R32[e] = float(r);
+            if (Rz) Rz[e] = ZT(r);
         }
         v[0] += r * r;
     }
@@ -460,15 +474,15 @@ __global__ void __launch_bounds__(256) bk_update_fin(BState* st, const double* _
     }
 }
 
-// flexible beta_j = (R.Z32 - R.Zold) / rz_j ; rz_j = R.Z32
-template <int MW>
-__global__ void __launch_bounds__(256) bk_beta(size_t len, const double* __restrict__ R, const float* __restrict__ Z32,
+// flexible beta_j = (R.Z - R.Zold) / rz_j ; rz_j = R.Z
+template <int MW, typename ZT>
+__global__ void __launch_bounds__(256) bk_beta(size_t len, const double* __restrict__ R, const ZT* __restrict__ Z,
                                              const double* __restrict__ Zold, const BState* st, double* partials) {
     if (st->all_done) return;
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double r = R[e];
-        v[0] += r * double(Z32[e]);
+        v[0] += r * double(Z[e]);
         v[1] += r * Zold[e];
     }
     colsum_write<2, MW>(v, partials);
@@ -497,16 +511,16 @@ __global__ void __launch_bounds__(256) bk_beta_fin(BState* st, const double* __r
     }
 }
 
-// P = Z32 + beta P ; Zold = Z32 (running columns only)
-template <int MW>
-__global__ void __launch_bounds__(256) bk_pupdate(size_t len, double* __restrict__ P, const float* __restrict__ Z32,
+// P = Z + beta P ; Zold = Z (running columns only)
+template <int MW, typename ZT>
+__global__ void __launch_bounds__(256) bk_pupdate(size_t len, double* __restrict__ P, const ZT* __restrict__ Z,
                                                 double* __restrict__ Zold, const BState* st) {
     if (st->all_done) return;
     const int j = threadIdx.x % MW;
     if (st->done[j]) return;
     const double beta = st->beta[j];
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
-        const double z = double(Z32[e]);
+        const double z = double(Z[e]);
         P[e] = z + beta * P[e];
         Zold[e] = z;
     }
@@ -517,12 +531,17 @@ __global__ void k_set_bstate(BState* st, double tol2) {
     st->all_done = 0;
 }
 
-template <int MW>
+// ZT: the V-cycle's precision (float: FP32 multigrid inside the FP64 recurrences; double: an
+// FP64 V-cycle on R itself, the plain FP64 form of the same algorithm)
+template <int MW, typename ZT>
 static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double* B, double* X, double rel_tol,
                                 int maxiter, int check_every) {
+    constexpr bool fp64 = std::is_same<ZT, double>::value;
     const size_t len = size_t(A->n) * MW;
     DevBuf<double> R(len), P(len), Q(len), Zold(len), part(size_t(2) * 64 * kBlkParts);
-    DevBuf<float> R32(len), Z32(len);
+    DevBuf<ZT> Rz(fp64 ? 0 : len), Z(len);
+    ZT* rz = fp64 ? nullptr : Rz.p;                 // the V-cycle's input copy of R
+    const ZT* vin = fp64 ? reinterpret_cast<const ZT*>(R.p) : Rz.p;
     DevBuf<BState> st(1);
     MK_HIP(hipMemsetAsync(st.p, 0, sizeof(BState), c->stream));
     const unsigned g = unsigned(std::min<size_t>(std::min<size_t>(size_t(fin_grid()), size_t(kBlkParts)),
@@ -530,11 +549,11 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
     const int ng = int(g);
     // the recurrence residual is driven to half the target so the true residual meets it
     hipLaunchKernelGGL(k_set_bstate, dim3(1), dim3(1), 0, c->stream, st.p, 0.25 * rel_tol * rel_tol);
-    hipLaunchKernelGGL((bk_init0<MW>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, R32.p, part.p);
+    hipLaunchKernelGGL((bk_init0<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, rz, part.p);
     hipLaunchKernelGGL((bk_init0_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-    mg_apply32_block(c, mg, MW, R32.p, Z32.p, nullptr);
-    hipLaunchKernelGGL((bk_init1<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                       (const float*)Z32.p, P.p, Zold.p, part.p);
+    mg_apply_block<ZT>(c, mg, MW, vin, Z.p, nullptr);
+    hipLaunchKernelGGL((bk_init1<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
+                       (const ZT*)Z.p, P.p, Zold.p, part.p);
     hipLaunchKernelGGL((bk_init1_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
     MK_HIP(hipGetLastError());
     BState h{};
@@ -550,14 +569,14 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
             hipLaunchKernelGGL((bk_alpha<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)P.p,
                                (const double*)Q.p, (const BState*)st.p, part.p);
             hipLaunchKernelGGL((bk_alpha_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-            hipLaunchKernelGGL((bk_update<MW>), dim3(g), dim3(256), 0, c->stream, len, X, R.p, R32.p,
+            hipLaunchKernelGGL((bk_update<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, X, R.p, rz,
                                (const double*)P.p, (const double*)Q.p, (const BState*)st.p, part.p);
             hipLaunchKernelGGL((bk_update_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-            mg_apply32_block(c, mg, MW, R32.p, Z32.p, &st.p->all_done);
-            hipLaunchKernelGGL((bk_beta<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                               (const float*)Z32.p, (const double*)Zold.p, (const BState*)st.p, part.p);
+            mg_apply_block<ZT>(c, mg, MW, vin, Z.p, &st.p->all_done);
+            hipLaunchKernelGGL((bk_beta<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
+                               (const ZT*)Z.p, (const double*)Zold.p, (const BState*)st.p, part.p);
             hipLaunchKernelGGL((bk_beta_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-            hipLaunchKernelGGL((bk_pupdate<MW>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const float*)Z32.p,
+            hipLaunchKernelGGL((bk_pupdate<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const ZT*)Z.p,
                                Zold.p, (const BState*)st.p);
         }
         iters += batch;
@@ -577,16 +596,20 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
 }
 
 BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
-                      int maxiter, int check_every) {
+                      int maxiter, int check_every, bool fp64_vcycle) {
     MK_CHECK(mg, MK_ERR_ARG, "block PCG needs the operator's multigrid hierarchy");
     MK_CHECK(!A->cplx && A->n == A->m, MK_ERR_ARG, "block PCG: square real operator");
+#define MK_BLOCK_PCG(MW)                                                                             \
+    return fp64_vcycle ? block_pcg_mw<MW, double>(c, A, mg, B, X, rel_tol, maxiter, check_every) \
+                       : block_pcg_mw<MW, float>(c, A, mg, B, X, rel_tol, maxiter, check_every);
     switch (mw) {
-        case 8: return block_pcg_mw<8>(c, A, mg, B, X, rel_tol, maxiter, check_every);
-        case 16: return block_pcg_mw<16>(c, A, mg, B, X, rel_tol, maxiter, check_every);
-        case 32: return block_pcg_mw<32>(c, A, mg, B, X, rel_tol, maxiter, check_every);
-        case 64: return block_pcg_mw<64>(c, A, mg, B, X, rel_tol, maxiter, check_every);
+        case 8: MK_BLOCK_PCG(8)
+        case 16: MK_BLOCK_PCG(16)
+        case 32: MK_BLOCK_PCG(32)
+        case 64: MK_BLOCK_PCG(64)
         default: throw Error(MK_ERR_ARG, "block PCG: block width must be 8, 16, 32 or 64");
     }
+#undef MK_BLOCK_PCG
 }
 
 }  // namespace mk

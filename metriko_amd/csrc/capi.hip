@@ -441,6 +441,9 @@ mk_status mk_init(mk_ctx** out, int device) {
         c->device = device;
         if (const char* g = std::getenv("METRIKO_GRAPHS")) c->use_graphs = std::atoi(g) != 0;
         MK_HIP(hipSetDevice(device));
+        // the allocation-free solve's scratch on this device (a failure is reported by that
+        // entry point, not here)
+        (void)mk::noalloc_setup(device, nullptr);
         MK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         // per-block partial sums of the two-launch grid reductions (up to kMaxK quantities of
         // kMaxParts blocks each)
@@ -700,7 +703,8 @@ mk_status mk_spd_solve_many(mk_ctx* c, const mk_op* A, int m, const double* B, d
         const size_t n = size_t(A->n), len = n * (A->cplx ? 2 : 1);
         int iters = 0, solves = 0;
         double worst = 0.0;
-        const bool batched = !A->cplx && A->mg && o.precond == MK_PRECOND_AMG && o.precision != MK_PREC_FP64;
+        // every precision mode batches the columns; MK_PREC_FP64 runs the V-cycle in FP64
+        const bool batched = !A->cplx && A->mg && o.precond == MK_PRECOND_AMG;
         if (batched && m > 0) {
             const int mw = m > 32 ? 64 : m > 16 ? 32 : m > 8 ? 16 : 8;
             DevBuf<double> dB(n * size_t(std::min(m, mw))), dX(n * size_t(std::min(m, mw)));
@@ -710,7 +714,8 @@ mk_status mk_spd_solve_many(mk_ctx* c, const mk_op* A, int m, const double* B, d
                 dB.upload(B + size_t(b0) * n, size_t(cnt) * n, c->stream);
                 hipLaunchKernelGGL(k_pack_cols, dim3(unsigned((n * mw + 255) / 256)), dim3(256), 0, c->stream, n, mw,
                                    cnt, (const double*)dB.p, Bp.p);
-                mk::BlockResult r = mk::block_pcg(c, A, A->mg, mw, Bp.p, Xp.p, o.tol, o.maxiter, o.check_every);
+                mk::BlockResult r = mk::block_pcg(c, A, A->mg, mw, Bp.p, Xp.p, o.tol, o.maxiter, o.check_every,
+                                                  o.precision == MK_PREC_FP64);
                 hipLaunchKernelGGL(k_unpack_block, dim3(unsigned((n * mw + 255) / 256)), dim3(256), 0, c->stream, n,
                                    mw, cnt, (const double*)Xp.p, dX.p);
                 dX.download(X + size_t(b0) * n, size_t(cnt) * n, c->stream);

@@ -273,9 +273,10 @@ std::vector<double> dense_solve(std::vector<double> M, std::vector<double> b, in
 // LMSolver::solve (levenberg_marquardt.h:140-231) with lambda = 0, verbose = false.
 // dir(prevx) writes the direction into d (device); energy(x) returns ||E(x)||^2.
 // Returns 1/0 like the reference; x_out receives the solution (device, may alias prevx buffer).
-template <typename Dir, typename Energy>
+// post(x) -> true ends the solve after an iteration (ST->post_iteration, levenberg_marquardt.h:221-224).
+template <typename Dir, typename Energy, typename Post>
 bool lm_solve(Ctx& K, size_t n, double* x0, double* x_out, int maxIter, double funcTol, Dir&& dir, Energy&& energy,
-              DevBuf<double>& prevx, DevBuf<double>& d, DevBuf<double>& trial) {
+              Post&& post, DevBuf<double>& prevx, DevBuf<double>& d, DevBuf<double>& trial) {
     prevx.ensure(n); d.ensure(n); trial.ensure(n);
     MK_HIP(hipMemcpyAsync(prevx.p, x0, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
     int it = 0;
@@ -291,6 +292,7 @@ bool lm_solve(Ctx& K, size_t n, double* x0, double* x_out, int maxIter, double f
             MK_HIP(hipMemcpyAsync(prevx.p, trial.p, n * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
             if (std::abs(e_prev - e_new) < funcTol) { ret = false; break; }
         }
+        if (post(prevx.p)) { ret = true; break; }
         it++;
         if (!(it <= maxIter)) { ret = false; break; }
     }
@@ -742,7 +744,8 @@ struct InjSolver {
         ws.sync(K.st());
         std::vector<double> M(static_cast<size_t>(m) * m), col(static_cast<size_t>(m));
         dd.ensure(size_t(m));
-        const bool blk = !jacobi && K.o.precision != MK_PREC_FP64 && h.op->mg;
+        // every precision mode batches the columns; MK_PREC_FP64 runs the V-cycle in FP64
+        const bool blk = !jacobi && h.op->mg;
         if (blk) {
             // the Woodbury columns as blocks of up to 64 right-hand sides (one matrix read per step)
             const int mwmax = m > 32 ? 64 : m > 16 ? 32 : m > 8 ? 16 : 8;
@@ -758,7 +761,8 @@ struct InjSolver {
                 hipLaunchKernelGGL(k_scatter_rows_packed, dim3(unsigned(mw)), dim3(64), 0, K.st(), mw, dsel.p, ws.ptr.p,
                                    ws.col.p, ws.val.p, Bp.p);
                 const mk::BlockResult br = mk::block_pcg(K.c, h.op, h.op->mg, mw, Bp.p, Xp.p, K.o.tol,
-                                                         std::min(K.o.maxiter, 1000), K.o.check_every);
+                                                         std::min(K.o.maxiter, 1000), K.o.check_every,
+                                                         K.o.precision == MK_PREC_FP64);
                 K.acc.iterations += br.iters;
                 K.acc.solves += cnt;
                 if (br.status == MK_ERR_ARG || br.status == MK_ERR_HIP || br.status == MK_ERR_ALLOC ||
@@ -1281,7 +1285,24 @@ void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, 
             dNF.alloc(size_t(ck.rows));
             dflag.alloc(size_t(ck.nF));
         }
+        // a host callback (mk_set_naive_iter_callback) is the iter_cb when set: it gets
+        // fullx = UFull x of the iterate's x-block
+        OpHolder hU;
+        DevBuf<double> dfull;
+        std::vector<double> hfull;
+        if (K.c->naive_cb) {
+            hU.op = make_rect_op(K.c, *in.UF, K.o);
+            hU.op->label = "igm_iter_cb";
+            dfull.alloc(size_t(in.nfull));
+            hfull.resize(size_t(in.nfull));
+        }
         auto post = [&](const double* v) {
+            if (K.c->naive_cb) {
+                mk::spmv64(K.c, hU.op, v, dfull.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
+                dfull.download(hfull.data(), hfull.size(), K.st());
+                K.sync();
+                return K.c->naive_cb(K.c->naive_user, hfull.data(), in.nfull) != 0;
+            }
             if (!checked) return false;
             mk::spmv64(K.c, hN.op, v, dNF.p, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
             hipLaunchKernelGGL(k_flips, dim3(nblk(size_t(ck.nF))), dim3(256), 0, K.st(), ck.nF, ck.N, didx.p, dNF.p,
@@ -1598,7 +1619,12 @@ extern "C" mk_status mk_iterative_rounding(
                 return wIn * wIn * a + wCn * wCn * b + wIn * wIn * fixres;
             };
             DevBuf<double> F2_(l1);
-            lm_solve(K, l1, fcur.p, F2_.p, 100, 10e-10, dir, energy, pv, dd, tr);
+            // iter_cb after every naive iteration: the x-block stays at XSmall in this pass, so
+            // the callback sees x0 = UFull XSmall each time
+            auto post = [&](const double*) {
+                return c->naive_cb ? c->naive_cb(c->naive_user, x0.data(), nfull) != 0 : false;
+            };
+            lm_solve(K, l1, fcur.p, F2_.p, 100, 10e-10, dir, energy, post, pv, dd, tr);
             S.naive_ms = now_ms() - t0;
             MK_LOG("[igm] naive LM %.0f ms\n", S.naive_ms);
 
@@ -1636,8 +1662,9 @@ extern "C" mk_status mk_iterative_rounding(
             std::vector<int> left = flags->round_seams ? integer : singular;
             bool roundedSingulars = false, rounded = false;
             // Batched path: every candidate the loop can fix gets its column up front, in blocks
-            // of up to 64 right-hand sides (FP64 PCG, FP32 V-cycle on the whole block).
-            if (o.precision != MK_PREC_FP64 && o.precond == MK_PRECOND_AMG && hA0.op->mg && nc_ > 0) {
+            // of up to 64 right-hand sides (FP64 PCG, V-cycle on the whole block: FP32, or FP64
+            // in MK_PREC_FP64).
+            if (o.precond == MK_PRECOND_AMG && hA0.op->mg && nc_ > 0) {
                 std::vector<int> need;  // candidate positions, in candidate order
                 std::vector<uint8_t> mark(nc_, 0);
                 for (int i : left) mark[size_t(candPos[i])] = 1;
@@ -1660,7 +1687,7 @@ extern "C" mk_status mk_iterative_rounding(
                     hipLaunchKernelGGL(k_scatter_rows_packed, dim3(unsigned(mw)), dim3(64), 0, K.st(), mw, dsel.p,
                                        cand.ptr.p, cand.col.p, cand.val.p, Bp.p);
                     mk::BlockResult br = mk::block_pcg(c, hA0.op, hA0.op->mg, mw, Bp.p, Xp.p, o.tol, o.maxiter,
-                                                       o.check_every);
+                                                       o.check_every, o.precision == MK_PREC_FP64);
                     K.acc.iterations += br.iters;
                     K.acc.solves += cnt;
                     S.batched_solves += cnt;
@@ -1806,7 +1833,8 @@ extern "C" mk_status mk_iterative_rounding(
                 };
                 {
                     const double t1 = now_ms();
-                    lm_solve(K, size_t(k), dxcur.p, dxlm.p, 100, 1e-7, dir, energy, pv2, dd2, tr2);
+                    lm_solve(K, size_t(k), dxcur.p, dxlm.p, 100, 1e-7, dir, energy, [](const double*) { return false; },
+                             pv2, dd2, tr2);
                     tm_lm += now_ms() - t1;
                 }
                 // post_checking (iter_rounding_loop.h:89-100)
@@ -1982,6 +2010,13 @@ mk_status mk_igm_injectivity_check(mk_ctx* c, int N, int nF, const int32_t* cut_
     d.col.assign(f_col, f_col + f_ptr[f_rows]);
     d.val.assign(f_val, f_val + f_ptr[f_rows]);
     c->inj_check = std::move(d);
+    return MK_OK;
+}
+
+mk_status mk_set_naive_iter_callback(mk_ctx* c, mk_naive_iter_cb cb, void* user) {
+    if (!c) return MK_ERR_ARG;
+    c->naive_cb = cb;
+    c->naive_user = cb ? user : nullptr;
     return MK_OK;
 }
 

@@ -35,7 +35,8 @@ struct MgLevel {
     int n = 0;
     DevBuf<double> b64, x64, r64, d64, y64;  // y: second iterate buffer of the fused smoother
     DevBuf<float> b32, x32, r32, d32, y32;
-    DevBuf<float> bb32, bx32, br32, bd32, by32;  // n x mw blocks for the batched V-cycle
+    DevBuf<float> bb32, bx32, br32, by32;  // n x mw blocks for the batched V-cycle (FP32)
+    DevBuf<double> bb64, bx64, br64, by64;  // the same for the FP64 batched V-cycle
 };
 
 struct Mg {
@@ -844,86 +845,97 @@ void vcycle(mk_ctx* c, Mg* g, int l, const T* b, T* x, const int* skip) {
     chebyshev<T>(c, g, L, b, x, false, skip);
 }
 
-// ---- batched V-cycle (n x mw row-major blocks, FP32) -------------------------------------------
-__global__ void k_bcheb_first(size_t len, int sh, const float* __restrict__ dinv, const float* __restrict__ r,
-                              float* __restrict__ d, float* __restrict__ x, float c0, int zero_start, const int* skip) {
+// ---- batched V-cycle (n x mw row-major blocks, FP32 or FP64) --------------------------------------
+// x = c0 D^-1 r: the first Chebyshev iterate from a zero start
+template <typename T>
+__global__ void k_bcheb_first(size_t len, int sh, const T* __restrict__ dinv, const T* __restrict__ r,
+                              T* __restrict__ x, T c0, const int* skip) {
     if (skip && *skip) return;
     const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e >= len) return;
-    const float v = c0 * dinv[e >> sh] * r[e];
-    d[e] = v;
-    if (zero_start) x[e] = v;
-    else x[e] += v;
-}
-__global__ void k_bcheb_next(size_t len, int sh, const float* __restrict__ dinv, const float* __restrict__ r,
-                             float* __restrict__ d, float* __restrict__ x, float a, float bcoef, const int* skip) {
-    if (skip && *skip) return;
-    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e >= len) return;
-    const float v = a * d[e] + bcoef * dinv[e >> sh] * r[e];
-    d[e] = v;
-    x[e] += v;
+    x[e] = c0 * dinv[e >> sh] * r[e];
 }
 // X = Inv B on the coarsest level, one thread per (row, column)
-__global__ void k_bdense(int nc, int sh, const float* __restrict__ inv, const float* __restrict__ b,
-                         float* __restrict__ x, const int* skip) {
+template <typename T>
+__global__ void k_bdense(int nc, int sh, const T* __restrict__ inv, const T* __restrict__ b, T* __restrict__ x,
+                         const int* skip) {
     if (skip && *skip) return;
     const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t mw = size_t(1) << sh;
     if (e >= size_t(nc) * mw) return;
     const size_t i = e >> sh, j = e & (mw - 1);
-    float a = 0.f;
+    T a = T(0);
     for (int k = 0; k < nc; k++) a += inv[i * size_t(nc) + k] * b[size_t(k) * mw + j];
     x[e] = a;
 }
 
-static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, const float* b, float* x,
-                            bool zero_start, const int* skip) {
+// the block buffers of a level in precision T
+template <typename T>
+struct BlockBufs { T *b, *x, *r, *y; };
+template <typename T>
+static BlockBufs<T> block_bufs(MgLevel& L) {
+    if constexpr (std::is_same<T, double>::value) return {L.bb64.p, L.bx64.p, L.br64.p, L.by64.p};
+    else return {L.bb32.p, L.bx32.p, L.br32.p, L.by32.p};
+}
+
+// Chebyshev(deg) smoothing of the block in the three-term form: after a zero start the first
+// iterate is c0 D^-1 b (k_bcheb_first) and every later step is one fused kernel (residual and
+// update) that writes over the buffer holding the iterate before the one it reads; the two
+// buffers x and y alternate, the first chosen so that the last step lands in x.
+template <typename T>
+static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, const T* b, T* x, bool zero_start,
+                            const int* skip) {
     const double lmax = L.lmax, lmin = L.lmax / tune().lmin_div;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho0 = 1.0 / sigma;
     const size_t len = size_t(L.n) * mw;
-    float* d = L.bd32.p;
-    float* y = L.by32.p;
-    // Every step after a zero start is one fused kernel (residual + update, x_in -> x_out);
-    // the iterate ping-pongs between x and y, and the first buffer is chosen so that the
-    // last step lands in x.
-    float* cur = x;
+    T* y = block_bufs<T>(L).y;
+    const T* dinv;
+    if constexpr (std::is_same<T, double>::value) dinv = L.A->dinv64.p;
+    else dinv = L.A->dinv32.p;
+    T* cur = x;
     int fused = g->deg;
     if (zero_start) {
         cur = ((g->deg - 1) % 2 == 0) ? x : y;
-        hipLaunchKernelGGL(k_bcheb_first, dim3(nb(len)), dim3(256), 0, c->stream, len, sh, (const float*)L.A->dinv32.p,
-                           b, d, cur, float(1.0 / theta), 1, skip);
+        hipLaunchKernelGGL((k_bcheb_first<T>), dim3(nb(len)), dim3(256), 0, c->stream, len, sh, dinv, b, cur,
+                           T(1.0 / theta), skip);
         fused = g->deg - 1;
     }
     for (int k = 0; k < fused; k++) {
-        float* out = (cur == x) ? y : x;
+        T* out = (cur == x) ? y : x;
         if (!zero_start && k == 0) {
-            spmm_cheb_launch(c, L.A, mw, cur, out, b, d, 0.0f, float(1.0 / theta), true, skip);
+            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(0), T(1.0 / theta), 5, skip);
         } else {
             const double rho1 = 1.0 / (2.0 * sigma - rho0);
-            spmm_cheb_launch(c, L.A, mw, cur, out, b, d, float(rho1 * rho0), float(2.0 * rho1 / delta), false, skip);
+            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(rho1 * rho0), T(2.0 * rho1 / delta),
+                                (zero_start && k == 0) ? 7 : 6, skip);
             rho0 = rho1;
         }
         cur = out;
     }
-    if (cur != x) copy_launch<float>(c, len, cur, x, skip);
+    if (cur != x) copy_launch<T>(c, len, cur, x, skip);
 }
 
-static void vcycle_block(mk_ctx* c, Mg* g, int l, int mw, int sh, const float* b, float* x, const int* skip) {
+template <typename T>
+static void vcycle_block(mk_ctx* c, Mg* g, int l, int mw, int sh, const T* b, T* x, const int* skip) {
     MgLevel& L = g->lv[size_t(l)];
     if (l + 1 == int(g->lv.size())) {
-        hipLaunchKernelGGL(k_bdense, dim3(nb(size_t(g->nc) * mw)), dim3(256), 0, c->stream, g->nc, sh,
-                           (const float*)g->inv32.p, b, x, skip);
+        const T* inv;
+        if constexpr (std::is_same<T, double>::value) inv = g->inv64.p;
+        else inv = g->inv32.p;
+        hipLaunchKernelGGL((k_bdense<T>), dim3(nb(size_t(g->nc) * mw)), dim3(256), 0, c->stream, g->nc, sh, inv, b, x,
+                           skip);
         return;
     }
-    chebyshev_block(c, g, L, mw, sh, b, x, true, skip);
-    spmm_launch<float>(c, L.A, mw, x, L.br32.p, b, SPMV_RESID_NORED, skip);
+    const BlockBufs<T> Lb = block_bufs<T>(L);
+    chebyshev_block<T>(c, g, L, mw, sh, b, x, true, skip);
+    spmm_launch<T>(c, L.A, mw, x, Lb.r, b, SPMV_RESID_NORED, skip);
     MgLevel& Cn = g->lv[size_t(l) + 1];
-    spmm_launch<float>(c, L.R, mw, L.br32.p, Cn.bb32.p, nullptr, SPMV_PLAIN, skip);
-    vcycle_block(c, g, l + 1, mw, sh, Cn.bb32.p, Cn.bx32.p, skip);
-    spmm_launch<float>(c, L.P, mw, Cn.bx32.p, x, nullptr, SPMV_ACCUM, skip);
-    chebyshev_block(c, g, L, mw, sh, b, x, false, skip);
+    const BlockBufs<T> Cb = block_bufs<T>(Cn);
+    spmm_launch<T>(c, L.R, mw, Lb.r, Cb.b, nullptr, SPMV_PLAIN, skip);
+    vcycle_block<T>(c, g, l + 1, mw, sh, Cb.b, Cb.x, skip);
+    spmm_launch<T>(c, L.P, mw, Cb.x, x, nullptr, SPMV_ACCUM, skip);
+    chebyshev_block<T>(c, g, L, mw, sh, b, x, false, skip);
 }
 
 // ---- host V-cycle / PCG (test and tuning aid; same algorithm as the device V-cycle) ----------
@@ -1081,18 +1093,26 @@ Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const
 }
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { vcycle<float>(c, g, 0, b, x, skip); }
-void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const int* skip) {
+template <typename T>
+void mg_apply_block(mk_ctx* c, Mg* g, int mw, const T* b, T* x, const int* skip) {
     MK_CHECK(!g->cplx, MK_ERR_UNSUPPORTED, "batched V-cycle: real hierarchies only");
     MK_CHECK(mw == 8 || mw == 16 || mw == 32 || mw == 64, MK_ERR_ARG, "batched V-cycle: block width");
     const int sh = mw == 8 ? 3 : mw == 16 ? 4 : mw == 32 ? 5 : 6;
     for (size_t l = 0; l < g->lv.size(); l++) {
         MgLevel& L = g->lv[l];
         const size_t len = size_t(L.n) * mw;
-        if (l > 0) { L.bb32.ensure(len); L.bx32.ensure(len); }
-        if (l + 1 < g->lv.size()) { L.br32.ensure(len); L.bd32.ensure(len); L.by32.ensure(len); }
+        if constexpr (std::is_same<T, double>::value) {
+            if (l > 0) { L.bb64.ensure(len); L.bx64.ensure(len); }
+            if (l + 1 < g->lv.size()) { L.br64.ensure(len); L.by64.ensure(len); }
+        } else {
+            if (l > 0) { L.bb32.ensure(len); L.bx32.ensure(len); }
+            if (l + 1 < g->lv.size()) { L.br32.ensure(len); L.by32.ensure(len); }
+        }
     }
-    vcycle_block(c, g, 0, mw, sh, b, x, skip);
+    vcycle_block<T>(c, g, 0, mw, sh, b, x, skip);
 }
+template void mg_apply_block<float>(mk_ctx*, Mg*, int, const float*, float*, const int*);
+template void mg_apply_block<double>(mk_ctx*, Mg*, int, const double*, double*, const int*);
 int mg_levels(const Mg* g) { return int(g->lv.size()); }
 uint64_t mg_uid(const Mg* g) { return g ? g->uid : 0; }
 void mg_set_labels(Mg* g, const std::string& base) {

@@ -112,12 +112,17 @@ Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const do
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip);
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
-// V-cycle on an n x mw row-major block (FP32)
-void mg_apply32_block(mk_ctx* c, Mg* g, int mw, const float* b, float* x, const int* skip);
+// V-cycle on an n x mw row-major block (T = float: FP32 values and vectors; double: FP64)
+template <typename T>
+void mg_apply_block(mk_ctx* c, Mg* g, int mw, const T* b, T* x, const int* skip);
 int mg_levels(const Mg* g);
 // process-unique id of a hierarchy (never reused after it is freed; keys captured graphs)
 uint64_t mg_uid(const Mg* g);
 void mg_set_labels(Mg* g, const std::string& base);
+// looks up the current device's copy of the allocation-free solve's static scratch (once per
+// device; mk_init calls it with the device current); a failure is kept and reported by
+// mk_spd_solve_noalloc
+mk_status noalloc_setup(int device, std::string* err);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
 void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
 mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector<int>&& col,
@@ -163,6 +168,9 @@ struct mk_ctx {
     int64_t graph_replays = 0, graph_captures = 0;
     // the next iterative rounding keeps its last LM round's system (mk_keep_igm_state)
     bool keep_igm_state = false;
+    // the naive LM pass's iter_cb (mk_set_naive_iter_callback); null = none
+    mk_naive_iter_cb naive_cb = nullptr;
+    void* naive_user = nullptr;
     struct mk_igm_state* igm_state = nullptr;
     mk_inj_check_data inj_check;
 };
@@ -253,13 +261,16 @@ template <typename T>
 void copy_launch(mk_ctx* c, size_t len, const T* x, T* y, const int* skip);
 template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip);
-// one Chebyshev smoothing step fused with its residual on a block of mw right-hand sides:
-// r = b - A x_in ; d = cs D^-1 r (first) or ca d + cs D^-1 r ; x_out = x_in + d
-void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const float* x_in, float* x_out, const float* b, float* d,
-                      float ca, float cs, bool first, const int* skip);
+// one three-term Chebyshev step fused with its residual on a block of mw right-hand sides,
+// r = b - A x_in: mode 5 x_out = x_in + cs D^-1 r; 6 x_out = x_in + ca (x_in - x_out) + cs D^-1 r
+// (x_out holds the previous iterate on entry); 7 x_out = (1 + ca) x_in + cs D^-1 r
+template <typename T>
+void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out, const T* b, T ca, T cs, int mode,
+                      const int* skip);
 // A X = B for mw right-hand sides (B, X: n x mw row-major, device); X starts from zero.
+// fp64_vcycle: the V-cycle in FP64 (the plain FP64 form of the same algorithm) instead of FP32
 BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
-                      int maxiter, int check_every);
+                      int maxiter, int check_every, bool fp64_vcycle = false);
 
 
 // values are multiplied by scale on upload
