@@ -21,18 +21,38 @@ def lap2d(nx, shift):
     return (sp.kron(T, sp.identity(nx)) + sp.kron(sp.identity(nx), T) + shift * sp.identity(nx * nx)).tocsr()
 
 
+def _hip():
+    """The HIP runtime the library already loaded (by its soname): the caller's own device
+    allocations and copies, made before and after the allocation-free call."""
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return hip
+
+
+H2D, D2H = 1, 2
+
+
 def noalloc_solve(ctx, op, b, o):
-    """b, x as device arrays owned by the caller (torch), then the allocation-free call."""
-    import torch
+    """b, x as device arrays owned by the caller, then the allocation-free call."""
+    hip = _hip()
     bp = op._pack(b)
-    bd = torch.tensor(bp, dtype=torch.float64, device="cuda")
-    xd = torch.zeros_like(bd)
-    torch.cuda.synchronize()
-    st = mk.Stats()
-    s = ctx.L.mk_spd_solve_noalloc(ctx.h, op.h, C.c_void_p(bd.data_ptr()), C.c_void_p(xd.data_ptr()),
-                                   C.byref(o), C.byref(st))
-    ctx._check(s, "mk_spd_solve_noalloc")
-    return op._unpack(xd.cpu().numpy()), mk.struct_dict(st)
+    nbytes = bp.size * 8
+    bd, xd = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(bd), nbytes) == 0 and hip.hipMalloc(C.byref(xd), nbytes) == 0
+    try:
+        x = np.zeros_like(bp)
+        assert hip.hipMemcpy(bd, bp.ctypes.data, nbytes, H2D) == 0
+        assert hip.hipMemcpy(xd, x.ctypes.data, nbytes, H2D) == 0
+        st = mk.Stats()
+        s = ctx.L.mk_spd_solve_noalloc(ctx.h, op.h, bd, xd, C.byref(o), C.byref(st))
+        ctx._check(s, "mk_spd_solve_noalloc")
+        assert hip.hipMemcpy(x.ctypes.data, xd, nbytes, D2H) == 0
+        return op._unpack(x), mk.struct_dict(st)
+    finally:
+        hip.hipFree(bd)
+        hip.hipFree(xd)
 
 
 @pytest.mark.parametrize("precond", [mk.PRECOND_JACOBI, mk.PRECOND_NONE])
