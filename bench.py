@@ -17,8 +17,9 @@ rounding loop).  ms_per_step is the whole pipeline (host mesh/seam/cut combinato
 Inputs enter through host memory (the C ABI takes host arrays).
 
 roofline: the kernel with the largest total time in one extra, event-timed step; achieved =
-its algorithmic bytes per launch / its average launch time (hipEvents on the library stream);
-traffic = HBM bytes per launch from the committed rocprofv3 --pmc summary when present.
+its algorithmic bytes summed over its launches / their summed time (hipEvents on the library
+stream); traffic = HBM bytes per launch from the committed rocprofv3 --pmc summary when present.
+field_spmv_fp64: the field operator's SpMV warm and with the caches flushed between launches.
 
 Prints ONE JSON line.  Only --gpus 1 is supported (one process, one GPU).
 """
@@ -87,20 +88,80 @@ def traffic_for(key):
 
 
 def as_specified(ctx, mk):
-    """The level-8 mesh with the unscaled N(0, 1e-3) jitter: field stage timed, then the size of
-    the IGM problem it leads to (host combinatorics)."""
+    """Configuration 5 as stated: the level-8 mesh with the unscaled N(0, 1e-3) jitter.  Its
+    field stage is timed in both precisions (operator assembly on the device, multigrid set-up,
+    50 solves), then the size of the IGM problem it leads to (host combinatorics); that IGM is
+    not run (16.9k singularities, 87.6k constraint rows, ~77.6k rounding rounds)."""
     V, F, u0 = workload(LEVEL, jitter=1e-3)
     he = mk.hmesh_halfedges(V, F)
-    t0 = time.perf_counter()
-    L = ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
-    x, st = ctx.smallest_eig(L, he["face_area"], u0, 50, mk.opts(precision=mk.PREC_MIXED_MG))
-    field_ms = 1e3 * (time.perf_counter() - t0)
+    res = {"jitter": 1e-3}
+    x = None
+    for name, prec in (("mixed", mk.PREC_MIXED_MG), ("fp64", mk.PREC_FP64)):
+        t0 = time.perf_counter()
+        L = ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
+        xs, st = ctx.smallest_eig(L, he["face_area"], u0, 50, mk.opts(precision=prec))
+        ms = 1e3 * (time.perf_counter() - t0)
+        L.close()
+        res[f"field_ms_{name}"] = round(ms, 1)
+        res[f"field_iters_{name}"] = st["iterations"]
+        res[f"field_rel_residual_{name}"] = st["rel_residual"]
+        if x is None:
+            x = xs
     c = mk.combinatorics(V, F, x)
-    return {"jitter": 1e-3, "field_ms": round(field_ms, 1), "field_iters": st["iterations"],
-            "singularities": c["nS"], "transitions": c["nT"], "constraint_rows": c["constraint_rows"],
-            "seam_edges": int(c["seam"].sum()),
-            "igm": "not run: the dense FullPivLU kernel of the constraint block and one greedy rounding "
-                   "round per integer variable (reference algorithm) do not finish at this size"}
+    res.update({"singularities": c["nS"], "transitions": c["nT"], "constraint_rows": c["constraint_rows"],
+                "seam_edges": int(c["seam"].sum()),
+                "igm": "not run: the dense FullPivLU kernel of the constraint block and one rounding round per "
+                       "integer variable (reference algorithm) do not finish at this size"})
+    return res
+
+
+def cold_spmv(ctx, mk, V, F, reps=20):
+    """The field operator's FP64 SpMV (complex, 1.31M rows, 5.24M entries, 152 MB algorithmic)
+    timed with hipEvents, warm (back to back) and cold (a 1 GiB buffer written between
+    repetitions, so neither the 256 MB Infinity Cache nor the L2s hold the operator or x)."""
+    import ctypes as C
+    # the HIP runtime the library loaded: a 1 GiB buffer written between repetitions
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    flush_bytes = 1 << 30
+    flush = C.c_void_p()
+    if hip.hipMalloc(C.byref(flush), flush_bytes) != 0:
+        raise RuntimeError("hipMalloc of the 1 GiB flush buffer failed")
+    he = mk.hmesh_halfedges(V, F)
+    L = ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
+    g = np.random.Generator(np.random.PCG64(7))
+    x = g.standard_normal(F.shape[0]) + 1j * g.standard_normal(F.shape[0])
+    out = {}
+    for name, cold in (("warm", False), ("cold", True)):
+        L.apply(x)
+        ctx.set_timing(True)
+        ctx.timing_report(reset=True)
+        for r in range(reps):
+            if cold:
+                hip.hipMemset(flush, r & 0xff, flush_bytes)
+                hip.hipDeviceSynchronize()
+            L.apply(x)
+        rep = [k for k in ctx.timing_report(reset=True) if k["key"] == "field_L:spmv:fp64"]
+        ctx.set_timing(False)
+        k = rep[0]
+        out[name] = {"us": round(1e3 * k["ms"] / k["calls"], 2), "bytes": round(k["bytes"] / k["calls"], 1),
+                     "GBps": round(k["bytes"] / (k["ms"] * 1e-3) / 1e9, 1)}
+    L.close()
+    hip.hipFree(flush)
+    out["frac_cold"] = round(out["cold"]["GBps"] / PEAK_HBM_GBS, 4)
+    return out
+
+
+def measured_cpu_config():
+    """The oracle's full run at the 1.31M-face workload on the GPU box's host, if its summary
+    (profiles/cpu_baseline_L8.json, written by oracle/time_full_config.py) is present."""
+    p = os.path.join(ROOT, "profiles", "cpu_baseline_L8.json")
+    try:
+        return json.load(open(p))
+    except (OSError, ValueError):
+        return None
 
 
 def run(ctx, V, F, u0, o):
@@ -118,6 +179,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-as-specified", action="store_true")
+    ap.add_argument("--no-cold-spmv", action="store_true")
     ap.add_argument("--level", type=int, default=LEVEL)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 1 GiB buffer three times (HBM counter calibration for rocprofv3 --pmc)")
@@ -155,8 +217,10 @@ def main():
         "value": round(statistics.median(solves), 1), "unit": "ms", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(statistics.median(walls), 1), "higher_is_better": False,
         "scaling": "weak", "vs_baseline": None, "dtype": "fp64 (fp32 multigrid)", "data": "synthetic",
-        "config": {"workload": f"bumpy icosphere L{args.level} ({V.shape[0]:,} v / {F.shape[0]:,} f, jitter "
-                               f"{jitter_for(args.level):g}), field + seamless IGM, ratio 0.03",
+        "config": {"workload": f"bumpy icosphere L{args.level} ({V.shape[0]:,} v / {F.shape[0]:,} f), field + "
+                               f"seamless IGM, ratio 0.03; DEVIATION from configuration 5: radial jitter "
+                               f"N(0, {jitter_for(args.level):g}) (the level-4 jitter scaled with the edge length) "
+                               f"instead of N(0, 1e-3), whose IGM is not run (see as_specified)",
                    "precision": "MK_PREC_MIXED_MG: FP64 flexible PCG, FP32 multigrid V-cycle, tol 1e-10",
                    "nS": info["nS"], "nT": info["nT"], "k": info["k"], "success": info["success"],
                    "field_ms": round(info["field"]["ms"] + info["field"]["setup_ms"], 1),
@@ -172,18 +236,21 @@ def main():
         run(ctx, V, F, u0, mixed)
         rep = ctx.timing_report(reset=True)
         ctx.set_timing(False)
+        # per kernel key: algorithmic bytes summed over its launches (launches of one key can
+        # move different amounts, e.g. the first Chebyshev step of a cycle) over the summed
+        # hipEvent time of those launches
         dom = max(rep, key=lambda r: r["ms"])
         avg_ms = dom["ms"] / max(dom["calls"], 1)
-        achieved = dom["bytes_per_call"] / (avg_ms * 1e-3) / 1e9
+        per_launch = dom["bytes"] / max(dom["calls"], 1)
+        achieved = dom["bytes"] / (dom["ms"] * 1e-3) / 1e9
         result["roofline"] = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(dom["key"]),
-            "kernel": dom["key"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": dom["bytes_per_call"],
+            "kernel": dom["key"], "avg_us": round(avg_ms * 1e3, 2), "bytes_per_launch": round(per_launch, 1),
             "launches": dom["calls"], "rows": dom["rows"]}
-        top = sorted(rep, key=lambda r: -r["ms"])[:6]
+        top = sorted(rep, key=lambda r: -r["ms"])[:8]
         result["top_kernels"] = [{"key": r["key"], "ms": round(r["ms"], 1), "calls": r["calls"],
-                                  "GBps": round(r["bytes_per_call"] / (r["ms"] / max(r["calls"], 1) * 1e-3) / 1e9, 1)}
-                                 for r in top]
+                                  "GBps": round(r["bytes"] / (r["ms"] * 1e-3) / 1e9, 1)} for r in top]
     if not args.no_fp64:
         o64, w64 = run(ctx, V, F, u0, fp64)
         a, b = outs[-1], o64
@@ -204,11 +271,20 @@ def main():
                 "decisions_same_order": a["decisions"] == b["decisions"]},
         }
         result["speedup_mixed_vs_fp64"] = round(result["fp64"]["solve_ms"] / result["value"], 3)
+    result["breakdown_ms"] = {
+        "field_setup": round(info["field"]["setup_ms"], 1), "field_solves": round(info["field"]["ms"], 1),
+        "igm_host_setup": round(info["igm"]["setup_ms"], 1), "igm_kkt": round(info["igm"]["kkt_ms"], 1),
+        "igm_naive": round(info["igm"]["naive_ms"], 1), "igm_rounding": round(info["igm"]["rounding_ms"], 1)}
     if not args.no_as_specified and args.level == LEVEL:
         result["as_specified"] = as_specified(ctx, mk)
+    if not args.no_cold_spmv:
+        result["field_spmv_fp64"] = cold_spmv(ctx, mk, V, F)
     ctx.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline()
+        full = measured_cpu_config()
+        if full:
+            result["cpu_baseline"]["measured_full_config"] = full
     print(json.dumps(result), flush=True)
 
 

@@ -9,7 +9,7 @@
 //   CholeskyWrapper (igm/iterative_rounding/iter_rounding.h:14-31)
 //   LMSolver + DiagonalDamping (solver/levenberg_marquardt.h:13-232)
 // Known answers: residuals of the solves; the smallest eigenvalue of the grid Laplacian
-// 4 - 2 cos(pi/(n+1)) - 2 cos(pi/(n+1)) + shift (Dirichlet n x n grid, M = I); the Rosenbrock
+// 4 - 4 cos(pi/(n+1)) + shift (Dirichlet n x n grid, M = I); the Rosenbrock
 // minimiser (1, 1).
 #include <cmath>
 #include <cstdio>
@@ -132,7 +132,7 @@ int main() {
         const double r = rel_residual(A, x, b);
         report("solveSquare", r <= 1e-10, r);
     }
-    const double lam_exact = 8.0 - 4.0 * std::cos(M_PI / (n + 1)) + 0.01;
+    const double lam_exact = 4.0 - 4.0 * std::cos(M_PI / (n + 1)) + 0.01;
     {   // solveSmallestEig(SprsC&, SprsC&): Rayleigh quotient -> smallest eigenvalue (M = I)
         SprsC L = grid<complex>(n, 0.01, 0.0, 0.0);
         SprsC M = SprsC::identity(L.rows());
