@@ -337,8 +337,9 @@ mk_status mk_autotune_spmv(mk_ctx* ctx, int n, const int64_t* rowptr, const int3
  * std::rand stream) and the u/v Poisson integration of its first direction g_u (root 0 in the
  * face basis) and g_v = n x g_u: min sum_f A_f |grad u - g_f|^2 with vertex 0 pinned to 0.
  * All meshes share one block-diagonal operator per stage (one launch per kernel for the whole
- * batch, field layout picked by mk_autotune_spmv on a row sample); every mesh's solves meet
- * opts->tol relative to its own right-hand side.  Outputs (either may be NULL per mesh):
+ * batch, the field and stiffness layouts each picked by mk_autotune_spmv on a row sample; every
+ * batch of PCG iterations replays one captured hipGraph over all meshes); every mesh's solves
+ * meet opts->tol relative to its own right-hand side.  Outputs (either may be NULL per mesh):
  * compressed[m] [2 nF[m]] interleaved, uv[m] [2 nV[m]] = (u, v) per vertex. */
 typedef struct {
     int meshes;
@@ -348,6 +349,11 @@ typedef struct {
     double ms_host_setup, ms_field, ms_poisson, ms_total;
     int tuned_slice_height, tuned_sigma, tuned_block;  /* layout picked for the field operator */
     double tuned_spmv_us;                              /* its SpMV time on the row sample */
+    int tuned_poisson_slice_height, tuned_poisson_sigma, tuned_poisson_block;  /* the stiffness operator's */
+    double tuned_poisson_spmv_us;
+    int64_t graph_replays;   /* PCG iteration batches replayed as one captured hipGraph each */
+    int64_t graph_captures;  /* graphs captured (one per operator and buffer set) */
+    int64_t eager_batches;   /* iteration batches launched kernel by kernel (graphs off or timing on) */
 } mk_batch_stats;
 mk_status mk_batch_run(mk_ctx* ctx, int nmesh, const int32_t* nV, const int32_t* nF, const double* const* V,
                        const int32_t* const* F, int N, int n_iter, const double* const* u0,

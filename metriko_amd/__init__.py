@@ -77,7 +77,10 @@ class BatchStats(C.Structure):
                 ("field_rel_residual", C.c_double), ("poisson_rel_residual", C.c_double),
                 ("ms_host_setup", C.c_double), ("ms_field", C.c_double), ("ms_poisson", C.c_double),
                 ("ms_total", C.c_double), ("tuned_slice_height", C.c_int), ("tuned_sigma", C.c_int),
-                ("tuned_block", C.c_int), ("tuned_spmv_us", C.c_double)]
+                ("tuned_block", C.c_int), ("tuned_spmv_us", C.c_double),
+                ("tuned_poisson_slice_height", C.c_int), ("tuned_poisson_sigma", C.c_int),
+                ("tuned_poisson_block", C.c_int), ("tuned_poisson_spmv_us", C.c_double),
+                ("graph_replays", C.c_int64), ("graph_captures", C.c_int64), ("eager_batches", C.c_int64)]
 
 
 def struct_dict(s):

@@ -366,30 +366,41 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
             for (size_t k = 0; k < d.scol.size(); k++) Sc[static_cast<size_t>(s0) + k] = int(v0) + d.scol[k];
             std::memcpy(Sv.data() + s0, d.sval.data(), sizeof(double) * d.sval.size());
         }
-        // ---- autotuned layout of the field operator (sample of its rows) ----
-        mk_solve_opts of = o;
-        {
-            const int ns = int(std::min<int64_t>(nFt, 262144));
+        // ---- autotuned layouts of the field and stiffness operators (samples of their rows) ----
+        // the leading rows (up to 262,144), couplings inside the sample kept
+        auto tune_layout = [&](int64_t nrows, const std::vector<int64_t>& P, const std::vector<int>& Cc,
+                               const std::vector<double>& Vv, int cplx, mk_solve_opts& best, double& us) {
+            const int ns = int(std::min<int64_t>(nrows, 262144));
+            const int w = cplx ? 2 : 1;
             std::vector<int64_t> sp(static_cast<size_t>(ns) + 1, 0);
             std::vector<int> sc;
             std::vector<double> sv;
             for (int i = 0; i < ns; i++) {
-                for (int64_t q = Lp[static_cast<size_t>(i)]; q < Lp[static_cast<size_t>(i) + 1]; q++)
-                    if (Lc[static_cast<size_t>(q)] < ns) {
-                        sc.push_back(Lc[static_cast<size_t>(q)]);
-                        sv.push_back(Lv[static_cast<size_t>(2 * q)]);
-                        sv.push_back(Lv[static_cast<size_t>(2 * q + 1)]);
+                for (int64_t q = P[static_cast<size_t>(i)]; q < P[static_cast<size_t>(i) + 1]; q++)
+                    if (Cc[static_cast<size_t>(q)] < ns) {
+                        sc.push_back(Cc[static_cast<size_t>(q)]);
+                        for (int k = 0; k < w; k++) sv.push_back(Vv[static_cast<size_t>(w * q + k)]);
                     }
                 sp[static_cast<size_t>(i) + 1] = int64_t(sc.size());
             }
+            MK_CHECK(mk_autotune_spmv(c, ns, sp.data(), sc.data(), sv.data(), cplx, &o, &best, &us) == MK_OK,
+                     MK_ERR_HIP, "autotune: " + c->last_error);
+        };
+        mk_solve_opts of = o, osl = o;
+        {
             double us = 0.0;
-            MK_CHECK(mk_autotune_spmv(c, ns, sp.data(), sc.data(), sv.data(), 1, &o, &of, &us) == MK_OK, MK_ERR_HIP,
-                     "autotune: " + c->last_error);
+            tune_layout(nFt, Lp, Lc, Lv, 1, of, us);
             S.tuned_slice_height = of.slice_height;
             S.tuned_sigma = of.sigma;
             S.tuned_block = of.block_size;
             S.tuned_spmv_us = us;
+            tune_layout(nVt, Sp, Sc, Sv, 0, osl, us);
+            S.tuned_poisson_slice_height = osl.slice_height;
+            S.tuned_poisson_sigma = osl.sigma;
+            S.tuned_poisson_block = osl.block_size;
+            S.tuned_poisson_spmv_us = us;
         }
+        const int64_t replays0 = c->graph_replays, captures0 = c->graph_captures, eager0 = c->eager_batches;
         S.ms_host_setup = now_ms() - t_all;
         // ---- field: batched inverse iteration ----
         double t0 = now_ms();
@@ -528,7 +539,7 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
             bu[0] = 0.0;  // pinned vertex
             bv[0] = 0.0;
         }
-        mk_op* Sop = mk::op_create_owned(c, int(nVt), std::move(Sp), std::move(Sc), std::move(Sv), o);
+        mk_op* Sop = mk::op_create_owned(c, int(nVt), std::move(Sp), std::move(Sc), std::move(Sv), osl);
         std::unique_ptr<mk_op, void (*)(mk_op*)> Shold(Sop, [](mk_op* p) { mk_op_destroy(p); });
         Sop->label = "batch_stiffness";
         mk::ensure_mg(c, Sop, o);
@@ -566,6 +577,9 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
         }
         S.ms_poisson = now_ms() - t0;
         S.ms_total = now_ms() - t_all;
+        S.graph_replays = c->graph_replays - replays0;
+        S.graph_captures = c->graph_captures - captures0;
+        S.eager_batches = c->eager_batches - eager0;
         if (stats) *stats = S;
         return MK_OK;
     } catch (const Error& e) {

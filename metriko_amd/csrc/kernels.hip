@@ -1008,6 +1008,7 @@ static void run_batch(mk_ctx* c, const mk_op* A_, const Mg* mg, std::vector<uint
     MK_CHECK(A->ctx == c, MK_ERR_ARG, "operator used with a context other than the one it was created on");
     if (!c->use_graphs || c->time_spmv || !graphable) {
         for (int b = 0; b < batch; b++) body();
+        c->eager_batches++;
         return;
     }
     GraphKeyParts kp;

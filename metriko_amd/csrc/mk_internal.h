@@ -165,7 +165,7 @@ struct mk_ctx {
     int64_t spmv_calls = 0;
     double spmv_bytes = 0.0;
     bool use_graphs = true;  // replay captured PCG iteration batches
-    int64_t graph_replays = 0, graph_captures = 0;
+    int64_t graph_replays = 0, graph_captures = 0, eager_batches = 0;
     // the next iterative rounding keeps its last LM round's system (mk_keep_igm_state)
     bool keep_igm_state = false;
     // the naive LM pass's iter_cb (mk_set_naive_iter_callback); null = none

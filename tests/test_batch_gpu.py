@@ -55,6 +55,9 @@ def test_batch_64_meshes(gpu_ctx):
     print(st)
     assert st["meshes"] == 64 and st["faces"] == 2708954
     assert st["field_rel_residual"] <= 1e-10 and st["poisson_rel_residual"] <= 1e-10
+    # every batch of PCG iterations over all 64 meshes replayed one captured hipGraph
+    assert st["graph_replays"] > 0 and st["eager_batches"] == 0
+    assert st["tuned_poisson_slice_height"] in (64, 128, 256) and st["tuned_poisson_spmv_us"] > 0.0
     checked = 0
     for (V, F), u0, o in zip(ms, u0s, out):
         m_area = Hmesh(V, F).faceArea if V.shape[0] <= 2562 else None
