@@ -19,7 +19,8 @@ Inputs enter through host memory (the C ABI takes host arrays).
 roofline: the kernel with the largest total time in one extra, event-timed step; achieved =
 its algorithmic bytes summed over its launches / their summed time (hipEvents on the library
 stream); traffic = HBM bytes per launch from the committed rocprofv3 --pmc summary when present.
-field_spmv_fp64: the field operator's SpMV warm and with the caches flushed between launches.
+field_spmv_fp64: the field operator's SpMV warm, after a cache flush, and streamed round robin
+over copies whose total exceeds the Infinity Cache.
 
 Prints ONE JSON line.  Only --gpus 1 is supported (one process, one GPU).
 """
@@ -75,6 +76,44 @@ def cpu_baseline():
     }
 
 
+def cpu_baseline_full(level=LEVEL):
+    """The CPU oracle (numpy/scipy SuperLU) once over the whole benchmark workload: the same
+    mesh, field, seam, cut and seamless IGM as one GPU step, on the host this runs on, one
+    thread (SuperLU has no threaded path; BLAS limited to one thread).  Prints each oracle stage
+    as it ends (minutes in total) and returns the summary written to gpurun_out/."""
+    import platform
+    import scipy
+    from threadpoolctl import threadpool_limits
+    from oracle import progress
+    from oracle.pipeline import run_pipeline
+    progress.hook = lambda m: print(m, file=sys.stderr, flush=True)
+    V, F, u0 = workload(level)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        r = run_pipeline(V, F, ratio=0.03, u0=u0)
+        wall = time.perf_counter() - t0
+    t = r["times"]
+    return {
+        "workload": f"bumpy icosphere L{level} ({V.shape[0]:,} v / {F.shape[0]:,} f), jitter N(0, {jitter_for(level):g}), "
+                    f"field + seamless IGM, ratio 0.03 (the GPU benchmark's step)",
+        "field_plus_igm_ms": round(1e3 * (t["field"] + t["igm"]), 1), "field_ms": round(1e3 * t["field"], 1),
+        "igm_ms": round(1e3 * t["igm"], 1), "hmesh_ms": round(1e3 * t["hmesh"], 1),
+        "combinatorics_ms": round(1e3 * t["combinatorics"], 1), "pipeline_wall_ms": round(1e3 * wall, 1),
+        "nS": int(r["nS"]), "nT": int(r["nT"]), "success": bool(r["success"]),
+        "rounding_decisions": len(r["decisions"]), "threads": 1, "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)), "cpu_model": model, "python": platform.python_version(),
+        "numpy": np.__version__, "scipy": scipy.__version__, "kind": "port",
+    }
+
+
 def traffic_for(key):
     """HBM bytes per launch of the kernel behind `key`, from profiles/traffic.json (written from
     the rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs), if present."""
@@ -115,12 +154,19 @@ def as_specified(ctx, mk):
     return res
 
 
-def cold_spmv(ctx, mk, V, F, reps=20):
-    """The field operator's FP64 SpMV (complex, 1.31M rows, 5.24M entries, 152 MB algorithmic)
-    timed with hipEvents, warm (back to back) and cold (a 1 GiB buffer written between
-    repetitions, so neither the 256 MB Infinity Cache nor the L2s hold the operator or x)."""
+def cold_spmv(ctx, mk, V, F, reps=24, copies=6):
+    """The field operator's FP64 SpMV (complex, 1.31M rows, 5.24M entries, 147 MB algorithmic)
+    timed with hipEvents in three ways:
+    warm   back to back on one operator (the operator stays in the 256 MB Infinity Cache);
+    flush  a 1 GiB buffer written (hipMemset) between repetitions: neither the Infinity Cache
+           nor the L2s hold the operator, but the flush leaves them full of dirty lines that
+           are written back while the SpMV runs;
+    stream round robin over `copies` separate copies of the operator (6 x 125 MB of matrix,
+           three times the Infinity Cache), so every SpMV reads its matrix from HBM, with no
+           flush traffic: the operator-streaming rate of a solver whose working set exceeds
+           the caches.
+    frac_cold is the streaming rate over the 8 TB/s peak."""
     import ctypes as C
-    # the HIP runtime the library loaded: a 1 GiB buffer written between repetitions
     hip = C.CDLL("libamdhip64.so.7")
     hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
@@ -130,33 +176,35 @@ def cold_spmv(ctx, mk, V, F, reps=20):
     if hip.hipMalloc(C.byref(flush), flush_bytes) != 0:
         raise RuntimeError("hipMalloc of the 1 GiB flush buffer failed")
     he = mk.hmesh_halfedges(V, F)
-    L = ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4)
+    Ls = [ctx.field_laplacian(he["face_halfs"], he["twin"], he["half_face"], he["farg"], 4) for _ in range(copies)]
     g = np.random.Generator(np.random.PCG64(7))
     x = g.standard_normal(F.shape[0]) + 1j * g.standard_normal(F.shape[0])
     out = {}
-    for name, cold in (("warm", False), ("cold", True)):
-        L.apply(x)
+    for name in ("warm", "flush", "stream"):
+        for L in Ls:
+            L.apply(x)
         ctx.set_timing(True)
         ctx.timing_report(reset=True)
         for r in range(reps):
-            if cold:
+            if name == "flush":
                 hip.hipMemset(flush, r & 0xff, flush_bytes)
                 hip.hipDeviceSynchronize()
-            L.apply(x)
+            (Ls[r % copies] if name == "stream" else Ls[0]).apply(x)
         rep = [k for k in ctx.timing_report(reset=True) if k["key"] == "field_L:spmv:fp64"]
         ctx.set_timing(False)
         k = rep[0]
         out[name] = {"us": round(1e3 * k["ms"] / k["calls"], 2), "bytes": round(k["bytes"] / k["calls"], 1),
                      "GBps": round(k["bytes"] / (k["ms"] * 1e-3) / 1e9, 1)}
-    L.close()
+    for L in Ls:
+        L.close()
     hip.hipFree(flush)
-    out["frac_cold"] = round(out["cold"]["GBps"] / PEAK_HBM_GBS, 4)
+    out["frac_cold"] = round(out["stream"]["GBps"] / PEAK_HBM_GBS, 4)
     return out
 
 
 def measured_cpu_config():
     """The oracle's full run at the 1.31M-face workload on the GPU box's host, if its summary
-    (profiles/cpu_baseline_L8.json, written by oracle/time_full_config.py) is present."""
+    (profiles/cpu_baseline_L8.json, the summary `bench.py --cpu-baseline-full` writes) is present."""
     p = os.path.join(ROOT, "profiles", "cpu_baseline_L8.json")
     try:
         return json.load(open(p))
@@ -181,6 +229,8 @@ def main():
     ap.add_argument("--no-as-specified", action="store_true")
     ap.add_argument("--no-cold-spmv", action="store_true")
     ap.add_argument("--level", type=int, default=LEVEL)
+    ap.add_argument("--cpu-baseline-full", action="store_true",
+                    help="time the CPU oracle once over the whole workload (minutes; no GPU use) and exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first copy a 1 GiB buffer three times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -188,6 +238,13 @@ def main():
         print(f"bench.py: only --gpus 1 is supported (one process drives one MI355X); got --gpus {args.gpus}. "
               f"The path shards by mesh with no communication; see DESIGN.md.", file=sys.stderr)
         sys.exit(2)
+    if args.cpu_baseline_full:
+        res = cpu_baseline_full(args.level)
+        os.makedirs(os.path.join(ROOT, "gpurun_out"), exist_ok=True)
+        with open(os.path.join(ROOT, "gpurun_out", f"cpu_baseline_L{args.level}.json"), "w") as f:
+            json.dump(res, f, indent=1)
+        print(json.dumps(res), flush=True)
+        return
     import metriko_amd as mk
     if args.calibrate_copy:
         import torch

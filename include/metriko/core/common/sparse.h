@@ -12,6 +12,7 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+#include "hvec.h"
 #include "typedef.h"
 
 namespace metriko {
@@ -28,9 +29,9 @@ namespace metriko {
     template<typename T>
     struct Csr {
         int nr = 0, nc = 0;
-        std::vector<int64_t> ptr{0};
-        std::vector<int> col;
-        std::vector<T> val;
+        HVec<int64_t> ptr{0};
+        HVec<int> col;
+        HVec<T> val;
 
         Csr() = default;
         Csr(int r, int c) : nr(r), nc(c), ptr(size_t(r) + 1, 0) {}
@@ -49,7 +50,7 @@ namespace metriko {
             std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
             for (auto& t : T_) { int64_t p = pos[t.r]++; c[p] = t.c; v[p] = t.v; }
             // rows are independent: sort + merge each (two passes: sizes, then values)
-            std::vector<int64_t> uq(size_t(nr) + 1, 0);
+            HVec<int64_t> uq(size_t(nr) + 1, 0);
 #pragma omp parallel if (T_.size() > 100000)
             {
                 std::vector<int> idx, cs;
@@ -93,7 +94,7 @@ namespace metriko {
         void prune(double ref = 0.0, double eps = 1e-12) {
             const double th = std::abs(ref) * eps;
             int64_t w = 0;
-            std::vector<int64_t> np(size_t(nr) + 1, 0);
+            HVec<int64_t> np(size_t(nr) + 1, 0);
             for (int i = 0; i < nr; i++) {
                 for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) {
                     if (std::abs(val[k]) > th) { col[w] = col[k]; val[w] = val[k]; w++; }
@@ -124,11 +125,29 @@ namespace metriko {
                 for (int i = q * step; i < r1; i++)
                     for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) cc[size_t(q)][size_t(col[k])]++;
             }
-            t.ptr.assign(size_t(nc) + 1, 0);
-            for (int j = 0; j < nc; j++) {
-                int64_t s = t.ptr[j];
-                for (int q = 0; q < nch; q++) { const int64_t x = cc[size_t(q)][size_t(j)]; cc[size_t(q)][size_t(j)] = s; s += x; }
-                t.ptr[j + 1] = s;
+            t.ptr.resize(size_t(nc) + 1);
+            t.ptr[0] = 0;
+            // offsets: column j's entries go chunk by chunk; the columns are split into nch
+            // ranges whose totals are scanned first, so the per-column pass runs in parallel
+            const int cstep = (nc + nch - 1) / std::max(nch, 1);
+            std::vector<int64_t> rtot(size_t(nch) + 1, 0);
+#pragma omp parallel for schedule(static, 1)
+            for (int q = 0; q < nch; q++) {
+                int64_t s = 0;
+                const int c1 = std::min(nc, (q + 1) * cstep);
+                for (int j = q * cstep; j < c1; j++)
+                    for (int p = 0; p < nch; p++) s += cc[size_t(p)][size_t(j)];
+                rtot[size_t(q) + 1] = s;
+            }
+            for (int q = 0; q < nch; q++) rtot[size_t(q) + 1] += rtot[size_t(q)];
+#pragma omp parallel for schedule(static, 1)
+            for (int q = 0; q < nch; q++) {
+                int64_t s = rtot[size_t(q)];
+                const int c1 = std::min(nc, (q + 1) * cstep);
+                for (int j = q * cstep; j < c1; j++) {
+                    for (int p = 0; p < nch; p++) { const int64_t x = cc[size_t(p)][size_t(j)]; cc[size_t(p)][size_t(j)] = s; s += x; }
+                    t.ptr[size_t(j) + 1] = s;
+                }
             }
 #pragma omp parallel for schedule(static, 1)
             for (int q = 0; q < nch; q++) {
@@ -190,7 +209,7 @@ namespace metriko {
             for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) w += B.ptr[A.col[ka] + 1] - B.ptr[A.col[ka]];
             return w;
         };
-        std::vector<int64_t> cnt(size_t(A.nr) + 1, 0);
+        HVec<int64_t> cnt(size_t(A.nr) + 1, 0);
 #pragma omp parallel if (par)
         {
             std::vector<int> mark, cols;
@@ -218,9 +237,9 @@ namespace metriko {
             }
         }
         for (int i = 0; i < A.nr; i++) cnt[size_t(i) + 1] += cnt[size_t(i)];
-        C.ptr = cnt;
-        C.col.resize(size_t(cnt[size_t(A.nr)]));
-        C.val.resize(size_t(cnt[size_t(A.nr)]));
+        C.ptr = std::move(cnt);
+        C.col.resize(size_t(C.ptr[size_t(A.nr)]));
+        C.val.resize(size_t(C.ptr[size_t(A.nr)]));
 #pragma omp parallel if (par)
         {
             std::vector<T> acc;
@@ -273,7 +292,7 @@ namespace metriko {
         if (A.nr != B.nr || A.nc != B.nc) throw std::invalid_argument("sparse add: shape mismatch");
         Csr<T> C(A.nr, A.nc);
         const bool par = A.nonZeros() + B.nonZeros() > 200000;
-        std::vector<int64_t> cnt(size_t(A.nr) + 1, 0);
+        HVec<int64_t> cnt(size_t(A.nr) + 1, 0);
 #pragma omp parallel for schedule(static, 4096) if (par)
         for (int i = 0; i < A.nr; i++) {
             int64_t p = A.ptr[i], q = B.ptr[i], n = 0;
@@ -286,9 +305,9 @@ namespace metriko {
             cnt[size_t(i) + 1] = n;
         }
         for (int i = 0; i < A.nr; i++) cnt[size_t(i) + 1] += cnt[size_t(i)];
-        C.ptr = cnt;
-        C.col.resize(size_t(cnt[size_t(A.nr)]));
-        C.val.resize(size_t(cnt[size_t(A.nr)]));
+        C.ptr = std::move(cnt);
+        C.col.resize(size_t(C.ptr[size_t(A.nr)]));
+        C.val.resize(size_t(C.ptr[size_t(A.nr)]));
 #pragma omp parallel for schedule(static, 4096) if (par)
         for (int i = 0; i < A.nr; i++) {
             int64_t p = A.ptr[i], q = B.ptr[i], w = C.ptr[i];

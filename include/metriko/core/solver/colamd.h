@@ -25,8 +25,8 @@ namespace colamd {
 
 // p[k] = the column placed k-th.  colptr / rowind: the pattern in compressed-column form, row
 // indices sorted and unique in each column.
-inline std::vector<int> order(int n_row, int n_col, const std::vector<int64_t>& colptr,
-                              const std::vector<int>& rowind) {
+template <typename PtrVec, typename IdxVec>
+std::vector<int> order(int n_row, int n_col, const PtrVec& colptr, const IdxVec& rowind) {
     constexpr int kEmpty = -1, kDeadPrincipal = -1, kDeadNonPrincipal = -2;
     const int64_t nnz = colptr[size_t(n_col)];
     // entries: column lists first, then row lists, then the pivot rows as they are formed

@@ -351,9 +351,12 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
         S.faces = nFt;
         S.verts = nVt;
         // union (block-diagonal) operators
-        std::vector<int64_t> Lp(static_cast<size_t>(nFt) + 1, 0), Sp(static_cast<size_t>(nVt) + 1, 0);
-        std::vector<int> Lc(static_cast<size_t>(lnz.back())), Sc(static_cast<size_t>(snz.back()));
-        std::vector<double> Lv(static_cast<size_t>(lnz.back()) * 2), Sv(static_cast<size_t>(snz.back())), mass(static_cast<size_t>(nFt));
+        std::vector<int64_t> Lp(static_cast<size_t>(nFt) + 1, 0);
+        metriko::HVec<int64_t> Sp(static_cast<size_t>(nVt) + 1, 0);
+        std::vector<int> Lc(static_cast<size_t>(lnz.back()));
+        metriko::HVec<int> Sc(static_cast<size_t>(snz.back()));
+        std::vector<double> Lv(static_cast<size_t>(lnz.back()) * 2), mass(static_cast<size_t>(nFt));
+        metriko::HVec<double> Sv(static_cast<size_t>(snz.back()));
 #pragma omp parallel for schedule(dynamic, 1)
         for (int m = 0; m < nmesh; m++) {
             const MeshData& d = D[static_cast<size_t>(m)];
@@ -368,8 +371,8 @@ mk_status mk_batch_run(mk_ctx* c, int nmesh, const int32_t* nV, const int32_t* n
         }
         // ---- autotuned layouts of the field and stiffness operators (samples of their rows) ----
         // the leading rows (up to 262,144), couplings inside the sample kept
-        auto tune_layout = [&](int64_t nrows, const std::vector<int64_t>& P, const std::vector<int>& Cc,
-                               const std::vector<double>& Vv, int cplx, mk_solve_opts& best, double& us) {
+        auto tune_layout = [&](int64_t nrows, const auto& P, const auto& Cc, const auto& Vv, int cplx,
+                               mk_solve_opts& best, double& us) {
             const int ns = int(std::min<int64_t>(nrows, 262144));
             const int w = cplx ? 2 : 1;
             std::vector<int64_t> sp(static_cast<size_t>(ns) + 1, 0);

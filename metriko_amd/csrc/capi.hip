@@ -251,9 +251,9 @@ std::vector<std::complex<double>> herm_pinv_solve(int m, const std::vector<std::
 void host_csr(mk_ctx* c, const mk_op* A, std::vector<int64_t>& rp, std::vector<int>& cc, std::vector<double>& vv) {
     const int vw = A->cplx ? 2 : 1;
     if (!A->h_ptr.empty()) {
-        rp = A->h_ptr;
-        cc = A->h_col;
-        vv = A->h_val;
+        rp.assign(A->h_ptr.begin(), A->h_ptr.end());
+        cc.assign(A->h_col.begin(), A->h_col.end());
+        vv.assign(A->h_val.begin(), A->h_val.end());
         return;
     }
     MK_CHECK(A->nmid == 0, MK_ERR_UNSUPPORTED, "operator export: no host copy of the medium-row part");
@@ -370,8 +370,8 @@ void stats_fill(mk_ctx* c, mk_stats* s) {
 namespace mk {
 // A real operator that takes over a host CSR (no copy of its arrays); same layout as
 // mk_op_create_real.
-mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector<int>&& col,
-                       std::vector<double>&& val, const mk_solve_opts& o) {
+mk_op* op_create_owned(mk_ctx* c, int n, metriko::HVec<int64_t>&& ptr, metriko::HVec<int>&& col,
+                       metriko::HVec<double>&& val, const mk_solve_opts& o) {
     MK_CHECK(n > 0 && ptr.size() == size_t(n) + 1 && col.size() == size_t(ptr[size_t(n)]) && val.size() == col.size(),
              MK_ERR_ARG, "operator arrays");
     MK_HIP(hipSetDevice(c->device));

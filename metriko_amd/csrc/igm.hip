@@ -199,9 +199,9 @@ struct Ctx {
 
 // Values of s A + d I for a matrix whose rows all hold their diagonal entry (E = X2F^T X2F
 // of a variable set every column of X2F touches); OpenMP over rows.
-std::vector<double> scaled_shifted(const SprsD& A, double s, double d) {
+metriko::HVec<double> scaled_shifted(const SprsD& A, double s, double d) {
     const int n = A.rows();
-    std::vector<double> v(A.val.size());
+    metriko::HVec<double> v(A.val.size());
     std::atomic<int> missing{0};
 #pragma omp parallel for schedule(static, 4096)
     for (int i = 0; i < n; i++) {
@@ -1275,9 +1275,9 @@ void injective_rounding(Ctx& K, const InjInputs& in, const mk_igm_flags* flags, 
         DevBuf<double> dNF, dflag;
         if (checked) {
             SprsD f2n(ck.rows, ck.cols);
-            f2n.ptr = ck.ptr;
-            f2n.col = ck.col;
-            f2n.val = ck.val;
+            f2n.ptr.assign(ck.ptr.begin(), ck.ptr.end());
+            f2n.col.assign(ck.col.begin(), ck.col.end());
+            f2n.val.assign(ck.val.begin(), ck.val.end());
             const SprsD Nf = f2n * (*in.UF);
             hN.op = make_rect_op(K.c, Nf, K.o);
             hN.op->label = "igm_inj_check";
@@ -1514,7 +1514,7 @@ extern "C" mk_status mk_iterative_rounding(
         const double wI = 1.0, wK = 10e4, wC = 0.01;
         {
             const double ts = now_ms();
-            hA0.op = mk::op_create_owned(c, k, std::vector<int64_t>(E.ptr), std::vector<int>(E.col),
+            hA0.op = mk::op_create_owned(c, k, metriko::HVec<int64_t>(E.ptr), metriko::HVec<int>(E.col),
                                          scaled_shifted(E, length * wI * wI, wC * wC), o);
             MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
             hA0.op->label = "igm_A0";

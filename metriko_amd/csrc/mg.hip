@@ -563,7 +563,7 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
             auto keep = [&](int i, int j) { return j == i || !(lng[i] || (lng[j] && !keep_long)); };
             Csr<T> Af(A.nr, A.nc);
             {
-                std::vector<int64_t>& np = Af.ptr;
+                auto& np = Af.ptr;
 #pragma omp parallel for schedule(static, 4096)
                 for (int i = 0; i < A.nr; i++) {
                     int64_t c = 0;

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include "metriko_hip.h"
 #include "graph_key.h"
+#include "metriko/core/common/hvec.h"
 
 namespace mk {
 
@@ -125,8 +126,8 @@ void mg_set_labels(Mg* g, const std::string& base);
 mk_status noalloc_setup(int device, std::string* err);
 // builds A's multigrid hierarchy when opts ask for MK_PRECOND_AMG (no-op otherwise)
 void ensure_mg(mk_ctx* c, const mk_op* A, const mk_solve_opts& o);
-mk_op* op_create_owned(mk_ctx* c, int n, std::vector<int64_t>&& ptr, std::vector<int>&& col,
-                       std::vector<double>&& val, const mk_solve_opts& o);
+mk_op* op_create_owned(mk_ctx* c, int n, metriko::HVec<int64_t>&& ptr, metriko::HVec<int>&& col,
+                       metriko::HVec<double>&& val, const mk_solve_opts& o);
 int64_t mg_nnz(const Mg* g);
 int mg_level_rows(const Mg* g, int l);
 
@@ -231,9 +232,9 @@ struct mk_op {
     std::vector<double> nullvec;
     int nnull = 0;
     // host CSR copy (sorted columns)
-    std::vector<int64_t> h_ptr;
-    std::vector<int> h_col;
-    std::vector<double> h_val;
+    metriko::HVec<int64_t> h_ptr;
+    metriko::HVec<int> h_col;
+    metriko::HVec<double> h_val;
     // host CSR used only by the multigrid set-up when the operator was assembled on the
     // device (its values may differ from the device's in the last bit; released after use)
     std::vector<int64_t> mg_ptr;

@@ -7,6 +7,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from . import progress
 from .hmesh import TWO_PI
 
 
@@ -111,10 +112,13 @@ def solve_smallest_eig(L, mass, u0, n_iter=50, return_history=False):
     x /= s; u = x.  No convergence test.
     """
     lu = spla.splu(L.tocsc())
+    progress.report(f"field: L factorised ({L.shape[0]} rows)")
     u = u0.astype(np.complex128).copy()
     x = u
     hist = []
-    for _ in range(n_iter):
+    for it in range(n_iter):
+        if it % 10 == 9:
+            progress.report(f"field: inverse iteration {it + 1}/{n_iter}")
         x = lu.solve(mass * u)
         s = np.sqrt(abs(np.vdot(x, mass * x)))
         x = x / s

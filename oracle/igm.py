@@ -30,6 +30,7 @@ import numpy as np
 import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
+from . import progress
 from .colamd import reduce_to_linearly_independent as _reduce_colamd, structural_product
 from .linalg import fullpiv_lu_kernel
 
@@ -261,6 +262,7 @@ class RosyParameterization:
         bb = cut.pos.max(axis=0) - cut.pos.min(axis=0)
         self.length = np.sqrt(bb @ bb) * self.gridscale / norm
         self.G2full = (G2 @ self.fullx2Nfunc).tocsr()
+        progress.report("IGM: constraints reduced, gradient operators assembled")
         self.ir = IterativeRounding(self.fixedIdcs, self.fixedVals, self.singularIdcs, self.integerIdcs,
                                     self.length, Cfull, self.G2full, N, n, cut.nF, self.seamless,
                                     self.roundSeams, self.rawF2Vec, literal=literal)
@@ -337,9 +339,11 @@ class IterativeRounding:
         f = X2F.T @ self.F2
         constMat = self.UFull[self.fixedIdcs, :]
         KKT = sp.bmat([[E, constMat.T], [constMat, None]], format="csc")
+        progress.report(f"IGM: kernel basis (rank {self.crank}) and KKT assembled")
         rhs = np.concatenate([f, self.fixedVals])
         self.E, self.f_rhs, self.constMat = E, f, constMat
         sol = spla.splu(KKT).solve(rhs)
+        progress.report(f"IGM: KKT solved ({KKT.shape[0]} rows)")
         self.XSmall = sol[:k]
         self.kkt_residual = np.linalg.norm(KKT @ sol - rhs) / max(np.linalg.norm(rhs), 1e-300)
         self.x0 = self.UFull @ self.XSmall
@@ -412,6 +416,7 @@ class IterativeRounding:
 
         XF = np.concatenate([self.XSmall, F2])
         xnaive, _ = self._lm(XF, obj_n, dir_n, 100, 10e-10, 10e-10, self.stats)
+        progress.report("IGM: naive LM pass done")
         self.x_naive = xnaive
         self.fullx = UF @ xnaive[:k]
         X_ = xnaive[:k].copy()
@@ -423,6 +428,7 @@ class IterativeRounding:
         X2F = _prune_zeros(G2 @ UF * self.length)
         A0 = (wI ** 2 * (X2F.T @ X2F) + wC ** 2 * sp.identity(k)).tocsc()
         A0lu = spla.splu(A0)
+        progress.report(f"IGM: A0 factorised ({A0.shape[0]} rows)")
         b_int = wI ** 2 * (X2F.T @ F2_)
         leftIdcs = list(self.integerIdcs if self.roundSeams else self.singularIdcs)
         roundedSingulars = False
@@ -454,6 +460,7 @@ class IterativeRounding:
                 leftIdcs = list(self.integerIdcs)
                 roundedSingulars = True
             self.stats["rounds"] += 1
+            progress.report(f"IGM: rounding round {self.stats['rounds']}")
             if not (minDif > 10e-7):
                 continue
             rounded = True

@@ -6,6 +6,7 @@ import time
 
 import numpy as np
 
+from . import progress
 from .field import FaceRosyField
 from .hmesh import Hmesh
 from .igm import RosyParameterization
@@ -24,15 +25,20 @@ def run_pipeline(V, F, N=4, ratio=0.03, u0=None, n_iter=50, literal=False, seaml
     t0 = time.perf_counter()
     mesh = Hmesh(V, F)
     t["hmesh"] = time.perf_counter() - t0
+    progress.report("hmesh built")
     t0 = time.perf_counter()
     rawf = FaceRosyField(mesh, N, u0=u0, n_iter=n_iter)
     t["field"] = time.perf_counter() - t0
     rawf.compute_matching("curl")
+    progress.report("matching and singularities done")
     t0 = time.perf_counter()
     seam = compute_seam(mesh, rawf.singular)
+    progress.report("seam done")
     cutm = compute_cut_mesh(mesh, seam)
+    progress.report("cut mesh built")
     cfield, cmatch = compute_combed_field(mesh, rawf.field, rawf.matching, seam, N)
     ext = compute_extrinsic_field(mesh, cfield)
+    progress.report("combed and extrinsic fields done")
     t["combinatorics"] = time.perf_counter() - t0
     rp = RosyParameterization(mesh, cutm, ext, rawf.singular, cmatch, seam, N, ratio)
     rp.seamless = seamless
@@ -40,6 +46,7 @@ def run_pipeline(V, F, N=4, ratio=0.03, u0=None, n_iter=50, literal=False, seaml
     rp.roundSeams = round_seams
     t0 = time.perf_counter()
     rp.setup()
+    progress.report("parameterization setup done")
     ok = rp.integ(literal=literal)
     t["igm"] = time.perf_counter() - t0
     uv = np.zeros((mesh.nF * 3, 2))
