@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
 
 template <typename T, int MW>
 static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, SpmvMode mode,
-                    const int* skip) {
+                    const int* skip, const SpmvTimer& tm) {
     unsigned g0 = 0;
     const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
@@ -197,8 +197,11 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
     const unsigned g2 = unsigned((wl + 3) / 4);
     // SELL rows, then (when there are any) the medium and long rows
 #define MK_SPMM_PARTS(M)                                                                                        \
-    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip); \
-    if (g2) hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip);
+    MK_TLAUNCH(tm, 1, g2 == 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip, \
+               (const T*)nullptr, T(0), T(0));                                                                          \
+    if (g2)                                                                                                             \
+        MK_TLAUNCH(tm, 0, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip,          \
+                   (const T*)nullptr, T(0), T(0));
     switch (mode) {
         case SPMV_PLAIN: MK_SPMM_PARTS(0) break;
         case SPMV_RESID_NORED: MK_SPMM_PARTS(3) break;
@@ -210,7 +213,7 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
 
 template <typename T, int MW>
 static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, const T* dinv, T ca,
-                         T cs, int mode, const int* skip) {
+                         T cs, int mode, const int* skip, const SpmvTimer& tm) {
     unsigned g0 = 0;
     const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
@@ -219,11 +222,11 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
     const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
     const unsigned g2 = unsigned((wl + 3) / 4);
 #define MK_CHEB_PARTS(M)                                                                                            \
-    hipLaunchKernelGGL((sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B, skip,     \
-                       dinv, ca, cs);                                                                               \
+    MK_TLAUNCH(tm, 1, g2 == 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B,   \
+               skip, dinv, ca, cs);                                                                                 \
     if (g2)                                                                                                         \
-        hipLaunchKernelGGL((sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip, dinv, \
-                           ca, cs);
+        MK_TLAUNCH(tm, 0, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip,     \
+                   dinv, ca, cs);
     switch (mode) {
         case 5: MK_CHEB_PARTS(5) break;
         case 6: MK_CHEB_PARTS(6) break;
@@ -252,10 +255,10 @@ void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out
     if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
     else { val = A->val32.p; dinv = A->dinv32.p; }
     switch (mw) {
-        case 8: spmm_cheb_mw<T, 8>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
-        case 16: spmm_cheb_mw<T, 16>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
-        case 32: spmm_cheb_mw<T, 32>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
-        case 64: spmm_cheb_mw<T, 64>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip); break;
+        case 8: spmm_cheb_mw<T, 8>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
+        case 16: spmm_cheb_mw<T, 16>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
+        case 32: spmm_cheb_mw<T, 32>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
+        case 64: spmm_cheb_mw<T, 64>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
         default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
     }
     MK_HIP(hipGetLastError());
@@ -277,10 +280,10 @@ void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64" : mw == 32 ? "spmm32" : mw == 16 ? "spmm16" : "spmm8",
                  sizeof(T) == 4);
     switch (mw) {
-        case 8: spmm_mw<T, 8>(c, A, val, X, Y, B, mode, skip); break;
-        case 16: spmm_mw<T, 16>(c, A, val, X, Y, B, mode, skip); break;
-        case 32: spmm_mw<T, 32>(c, A, val, X, Y, B, mode, skip); break;
-        case 64: spmm_mw<T, 64>(c, A, val, X, Y, B, mode, skip); break;
+        case 8: spmm_mw<T, 8>(c, A, val, X, Y, B, mode, skip, tm); break;
+        case 16: spmm_mw<T, 16>(c, A, val, X, Y, B, mode, skip, tm); break;
+        case 32: spmm_mw<T, 32>(c, A, val, X, Y, B, mode, skip, tm); break;
+        case 64: spmm_mw<T, 64>(c, A, val, X, Y, B, mode, skip, tm); break;
         default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
     }
     MK_HIP(hipGetLastError());

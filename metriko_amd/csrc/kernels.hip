@@ -738,8 +738,8 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
     SpmvTimer tm(c, bytes, A, kname[int(mode)], sizeof(T) == 4);
     const unsigned gr = std::max(1u, std::min(g, fin_grid()));  // reducing modes: capped grid
 #define MK_SPMV(CPLX, MODE)                                                                                \
-    hipLaunchKernelGGL((sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
-                       c->stream, D, x, y, b, c->partials.p, skip)
+    MK_TLAUNCH(tm, 1, 1, (sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
+               c->stream, D, x, y, b, c->partials.p, skip)
 #define MK_SPMV_ALL(CPLX)                            \
     switch (mode) {                                  \
         case SPMV_PLAIN: MK_SPMV(CPLX, 0); break;    \
@@ -794,11 +794,11 @@ void cheb_launch(mk_ctx* c, const mk_op* A, const T* x_in, T* x_out, const T* b,
                                                                                       double(sizeof(T)));
     SpmvTimer tm(c, bytes, A, "cheb", sizeof(T) == 4);
     if (A->cplx)
-        hipLaunchKernelGGL((sell_cheb<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv, a,
-                           s, first, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_cheb<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
+                   a, s, first, skip);
     else
-        hipLaunchKernelGGL((sell_cheb<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
-                           a, s, first, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_cheb<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
+                   a, s, first, skip);
     MK_HIP(hipGetLastError());
 }
 template void cheb_launch<double>(mk_ctx*, const mk_op*, const double*, double*, const double*, double*, double,
@@ -856,11 +856,11 @@ void cheb0_launch(mk_ctx* c, const mk_op* A, const T* b, T* x_out, T* d, T c0, T
     const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) + double(A->n) * (vb + double(sizeof(T)));
     SpmvTimer tm(c, bytes, A, "cheb0", sizeof(T) == 4);
     if (A->cplx)
-        hipLaunchKernelGGL((sell_cheb0<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0, a,
-                           s, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_cheb0<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
+                   a, s, skip);
     else
-        hipLaunchKernelGGL((sell_cheb0<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0, a,
-                           s, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_cheb0<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
+                   a, s, skip);
     MK_HIP(hipGetLastError());
 }
 template void cheb0_launch<double>(mk_ctx*, const mk_op*, const double*, double*, double*, double, double, double,
@@ -902,9 +902,9 @@ void spmv2_launch(mk_ctx* c, const mk_op* A, int m1, const T* u, const T* v, T* 
     const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
     SpmvTimer tm(c, sizeof(T) == 8 ? A->bytes64 : A->bytes32, A, "restrict_resid", sizeof(T) == 4);
     if (A->cplx)
-        hipLaunchKernelGGL((sell_spmv2<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_spmv2<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
     else
-        hipLaunchKernelGGL((sell_spmv2<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+        MK_TLAUNCH(tm, 1, 1, (sell_spmv2<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
     MK_HIP(hipGetLastError());
 }
 template void spmv2_launch<double>(mk_ctx*, const mk_op*, int, const double*, const double*, double*, const int*);
@@ -985,11 +985,11 @@ static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, 
     {
         SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
         if (A->cplx)
-            hipLaunchKernelGGL((cg_spmv<T, true>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
-                               (const CgState*)c->state.p, c->partials.p);
+            MK_TLAUNCH(tm, 1, 1, (cg_spmv<T, true>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+                       (const CgState*)c->state.p, c->partials.p);
         else
-            hipLaunchKernelGGL((cg_spmv<T, false>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
-                               (const CgState*)c->state.p, c->partials.p);
+            MK_TLAUNCH(tm, 1, 1, (cg_spmv<T, false>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+                       (const CgState*)c->state.p, c->partials.p);
     }
     finish<1, FinCgAlpha>(c, gr);
 }
@@ -1217,11 +1217,11 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         {
             SpmvTimer tm(c, A->bytes64 + double(len) * (4.0 + 8.0 * 5.0), A, "cg_spmv", false);
             if (A->cplx)
-                hipLaunchKernelGGL((cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
-                                   w.p.p, w.q.p, w.z.p, st, c->partials.p);
+                MK_TLAUNCH(tm, 1, 1, (cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D,
+                           (const float*)w.z32.p, w.p.p, w.q.p, w.z.p, st, c->partials.p);
             else
-                hipLaunchKernelGGL((cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D, (const float*)w.z32.p,
-                                   w.p.p, w.q.p, w.z.p, st, c->partials.p);
+                MK_TLAUNCH(tm, 1, 1, (cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D,
+                           (const float*)w.z32.p, w.p.p, w.q.p, w.z.p, st, c->partials.p);
         }
         hipLaunchKernelGGL(cgm_spmv_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gsr));
         hipLaunchKernelGGL(cgm_update, dim3(gl), dim3(256), 0, c->stream, len, x, w.r.p, w.r32.p,

@@ -12,6 +12,7 @@
 // in D^-1 A on [rho/30, 1.1 rho]; every operator is a SELL-C-sigma matrix and every
 // kernel honours the CG "done" flag so a converged solve costs nothing further.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -146,7 +147,7 @@ double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
     const int n = A.nr;
     std::mt19937_64 g(12345);
     std::uniform_real_distribution<double> U(-1.0, 1.0);
-    std::vector<T> x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
+    metriko::HVec<T> x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
     for (auto& v : x) {
         if constexpr (std::is_same<T, double>::value) v = U(g);
         else v = cd(U(g), U(g));
@@ -166,6 +167,46 @@ double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
         for (int i = 0; i < n; i++) x[i] = y[i] / nrm;
     }
     return lam;
+}
+
+// A <- (A + A^H) / 2, each entry computed as 0.5 a_ij + 0.5 conj(a_ji).  In place when the
+// pattern is structurally symmetric (R A P of a structurally symmetric A with R = P^H always
+// is): the owner of each pair i < j finds (j, i) by binary search in row j (sorted columns) and
+// writes both entries, so no thread touches another's pair.  Otherwise through the transpose.
+template <typename T>
+void hermitian_average(Csr<T>& A) {
+    const int n = A.nr;
+    auto partner = [&](int i, int j) -> int64_t {  // position of (j, i), or -1
+        const int* b = A.col.data() + A.ptr[size_t(j)];
+        const int* e = A.col.data() + A.ptr[size_t(j) + 1];
+        const int* p = std::lower_bound(b, e, i);
+        return (p != e && *p == i) ? int64_t(p - A.col.data()) : -1;
+    };
+    std::atomic<int> asym{0};
+#pragma omp parallel for schedule(static, 4096) if (A.nonZeros() > 200000)
+    for (int i = 0; i < n; i++)
+        for (int64_t k = A.ptr[size_t(i)]; k < A.ptr[size_t(i) + 1]; k++)
+            if (A.col[size_t(k)] > i && partner(i, A.col[size_t(k)]) < 0) asym.store(1, std::memory_order_relaxed);
+    if (asym.load() || A.nr != A.nc) {
+        Csr<T> AH = conj_transpose(A);
+        A = metriko::add(A, T(0.5), AH, T(0.5));
+        return;
+    }
+#pragma omp parallel for schedule(static, 4096) if (A.nonZeros() > 200000)
+    for (int i = 0; i < n; i++)
+        for (int64_t k = A.ptr[size_t(i)]; k < A.ptr[size_t(i) + 1]; k++) {
+            const int j = A.col[size_t(k)];
+            if (j < i) continue;
+            if (j == i) {
+                const T a = A.val[size_t(k)];
+                A.val[size_t(k)] = T(0.5) * a + T(0.5) * conjv(a);
+                continue;
+            }
+            const int64_t q = partner(i, j);
+            const T aij = A.val[size_t(k)], aji = A.val[size_t(q)];
+            A.val[size_t(k)] = T(0.5) * aij + T(0.5) * conjv(aji);
+            A.val[size_t(q)] = T(0.5) * aji + T(0.5) * conjv(aij);
+        }
 }
 
 // rows much longer than typical: > max(32, 8 x median length)
@@ -629,8 +670,7 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
                 lap("residual restriction");
             }
             // exact Hermitian symmetry of the coarse operator
-            Csr<T> AcH = conj_transpose(Ac);
-            Ac = metriko::add(Ac, T(0.5), AcH, T(0.5));
+            hermitian_average(Ac);
             H.A.push_back(std::move(A));
             H.P.push_back(std::move(P));
             H.R.push_back(std::move(R));

@@ -3,6 +3,7 @@
 #pragma once
 #include <algorithm>
 #include <string>
+#include <hip/hip_ext.h>
 #include "mk_internal.h"
 
 namespace mk {
@@ -248,6 +249,8 @@ void timing_flush(mk_ctx* c);
 // "<operator label>:<kernel>:<fp32|fp64>".
 struct SpmvTimer {
     mk_ctx* c;
+    // with timing on: the start / end stamps the timed launches carry (MK_TLAUNCH)
+    hipEvent_t e0 = nullptr, e1 = nullptr;
     SpmvTimer(mk_ctx* c_, double bytes, const mk_op* A = nullptr, const char* kernel = "spmv", bool fp32 = false)
         : c(c_) {
         c->spmv_calls++;
@@ -265,13 +268,24 @@ struct SpmvTimer {
             }
             it->second.bytes += bytes;
             c->ev_key.push_back(key);
-            MK_HIP(hipEventRecord(next_event(c), c->stream));
+            e0 = next_event(c);
+            e1 = next_event(c);
         }
     }
-    ~SpmvTimer() {
-        if (c->time_spmv) (void)hipEventRecord(next_event(c), c->stream);
-    }
 };
+
+// Launches kernel K on STREAM.  With timing on, the launch carries the timer's start stamp
+// (FIRST) and / or end stamp (LAST) through hipExtLaunchKernelGGL: the stamps are taken at
+// the kernel's own start and end, so the timed interval holds no event packet; a timed
+// quantity launched as two kernels passes FIRST to the first and LAST to the second.
+#define MK_TLAUNCH(tm, FIRST, LAST, K, G, B, SH, STREAM, ...)                                             \
+    do {                                                                                                  \
+        if ((tm).e0)                                                                                      \
+            hipExtLaunchKernelGGL(K, G, B, SH, STREAM, (FIRST) ? (tm).e0 : nullptr, (LAST) ? (tm).e1 : nullptr, \
+                                  0, __VA_ARGS__);                                                        \
+        else                                                                                              \
+            hipLaunchKernelGGL(K, G, B, SH, STREAM, __VA_ARGS__);                                         \
+    } while (0)
 
 template <typename T>
 static SellDev<T> sell_dev(const mk_op* A, const T* val, int block, unsigned* grid) {

@@ -72,3 +72,34 @@ def test_gpu_pipeline_torus_204800(gpu_ctx, precision):
     assert_decisions_in_order(out["decisions"], ref)
     uv = np.asarray(out["uv"]).reshape(-1, 2)[g["uv_corners"]]
     assert np.abs(uv - g["uv"]).max() / np.abs(g["uv"]).max() < 1e-8
+
+
+@pytest.mark.gpu
+def test_gpu_pipeline_l8_matches_oracle_fixture(gpu_ctx):
+    """The benchmark's 1,310,720-face mesh (bench.workload(8)) against the oracle's fixture
+    (tests/golden/make_golden.py --l8; 12 minutes of single-thread SuperLU on the CPU):
+    singularities, seam, cut-mesh indices (checksum), nS / nT and the 74 rounding decisions
+    bit-exact, the field within 1e-8 up to a global phase on every 500th face and uv within
+    1e-8 relative on every 1500th corner, in the default precision (FP32 multigrid inside FP64
+    PCG)."""
+    from bench import workload
+    from tests.helpers import cut_hash
+    g = np.load(os.path.join(GOLDEN, "bumpy_L8_oracle.npz"), allow_pickle=False)
+    V, F, u0 = workload(8)
+    o = mk.opts(precision=mk.PREC_MIXED_MG)
+    out = gpu_ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=o, igm_opts=o)
+    err = phase_error(out["compressed"][g["field_faces"]], g["field"])
+    uv = np.asarray(out["uv"]).reshape(-1, 2)[g["uv_corners"]]
+    uv_err = np.abs(uv - g["uv"]).max() / np.abs(g["uv"]).max()
+    print(f"[L8 oracle] field phase {err:.2e}, uv {uv_err:.2e}, decisions {len(out['decisions'])}", flush=True)
+    assert err < 1e-8
+    sing = np.nonzero(out["singular"])[0]
+    assert np.array_equal(sing, g["singular_v"]) and np.array_equal(out["singular"][sing], g["singular_idx"])
+    assert np.array_equal(np.nonzero(out["seam"])[0], g["seam_edges"])
+    assert out["info"]["cutV"] == int(g["cutV"]) and cut_hash(out["cut_idx"]) == int(g["cut_hash"])
+    assert (out["info"]["nS"], out["info"]["nT"]) == (int(g["nS"]), int(g["nT"]))
+    assert out["info"]["success"] == int(g["success"])
+    ref = [(i, v, m) for (i, v), m in zip(g["decisions"].tolist(), g["decision_margin"].tolist())]
+    assert_decisions_in_order(out["decisions"], ref)
+    assert uv_err < 1e-8
+

@@ -179,6 +179,34 @@ def test_graph_replay_matches_eager(gpu_ctx, precision):
 
 
 @pytest.mark.gpu
+def test_kernel_timing_leaves_results_unchanged(gpu_ctx):
+    """Per-kernel timing (start / end stamps carried by the launches) changes no result: a
+    multigrid solve and a block solve with timing on equal the same solves with it off, bit for
+    bit, and the report holds every timed kernel with its calls, time and bytes."""
+    nx = 60
+    T = sp.diags([-np.ones(nx - 1), 2.0 * np.ones(nx), -np.ones(nx - 1)], [-1, 0, 1])
+    A = (sp.kron(T, sp.identity(nx)) + sp.kron(sp.identity(nx), T) + 1e-3 * sp.identity(nx * nx)).tocsr()
+    op = gpu_ctx.op_real(A)
+    o = mk.opts(precision=mk.PREC_MIXED_MG)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(A.shape[0])
+    B = rng.standard_normal((A.shape[0], 40))
+    x_off = op.solve(b, o=o)[0]
+    X_off = op.solve_many(B, o=o)[0]
+    try:
+        gpu_ctx.set_timing(True)
+        gpu_ctx.timing_report(reset=True)
+        x_on = op.solve(b, o=o)[0]
+        X_on = op.solve_many(B, o=o)[0]
+        rep = gpu_ctx.timing_report(reset=True)
+    finally:
+        gpu_ctx.set_timing(False)
+    assert np.array_equal(x_on, x_off) and np.array_equal(X_on, X_off)
+    assert rep and all(k["calls"] > 0 and k["ms"] > 0.0 and k["bytes"] > 0.0 for k in rep)
+    assert any(":spmm" in k["key"] for k in rep) and any(":cheb" in k["key"] for k in rep)
+
+
+@pytest.mark.gpu
 def test_operator_rejects_foreign_context(gpu_ctx):
     """Captured graphs are cached on the operator and hold its context's buffers, so an operator
     is only run on the context it was created on: another context gets MK_ERR_ARG, and the
