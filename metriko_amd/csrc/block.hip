@@ -74,6 +74,56 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
 // wavefront whose row groups split the entries.  PART 1 / 2: only the SELL rows / only the
 // medium and long rows (launched with nw_sell = 0), so the SELL launch does not carry the
 // registers of the eight-gather chains and runs at full occupancy.
+// Stores row `row` of the result (lane j of its LPR lanes): Y = acc (MODE 0), B - acc (3),
+// Y + acc (4), the Chebyshev updates (5-7, x_k = xd when the row's own entry was gathered).
+template <typename T, int MW, int MODE, typename V>
+__device__ __forceinline__ void spmm_store(const V& acc, int row, int j, const V* __restrict__ Xv, T* __restrict__ Y,
+                                           const T* __restrict__ B, const T* __restrict__ dinv, T ca, T cs,
+                                           bool have_xd, const V& xd) {
+    constexpr int LPR = MW / Vec16<T>::N;
+    V* Yv = reinterpret_cast<V*>(Y);
+    const size_t o = size_t(row) * LPR + j;
+    if constexpr (MODE == 0) Yv[o] = acc;
+    if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
+    if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
+    if constexpr (MODE >= 5) {
+        const V xo = have_xd ? xd : Xv[o];
+        const V r = vsub(reinterpret_cast<const V*>(B)[o], acc);
+        const T sd = cs * dinv[row];
+        if constexpr (MODE == 5) Yv[o] = vaxpy(T(1), xo, sd, r);
+        if constexpr (MODE == 7) Yv[o] = vaxpy(T(1) + ca, xo, sd, r);
+        if constexpr (MODE == 6) Yv[o] = vaxpy(T(1), vaxpy(T(1) + ca, xo, -ca, Yv[o]), sd, r);
+    }
+}
+
+// Sum over entries [k0, k1) of a CSR row (stride R between a lane group's entries, starting
+// at k0 + sub): eight independent gathers in flight per lane.
+template <typename T, int R, int LPR, typename V>
+__device__ __forceinline__ V chain_sum(int64_t k0, int64_t k1, int sub, int j, const int* __restrict__ cols,
+                                       const T* __restrict__ vals, const V* __restrict__ Xv) {
+    V acc, a1, a2, a3, a4, a5, a6, a7;
+    vzero(acc); vzero(a1); vzero(a2); vzero(a3); vzero(a4); vzero(a5); vzero(a6); vzero(a7);
+    int64_t k = k0 + sub;
+    for (; k + 7 * R < k1; k += 8 * R) {
+        const int c0 = cols[k], c1 = cols[k + R], c2 = cols[k + 2 * R], c3 = cols[k + 3 * R];
+        const int c4 = cols[k + 4 * R], c5 = cols[k + 5 * R], c6 = cols[k + 6 * R], c7 = cols[k + 7 * R];
+        const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
+        const V x2 = Xv[size_t(c2) * LPR + j], x3 = Xv[size_t(c3) * LPR + j];
+        const V x4 = Xv[size_t(c4) * LPR + j], x5 = Xv[size_t(c5) * LPR + j];
+        const V x6 = Xv[size_t(c6) * LPR + j], x7 = Xv[size_t(c7) * LPR + j];
+        vfma(acc, vals[k], x0);
+        vfma(a1, vals[k + R], x1);
+        vfma(a2, vals[k + 2 * R], x2);
+        vfma(a3, vals[k + 3 * R], x3);
+        vfma(a4, vals[k + 4 * R], x4);
+        vfma(a5, vals[k + 5 * R], x5);
+        vfma(a6, vals[k + 6 * R], x6);
+        vfma(a7, vals[k + 7 * R], x7);
+    }
+    for (; k < k1; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
+    return vadd(vadd(vadd(acc, a1), vadd(a2, a3)), vadd(vadd(a4, a5), vadd(a6, a7)));
+}
+
 template <typename T, int MW, int MODE, int PART = 0>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
                                                T* __restrict__ Y, const T* __restrict__ B, const int* skip,
@@ -127,62 +177,44 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
     } else {
         if constexpr (PART == 1) return;
         int lw = gw - nw_sell;
-        const int* rows = A.mid_rows;
-        const int64_t* ptr = A.mid_ptr;
-        const int* cols = A.mid_col;
-        const T* vals = A.mid_val;
-        if (lw >= A.nmid) {
-            lw -= A.nmid;
-            if (lw >= A.nlong) return;  // uniform inside the wavefront
-            rows = A.long_rows;
-            ptr = A.long_ptr;
-            cols = A.long_col;
-            vals = A.long_val;
-        }
-        // a medium or long row is one wavefront's serial chain: eight independent gathers in
-        // flight per lane keep it off the kernel's critical path
-        {
-            V a1, a2, a3, a4, a5, a6, a7;
-            vzero(a1); vzero(a2); vzero(a3); vzero(a4); vzero(a5); vzero(a6); vzero(a7);
-            int64_t k = ptr[lw] + sub;
-            const int64_t e = ptr[lw + 1];
-            for (; k + 7 * R < e; k += 8 * R) {
-                const int c0 = cols[k], c1 = cols[k + R], c2 = cols[k + 2 * R], c3 = cols[k + 3 * R];
-                const int c4 = cols[k + 4 * R], c5 = cols[k + 5 * R], c6 = cols[k + 6 * R], c7 = cols[k + 7 * R];
-                const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
-                const V x2 = Xv[size_t(c2) * LPR + j], x3 = Xv[size_t(c3) * LPR + j];
-                const V x4 = Xv[size_t(c4) * LPR + j], x5 = Xv[size_t(c5) * LPR + j];
-                const V x6 = Xv[size_t(c6) * LPR + j], x7 = Xv[size_t(c7) * LPR + j];
-                vfma(acc, vals[k], x0);
-                vfma(a1, vals[k + R], x1);
-                vfma(a2, vals[k + 2 * R], x2);
-                vfma(a3, vals[k + 3 * R], x3);
-                vfma(a4, vals[k + 4 * R], x4);
-                vfma(a5, vals[k + 5 * R], x5);
-                vfma(a6, vals[k + 6 * R], x6);
-                vfma(a7, vals[k + 7 * R], x7);
-            }
-            for (; k < e; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
-            acc = vadd(vadd(vadd(acc, a1), vadd(a2, a3)), vadd(vadd(a4, a5), vadd(a6, a7)));
-        }
+        if (lw < A.nmid) {  // a medium row: one wavefront, its R lane groups split the entries
+            acc = chain_sum<T, R, LPR>(A.mid_ptr[lw], A.mid_ptr[lw + 1], sub, j, A.mid_col, A.mid_val, Xv);
 #pragma unroll
-        for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
-        if (sub != 0) return;
-        row = rows[lw];
+            for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
+            if (sub != 0) return;
+            row = A.mid_rows[lw];
+        } else {  // a segment of a long row: its partial sum, combined by sell_spmm_long
+            lw -= A.nmid;
+            if (lw >= A.nseg) return;  // uniform inside the wavefront
+            acc = chain_sum<T, R, LPR>(A.seg_beg[lw], A.seg_end[lw], sub, j, A.long_col, A.long_val, Xv);
+#pragma unroll
+            for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
+            if (sub == 0) reinterpret_cast<V*>(A.seg_part)[size_t(lw) * LPR + j] = acc;
+            return;
+        }
     }
-    V* Yv = reinterpret_cast<V*>(Y);
-    const size_t o = size_t(row) * LPR + j;
-    if constexpr (MODE == 0) Yv[o] = acc;
-    if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
-    if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
-    if constexpr (MODE >= 5) {
-        const V xo = have_xd ? xd : Xv[o];
-        const V r = vsub(reinterpret_cast<const V*>(B)[o], acc);
-        const T sd = cs * dinv[row];
-        if constexpr (MODE == 5) Yv[o] = vaxpy(T(1), xo, sd, r);
-        if constexpr (MODE == 7) Yv[o] = vaxpy(T(1) + ca, xo, sd, r);
-        if constexpr (MODE == 6) Yv[o] = vaxpy(T(1), vaxpy(T(1) + ca, xo, -ca, Yv[o]), sd, r);
-    }
+    spmm_store<T, MW, MODE>(acc, row, j, Xv, Y, B, dinv, ca, cs, have_xd, xd);
+}
+
+// Long rows of the multi-column kernels: one wavefront per row adds its segments' partial
+// sums in segment order (lanes j < LPR) and stores the row as sell_spmm does.
+template <typename T, int MW, int MODE>
+__global__ void __launch_bounds__(256) sell_spmm_long(SellDev<T> A, const T* __restrict__ X, T* __restrict__ Y,
+                                                    const T* __restrict__ B, const int* skip,
+                                                    const T* __restrict__ dinv, T ca, T cs) {
+    if (skip && *skip) return;
+    using V = typename Vec16<T>::type;
+    constexpr int LPR = MW / Vec16<T>::N;
+    const int w = int(blockIdx.x) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
+    const int j = int(threadIdx.x & 63);
+    if (w >= A.nlong || j >= LPR) return;
+    const V* __restrict__ P = reinterpret_cast<const V*>(A.seg_part);
+    V acc;
+    vzero(acc);
+    for (int q = A.lseg_ptr[w]; q < A.lseg_ptr[w + 1]; q++) acc = vadd(acc, P[size_t(q) * LPR + j]);
+    V xd;
+    vzero(xd);
+    spmm_store<T, MW, MODE>(acc, A.long_rows[w], j, reinterpret_cast<const V*>(X), Y, B, dinv, ca, cs, false, xd);
 }
 
 template <typename T, int MW>
@@ -192,15 +224,20 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
     const SellDev<T> D = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
     const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
-    const int64_t wl = int64_t(A->nmid) + A->nlong;
+    const int64_t wl = int64_t(A->nmid) + A->nseg;
     const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
     const unsigned g2 = unsigned((wl + 3) / 4);
-    // SELL rows, then (when there are any) the medium and long rows
+    const unsigned g3 = unsigned((int64_t(A->nlong) + 3) / 4);
+    // SELL rows, then (when there are any) the medium rows and long-row segments, then the
+    // long rows' sums
 #define MK_SPMM_PARTS(M)                                                                                        \
-    MK_TLAUNCH(tm, 1, g2 == 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip, \
+    MK_TLAUNCH(tm, 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip, \
                (const T*)nullptr, T(0), T(0));                                                                          \
     if (g2)                                                                                                             \
-        MK_TLAUNCH(tm, 0, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip,          \
+        MK_TLAUNCH(tm, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip,          \
+                   (const T*)nullptr, T(0), T(0));                                                                      \
+    if (g3)                                                                                                             \
+        MK_TLAUNCH(tm, 2, (sell_spmm_long<T, MW, M>), dim3(g3), dim3(256), 0, c->stream, D, X, Y, B, skip,            \
                    (const T*)nullptr, T(0), T(0));
     switch (mode) {
         case SPMV_PLAIN: MK_SPMM_PARTS(0) break;
@@ -218,15 +255,19 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
     const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
     const int nw_sell = int((int64_t(A->n_short) + R - 1) / R);
-    const int64_t wl = int64_t(A->nmid) + A->nlong;
+    const int64_t wl = int64_t(A->nmid) + A->nseg;
     const unsigned g1 = unsigned(std::max<int64_t>(1, (int64_t(nw_sell) + 3) / 4));
     const unsigned g2 = unsigned((wl + 3) / 4);
+    const unsigned g3 = unsigned((int64_t(A->nlong) + 3) / 4);
 #define MK_CHEB_PARTS(M)                                                                                            \
-    MK_TLAUNCH(tm, 1, g2 == 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B,   \
+    MK_TLAUNCH(tm, 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B,   \
                skip, dinv, ca, cs);                                                                                 \
     if (g2)                                                                                                         \
-        MK_TLAUNCH(tm, 0, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip,     \
-                   dinv, ca, cs);
+        MK_TLAUNCH(tm, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip,     \
+                   dinv, ca, cs);                                                                                   \
+    if (g3)                                                                                                         \
+        MK_TLAUNCH(tm, 2, (sell_spmm_long<T, MW, M>), dim3(g3), dim3(256), 0, c->stream, Dd, X, Y, B, skip, dinv, \
+                   ca, cs);
     switch (mode) {
         case 5: MK_CHEB_PARTS(5) break;
         case 6: MK_CHEB_PARTS(6) break;
@@ -249,7 +290,7 @@ void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out
     const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * s + double(A->m) * vb +
                          double(A->n) * vb * (mode == 6 ? 3.0 : 2.0) + double(A->n) * s;
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64_cheb" : mw == 32 ? "spmm32_cheb" : mw == 16 ? "spmm16_cheb" :
-                 "spmm8_cheb", sizeof(T) == 4);
+                 "spmm8_cheb", sizeof(T) == 4, 3);
     const T* val;
     const T* dinv;
     if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
@@ -278,7 +319,7 @@ void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B
     const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * sizeof(T) +
                          double(A->m) * vb + double(A->n) * vb * (mode == SPMV_PLAIN ? 1.0 : 2.0);
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64" : mw == 32 ? "spmm32" : mw == 16 ? "spmm16" : "spmm8",
-                 sizeof(T) == 4);
+                 sizeof(T) == 4, 3);
     switch (mw) {
         case 8: spmm_mw<T, 8>(c, A, val, X, Y, B, mode, skip, tm); break;
         case 16: spmm_mw<T, 16>(c, A, val, X, Y, B, mode, skip, tm); break;

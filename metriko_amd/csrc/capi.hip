@@ -358,6 +358,7 @@ void stats_reset(mk_ctx* c) {
     c->spmv_calls = 0;
     c->spmv_bytes = 0.0;
     c->ev_used = 0;
+    c->ev_key.clear();
 }
 void stats_fill(mk_ctx* c, mk_stats* s) {
     timing_flush(c);

@@ -37,9 +37,11 @@ void timing_flush(mk_ctx* c) {
         float ms = 0.f;
         MK_HIP(hipEventElapsedTime(&ms, c->ev_pool[i], c->ev_pool[i + 1]));
         c->spmv_ms += ms;
-        const std::string& key = c->ev_key[i / 2];
+        const std::string& k = c->ev_key[i / 2];
+        const bool cont = !k.empty() && k[0] == '+';  // second launch of the previous call
+        const std::string key = cont ? k.substr(1) : k;
         auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
-        if (it != c->kstats.end()) { it->second.calls++; it->second.ms += ms; }
+        if (it != c->kstats.end()) { if (!cont) it->second.calls++; it->second.ms += ms; }
     }
     c->ev_used = 0;
     c->ev_key.clear();
@@ -513,6 +515,41 @@ __global__ void k_set_state_dev(CgState* st, const double* bb, double tol2) {
 // host side
 
 
+// SELL fill: thread (slice s, lane l) writes row order[s C + l] (or s C + l) as column l of
+// its slice, entry j at slice_ptr[s] + j C + l; past the row's end a padding entry (its own
+// column when that is a valid column, value 0).  VW values per entry (2: complex).
+template <int VW>
+__global__ void __launch_bounds__(256) k_sell_fill(int ns, int C, int n_rows, int m, const int64_t* __restrict__ sptr,
+                                                   const int* __restrict__ sw, const int* __restrict__ order,
+                                                   const int64_t* __restrict__ rowptr, const int* __restrict__ col,
+                                                   const double* __restrict__ val, double scale, int* __restrict__ ocol,
+                                                   double* __restrict__ ov64, float* __restrict__ ov32) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= int64_t(ns) * C) return;
+    const int s = int(t / C), l = int(t - int64_t(s) * C);
+    const int r = s * C + l;
+    const int row = r < n_rows ? (order ? order[r] : r) : -1;
+    const int64_t b = row >= 0 ? rowptr[row] : 0;
+    const int64_t len = row >= 0 ? rowptr[row + 1] - b : 0;
+    const int w = sw[s];
+    for (int j = 0; j < w; j++) {
+        const int64_t e = sptr[s] + int64_t(j) * C + l;
+        if (j < len) {
+            ocol[e] = col[b + j];
+#pragma unroll
+            for (int q = 0; q < VW; q++) {
+                const double v = scale * val[(b + j) * VW + q];
+                ov64[e * VW + q] = v;
+                ov32[e * VW + q] = float(v);
+            }
+        } else {
+            ocol[e] = (row >= 0 && row < m) ? row : 0;
+#pragma unroll
+            for (int q = 0; q < VW; q++) { ov64[e * VW + q] = 0.0; ov32[e * VW + q] = 0.0f; }
+        }
+    }
+}
+
 void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma, double scale) {
     const int n = op->n;
     const int vw = op->cplx ? 2 : 1;
@@ -605,32 +642,6 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     for (int s = 0; s < ns; s++) sptr[s + 1] = sptr[s] + int64_t(sw[s]) * C;
     const int64_t stored = sptr[ns];
     const double t_layout = lap_ms();
-    // filled completely below (padding included): no zero-initialisation pass
-    std::unique_ptr<int[]> hcol(new int[size_t(std::max<int64_t>(stored, 1))]);
-    std::unique_ptr<double[]> hval(new double[size_t(std::max<int64_t>(stored, 1)) * vw]);
-    std::unique_ptr<float[]> hval32(new float[size_t(std::max<int64_t>(stored, 1)) * vw]);
-#pragma omp parallel for schedule(dynamic, 64) if (stored > 200000)
-    for (int s = 0; s < ns; s++) {
-        for (int l = 0; l < C; l++) {
-            const int r = s * C + l;
-            const int row = r < n_rows ? order[r] : -1;
-            const int64_t len = row >= 0 ? rowptr[row + 1] - rowptr[row] : 0;
-            for (int j = 0; j < sw[s]; j++) {
-                const int64_t e = sptr[s] + int64_t(j) * C + l;
-                if (j < len) {
-                    const int64_t k = rowptr[row] + j;
-                    hcol[e] = col[k];
-                    for (int c = 0; c < vw; c++) {
-                        hval[e * vw + c] = scale * val[k * vw + c];
-                        hval32[e * vw + c] = float(scale * val[k * vw + c]);
-                    }
-                } else {
-                    hcol[e] = (row >= 0 && row < op->m) ? row : 0;  // padding: valid column, zero value
-                    for (int c = 0; c < vw; c++) { hval[e * vw + c] = 0.0; hval32[e * vw + c] = 0.0f; }
-                }
-            }
-        }
-    }
     // inverse diagonal (real part); rectangular operators keep 1
     std::vector<double> dinv(n, 1.0);
     std::vector<float> dinv32(n, 1.0f);
@@ -680,10 +691,6 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     }
     // keep every uploaded array non-empty (an operator may consist of long rows only)
     if (sw.empty()) sw.push_back(0);
-    if (stored == 0) {
-        hcol[0] = 0;
-        for (int c = 0; c < vw; c++) { hval[c] = 0.0; hval32[c] = 0.0f; }
-    }
     if (order.empty()) order.push_back(0);
     const double t_fill = lap_ms();
     hipStream_t st = op->ctx->stream;
@@ -694,13 +701,40 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->slice_ptr.upload(sptr.data(), sptr.size(), st);
     op->slice_w.upload(sw.data(), std::max<size_t>(sw.size(), 1), st);
     const size_t nst = size_t(std::max<int64_t>(stored, 1));
-    op->col.upload(hcol.get(), nst, st);
-    op->val64.upload(hval.get(), nst * vw, st);
-    op->val32.upload(hval32.get(), nst * vw, st);
     op->dinv64.upload(dinv.data(), dinv.size(), st);
     op->dinv32.upload(dinv32.data(), dinv32.size(), st);
     if (permuted) op->perm.upload(order.data(), std::max<size_t>(order.size(), 1), st);
     else op->perm.release();
+    // the SELL arrays are filled on the device from the CSR arrays (one upload of the CSR
+    // instead of a host fill and an upload of three padded arrays)
+    op->col.alloc(nst);
+    op->val64.alloc(nst * vw);
+    op->val32.alloc(nst * vw);
+    if (stored > 0) {
+        const int64_t nnz_all = rowptr[n];
+        DevBuf<int64_t> drp;
+        DevBuf<int> dcol;
+        DevBuf<double> dval;
+        drp.upload(rowptr, size_t(n) + 1, st);
+        dcol.upload(col, size_t(std::max<int64_t>(nnz_all, 1)), st);
+        dval.upload(val, size_t(std::max<int64_t>(nnz_all, 1)) * vw, st);
+        const size_t nthr = size_t(ns) * size_t(C);
+        const unsigned g = unsigned((nthr + 255) / 256);
+        if (vw == 2)
+            hipLaunchKernelGGL((k_sell_fill<2>), dim3(g), dim3(256), 0, st, ns, C, n_rows, op->m, op->slice_ptr.p,
+                               op->slice_w.p, (const int*)(permuted ? op->perm.p : nullptr), (const int64_t*)drp.p,
+                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p);
+        else
+            hipLaunchKernelGGL((k_sell_fill<1>), dim3(g), dim3(256), 0, st, ns, C, n_rows, op->m, op->slice_ptr.p,
+                               op->slice_w.p, (const int*)(permuted ? op->perm.p : nullptr), (const int64_t*)drp.p,
+                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p);
+        MK_HIP(hipGetLastError());
+        MK_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
+    } else {
+        MK_HIP(hipMemsetAsync(op->col.p, 0, nst * sizeof(int), st));
+        MK_HIP(hipMemsetAsync(op->val64.p, 0, nst * vw * sizeof(double), st));
+        MK_HIP(hipMemsetAsync(op->val32.p, 0, nst * vw * sizeof(float), st));
+    }
     op->nmid = nmid;
     op->mid_lanes = mlanes;
     if (nmid) {
@@ -716,6 +750,22 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         op->long_col.upload(lcol.data(), lcol.size(), st);
         op->long_val64.upload(lval.data(), lval.size(), st);
         op->long_val32.upload(lval32.data(), lval32.size(), st);
+        std::vector<int64_t> sb, se;
+        std::vector<int> lsp(size_t(nlong) + 1, 0);
+        for (int w = 0; w < nlong; w++) {
+            for (int64_t k = lptr[w]; k < lptr[w + 1]; k += kLongSeg) {
+                sb.push_back(k);
+                se.push_back(std::min<int64_t>(k + kLongSeg, lptr[w + 1]));
+            }
+            lsp[size_t(w) + 1] = int(sb.size());
+        }
+        op->nseg = int(sb.size());
+        op->seg_beg.upload(sb.data(), std::max<size_t>(sb.size(), 1), st);
+        op->seg_end.upload(se.data(), std::max<size_t>(se.size(), 1), st);
+        op->lseg_ptr.upload(lsp.data(), lsp.size(), st);
+        op->seg_part.alloc(std::max<size_t>(sb.size(), 1) * 64);
+    } else {
+        op->nseg = 0;
     }
     MK_HIP(hipStreamSynchronize(st));
     if (n > 1000000)
@@ -738,7 +788,7 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
     SpmvTimer tm(c, bytes, A, kname[int(mode)], sizeof(T) == 4);
     const unsigned gr = std::max(1u, std::min(g, fin_grid()));  // reducing modes: capped grid
 #define MK_SPMV(CPLX, MODE)                                                                                \
-    MK_TLAUNCH(tm, 1, 1, (sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
+    MK_TLAUNCH(tm, 0, (sell_spmv<T, CPLX, MODE>), dim3((MODE == 1 || MODE == 2) ? gr : g), dim3(A->block), 0, \
                c->stream, D, x, y, b, c->partials.p, skip)
 #define MK_SPMV_ALL(CPLX)                            \
     switch (mode) {                                  \
@@ -794,10 +844,10 @@ void cheb_launch(mk_ctx* c, const mk_op* A, const T* x_in, T* x_out, const T* b,
                                                                                       double(sizeof(T)));
     SpmvTimer tm(c, bytes, A, "cheb", sizeof(T) == 4);
     if (A->cplx)
-        MK_TLAUNCH(tm, 1, 1, (sell_cheb<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
+        MK_TLAUNCH(tm, 0, (sell_cheb<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
                    a, s, first, skip);
     else
-        MK_TLAUNCH(tm, 1, 1, (sell_cheb<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
+        MK_TLAUNCH(tm, 0, (sell_cheb<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, x_in, x_out, b, d, dinv,
                    a, s, first, skip);
     MK_HIP(hipGetLastError());
 }
@@ -856,10 +906,10 @@ void cheb0_launch(mk_ctx* c, const mk_op* A, const T* b, T* x_out, T* d, T c0, T
     const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) + double(A->n) * (vb + double(sizeof(T)));
     SpmvTimer tm(c, bytes, A, "cheb0", sizeof(T) == 4);
     if (A->cplx)
-        MK_TLAUNCH(tm, 1, 1, (sell_cheb0<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
+        MK_TLAUNCH(tm, 0, (sell_cheb0<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
                    a, s, skip);
     else
-        MK_TLAUNCH(tm, 1, 1, (sell_cheb0<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
+        MK_TLAUNCH(tm, 0, (sell_cheb0<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, b, dinv, x_out, d, c0,
                    a, s, skip);
     MK_HIP(hipGetLastError());
 }
@@ -902,9 +952,9 @@ void spmv2_launch(mk_ctx* c, const mk_op* A, int m1, const T* u, const T* v, T* 
     const SellDev<T> D = sell_dev<T>(A, val, A->block, &g);
     SpmvTimer tm(c, sizeof(T) == 8 ? A->bytes64 : A->bytes32, A, "restrict_resid", sizeof(T) == 4);
     if (A->cplx)
-        MK_TLAUNCH(tm, 1, 1, (sell_spmv2<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+        MK_TLAUNCH(tm, 0, (sell_spmv2<T, true>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
     else
-        MK_TLAUNCH(tm, 1, 1, (sell_spmv2<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
+        MK_TLAUNCH(tm, 0, (sell_spmv2<T, false>), dim3(g), dim3(A->block), 0, c->stream, D, m1, u, v, y, skip);
     MK_HIP(hipGetLastError());
 }
 template void spmv2_launch<double>(mk_ctx*, const mk_op*, int, const double*, const double*, double*, const int*);
@@ -985,10 +1035,10 @@ static void launch_cg_spmv(mk_ctx* c, const mk_op* A, const T* val, const T* p, 
     {
         SpmvTimer tm(c, bytes, A, "cg_spmv", sizeof(T) == 4);
         if (A->cplx)
-            MK_TLAUNCH(tm, 1, 1, (cg_spmv<T, true>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+            MK_TLAUNCH(tm, 0, (cg_spmv<T, true>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
                        (const CgState*)c->state.p, c->partials.p);
         else
-            MK_TLAUNCH(tm, 1, 1, (cg_spmv<T, false>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
+            MK_TLAUNCH(tm, 0, (cg_spmv<T, false>), dim3(gr), dim3(A->block), 0, c->stream, D, p, q,
                        (const CgState*)c->state.p, c->partials.p);
     }
     finish<1, FinCgAlpha>(c, gr);
@@ -1217,10 +1267,10 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         {
             SpmvTimer tm(c, A->bytes64 + double(len) * (4.0 + 8.0 * 5.0), A, "cg_spmv", false);
             if (A->cplx)
-                MK_TLAUNCH(tm, 1, 1, (cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D,
+                MK_TLAUNCH(tm, 0, (cgm_spmv<true>), dim3(gsr), dim3(A->block), 0, c->stream, D,
                            (const float*)w.z32.p, w.p.p, w.q.p, w.z.p, st, c->partials.p);
             else
-                MK_TLAUNCH(tm, 1, 1, (cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D,
+                MK_TLAUNCH(tm, 0, (cgm_spmv<false>), dim3(gsr), dim3(A->block), 0, c->stream, D,
                            (const float*)w.z32.p, w.p.p, w.q.p, w.z.p, st, c->partials.p);
         }
         hipLaunchKernelGGL(cgm_spmv_fin, dim3(1), dim3(1024), 0, c->stream, st, (const double*)c->partials.p, int(gsr));

@@ -223,6 +223,12 @@ struct mk_op {
     mk::DevBuf<int> long_col;
     mk::DevBuf<double> long_val64;
     mk::DevBuf<float> long_val32;
+    // the long rows cut into segments of at most kLongSeg entries for the multi-column kernels
+    // (one wavefront per segment, partial sums combined per row in segment order)
+    int nseg = 0;
+    mk::DevBuf<int64_t> seg_beg, seg_end;  // entry range of each segment in long_col / long_val
+    mk::DevBuf<int> lseg_ptr;              // segments of long row w: [lseg_ptr[w], lseg_ptr[w + 1])
+    mk::DevBuf<double> seg_part;           // nseg x 64 partial sums (FP64 or FP32 entries)
     mk::DevBuf<double> dinv64;  // inverse diagonal (real part for complex)
     mk::DevBuf<float> dinv32;
     int block = 256;

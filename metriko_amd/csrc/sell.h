@@ -14,6 +14,9 @@ static constexpr int kMaxK = 4;
 // Longest row kept in the SELL part; longer rows go to the medium (lane group per row) or
 // long (wavefront per row) parts at upload, so SELL slice widths never exceed it.
 static constexpr int kSellMaxLen = 12;
+// Entries per segment of a long row in the multi-column kernels (a 1,330-entry row: 11
+// wavefronts instead of one serial chain)
+static constexpr int kLongSeg = 128;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -104,6 +107,11 @@ struct SellDev {
     const int64_t* long_ptr;
     const int* long_col;
     const T* long_val;
+    int nseg;                // long-row segments (multi-column kernels)
+    const int64_t* seg_beg;
+    const int64_t* seg_end;
+    const int* lseg_ptr;
+    T* seg_part;
 };
 
 template <typename T>
@@ -249,14 +257,19 @@ void timing_flush(mk_ctx* c);
 // "<operator label>:<kernel>:<fp32|fp64>".
 struct SpmvTimer {
     mk_ctx* c;
-    // with timing on: the start / end stamps the timed launches carry (MK_TLAUNCH)
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    SpmvTimer(mk_ctx* c_, double bytes, const mk_op* A = nullptr, const char* kernel = "spmv", bool fp32 = false)
-        : c(c_) {
+    // with timing on: start / end stamps for each of up to three launches of one timed
+    // quantity (MK_TLAUNCH part 0, 1, 2); the durations are added (the intervals between the
+    // launches are not counted)
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int parts = 1;
+    mutable unsigned used = 0;
+    SpmvTimer(mk_ctx* c_, double bytes, const mk_op* A = nullptr, const char* kernel = "spmv", bool fp32 = false,
+              int nparts = 1)
+        : c(c_), parts(nparts) {
         c->spmv_calls++;
         c->spmv_bytes += bytes;
         if (c->time_spmv) {
-            if (c->ev_used + 2 > 4096) timing_flush(c);
+            if (c->ev_used + 2 * size_t(parts) > 4096) timing_flush(c);
             std::string key = std::string(A ? A->label : "op") + ":" + kernel + ":" + (fp32 ? "fp32" : "fp64");
             auto it = std::find_if(c->kstats.begin(), c->kstats.end(), [&](const auto& p) { return p.first == key; });
             if (it == c->kstats.end()) {
@@ -267,24 +280,34 @@ struct SpmvTimer {
                 it = c->kstats.end() - 1;
             }
             it->second.bytes += bytes;
-            c->ev_key.push_back(key);
-            e0 = next_event(c);
-            e1 = next_event(c);
+            for (int q = 0; q < parts; q++) {
+                c->ev_key.push_back(q ? "+" + key : key);  // "+": time added to the call, no extra call
+                ev[2 * q] = next_event(c);
+                ev[2 * q + 1] = next_event(c);
+            }
         }
+    }
+    ~SpmvTimer() {
+        for (int q = 0; q < parts; q++)  // a part that was not launched: an empty interval
+            if (ev[2 * q] && !(used & (1u << q))) {
+                (void)hipEventRecord(ev[2 * q], c->stream);
+                (void)hipEventRecord(ev[2 * q + 1], c->stream);
+            }
     }
 };
 
-// Launches kernel K on STREAM.  With timing on, the launch carries the timer's start stamp
-// (FIRST) and / or end stamp (LAST) through hipExtLaunchKernelGGL: the stamps are taken at
-// the kernel's own start and end, so the timed interval holds no event packet; a timed
-// quantity launched as two kernels passes FIRST to the first and LAST to the second.
-#define MK_TLAUNCH(tm, FIRST, LAST, K, G, B, SH, STREAM, ...)                                             \
-    do {                                                                                                  \
-        if ((tm).e0)                                                                                      \
-            hipExtLaunchKernelGGL(K, G, B, SH, STREAM, (FIRST) ? (tm).e0 : nullptr, (LAST) ? (tm).e1 : nullptr, \
-                                  0, __VA_ARGS__);                                                        \
-        else                                                                                              \
-            hipLaunchKernelGGL(K, G, B, SH, STREAM, __VA_ARGS__);                                         \
+// Launches kernel K on STREAM.  With timing on, the launch carries the timer's start and end
+// stamps of part PART through hipExtLaunchKernelGGL: they are taken at the kernel's own start
+// and end, so the timed interval holds no event packet.
+#define MK_TLAUNCH(tm, PART, K, G, B, SH, STREAM, ...)                                                  \
+    do {                                                                                                \
+        if ((tm).ev[0]) {                                                                               \
+            (tm).used |= 1u << (PART);                                                                  \
+            hipExtLaunchKernelGGL(K, G, B, SH, STREAM, (tm).ev[2 * (PART)], (tm).ev[2 * (PART) + 1], 0, \
+                                  __VA_ARGS__);                                                         \
+        } else {                                                                                        \
+            hipLaunchKernelGGL(K, G, B, SH, STREAM, __VA_ARGS__);                                       \
+        }                                                                                               \
     } while (0)
 
 template <typename T>
@@ -302,6 +325,11 @@ static SellDev<T> sell_dev(const mk_op* A, const T* val, int block, unsigned* gr
     d.long_rows = A->long_rows.p;
     d.long_ptr = A->long_ptr.p;
     d.long_col = A->long_col.p;
+    d.nseg = A->nseg;
+    d.seg_beg = A->seg_beg.p;
+    d.seg_end = A->seg_end.p;
+    d.lseg_ptr = A->lseg_ptr.p;
+    d.seg_part = reinterpret_cast<T*>(A->seg_part.p);
     if constexpr (std::is_same<T, double>::value) d.long_val = A->long_val64.p;
     else d.long_val = A->long_val32.p;
     d.nmid = A->nmid;
