@@ -415,15 +415,14 @@ __global__ void __launch_bounds__(256) bk_init0_fin(BState* st, const double* __
     }
 }
 
-// P = Zold = Z ; rz_j = R.Z ; rr_j = R.R ; done flags
+// P = Z ; rz_j = R.Z ; rr_j = R.R ; done flags
 template <int MW, typename ZT>
 __global__ void __launch_bounds__(256) bk_init1(size_t len, const double* __restrict__ R, const ZT* __restrict__ Z,
-                                              double* __restrict__ P, double* __restrict__ Zold, double* partials) {
+                                              double* __restrict__ P, double* partials) {
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double z = double(Z[e]), r = R[e];
         P[e] = z;
-        Zold[e] = z;
         v[0] += r * z;
         v[1] += r * r;
     }
@@ -521,13 +520,13 @@ __global__ void __launch_bounds__(256) bk_update_fin(BState* st, const double* _
 // flexible beta_j = (R.Z - R.Zold) / rz_j ; rz_j = R.Z
 template <int MW, typename ZT>
 __global__ void __launch_bounds__(256) bk_beta(size_t len, const double* __restrict__ R, const ZT* __restrict__ Z,
-                                             const double* __restrict__ Zold, const BState* st, double* partials) {
+                                             const ZT* __restrict__ Zold, const BState* st, double* partials) {
     if (st->all_done) return;
     double v[2] = {0.0, 0.0};
     for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
         const double r = R[e];
         v[0] += r * double(Z[e]);
-        v[1] += r * Zold[e];
+        v[1] += r * double(Zold[e]);
     }
     colsum_write<2, MW>(v, partials);
 }
@@ -555,19 +554,16 @@ __global__ void __launch_bounds__(256) bk_beta_fin(BState* st, const double* __r
     }
 }
 
-// P = Z + beta P ; Zold = Z (running columns only)
+// P = Z + beta P (running columns only)
 template <int MW, typename ZT>
 __global__ void __launch_bounds__(256) bk_pupdate(size_t len, double* __restrict__ P, const ZT* __restrict__ Z,
-                                                double* __restrict__ Zold, const BState* st) {
+                                                const BState* st) {
     if (st->all_done) return;
     const int j = threadIdx.x % MW;
     if (st->done[j]) return;
     const double beta = st->beta[j];
-    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256) {
-        const double z = double(Z[e]);
-        P[e] = z + beta * P[e];
-        Zold[e] = z;
-    }
+    for (size_t e = size_t(blockIdx.x) * 256 + threadIdx.x; e < len; e += size_t(gridDim.x) * 256)
+        P[e] = double(Z[e]) + beta * P[e];
 }
 
 __global__ void k_set_bstate(BState* st, double tol2) {
@@ -582,8 +578,12 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
                                 int maxiter, int check_every) {
     constexpr bool fp64 = std::is_same<ZT, double>::value;
     const size_t len = size_t(A->n) * MW;
-    DevBuf<double> R(len), P(len), Q(len), Zold(len), part(size_t(2) * 64 * kBlkParts);
-    DevBuf<ZT> Rz(fp64 ? 0 : len), Z(len);
+    DevBuf<double> R(len), P(len), Q(len), part(size_t(2) * 64 * kBlkParts);
+    // two preconditioned residual blocks, swapped every iteration: the flexible beta needs the
+    // previous one (kept in the V-cycle's own precision, so no copy is made)
+    DevBuf<ZT> Rz(fp64 ? 0 : len), Za(len), Zb(len);
+    ZT* zc = Za.p;  // the V-cycle's current output
+    ZT* zo = Zb.p;  // the previous one
     ZT* rz = fp64 ? nullptr : Rz.p;                 // the V-cycle's input copy of R
     const ZT* vin = fp64 ? reinterpret_cast<const ZT*>(R.p) : Rz.p;
     DevBuf<BState> st(1);
@@ -595,9 +595,10 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
     hipLaunchKernelGGL(k_set_bstate, dim3(1), dim3(1), 0, c->stream, st.p, 0.25 * rel_tol * rel_tol);
     hipLaunchKernelGGL((bk_init0<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, B, X, R.p, rz, part.p);
     hipLaunchKernelGGL((bk_init0_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-    mg_apply_block<ZT>(c, mg, MW, vin, Z.p, nullptr);
+    mg_apply_block<ZT>(c, mg, MW, vin, zc, nullptr);
     hipLaunchKernelGGL((bk_init1<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                       (const ZT*)Z.p, P.p, Zold.p, part.p);
+                       (const ZT*)zc, P.p, part.p);
+    std::swap(zc, zo);
     hipLaunchKernelGGL((bk_init1_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
     MK_HIP(hipGetLastError());
     BState h{};
@@ -616,12 +617,13 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
             hipLaunchKernelGGL((bk_update<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, X, R.p, rz,
                                (const double*)P.p, (const double*)Q.p, (const BState*)st.p, part.p);
             hipLaunchKernelGGL((bk_update_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-            mg_apply_block<ZT>(c, mg, MW, vin, Z.p, &st.p->all_done);
+            mg_apply_block<ZT>(c, mg, MW, vin, zc, &st.p->all_done);
             hipLaunchKernelGGL((bk_beta<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, (const double*)R.p,
-                               (const ZT*)Z.p, (const double*)Zold.p, (const BState*)st.p, part.p);
+                               (const ZT*)zc, (const ZT*)zo, (const BState*)st.p, part.p);
             hipLaunchKernelGGL((bk_beta_fin<MW>), dim3(1), dim3(256), 0, c->stream, st.p, (const double*)part.p, ng);
-            hipLaunchKernelGGL((bk_pupdate<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const ZT*)Z.p,
-                               Zold.p, (const BState*)st.p);
+            hipLaunchKernelGGL((bk_pupdate<MW, ZT>), dim3(g), dim3(256), 0, c->stream, len, P.p, (const ZT*)zc,
+                               (const BState*)st.p);
+            std::swap(zc, zo);
         }
         iters += batch;
         MK_HIP(hipGetLastError());

@@ -216,30 +216,28 @@ metriko::HVec<double> scaled_shifted(const SprsD& A, double s, double d) {
     return v;
 }
 
-mk_op* make_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o) {
-    mk_op* op = nullptr;
-    mk_status s = mk_op_create_real(c, A.rows(), A.ptr.data(), A.col.data(), A.val.data(), &o, &op);
-    if (s != MK_OK) throw Error(s, "operator upload failed: " + c->last_error);
-    return op;
-}
 // rectangular operator (no multigrid, no diagonal): hybrid SELL upload of a host CSR
-mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o, double scale = 1.0) {
+// device operator (no host copy kept) from CSR arrays, values times scale
+mk_op* make_op_arrays(mk_ctx* c, int rows, int cols, const int64_t* ptr, const int* col, const double* val,
+                      const mk_solve_opts& o, double scale = 1.0) {
     auto* op = new mk_op();
     try {
         op->ctx = c;
-        op->n = A.rows();
-        op->m = A.cols();
+        op->n = rows;
+        op->m = cols;
         op->cplx = false;
-        op->nnz = A.nonZeros();
+        op->nnz = ptr[rows];
         op->block = o.block_size;
         const bool csr = o.format == MK_FMT_CSR;
-        mk::build_sell(op, A.ptr.data(), A.col.data(), A.val.data(), csr ? 64 : o.slice_height, csr ? 1 : o.sigma,
-                       scale);
+        mk::build_sell(op, ptr, col, val, csr ? 64 : o.slice_height, csr ? 1 : o.sigma, scale);
     } catch (...) {
         delete op;
         throw;
     }
     return op;
+}
+mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o, double scale = 1.0) {
+    return make_op_arrays(c, A.rows(), A.cols(), A.ptr.data(), A.col.data(), A.val.data(), o, scale);
 }
 struct OpHolder {
     mk_op* op = nullptr;
@@ -1499,7 +1497,29 @@ extern "C" mk_status mk_iterative_rounding(
             for (int64_t q = E.ptr[i]; q < E.ptr[i + 1]; q++) if (E.col[q] == i) dmean += E.val[q];
         dmean = k ? dmean / k : 1.0;
         if (!(dmean > 0.0)) dmean = 1.0;
-        SprsD Eaug = E + (Cm.transpose() * Cm).scaled(dmean);
+        // Eaug = E + dmean Cm^T Cm: the penalty's few entries added to a copy of E's values
+        // when they lie in E's pattern (the fixed DOFs' own entries), else the sparse sum
+        const SprsD Pen = (Cm.transpose() * Cm).scaled(dmean);
+        metriko::HVec<double> eaug_val;
+        SprsD EaugSum;
+        bool in_pattern = true;
+        {
+            std::vector<std::pair<int64_t, double>> adds;
+            for (int i = 0; i < Pen.rows() && in_pattern; i++)
+                for (int64_t q = Pen.ptr[i]; q < Pen.ptr[i + 1] && in_pattern; q++) {
+                    const int* b = E.col.data() + E.ptr[i];
+                    const int* e = E.col.data() + E.ptr[i + 1];
+                    const int* f = std::lower_bound(b, e, Pen.col[q]);
+                    if (f == e || *f != Pen.col[q]) in_pattern = false;
+                    else adds.emplace_back(int64_t(f - E.col.data()), Pen.val[q]);
+                }
+            if (in_pattern) {
+                eaug_val = E.val;
+                for (const auto& a : adds) eaug_val[size_t(a.first)] = 1.0 * eaug_val[size_t(a.first)] + 1.0 * a.second;
+            } else {
+                EaugSum = E + Pen;
+            }
+        }
         std::vector<double> rhs(fr);
         {
             std::vector<double> ctg(size_t(k), 0.0);
@@ -1530,7 +1550,8 @@ extern "C" mk_status mk_iterative_rounding(
             t0 += now_ms() - ts;  // the KKT time excludes the A0 operator and hierarchy set-up
         }
         MK_LOG("[igm] KKT: Eaug assembled %.0f ms\n", now_ms() - t0);
-        hE.op = make_op(c, Eaug, o);
+        hE.op = in_pattern ? make_op_arrays(c, k, k, E.ptr.data(), E.col.data(), eaug_val.data(), o)
+                           : make_rect_op(c, EaugSum, o);
         MK_LOG("[igm] KKT: Eaug uploaded %.0f ms\n", now_ms() - t0);
         hE.op->label = "igm_Eaug";
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
