@@ -245,9 +245,9 @@ inline bool compatible(const std::complex<double>&, const std::vector<std::compl
 // Strength graph and greedy aggregation (seed / attach / leftover passes).
 template <typename T>
 struct Aggregation {
-    std::vector<int64_t> sptr;
-    std::vector<int> scol;
-    std::vector<T> sval;
+    metriko::HVec<int64_t> sptr;
+    metriko::HVec<int> scol;
+    metriko::HVec<T> sval;
     std::vector<int> agg, root;
     int nagg = 0;
 };
@@ -257,9 +257,9 @@ Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double t
                          const std::vector<T>* B = nullptr, int nb = 0) {
     Aggregation<T> G;
     const int n = A.nr;
-    std::vector<int64_t>& sptr = G.sptr;
-    std::vector<int>& scol = G.scol;
-    std::vector<T>& sval = G.sval;
+    auto& sptr = G.sptr;
+    auto& scol = G.scol;
+    auto& sval = G.sval;
     sptr.assign(size_t(n) + 1, 0);
     // long rows (kernel-basis / transition DOFs coupling to a whole seam) stay out of the
     // strength graph: they become singleton aggregates and are not smoothed into others
@@ -300,8 +300,11 @@ Aggregation<T> aggregate(const Csr<T>& A, const std::vector<double>& d, double t
         nagg++;
     }
     std::vector<int> agg1 = agg;
-    for (int i = 0; i < n && tune().attach; i++) {  // attach to the strongest seeded neighbour
-        if (agg[i] != -1) continue;
+    const bool attach = tune().attach != 0;
+    // attach to the strongest seeded neighbour (each node reads only the seed pass's result)
+#pragma omp parallel for schedule(static, 4096) if (n > 100000)
+    for (int i = 0; i < n; i++) {
+        if (!attach || agg[i] != -1) continue;
         int best = -1;
         double bv = -1.0;
         for (int64_t k = sptr[i]; k < sptr[i + 1]; k++) {
@@ -326,9 +329,9 @@ template <typename T>
 Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, int& nagg) {
     const int n = A.nr;
     const Aggregation<T> G = aggregate(A, d, theta);
-    const std::vector<int64_t>& sptr = G.sptr;
-    const std::vector<int>& scol = G.scol;
-    const std::vector<T>& sval = G.sval;
+    const auto& sptr = G.sptr;
+    const auto& scol = G.scol;
+    const auto& sval = G.sval;
     const std::vector<int>& agg = G.agg;
     const std::vector<int>& root = G.root;
     nagg = G.nagg;
@@ -387,56 +390,81 @@ Csr<T> tentative_qr(const Csr<T>& A, const std::vector<double>& d, double theta,
     std::vector<double> scale(static_cast<size_t>(nb), 0.0);
     for (int c = 0; c < nb; c++)
         for (int i = 0; i < n; i++) scale[c] = std::max(scale[c], mag(B[size_t(c) * n + i]));
-    std::vector<metriko::Triplet<T>> tr;
-    tr.reserve(size_t(n) * nb);
-    std::vector<std::vector<T>> bc_rows;  // coarse rows of B, one vector of nb entries per coarse dof
-    nc = 0;
-    std::vector<T> Q, v;
-    for (int a = 0; a < G.nagg; a++) {
+    // per aggregate (independent, in parallel): the orthonormal columns Q (r of them, m rows
+    // each, at qoff = mptr[a] * nb) and the coarse rows R = Q^H B (r x nb at roff = a * nb * nb)
+    const int nagg = G.nagg;
+    metriko::HVec<T> Qall(size_t(n) * size_t(nb)), Rall(size_t(nagg) * size_t(nb) * size_t(nb));
+    std::vector<int> rcount(static_cast<size_t>(nagg));
+#pragma omp parallel if (n > 100000)
+    {
+        std::vector<T> v;
+#pragma omp for schedule(dynamic, 256)
+        for (int a = 0; a < nagg; a++) {
+            const int m = mptr[a + 1] - mptr[a];
+            const int* rows = mem.data() + mptr[a];
+            T* Q = Qall.data() + size_t(mptr[a]) * nb;
+            int r = 0;
+            for (int c = 0; c < nb; c++) {
+                v.assign(static_cast<size_t>(m), T(0));
+                for (int i = 0; i < m; i++) v[i] = B[size_t(c) * n + rows[i]];
+                double n0 = 0.0;
+                for (int i = 0; i < m; i++) n0 += mag2(v[i]);
+                n0 = std::sqrt(n0);
+                for (int pass = 0; pass < 2; pass++)
+                    for (int q = 0; q < r; q++) {
+                        T s = T(0);
+                        for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * v[i];
+                        for (int i = 0; i < m; i++) v[i] -= s * Q[size_t(q) * m + i];
+                    }
+                double nv = 0.0;
+                for (int i = 0; i < m; i++) nv += mag2(v[i]);
+                nv = std::sqrt(nv);
+                if (!(nv > 1e-10 * std::max(n0, scale[c] * 1e-6)) || nv == 0.0) continue;
+                for (int i = 0; i < m; i++) Q[size_t(r) * m + i] = v[i] / nv;
+                r++;
+            }
+            if (r == 0) {
+                for (int i = 0; i < m; i++) Q[i] = T(1.0 / std::sqrt(double(m)));
+                r = 1;
+            }
+            T* R = Rall.data() + size_t(a) * nb * nb;
+            for (int q = 0; q < r; q++)
+                for (int c = 0; c < nb; c++) {
+                    T s = T(0);
+                    for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * B[size_t(c) * n + rows[i]];
+                    R[size_t(q) * nb + c] = s;
+                }
+            rcount[size_t(a)] = r;
+        }
+    }
+    // coarse numbering: aggregate a owns coarse columns [coff[a], coff[a] + r_a)
+    std::vector<int> coff(size_t(nagg) + 1, 0);
+    for (int a = 0; a < nagg; a++) coff[size_t(a) + 1] = coff[size_t(a)] + rcount[size_t(a)];
+    nc = coff[size_t(nagg)];
+    // P: fine row rows[i] of aggregate a holds Q[q][i] in column coff[a] + q (q ascending)
+    Csr<T> P(n, nc);
+    for (int i = 0; i < n; i++) P.ptr[size_t(i) + 1] = rcount[size_t(G.agg[i])];
+    for (int i = 0; i < n; i++) P.ptr[size_t(i) + 1] += P.ptr[size_t(i)];
+    P.col.resize(size_t(P.ptr[size_t(n)]));
+    P.val.resize(size_t(P.ptr[size_t(n)]));
+#pragma omp parallel for schedule(dynamic, 256) if (n > 100000)
+    for (int a = 0; a < nagg; a++) {
         const int m = mptr[a + 1] - mptr[a];
         const int* rows = mem.data() + mptr[a];
-        Q.clear();
-        int r = 0;
-        for (int c = 0; c < nb; c++) {
-            v.assign(static_cast<size_t>(m), T(0));
-            for (int i = 0; i < m; i++) v[i] = B[size_t(c) * n + rows[i]];
-            double n0 = 0.0;
-            for (int i = 0; i < m; i++) n0 += mag2(v[i]);
-            n0 = std::sqrt(n0);
-            for (int pass = 0; pass < 2; pass++)
-                for (int q = 0; q < r; q++) {
-                    T s = T(0);
-                    for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * v[i];
-                    for (int i = 0; i < m; i++) v[i] -= s * Q[size_t(q) * m + i];
-                }
-            double nv = 0.0;
-            for (int i = 0; i < m; i++) nv += mag2(v[i]);
-            nv = std::sqrt(nv);
-            if (!(nv > 1e-10 * std::max(n0, scale[c] * 1e-6)) || nv == 0.0) continue;
-            for (int i = 0; i < m; i++) Q.push_back(v[i] / nv);
-            r++;
-        }
-        if (r == 0) {
-            for (int i = 0; i < m; i++) Q.push_back(T(1.0 / std::sqrt(double(m))));
-            r = 1;
-        }
-        for (int q = 0; q < r; q++) {
-            std::vector<T> row(static_cast<size_t>(nb), T(0));
-            for (int c = 0; c < nb; c++) {
-                T s = T(0);
-                for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * B[size_t(c) * n + rows[i]];
-                row[c] = s;
+        const T* Q = Qall.data() + size_t(mptr[a]) * nb;
+        for (int i = 0; i < m; i++) {
+            int64_t w = P.ptr[size_t(rows[i])];
+            for (int q = 0; q < rcount[size_t(a)]; q++, w++) {
+                P.col[size_t(w)] = coff[size_t(a)] + q;
+                P.val[size_t(w)] = Q[size_t(q) * m + i];
             }
-            for (int i = 0; i < m; i++) tr.emplace_back(rows[i], nc + q, Q[size_t(q) * m + i]);
-            bc_rows.push_back(std::move(row));
         }
-        nc += r;
     }
-    Csr<T> P(n, nc);
-    P.setFromTriplets(tr);
     Bc.assign(size_t(nc) * nb, T(0));
-    for (int i = 0; i < nc; i++)
-        for (int c = 0; c < nb; c++) Bc[size_t(c) * nc + i] = bc_rows[i][c];
+    for (int a = 0; a < nagg; a++)
+        for (int q = 0; q < rcount[size_t(a)]; q++)
+            for (int c = 0; c < nb; c++)
+                Bc[size_t(c) * nc + coff[size_t(a)] + q] = Rall[(size_t(a) * nb + q) * nb + c];
     return P;
 }
 
@@ -513,14 +541,13 @@ struct HostHier {
 };
 
 template <typename T>
-HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std::vector<T>* B0 = nullptr,
+HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector<T>* B0 = nullptr,
                        int nb = 0) {
     HostHier<T> H;
     const bool cplx = !std::is_same<T, double>::value;
-    Csr<T> A = A0;
     std::vector<T> B;
     if (B0 && nb > 0) {
-        if (B0->size() != size_t(A0.nr) * size_t(nb)) throw Error(MK_ERR_ARG, "multigrid: near-null block size mismatch");
+        if (B0->size() != size_t(A.nr) * size_t(nb)) throw Error(MK_ERR_ARG, "multigrid: near-null block size mismatch");
         B = *B0;
     } else {
         nb = 0;
@@ -680,9 +707,9 @@ HostHier<T> build_host(const Csr<T>& A0, double theta, int coarse_max, const std
 }
 
 template <typename T>
-Mg* build_impl(mk_ctx* c, const Csr<T>& A0, const mk_solve_opts& o, double theta, int coarse_max,
+Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int coarse_max,
                const std::vector<T>* nullvec = nullptr, int nnull = 0) {
-    HostHier<T> H = build_host<T>(A0, theta, coarse_max, nullvec, nnull);
+    HostHier<T> H = build_host<T>(std::move(A0), theta, coarse_max, nullvec, nnull);
     const auto t_up = std::chrono::steady_clock::now();
     auto* g = new Mg();
     try {
@@ -1087,7 +1114,7 @@ extern "C" mk_status mk_mg_host_pcg(int n, const int64_t* ptr, const int32_t* co
             for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
             std::vector<cd> B;
             for (size_t q = 0; q < size_t(n) * size_t(nnull); q++) B.emplace_back(nullvec[2 * q], nullvec[2 * q + 1]);
-            HostHier<cd> H = build_host<cd>(A, tune().theta_cplx, tune().coarse_cplx, nnull ? &B : nullptr, nnull);
+            HostHier<cd> H = build_host<cd>(Csr<cd>(A), tune().theta_cplx, tune().coarse_cplx, nnull ? &B : nullptr, nnull);
             if (levels) *levels = int(H.A.size());
             int it = host_pcg<cd>(A, H, reinterpret_cast<const cd*>(b), reinterpret_cast<cd*>(x), tol, maxit, relres);
             if (iters) *iters = it;
@@ -1098,7 +1125,8 @@ extern "C" mk_status mk_mg_host_pcg(int n, const int64_t* ptr, const int32_t* co
             A.val.assign(val, val + ptr[n]);
             std::vector<double> B;
             if (nnull > 0) B.assign(nullvec, nullvec + size_t(n) * nnull);
-            HostHier<double> H = build_host<double>(A, tune().theta_real, tune().coarse_real, nnull ? &B : nullptr, nnull);
+            HostHier<double> H = build_host<double>(Csr<double>(A), tune().theta_real, tune().coarse_real,
+                                                    nnull ? &B : nullptr, nnull);
             if (levels) *levels = int(H.A.size());
             int it = host_pcg<double>(A, H, b, x, tol, maxit, relres);
             if (iters) *iters = it;
@@ -1120,7 +1148,7 @@ Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const do
     A.ptr.assign(ptr, ptr + n + 1);
     A.col.assign(col, col + ptr[n]);
     A.val.assign(val, val + ptr[n]);
-    return build_impl<double>(c, A, o, tune().theta_real, tune().coarse_real, nullvec, nnull);
+    return build_impl<double>(c, std::move(A), o, tune().theta_real, tune().coarse_real, nullvec, nnull);
 }
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val,
                      const mk_solve_opts& o) {
@@ -1129,7 +1157,7 @@ Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const
     A.col.assign(col, col + ptr[n]);
     A.val.resize(size_t(ptr[n]));
     for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
-    return build_impl<cd>(c, A, o, tune().theta_cplx, tune().coarse_cplx);
+    return build_impl<cd>(c, std::move(A), o, tune().theta_cplx, tune().coarse_cplx);
 }
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip) { vcycle<float>(c, g, 0, b, x, skip); }
