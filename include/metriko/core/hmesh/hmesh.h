@@ -18,6 +18,7 @@
 #ifndef METRIKO_HMESH_H
 #define METRIKO_HMESH_H
 #include <algorithm>
+#include "metriko/core/common/parallel.h"
 #include "metriko/core/common/typedef.h"
 
 namespace metriko {
@@ -147,7 +148,7 @@ namespace metriko {
                 if (a > b) std::swap(a, b);
                 keys[size_t(3 * int64_t(f) + i)] = {(a << 32) | b, 3 * int64_t(f) + i};
             }
-        std::sort(keys.begin(), keys.end());
+        metriko::parallel_sort(keys);
         std::vector<int> ev0, ev1, ef0, ef1;
         ev0.reserve(size_t(nS / 2 + 16));
         face2edge = MatXi::Constant(nF, 3, -1);
@@ -206,13 +207,15 @@ namespace metriko {
         }
         std::vector<int> HV(numH), HE(numH), HF(numH), twinH(numH, -1);
         std::vector<int> VH(nV, -1);
+        // each edge writes only its own halfedges and their (face, side) slots; the loops that
+        // follow write disjoint entries too, so the parallel loops give the serial results
+#pragma omp parallel for schedule(static, 4096) if (nE > 100000)
         for (int e = 0; e < nE; e++) {
             for (int k = 0; k < 2; k++) {
                 const int h = EH(e, k);
                 if (h == -1) continue;
                 HE[h] = e;
                 HV[h] = edge2vert(e, k);
-                VH[edge2vert(e, k)] = h;
                 const int f = edge2face(e, k);
                 int j = -1;
                 for (int s = 0; s < 3; s++) if (face2edge(f, s) == e) j = s;
@@ -221,11 +224,15 @@ namespace metriko {
             }
             if (EH(e, 0) != -1 && EH(e, 1) != -1) { twinH[EH(e, 0)] = EH(e, 1); twinH[EH(e, 1)] = EH(e, 0); }
         }
+        // a vertex keeps the last halfedge assigned to it in edge order: the largest h
+        for (int h = 0; h < numH; h++) VH[HV[h]] = h;
+#pragma omp parallel for schedule(static, 4096) if (nF > 100000)
         for (int f = 0; f < nF; f++)
             for (int j = 0; j < 3; j++) { next[FH(f, j)] = FH(f, (j + 1) % 3); prev[FH(f, (j + 1) % 3)] = FH(f, j); }
         for (int v = 0; v < nV; v++) vert2half[v] = VH[v];
         for (int e = 0; e < nE; e++) edge2half[e] = EH(e, 0);
         for (int f = 0; f < nF; f++) face2half[f] = FH(f, 0);
+#pragma omp parallel for schedule(static, 4096) if (numH > 100000)
         for (int h = 0; h < numH; h++) {
             head[h] = HV[next[h]]; tail[h] = HV[h]; edge[h] = HE[h]; face[h] = HF[h];
             if (twinH[h] != -1) twin[h] = twinH[h];
@@ -251,13 +258,23 @@ namespace metriko {
         nL = int(loop2half.size());
         for (int e = 0; e < nE; e++) if (edge2half[e] == -1) edge2half[e] = twin[EH(e, 1)];
         for (int i = 0; i < nL; i++) loops.push_back(Loop{{i, this}});
-        faces.reserve(nF); verts.reserve(nV); edges.reserve(nE); halfs.reserve(nH); crnrs.reserve(nC);
-        for (int i = 0; i < nF; i++) faces.push_back(Face{{i, this}});
-        for (int i = 0; i < nV; i++) verts.push_back(Vert{{i, this}});
-        for (int i = 0; i < nE; i++) edges.push_back(Edge{{i, this}});
-        for (int i = 0; i < nH; i++) halfs.push_back(Half{{i, this}});
-        for (int i = 0; i < nC; i++) crnrs.push_back(Crnr{{i, this}});
+        faces.resize(size_t(nF)); verts.resize(size_t(nV)); edges.resize(size_t(nE)); halfs.resize(size_t(nH));
+        crnrs.resize(size_t(nC));
+#pragma omp parallel if (nH > 100000)
+        {
+#pragma omp for schedule(static, 16384) nowait
+            for (int i = 0; i < nF; i++) faces[size_t(i)] = Face{{i, this}};
+#pragma omp for schedule(static, 16384) nowait
+            for (int i = 0; i < nV; i++) verts[size_t(i)] = Vert{{i, this}};
+#pragma omp for schedule(static, 16384) nowait
+            for (int i = 0; i < nE; i++) edges[size_t(i)] = Edge{{i, this}};
+#pragma omp for schedule(static, 16384) nowait
+            for (int i = 0; i < nH; i++) halfs[size_t(i)] = Half{{i, this}};
+#pragma omp for schedule(static, 16384)
+            for (int i = 0; i < nC; i++) crnrs[size_t(i)] = Crnr{{i, this}};
+        }
         // corners (hmesh.h:430-442)
+#pragma omp parallel for schedule(static, 4096) if (nF > 100000)
         for (int f = 0; f < nF; f++)
             for (int it = 0; it < 3; it++) {
                 const int vid = F(f, it), cid = f * 3 + it;
@@ -276,7 +293,10 @@ namespace metriko {
         faceArea = VecXd(nF); halfCotan = VecXd(nH); edgeCotan = VecXd(nE); angleDefect = VecXd(nV);
         dihedralArg = VecXd(nH); heArgOnVert = VecXd(nH); heArgOnFace = VecXd(nH);
         baryDualArea = VecXd(nV); circDualArea = VecXd(nV);
+        // every loop below computes each element from its own inputs, so the OpenMP loops give
+        // the serial results bit for bit (the vertex normal sums faces in ascending order)
         // face frames and areas (hmesh.h:461-478)
+#pragma omp parallel for schedule(static, 4096) if (nF > 100000)
         for (int f = 0; f < nF; f++) {
             const int h = face2half[f];
             Row3d x = hvec(h), t = hvec(prev[h]) * -1.0, n = x.cross(t);
@@ -287,6 +307,7 @@ namespace metriko {
             faceArea[f] = n.norm() * 0.5;
         }
         // halfedge argument in the face frame (hmesh.h:481-489)
+#pragma omp parallel for schedule(static, 4096) if (nF > 100000)
         for (int f = 0; f < nF; f++) {
             double sum = 0;
             int h = face2half[f];
@@ -297,10 +318,33 @@ namespace metriko {
                 heArgOnFace[h] = sum;
             }
         }
-        // vertex frames (hmesh.h:492-506)
-        for (int f = 0; f < nF; f++)
-            for (int p = 0; p < 3; p++)
-                for (int c = 0; c < 3; c++) vertNormal(idx(f, p), c) += faceNormal(f, c) * faceArea[f];
+        // vertex frames (hmesh.h:492-506): area-weighted face normals summed per vertex over its
+        // faces in ascending face order (the order of the reference's loop over faces)
+        int64_t fan_corners = 0;
+#pragma omp parallel if (nV > 100000) reduction(+ : fan_corners)
+        {
+            std::vector<int> fs;
+#pragma omp for schedule(static, 4096)
+            for (int v = 0; v < nV; v++) {
+                fs.clear();
+                if (vert2half[v] >= 0)
+                    forVertCcw(vert2half[v], [&](int h) {
+                        if (face[h] != -1) fs.push_back(face[h]);
+                        return true;
+                    });
+                std::sort(fs.begin(), fs.end());
+                fan_corners += int64_t(fs.size());
+                for (int f : fs)
+                    for (int c = 0; c < 3; c++) vertNormal(v, c) += faceNormal(f, c) * faceArea[f];
+            }
+        }
+        if (fan_corners != 3 * int64_t(nF)) {  // a vertex whose faces are not one fan: the face loop
+            vertNormal = MatXd::Zero(nV, 3);
+            for (int f = 0; f < nF; f++)
+                for (int p = 0; p < 3; p++)
+                    for (int c = 0; c < 3; c++) vertNormal(idx(f, p), c) += faceNormal(f, c) * faceArea[f];
+        }
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
         for (int v = 0; v < nV; v++) {
             Row3d n = row3(vertNormal, v).normalized();
             set_row3(vertNormal, v, n);
@@ -310,6 +354,7 @@ namespace metriko {
             set_row3(vertBasisY, v, n.cross(x));
         }
         // dihedral angles and cotangents (hmesh.h:509-533)
+#pragma omp parallel for schedule(static, 4096) if (nH > 100000)
         for (int h = 0; h < nH; h++) {
             if (edgeIsBoundary(edge[h])) { dihedralArg[h] = 0; }
             else {
@@ -321,8 +366,10 @@ namespace metriko {
             Row3d vn = hvec(next[h]), vp = hvec(prev[h]) * -1.0;
             halfCotan[h] = vp.dot(vn) / vp.cross(vn).norm();
         }
+#pragma omp parallel for schedule(static, 4096) if (nE > 100000)
         for (int e = 0; e < nE; e++) edgeCotan[e] = (halfCotan[edge2half[e]] + halfCotan[twin[edge2half[e]]]) * 0.5;
         // dual areas, vertex arguments, angle defects (hmesh.h:536-568)
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
         for (int v = 0; v < nV; v++) {
             double a1 = 0, a2 = 0, sum = 0;
             forVertCcw(vert2half[v], [&](int h) {

@@ -305,16 +305,40 @@ namespace metriko {
     // ---- compute_cut_mesh (parameterization.h:126-152) -------------------------------------
     inline std::unique_ptr<Hmesh> compute_cut_mesh(const Hmesh& m, const std::vector<bool>& seam) {
         MatXi cutF(m.nF, 3);
-        std::vector<int> cut2v;
-        for (int v = 0; v < m.nV; v++) {
+        // vertex v opens a new copy at its fan's start and at every seam crossing; the copies
+        // are numbered in vertex order, so each vertex's count is taken first (in parallel),
+        // then the fans are walked again with their first number
+        const int nV = m.nV;
+        auto start = [&](int v) {
             int bgn = m.vert2half[v];
             if (m.isBV[v]) m.forVertCw(m.vert2half[v], [&](int h) { if (m.edgeIsBoundary(m.edge[h])) { bgn = h; return false; } return true; });
             else m.forVertCw(m.vert2half[v], [&](int h) { if (seam[size_t(m.edge[h])]) { bgn = h; return false; } return true; });
+            return bgn;
+        };
+        std::vector<int> first(size_t(nV) + 1, 0);
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
+        for (int v = 0; v < nV; v++) {
+            if (m.vert2half[v] < 0) continue;
+            const int bgn = start(v);
+            int c = 0;
+            m.forVertCcw(bgn, [&](int h) {
+                if (m.face[h] != -1 && (seam[size_t(m.edge[h])] || h == bgn)) c++;
+                return true;
+            });
+            first[size_t(v) + 1] = c;
+        }
+        for (int v = 0; v < nV; v++) first[size_t(v) + 1] += first[size_t(v)];
+        std::vector<int> cut2v(static_cast<size_t>(first[size_t(nV)]));
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
+        for (int v = 0; v < nV; v++) {
+            if (m.vert2half[v] < 0) continue;
+            const int bgn = start(v);
+            int id = first[size_t(v)] - 1;
             m.forVertCcw(bgn, [&](int h) {
                 if (m.face[h] == -1) return true;
-                if (seam[size_t(m.edge[h])] || h == bgn) cut2v.push_back(v);
+                if (seam[size_t(m.edge[h])] || h == bgn) cut2v[size_t(++id)] = v;
                 const int f = m.face[h];
-                for (int j = 0; j < 3; j++) if (m.idx(f, j) == v) cutF(f, j) = int(cut2v.size()) - 1;
+                for (int j = 0; j < 3; j++) if (m.idx(f, j) == v) cutF(f, j) = id;
                 return true;
             });
         }

@@ -99,6 +99,7 @@ namespace metriko {
         // computeSingular (face_rosy_field.h:100-107)
         void computeSingular() {
             singular = VecXi::Ones(mesh.nV);
+#pragma omp parallel for schedule(static, 4096) if (mesh.nV > 100000)
             for (int v = 0; v < mesh.nV; v++) {
                 if (mesh.isBV[v]) continue;
                 double sum = mesh.angleDefect[v] * rosyN;
@@ -114,6 +115,7 @@ namespace metriko {
         // computeConnection (face_rosy_field.h:109-117)
         void computeConnection() {
             connection = VecXc::Zero(mesh.nE);
+#pragma omp parallel for schedule(static, 4096) if (mesh.nE > 100000)
             for (int e = 0; e < mesh.nE; e++) {
                 if (mesh.edgeIsBoundary(e)) continue;
                 const Row3d v = mesh.hvec(mesh.edge2half[e]).normalized();
@@ -156,6 +158,7 @@ namespace metriko {
         // face_rosy_field.h:37-43: principal N-th root times the N rotations
         void roots() {
             field.resize(mesh.nF, rosyN);
+#pragma omp parallel for schedule(static, 4096) if (mesh.nF > 100000)
             for (int f = 0; f < mesh.nF; f++) {
                 const double th = std::arg(compressed[f]);
                 const complex base = std::polar(1.0, th / rosyN);
@@ -168,6 +171,8 @@ namespace metriko {
         // cMatching (face_rosy_field.h:169-202)
         void cMatching() {
             const int N = rosyN;
+            // every edge from its own two faces: the parallel loop gives the serial result
+#pragma omp parallel for schedule(static, 4096) if (mesh.nE > 100000)
             for (int e = 0; e < mesh.nE; e++) {
                 if (mesh.edgeIsBoundary(e)) continue;
                 const Row3d t = mesh.hvec(mesh.edge2half[e]).normalized();
@@ -192,6 +197,7 @@ namespace metriko {
         // pMatching (face_rosy_field.h:138-167)
         void pMatching() {
             const int N = rosyN;
+#pragma omp parallel for schedule(static, 4096) if (mesh.nE > 100000)
             for (int e = 0; e < mesh.nE; e++) {
                 if (mesh.edgeIsBoundary(e)) continue;
                 const int f0 = mesh.edge2face(e, 0), f1 = mesh.edge2face(e, 1);
@@ -316,20 +322,33 @@ namespace metriko {
         // igl::adjacency_list: sorted unique neighbours
         std::vector<int> aptr(size_t(m.nV) + 1, 0), adj;
         {
-            std::vector<std::vector<int>> A(size_t(m.nV));
+            // both ends of every side, bucketed per vertex, then each bucket sorted and
+            // deduplicated (in parallel: buckets are independent)
+            const int nV = m.nV;
+            std::vector<int> cnt(size_t(nV) + 1, 0);
+            for (int f = 0; f < m.nF; f++)
+                for (int j = 0; j < 3; j++) { cnt[size_t(m.idx(f, j)) + 1]++; cnt[size_t(m.idx(f, (j + 1) % 3)) + 1]++; }
+            for (int v = 0; v < nV; v++) cnt[size_t(v) + 1] += cnt[size_t(v)];
+            std::vector<int> all(static_cast<size_t>(cnt[size_t(nV)])), pos(cnt.begin(), cnt.end() - 1);
             for (int f = 0; f < m.nF; f++)
                 for (int j = 0; j < 3; j++) {
-                    const int s = m.idx(f, j), d = m.idx(f, (j + 1) % 3);
-                    A[s].push_back(d);
-                    A[d].push_back(s);
+                    const int a = m.idx(f, j), d = m.idx(f, (j + 1) % 3);
+                    all[size_t(pos[size_t(a)]++)] = d;
+                    all[size_t(pos[size_t(d)]++)] = a;
                 }
-            for (int v = 0; v < m.nV; v++) {
-                auto& a = A[v];
-                std::sort(a.begin(), a.end());
-                a.erase(std::unique(a.begin(), a.end()), a.end());
-                aptr[v + 1] = aptr[v] + int(a.size());
-                adj.insert(adj.end(), a.begin(), a.end());
+            std::vector<int> uniq(size_t(nV), 0);
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
+            for (int v = 0; v < nV; v++) {
+                auto b = all.begin() + cnt[size_t(v)], e = all.begin() + cnt[size_t(v) + 1];
+                std::sort(b, e);
+                uniq[size_t(v)] = int(std::unique(b, e) - b);
             }
+            for (int v = 0; v < nV; v++) aptr[size_t(v) + 1] = aptr[size_t(v)] + uniq[size_t(v)];
+            adj.resize(size_t(aptr[size_t(nV)]));
+#pragma omp parallel for schedule(static, 4096) if (nV > 100000)
+            for (int v = 0; v < nV; v++)
+                std::copy(all.begin() + cnt[size_t(v)], all.begin() + cnt[size_t(v)] + uniq[size_t(v)],
+                          adj.begin() + aptr[size_t(v)]);
         }
         std::vector<uint8_t> inCut(size_t(m.nV), 0);
         for (int k = 0; k < 3 * m.nF; k++) if (cuts[size_t(k)]) inCut[m.idx(k / 3, k % 3)] = 1;
@@ -427,6 +446,7 @@ namespace metriko {
             }
         }
         F->matching = VecXi::Constant(mesh.nE, -1);
+#pragma omp parallel for schedule(static, 4096) if (mesh.nE > 100000)
         for (int e = 0; e < mesh.nE; e++) {
             if (mesh.edgeIsBoundary(e)) continue;
             F->matching[e] = (turns[mesh.edge2face(e, 0)] - turns[mesh.edge2face(e, 1)] + field.matching[e] + 1000 * N) % N;
@@ -438,6 +458,7 @@ namespace metriko {
     inline MatXd compute_extrinsic_field(const Hmesh& mesh, const FaceRosyField& field, int rosyN) {
         (void)rosyN;
         MatXd ext(mesh.nF, 12);
+#pragma omp parallel for schedule(static, 4096) if (mesh.nF > 100000)
         for (int f = 0; f < mesh.nF; f++)
             for (int j = 0; j < 4; j++) {
                 const complex c = field.field(f, j);
