@@ -11,6 +11,9 @@
 #define METRIKO_PARAMETERIZATION_H
 #include <memory>
 #include <set>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include "metriko/core/hmesh/hmesh.h"
 #include "metriko/core/solver/matrix_ops.h"
 #include "iterative_rounding/iter_rounding.h"
@@ -143,60 +146,83 @@ namespace metriko {
         std::vector<PM> cpm(static_cast<size_t>(N));
         cpm[0] = ident(1);
         for (int i = 1; i < N; i++) cpm[i] = mul(unit, cpm[i - 1]);
-        std::vector<TripD> vT, cT;
         auto assign_block = [&](std::vector<TripD>& T, const PM& m, int r0, int c0) {
             for (int i = 0; i < N; i++) for (int j = 0; j < N; j++)
                 if (m[size_t(i * N + j)] != 0) T.emplace_back(r0 + i, c0 + j, double(m[size_t(i * N + j)]));
         };
-        int cur = 0;
-        std::vector<PM> permMats;
-        std::vector<int> permIdcs;
-        for (int v = 0; v < raw.nV; v++) {
-            permMats.assign(1, ident(1));
-            permIdcs.assign(1, v);
-            int iVcut = -1;
-            const int start = raw.isBV[v] ? find_first_bndr_he(v) : find_first_seam_he(v);
-            raw.forVertCcw(start, [&](int h) {
-                if (raw.face[h] == -1) return false;
-                const int f = raw.face[h];
-                int jVcut = -1;
-                for (int j = 0; j < 3; j++) if (raw.idx(f, j) == v) jVcut = cut.idx(f, j);
-                if (jVcut != iVcut) {
-                    iVcut = jVcut;
-                    for (size_t i = 0; i < permIdcs.size(); i++) assign_block(vT, permMats[i], N * iVcut, N * permIdcs[i]);
-                }
-                const int hn = raw.twin[raw.prev[h]];
-                if (raw.face[hn] != -1 && seam[size_t(raw.edge[hn])]) {
-                    const PM& p = cpm[size_t(he2matching[hn])];
-                    const int t = he2transidx[hn];
-                    if (t > 0) {
-                        for (auto& m : permMats) m = mul(p, m);
-                        permMats.push_back(ident(1));
-                        permIdcs.push_back(raw.nV + t - 1);
-                    } else {
-                        permMats.push_back(ident(-1));
-                        permIdcs.push_back(raw.nV - t - 1);
-                        for (auto& m : permMats) m = mul(p, m);
+        // vertices in contiguous ranges, one per thread; each range collects its own triplets
+        // and numbers its constraint rows from 0, then the ranges are offset and concatenated
+        // in vertex order: the serial loop's triplets, rows and numbering
+        int nth = 1;
+#ifdef _OPENMP
+        if (raw.nV > 100000) nth = std::max(1, omp_get_max_threads());
+#endif
+        std::vector<std::vector<TripD>> vTs(static_cast<size_t>(nth)), cTs(static_cast<size_t>(nth));
+        std::vector<int> curs(static_cast<size_t>(nth), 0);
+#pragma omp parallel for schedule(static, 1) num_threads(nth) if (nth > 1)
+        for (int t = 0; t < nth; t++) {
+            const int v0 = int(int64_t(raw.nV) * t / nth), v1 = int(int64_t(raw.nV) * (t + 1) / nth);
+            std::vector<TripD>& vTl = vTs[size_t(t)];
+            std::vector<TripD>& cTl = cTs[size_t(t)];
+            int cur = 0;
+            std::vector<PM> permMats;
+            std::vector<int> permIdcs;
+            for (int v = v0; v < v1; v++) {
+                permMats.assign(1, ident(1));
+                permIdcs.assign(1, v);
+                int iVcut = -1;
+                const int start = raw.isBV[v] ? find_first_bndr_he(v) : find_first_seam_he(v);
+                raw.forVertCcw(start, [&](int h) {
+                    if (raw.face[h] == -1) return false;
+                    const int f = raw.face[h];
+                    int jVcut = -1;
+                    for (int j = 0; j < 3; j++) if (raw.idx(f, j) == v) jVcut = cut.idx(f, j);
+                    if (jVcut != iVcut) {
+                        iVcut = jVcut;
+                        for (size_t i = 0; i < permIdcs.size(); i++) assign_block(vTl, permMats[i], N * iVcut, N * permIdcs[i]);
+                    }
+                    const int hn = raw.twin[raw.prev[h]];
+                    if (raw.face[hn] != -1 && seam[size_t(raw.edge[hn])]) {
+                        const PM& p = cpm[size_t(he2matching[hn])];
+                        const int tr = he2transidx[hn];
+                        if (tr > 0) {
+                            for (auto& m : permMats) m = mul(p, m);
+                            permMats.push_back(ident(1));
+                            permIdcs.push_back(raw.nV + tr - 1);
+                        } else {
+                            permMats.push_back(ident(-1));
+                            permIdcs.push_back(raw.nV - tr - 1);
+                            for (auto& m : permMats) m = mul(p, m);
+                        }
+                    }
+                    return true;
+                });
+                if (!raw.isBV[v]) {
+                    std::vector<int> idcs(permIdcs);
+                    std::sort(idcs.begin(), idcs.end());
+                    idcs.erase(std::unique(idcs.begin(), idcs.end()), idcs.end());
+                    std::vector<PM> mats(idcs.size(), PM(size_t(N * N), 0));
+                    bool any = false;
+                    for (size_t j = 0; j < idcs.size(); j++) {
+                        for (size_t k = 0; k < permIdcs.size(); k++)
+                            if (idcs[j] == permIdcs[k]) for (int q = 0; q < N * N; q++) mats[j][size_t(q)] += permMats[k][size_t(q)];
+                        if (idcs[j] == v) for (int i = 0; i < N; i++) mats[j][size_t(i * N + i)] -= 1;
+                        for (int q = 0; q < N * N; q++) if (mats[j][size_t(q)] != 0) any = true;
+                    }
+                    if (any) {
+                        for (size_t j = 0; j < mats.size(); j++) assign_block(cTl, mats[j], N * cur, N * idcs[j]);
+                        cur++;
                     }
                 }
-                return true;
-            });
-            if (!raw.isBV[v]) {
-                std::set<int> tmp(permIdcs.begin(), permIdcs.end());
-                std::vector<int> idcs(tmp.begin(), tmp.end());
-                std::vector<PM> mats(idcs.size(), PM(size_t(N * N), 0));
-                bool any = false;
-                for (size_t j = 0; j < idcs.size(); j++) {
-                    for (size_t k = 0; k < permIdcs.size(); k++)
-                        if (idcs[j] == permIdcs[k]) for (int q = 0; q < N * N; q++) mats[j][size_t(q)] += permMats[k][size_t(q)];
-                    if (idcs[j] == v) for (int i = 0; i < N; i++) mats[j][size_t(i * N + i)] -= 1;
-                    for (int q = 0; q < N * N; q++) if (mats[j][size_t(q)] != 0) any = true;
-                }
-                if (any) {
-                    for (size_t j = 0; j < mats.size(); j++) assign_block(cT, mats[j], N * cur, N * idcs[j]);
-                    cur++;
-                }
             }
+            curs[size_t(t)] = cur;
+        }
+        std::vector<TripD> vT, cT;
+        int cur = 0;
+        for (int t = 0; t < nth; t++) {
+            vT.insert(vT.end(), vTs[size_t(t)].begin(), vTs[size_t(t)].end());
+            for (const TripD& q : cTs[size_t(t)]) cT.emplace_back(q.r + N * cur, q.c, q.v);
+            cur += curs[size_t(t)];
         }
         vtrans2cut.resize(N * cut.nV, N * nR);
         vtrans2cut.setFromTriplets(vT);

@@ -337,32 +337,56 @@ Csr<T> tentative(const Csr<T>& A, const std::vector<double>& d, double theta, in
     nagg = G.nagg;
     std::vector<T> B(size_t(n), T(0));
     std::vector<uint8_t> seen(size_t(n), 0);
-    std::vector<int> q;
-    for (int a = 0; a < nagg; a++) {
-        const int r0 = root[a];
-        B[r0] = T(1);
-        seen[r0] = 1;
-        q.assign(1, r0);
-        for (size_t h = 0; h < q.size(); h++) {
-            const int u = q[h];
-            for (int64_t k = sptr[u]; k < sptr[u + 1]; k++) {
-                const int j = scol[k];
-                if (seen[j] || agg[j] != a) continue;
-                seen[j] = 1;
-                // complex (connection Laplacian): follow the transport phase; real (Laplacian
-                // of the uv coordinates): translations are the near-null space -> constants
-                if (std::is_same<T, double>::value && !tune().phase_real) B[j] = B[u];
-                else B[j] = -conjv(sval[k]) / mag(sval[k]) * B[u];
-                q.push_back(j);
+    const bool follow_phase = !(std::is_same<T, double>::value && !tune().phase_real);
+    // one BFS per aggregate from its root; a BFS stays inside its aggregate, so the
+    // aggregates run in parallel and every node is written by its own aggregate's BFS only
+#pragma omp parallel if (n > 100000)
+    {
+        std::vector<int> q;
+#pragma omp for schedule(dynamic, 256)
+        for (int a = 0; a < nagg; a++) {
+            const int r0 = root[a];
+            B[r0] = T(1);
+            seen[r0] = 1;
+            q.assign(1, r0);
+            for (size_t h = 0; h < q.size(); h++) {
+                const int u = q[h];
+                for (int64_t k = sptr[u]; k < sptr[u + 1]; k++) {
+                    const int j = scol[k];
+                    if (agg[j] != a || seen[j]) continue;
+                    seen[j] = 1;
+                    // complex (connection Laplacian): follow the transport phase; real (Laplacian
+                    // of the uv coordinates): translations are the near-null space -> constants
+                    if (!follow_phase) B[j] = B[u];
+                    else B[j] = -conjv(sval[k]) / mag(sval[k]) * B[u];
+                    q.push_back(j);
+                }
             }
         }
     }
-    for (int i = 0; i < n; i++) if (!seen[i]) B[i] = T(1);
+    // members of each aggregate in ascending order: the norms sum in node order
+    std::vector<int> mptr(size_t(nagg) + 1, 0), mem(static_cast<size_t>(n));
+    for (int i = 0; i < n; i++) mptr[size_t(agg[i]) + 1]++;
+    for (int a = 0; a < nagg; a++) mptr[size_t(a) + 1] += mptr[size_t(a)];
+    {
+        std::vector<int> fill(mptr.begin(), mptr.end() - 1);
+        for (int i = 0; i < n; i++) mem[size_t(fill[size_t(agg[i])]++)] = i;
+    }
     std::vector<double> nrm(size_t(nagg), 0.0);
-    for (int i = 0; i < n; i++) nrm[agg[i]] += mag2(B[i]);
+#pragma omp parallel for schedule(static, 1024) if (n > 100000)
+    for (int a = 0; a < nagg; a++) {
+        double s = 0.0;
+        for (int q = mptr[size_t(a)]; q < mptr[size_t(a) + 1]; q++) {
+            const int i = mem[size_t(q)];
+            if (!seen[size_t(i)]) B[size_t(i)] = T(1);
+            s += mag2(B[size_t(i)]);
+        }
+        nrm[size_t(a)] = s;
+    }
     Csr<T> P(n, nagg);
     P.col.resize(size_t(n));
     P.val.resize(size_t(n));
+#pragma omp parallel for schedule(static, 4096) if (n > 100000)
     for (int i = 0; i < n; i++) {
         P.col[i] = agg[i];
         P.val[i] = B[i] / std::sqrt(nrm[agg[i]]);
