@@ -257,23 +257,43 @@ namespace metriko {
         fullx2Nfunc = vtrans2cut * uncompress;
         SprsD Cfull = constraint * uncompress;
         reduce_to_linearly_independent(Cfull);
-        // G2: per cut face and corner, gradient of the hat function in the face frame
-        std::vector<TripD> iT;
-        iT.reserve(size_t(cut.nF) * 3 * N * 2);
-        for (int f = 0; f < cut.nF; f++) {
-            const Row3d nrm = row3(cut.faceNormal, f), bx = row3(cut.faceBasisX, f), by = row3(cut.faceBasisY, f);
-            int h = cut.face2half[f];
-            for (int k3 = 0; k3 < 3; k3++, h = cut.next[h]) {
-                const Row3d g = nrm.cross(cut.hvec(cut.next[h])) / (cut.faceArea[f] * 2);
+        // G2: per cut face and corner, gradient of the hat function in the face frame.  Rows
+        // 2N f + 2k + {0, 1} hold one entry per corner of face f (column N tail + k); written
+        // straight into CSR, each row's three entries in ascending column order (what the
+        // triplet assembly gives: a face's corners are distinct cut vertices), faces in parallel
+        SprsD G2(2 * N * cut.nF, N * cut.nV);
+        {
+            const int nrows = 2 * N * cut.nF;
+#pragma omp parallel for schedule(static, 16384) if (nrows > 100000)
+            for (int r = 0; r <= nrows; r++) G2.ptr[size_t(r)] = 3 * int64_t(r);
+            G2.col.resize(size_t(3) * size_t(nrows));
+            G2.val.resize(size_t(3) * size_t(nrows));
+#pragma omp parallel for schedule(static, 4096) if (cut.nF > 100000)
+            for (int f = 0; f < cut.nF; f++) {
+                const Row3d nrm = row3(cut.faceNormal, f), bx = row3(cut.faceBasisX, f), by = row3(cut.faceBasisY, f);
+                int h = cut.face2half[f];
+                int tl[3];
+                double gx[3], gy[3];
+                for (int k3 = 0; k3 < 3; k3++, h = cut.next[h]) {
+                    const Row3d g = nrm.cross(cut.hvec(cut.next[h])) / (cut.faceArea[f] * 2);
+                    tl[k3] = cut.tail[h];
+                    gx[k3] = g.dot(bx);
+                    gy[k3] = g.dot(by);
+                }
+                int o[3] = {0, 1, 2};  // corners by ascending cut vertex
+                std::stable_sort(o, o + 3, [&](int a, int b) { return tl[a] < tl[b]; });
                 for (int k = 0; k < N; k++) {
-                    const int c = N * cut.tail[h] + k, ir = 2 * N * f + k * 2;
-                    iT.emplace_back(ir + 0, c, g.dot(bx));
-                    iT.emplace_back(ir + 1, c, g.dot(by));
+                    const int64_t r0 = 2 * int64_t(N) * f + 2 * k;
+                    for (int q = 0; q < 3; q++) {
+                        const int c = N * tl[o[q]] + k;
+                        G2.col[size_t(3 * r0 + q)] = c;
+                        G2.val[size_t(3 * r0 + q)] = gx[o[q]];
+                        G2.col[size_t(3 * (r0 + 1) + q)] = c;
+                        G2.val[size_t(3 * (r0 + 1) + q)] = gy[o[q]];
+                    }
                 }
             }
         }
-        SprsD G2(2 * N * cut.nF, N * cut.nV);
-        G2.setFromTriplets(iT);
         // intrinsic field (integ.ipp:30-43)
         double norm = 0.0;
         for (int i = 0; i < cut.nF; i++)
@@ -284,6 +304,7 @@ namespace metriko {
             }
         norm /= double(N) * cut.nF;
         VecXd F2(2 * N * cut.nF);
+#pragma omp parallel for schedule(static, 4096) if (cut.nF > 100000)
         for (int f = 0; f < cut.nF; f++)
             for (int i = 0; i < N; i++) {
                 double a = 0, b = 0;
@@ -321,7 +342,9 @@ namespace metriko {
         nfn.resize(cut.nV, N);
         cfn.resize(raw.nF, N * 3);
         VecXd NF = fullx2Nfunc * fullx;
+#pragma omp parallel for schedule(static, 4096) if (cut.nV > 100000)
         for (int i = 0; i < cut.nV; i++) for (int k = 0; k < N; k++) nfn(i, k) = NF[N * i + k];
+#pragma omp parallel for schedule(static, 4096) if (raw.nF > 100000)
         for (int i = 0; i < raw.nF; i++)
             for (int j = 0; j < 3; j++)
                 for (int k = 0; k < N; k++) cfn(i, N * j + k) = nfn(cut.idx(i, j), k);

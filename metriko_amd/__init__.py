@@ -750,8 +750,7 @@ class Context:
         """example_igm in one call (mk_pipeline_run); returns outputs and per-stage info."""
         V, F = _mesh_arrays(V, F)
         nV, nF = V.shape[0], F.shape[0]
-        nE = nV + nF + 64  # upper bound placeholder; exact nE comes back in info
-        nE = hmesh_export(V, F)["nE"]
+        nE = 3 * nF  # at least the edge count (every edge has a side); the exact count comes back in info
         comp = np.zeros(2 * nF)
         sing = np.zeros(nV, np.int32)
         mat = np.zeros(nE, np.int32)
@@ -774,8 +773,9 @@ class Context:
                                            _ptr(cut, _I32), _ptr(uv, _D), cap, _ptr(fullx, _D), dcap, _ptr(di, _I32),
                                            _ptr(dv, _I32), C.byref(info)), "mk_pipeline_run")
         inf = struct_dict(info)
-        return {"compressed": comp[0::2] + 1j * comp[1::2], "singular": sing, "matching": mat,
-                "seam": seam.astype(bool), "cut_idx": cut, "uv": uv, "fullx": fullx[:inf["nfullx"]],
+        ne = inf["nE"]
+        return {"compressed": comp[0::2] + 1j * comp[1::2], "singular": sing, "matching": mat[:ne].copy(),
+                "seam": seam[:ne].astype(bool), "cut_idx": cut, "uv": uv, "fullx": fullx[:inf["nfullx"]],
                 "decisions": list(zip(di[:inf["ndec"]].tolist(), dv[:inf["ndec"]].tolist())), "info": inf}
 
 
