@@ -152,27 +152,32 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
         const int64_t base = A.slice_ptr[s] + lr;
         const int w = A.slice_w[s];
         row = A.perm ? A.perm[r] : r;
-        int k = 0;
-        for (; k + 1 < w; k += 2) {  // two independent gathers in flight
-            const int64_t e0 = base + int64_t(k) * A.C, e1 = e0 + A.C;
-            const int c0 = A.col[e0], c1 = A.col[e1];
-            const T v0 = A.val[e0], v1 = A.val[e1];
-            const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
-            vfma(acc, v0, x0);
-            vfma(acc, v1, x1);
-            if constexpr (MODE >= 5) {
-                if (c0 == row) { xd = x0; have_xd = true; }
-                if (c1 == row) { xd = x1; have_xd = true; }
+        // the row's column indices and values first (w <= kSellMaxLen, uniform in the
+        // wavefront), then its gathers kG at a time in flight, summed in entry order
+        int ci[kSellMaxLen];
+        T cv[kSellMaxLen];
+#pragma unroll
+        for (int k = 0; k < kSellMaxLen; k++)
+            if (k < w) {
+                ci[k] = A.col[base + int64_t(k) * A.C];
+                cv[k] = A.val[base + int64_t(k) * A.C];
             }
-        }
-        if (k < w) {
-            const int64_t e = base + int64_t(k) * A.C;
-            const int c0 = A.col[e];
-            const V x0 = Xv[size_t(c0) * LPR + j];
-            vfma(acc, A.val[e], x0);
-            if constexpr (MODE >= 5) {
-                if (c0 == row) { xd = x0; have_xd = true; }
-            }
+        constexpr int kG = 8;
+#pragma unroll
+        for (int k0 = 0; k0 < kSellMaxLen; k0 += kG) {
+            if (k0 >= w) break;
+            V xg[kG];
+#pragma unroll
+            for (int u = 0; u < kG; u++)
+                if (k0 + u < kSellMaxLen && k0 + u < w) xg[u] = Xv[size_t(ci[k0 + u]) * LPR + j];
+#pragma unroll
+            for (int u = 0; u < kG; u++)
+                if (k0 + u < kSellMaxLen && k0 + u < w) {
+                    vfma(acc, cv[k0 + u], xg[u]);
+                    if constexpr (MODE >= 5) {
+                        if (ci[k0 + u] == row) { xd = xg[u]; have_xd = true; }
+                    }
+                }
         }
     } else {
         if constexpr (PART == 1) return;
