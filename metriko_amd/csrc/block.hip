@@ -608,12 +608,25 @@ static BlockResult block_pcg_mw(mk_ctx* c, const mk_op* A, Mg* mg, const double*
     MK_HIP(hipGetLastError());
     BState h{};
     int iters = 0;
+    // the batch ends where the slowest running column is predicted to converge
+    const int interval = c->time_spmv ? 1 : std::max(1, check_every);
+    int prev_iters = -1;
+    double prev_rr[MW] = {};
     for (;;) {
         MK_HIP(hipMemcpyAsync(&h, st.p, sizeof(BState), hipMemcpyDeviceToHost, c->stream));
         MK_HIP(hipStreamSynchronize(c->stream));
         timing_flush(c);
         if (h.all_done || iters >= maxiter) break;
-        const int batch = std::min(check_every, maxiter - iters);
+        int plan = 0;
+        for (int j = 0; j < MW && plan < interval; j++) {
+            if (h.done[j]) continue;
+            const double nd = BatchPlanner::need(iters, h.rr[j], prev_iters, prev_rr[j], h.tol2 * h.bb[j]);
+            plan = std::max(plan, BatchPlanner::clamp_need(nd, interval));
+        }
+        if (plan == 0) plan = interval;
+        for (int j = 0; j < MW; j++) prev_rr[j] = h.rr[j];
+        prev_iters = iters;
+        const int batch = std::min(plan, maxiter - iters);
         for (int k = 0; k < batch; k++) {
             spmm_launch<double>(c, A, MW, P.p, Q.p, nullptr, SPMV_PLAIN, &st.p->all_done);
             hipLaunchKernelGGL((bk_alpha<MW>), dim3(g), dim3(256), 0, c->stream, len, (const double*)P.p,

@@ -128,6 +128,19 @@ static inline void finish(mk_ctx* c, unsigned nparts, double* out = nullptr, con
                        int(nparts), out, skip);
 }
 
+// One vector's element `row` (complex: two entries), loaded by spmv_row_gp's early hook.
+template <typename T, bool CPLX>
+struct RowElem {
+    static constexpr int W = CPLX ? 2 : 1;
+    const T* __restrict__ src;   // nullptr: nothing to load
+    T v[W];
+    __device__ __forceinline__ void operator()(int row) {
+        if (!src) return;
+#pragma unroll
+        for (int q = 0; q < W; q++) v[q] = src[W * row + q];
+    }
+};
+
 template <typename T, bool CPLX, int MODE>
 __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restrict__ x, T* __restrict__ y,
                                                const T* __restrict__ b, double* partials, const int* skip) {
@@ -137,20 +150,22 @@ __global__ void __launch_bounds__(256) sell_spmv(SellDev<T> A, const T* __restri
     for (int vb = blockIdx.x; vb < A.nvb; vb += gridDim.x) {
         int row = 0;
         T ar, ai;
-        if (!spmv_row<T, CPLX>(A, x, vb, row, ar, ai)) continue;
+        // the row's own element of b (modes 2, 3), y (4) or x (1), fetched early
+        RowElem<T, CPLX> re{MODE == 2 || MODE == 3 ? b : MODE == 4 ? y : MODE == 1 ? x : nullptr};
+        if (!spmv_row_gp<T, CPLX>(A, PlainGather<T, CPLX>{x}, vb, row, ar, ai, re)) continue;
         if constexpr (CPLX) {
-            if constexpr (MODE == 2 || MODE == 3) { ar = b[2 * row] - ar; ai = b[2 * row + 1] - ai; }
-            if constexpr (MODE == 4) { ar += y[2 * row]; ai += y[2 * row + 1]; }
+            if constexpr (MODE == 2 || MODE == 3) { ar = re.v[0] - ar; ai = re.v[1] - ai; }
+            if constexpr (MODE == 4) { ar += re.v[0]; ai += re.v[1]; }
             y[2 * row] = ar;
             y[2 * row + 1] = ai;
-            if constexpr (MODE == 1) contrib += double(x[2 * row]) * double(ar) + double(x[2 * row + 1]) * double(ai);
+            if constexpr (MODE == 1) contrib += double(re.v[0]) * double(ar) + double(re.v[1]) * double(ai);
             if constexpr (MODE == 2) contrib += double(ar) * double(ar) + double(ai) * double(ai);
         } else {
             T a = ar;
-            if constexpr (MODE == 2 || MODE == 3) a = b[row] - a;
-            if constexpr (MODE == 4) a += y[row];
+            if constexpr (MODE == 2 || MODE == 3) a = re.v[0] - a;
+            if constexpr (MODE == 4) a += re.v[0];
             y[row] = a;
-            if constexpr (MODE == 1) contrib += double(x[row]) * double(a);
+            if constexpr (MODE == 1) contrib += double(re.v[0]) * double(a);
             if constexpr (MODE == 2) contrib += double(a) * double(a);
         }
     }
@@ -806,6 +821,27 @@ void spmv_launch(mk_ctx* c, const mk_op* A, const T* val, const T* x, T* y, cons
 }
 // One Chebyshev step fused with the residual it needs, row-local:
 //   r = b - A x_in ; d = a d + s D^-1 r (first step: d = s D^-1 r) ; x_out = x_in + d.
+// The row-local operands of a Chebyshev step (element row of b, x_in, d and the inverse
+// diagonal), loaded while the row's gathers are in flight.
+template <typename T, bool CPLX>
+struct RowOperands {
+    static constexpr int W = CPLX ? 2 : 1;
+    const T* __restrict__ b;
+    const T* __restrict__ x;
+    const T* __restrict__ d;   // nullptr: not needed
+    const T* __restrict__ dinv;
+    T bv[W], xv[W], dv[W], di;
+    __device__ __forceinline__ void operator()(int row) {
+        di = dinv[row];
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+            bv[q] = b[W * row + q];
+            if (x) xv[q] = x[W * row + q];
+            dv[q] = d ? d[W * row + q] : T(0);
+        }
+    }
+};
+
 template <typename T, bool CPLX>
 __global__ void __launch_bounds__(256) sell_cheb(SellDev<T> A, const T* __restrict__ x_in, T* __restrict__ x_out,
                                                const T* __restrict__ b, T* __restrict__ d,
@@ -813,20 +849,21 @@ __global__ void __launch_bounds__(256) sell_cheb(SellDev<T> A, const T* __restri
     if (skip && *skip) return;
     int row = 0;
     T ar, ai;
-    if (!spmv_row<T, CPLX>(A, x_in, blockIdx.x, row, ar, ai)) return;
-    const T sd = s * dinv[row];
+    RowOperands<T, CPLX> ro{b, x_in, first ? nullptr : d, dinv};
+    if (!spmv_row_gp<T, CPLX>(A, PlainGather<T, CPLX>{x_in}, blockIdx.x, row, ar, ai, ro)) return;
+    const T sd = s * ro.di;
     if constexpr (CPLX) {
-        T d0 = sd * (b[2 * row] - ar), d1 = sd * (b[2 * row + 1] - ai);
-        if (!first) { d0 += a * d[2 * row]; d1 += a * d[2 * row + 1]; }
+        T d0 = sd * (ro.bv[0] - ar), d1 = sd * (ro.bv[1] - ai);
+        if (!first) { d0 += a * ro.dv[0]; d1 += a * ro.dv[1]; }
         d[2 * row] = d0;
         d[2 * row + 1] = d1;
-        x_out[2 * row] = x_in[2 * row] + d0;
-        x_out[2 * row + 1] = x_in[2 * row + 1] + d1;
+        x_out[2 * row] = ro.xv[0] + d0;
+        x_out[2 * row + 1] = ro.xv[1] + d1;
     } else {
-        T d0 = sd * (b[row] - ar);
-        if (!first) d0 += a * d[row];
+        T d0 = sd * (ro.bv[0] - ar);
+        if (!first) d0 += a * ro.dv[0];
         d[row] = d0;
-        x_out[row] = x_in[row] + d0;
+        x_out[row] = ro.xv[0] + d0;
     }
 }
 
@@ -877,18 +914,19 @@ __global__ void __launch_bounds__(256) sell_cheb0(SellDev<T> A, const T* __restr
     if (skip && *skip) return;
     int row = 0;
     T ar, ai;
-    if (!spmv_row_g<T, CPLX>(A, ScaledGather<T, CPLX>{b, dinv, c0}, blockIdx.x, row, ar, ai)) return;
-    const T di = dinv[row], f = c0 * di, sd = s * di;
+    RowOperands<T, CPLX> ro{b, nullptr, nullptr, dinv};
+    if (!spmv_row_gp<T, CPLX>(A, ScaledGather<T, CPLX>{b, dinv, c0}, blockIdx.x, row, ar, ai, ro)) return;
+    const T di = ro.di, f = c0 * di, sd = s * di;
     if constexpr (CPLX) {
-        const T x0 = f * b[2 * row], x1 = f * b[2 * row + 1];
-        const T d0 = a * x0 + sd * (b[2 * row] - ar), d1 = a * x1 + sd * (b[2 * row + 1] - ai);
+        const T x0 = f * ro.bv[0], x1 = f * ro.bv[1];
+        const T d0 = a * x0 + sd * (ro.bv[0] - ar), d1 = a * x1 + sd * (ro.bv[1] - ai);
         d[2 * row] = d0;
         d[2 * row + 1] = d1;
         x_out[2 * row] = x0 + d0;
         x_out[2 * row + 1] = x1 + d1;
     } else {
-        const T x0 = f * b[row];
-        const T d0 = a * x0 + sd * (b[row] - ar);
+        const T x0 = f * ro.bv[0];
+        const T d0 = a * x0 + sd * (ro.bv[0] - ar);
         d[row] = d0;
         x_out[row] = x0 + d0;
     }
@@ -1108,6 +1146,15 @@ static void run_batch(mk_ctx* c, const mk_op* A_, const Mg* mg, std::vector<uint
     c->graph_replays++;
 }
 
+// m iterations as captured batches: the check interval itself, or power-of-two pieces (a few
+// graph sizes per operator, each captured once and replayed by every later solve)
+template <typename F>
+static void run_batches(int m, int interval, F&& batch) {
+    if (m == interval) { batch(m); return; }
+    for (int p = 1 << 30; m > 0; p >>= 1)
+        if (p <= m) { batch(p); m -= p; }
+}
+
 // one PCG iteration with an external (multigrid) preconditioner
 template <typename T>
 static void cg_iter_mg(mk_ctx* c, const mk_op* A, const T* val, Mg* mg, T* x, T* r, T* z, T* p, T* q, double bytes) {
@@ -1163,15 +1210,18 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
     MK_HIP(hipGetLastError());
     if (mg) {
         CgState h{};
+        BatchPlanner plan(c->time_spmv ? 1 : check_every);
         for (;;) {
             MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
             MK_HIP(hipStreamSynchronize(c->stream));
             timing_flush(c);
             if (h.done || h.it >= maxiter) break;
-            const int batch = std::min(check_every, maxiter - h.it);
-            run_batch(c, A, mg, {uintptr_t(sizeof(T)), uintptr_t(val), uintptr_t(x), uintptr_t(r),
-                                 uintptr_t(z), uintptr_t(p), uintptr_t(q)},
-                      batch, batch == check_every, [&] { cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes); });
+            const int batch = std::min(plan.next(h.it, h.rr, h.tol2 * h.bb), maxiter - h.it);
+            run_batches(batch, check_every, [&](int b) {
+                run_batch(c, A, mg, {uintptr_t(sizeof(T)), uintptr_t(val), uintptr_t(x), uintptr_t(r),
+                                     uintptr_t(z), uintptr_t(p), uintptr_t(q)},
+                          b, true, [&] { cg_iter_mg<T>(c, A, val, mg, x, r, z, p, q, bytes); });
+            });
             MK_HIP(hipGetLastError());
         }
         CgResult res{h.status, h.it, h.rr, h.bb};
@@ -1180,12 +1230,13 @@ static CgResult cg_run(mk_ctx* c, const mk_op* A, const T* val, const T* dinv, T
     }
     CgState h{};
     int launched = 0;
+    BatchPlanner plan(c->time_spmv ? 1 : check_every);
     for (;;) {
         MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         MK_HIP(hipStreamSynchronize(c->stream));
         timing_flush(c);
         if (h.done || h.it >= maxiter) break;
-        const int batch = std::min(check_every, maxiter - h.it);
+        const int batch = std::min(plan.next(h.it, h.rr, h.tol2 * h.bb), maxiter - h.it);
         for (int b = 0; b < batch; b++) {
             launch_cg_spmv<T>(c, A, val, p, q, bytes);
             if (A->cplx)
@@ -1280,16 +1331,19 @@ CgResult cg64_mg32(mk_ctx* c, const mk_op* A, const double* b, double* x, double
         mg_apply32(c, mg, w.r32.p, w.z32.p, &st->done);
     };
     CgState h{};
+    BatchPlanner plan(c->time_spmv ? 1 : check_every);
     for (;;) {
         MK_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, c->stream));
         MK_HIP(hipStreamSynchronize(c->stream));
         timing_flush(c);
         if (h.done || h.it >= maxiter) break;
-        const int batch = std::min(check_every, maxiter - h.it);
-        run_batch(c, A, mg, {uintptr_t(4), uintptr_t(A->val64.p), uintptr_t(x), uintptr_t(w.r.p),
-                             uintptr_t(w.r32.p), uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p),
-                             uintptr_t(w.q.p)},
-                  batch, batch == check_every, iteration);
+        const int batch = std::min(plan.next(h.it, h.rr, h.tol2 * h.bb), maxiter - h.it);
+        run_batches(batch, check_every, [&](int b) {
+            run_batch(c, A, mg, {uintptr_t(4), uintptr_t(A->val64.p), uintptr_t(x), uintptr_t(w.r.p),
+                                 uintptr_t(w.r32.p), uintptr_t(w.z32.p), uintptr_t(w.z.p), uintptr_t(w.p.p),
+                                 uintptr_t(w.q.p)},
+                      b, true, iteration);
+        });
         MK_HIP(hipGetLastError());
     }
     CgResult res{h.status, h.it, h.rr, h.bb};

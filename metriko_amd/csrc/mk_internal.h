@@ -1,6 +1,8 @@
 // Internal declarations of the metriko HIP library (not part of the C ABI).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdint>
 #include <stdexcept>
@@ -87,6 +89,38 @@ struct CgState {
     int status;     // MK_OK / MK_ERR_NOT_SPD / MK_ERR_NAN
     int it;         // iterations taken
     int first;      // 1 until the first beta of the fused FP32-multigrid iteration
+};
+
+// Iterations to launch before the next host convergence check.  Iterations launched past
+// convergence return at once, but each still costs its few dozen launches (about a third of
+// a multigrid PCG iteration), so after the first check interval the count is predicted: the
+// residual's geometric decrease per iteration since the previous check, extrapolated to the
+// target (rounded up), within [1, interval].  An iteration stops where it would stop with
+// any batching, so results do not depend on the prediction.  The solvers use interval 1 in
+// kernel-timing mode, so the per-kernel averages hold no launch past convergence.
+struct BatchPlanner {
+    int interval;
+    int prev_it = -1;
+    double prev_rr = 0.0;
+    explicit BatchPlanner(int iv) : interval(iv > 0 ? iv : 1) {}
+    // iterations needed from rr to reach target at the rate seen since (prev_it, prev_rr); 0 =
+    // no prediction
+    static double need(int it, double rr, int pit, double prr, double target) {
+        if (pit < 0 || it <= pit || !(rr > target) || !(prr > 0.0) || !(target > 0.0)) return 0.0;
+        const double rate = std::log(rr / prr) / double(it - pit);
+        if (!(rate < -1e-3)) return 0.0;
+        return std::log(target / rr) / rate;
+    }
+    static int clamp_need(double nd, int interval) {
+        if (!(nd > 0.0) || nd >= double(interval)) return interval;
+        return std::max(1, int(std::ceil(nd)));
+    }
+    int next(int it, double rr, double target) {
+        const int b = clamp_need(need(it, rr, prev_it, prev_rr, target), interval);
+        prev_it = it;
+        prev_rr = rr;
+        return b;
+    }
 };
 
 // per-context scratch vectors

@@ -132,10 +132,18 @@ struct PlainGather {
     }
 };
 
+// No row-local operands to fetch early.
+struct NoPrefetch {
+    __device__ __forceinline__ void operator()(int) {}
+};
+
 // Computes one row of A x for virtual block vb (uniform in the block), x given by a gather.
-// Returns true in the thread that owns the result (row, ar, ai).
-template <typename T, bool CPLX, typename G>
-__device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int vb, int& row, T& ar, T& ai) {
+// Returns true in the thread that owns the result (row, ar, ai).  pre(row) is called as soon
+// as the row is known -- for the SELL rows before their column loads -- so that the caller's
+// row-local operands (right-hand side, diagonal, vectors updated in place) are in flight
+// together with the gathers instead of after them.
+template <typename T, bool CPLX, typename G, typename P>
+__device__ __forceinline__ bool spmv_row_gp(const SellDev<T>& A, const G& g, int vb, int& row, T& ar, T& ai, P& pre) {
     ar = 0;
     ai = 0;
     if (vb < A.nb_sell) {
@@ -145,6 +153,7 @@ __device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int 
         const int64_t base = A.slice_ptr[s] + lane;
         const int w = A.slice_w[s];
         row = A.perm ? A.perm[r] : r;
+        pre(row);
         // every column index first (w <= kSellMaxLen), then the gathers: two memory latencies
         // per row instead of two per entry
         int ci[kSellMaxLen];
@@ -193,6 +202,7 @@ __device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int 
         }
         if (m >= A.nmid || sub != 0) return false;
         row = A.mid_rows[m];
+        pre(row);
         return true;
     }
     const int w = (vb - A.nb_sell - A.nb_mid) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
@@ -213,9 +223,15 @@ __device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int 
         ar = wave_sum_t(ar);
         if constexpr (CPLX) ai = wave_sum_t(ai);
         row = A.long_rows[w];
+        if (lane == 0) pre(row);
         return lane == 0;
     }
     return false;
+}
+template <typename T, bool CPLX, typename G>
+__device__ __forceinline__ bool spmv_row_g(const SellDev<T>& A, const G& g, int vb, int& row, T& ar, T& ai) {
+    NoPrefetch none;
+    return spmv_row_gp<T, CPLX>(A, g, vb, row, ar, ai, none);
 }
 
 
