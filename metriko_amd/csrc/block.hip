@@ -62,6 +62,10 @@ __device__ __forceinline__ float4 vaxpy(float a, const float4& x, float b, const
 __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, const double2& y) {
     return make_double2(a * x.x + b * y.x, a * x.y + b * y.y);
 }
+__device__ __forceinline__ float4 vscale(float a, const float4& x) {
+    return make_float4(a * x.x, a * x.y, a * x.z, a * x.w);
+}
+__device__ __forceinline__ double2 vscale(double a, const double2& x) { return make_double2(a * x.x, a * x.y); }
 
 // Y = A X (MODE 0), Y = B - A X (3), Y += A X (4) for MW right-hand sides.  MODES 5-7: one
 // Chebyshev smoothing step fused with the residual it needs, in the three-term form (the
@@ -69,6 +73,8 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
 // Y != X:  r = B - A X;  5: Y = X + cs D^-1 r (first step from a given x);
 // 6: Y = X + ca (X - Y) + cs D^-1 r (Y holds x_{k-1} on entry and x_{k+1} on exit);
 // 7: Y = (1 + ca) X + cs D^-1 r (the step after x_1 from a zero start, x_0 = 0).
+// 8: mode 7 with x_1 = c0 D^-1 B formed inside the gathers (X unused, x_1 never stored).
+// 9: mode 6 with x_{k-1} = x_1 = c0 D^-1 B formed from the row's own B (Y is not read).
 // A lane carries one 16-byte vector of columns (LPR = MW / N lanes per row), a wavefront
 // covers 64 / LPR rows of the SELL part at a time; medium and long rows take a whole
 // wavefront whose row groups split the entries.  PART 1 / 2: only the SELL rows / only the
@@ -78,7 +84,7 @@ __device__ __forceinline__ double2 vaxpy(double a, const double2& x, double b, c
 // Y + acc (4), the Chebyshev updates (5-7, x_k = xd when the row's own entry was gathered).
 template <typename T, int MW, int MODE, typename V>
 __device__ __forceinline__ void spmm_store(const V& acc, int row, int j, const V* __restrict__ Xv, T* __restrict__ Y,
-                                           const T* __restrict__ B, const T* __restrict__ dinv, T ca, T cs,
+                                           const T* __restrict__ B, const T* __restrict__ dinv, T ca, T cs, T c0,
                                            bool have_xd, const V& xd) {
     constexpr int LPR = MW / Vec16<T>::N;
     V* Yv = reinterpret_cast<V*>(Y);
@@ -87,30 +93,43 @@ __device__ __forceinline__ void spmm_store(const V& acc, int row, int j, const V
     if constexpr (MODE == 3) Yv[o] = vsub(reinterpret_cast<const V*>(B)[o], acc);
     if constexpr (MODE == 4) Yv[o] = vadd(Yv[o], acc);
     if constexpr (MODE >= 5) {
-        const V xo = have_xd ? xd : Xv[o];
-        const V r = vsub(reinterpret_cast<const V*>(B)[o], acc);
+        const V bo = reinterpret_cast<const V*>(B)[o];
+        const V xo = have_xd ? xd : MODE == 8 ? vscale(c0 * dinv[row], bo) : Xv[o];
+        const V r = vsub(bo, acc);
         const T sd = cs * dinv[row];
         if constexpr (MODE == 5) Yv[o] = vaxpy(T(1), xo, sd, r);
-        if constexpr (MODE == 7) Yv[o] = vaxpy(T(1) + ca, xo, sd, r);
+        if constexpr (MODE == 7 || MODE == 8) Yv[o] = vaxpy(T(1) + ca, xo, sd, r);
         if constexpr (MODE == 6) Yv[o] = vaxpy(T(1), vaxpy(T(1) + ca, xo, -ca, Yv[o]), sd, r);
+        if constexpr (MODE == 9)
+            Yv[o] = vaxpy(T(1), vaxpy(T(1) + ca, xo, -ca, vscale(c0 * dinv[row], bo)), sd, r);
     }
+}
+
+// Gathered row c of the iterate: X's (lane vector j), or in mode 8 x_1 = c0 D^-1 B formed on
+// the fly (the same product the separate first step would have stored).
+template <typename T, int MODE, int LPR, typename V>
+__device__ __forceinline__ V gather_row(const V* __restrict__ Gv, const T* __restrict__ dinv, T c0, int c, int j) {
+    const V g = Gv[size_t(c) * LPR + j];
+    if constexpr (MODE == 8) return vscale(c0 * dinv[c], g);
+    else return g;
 }
 
 // Sum over entries [k0, k1) of a CSR row (stride R between a lane group's entries, starting
 // at k0 + sub): eight independent gathers in flight per lane.
-template <typename T, int R, int LPR, typename V>
+template <typename T, int R, int LPR, int MODE, typename V>
 __device__ __forceinline__ V chain_sum(int64_t k0, int64_t k1, int sub, int j, const int* __restrict__ cols,
-                                       const T* __restrict__ vals, const V* __restrict__ Xv) {
+                                       const T* __restrict__ vals, const V* __restrict__ Xv,
+                                       const T* __restrict__ dinv, T s0) {
     V acc, a1, a2, a3, a4, a5, a6, a7;
     vzero(acc); vzero(a1); vzero(a2); vzero(a3); vzero(a4); vzero(a5); vzero(a6); vzero(a7);
     int64_t k = k0 + sub;
     for (; k + 7 * R < k1; k += 8 * R) {
         const int c0 = cols[k], c1 = cols[k + R], c2 = cols[k + 2 * R], c3 = cols[k + 3 * R];
         const int c4 = cols[k + 4 * R], c5 = cols[k + 5 * R], c6 = cols[k + 6 * R], c7 = cols[k + 7 * R];
-        const V x0 = Xv[size_t(c0) * LPR + j], x1 = Xv[size_t(c1) * LPR + j];
-        const V x2 = Xv[size_t(c2) * LPR + j], x3 = Xv[size_t(c3) * LPR + j];
-        const V x4 = Xv[size_t(c4) * LPR + j], x5 = Xv[size_t(c5) * LPR + j];
-        const V x6 = Xv[size_t(c6) * LPR + j], x7 = Xv[size_t(c7) * LPR + j];
+        const V x0 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c0, j), x1 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c1, j);
+        const V x2 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c2, j), x3 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c3, j);
+        const V x4 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c4, j), x5 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c5, j);
+        const V x6 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c6, j), x7 = gather_row<T, MODE, LPR>(Xv, dinv, s0, c7, j);
         vfma(acc, vals[k], x0);
         vfma(a1, vals[k + R], x1);
         vfma(a2, vals[k + 2 * R], x2);
@@ -120,20 +139,22 @@ __device__ __forceinline__ V chain_sum(int64_t k0, int64_t k1, int sub, int j, c
         vfma(a6, vals[k + 6 * R], x6);
         vfma(a7, vals[k + 7 * R], x7);
     }
-    for (; k < k1; k += R) vfma(acc, vals[k], Xv[size_t(cols[k]) * LPR + j]);
+    for (; k < k1; k += R) vfma(acc, vals[k], gather_row<T, MODE, LPR>(Xv, dinv, s0, cols[k], j));
     return vadd(vadd(vadd(acc, a1), vadd(a2, a3)), vadd(vadd(a4, a5), vadd(a6, a7)));
 }
 
 template <typename T, int MW, int MODE, int PART = 0>
 __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, const T* __restrict__ X,
                                                T* __restrict__ Y, const T* __restrict__ B, const int* skip,
-                                               const T* __restrict__ dinv = nullptr, T ca = T(0), T cs = T(0)) {
+                                               const T* __restrict__ dinv = nullptr, T ca = T(0), T cs = T(0),
+                                               T c0 = T(0)) {
     if (skip && *skip) return;
     using V = typename Vec16<T>::type;
     constexpr int N = Vec16<T>::N;
     constexpr int LPR = MW / N;   // lanes per row
     constexpr int R = 64 / LPR;   // rows per wavefront
-    const V* __restrict__ Xv = reinterpret_cast<const V*>(X);
+    // the gathered block: X, or B in mode 8 (scaled by c0 D^-1 as it is gathered)
+    const V* __restrict__ Xv = reinterpret_cast<const V*>(MODE == 8 ? B : X);
     const int lane = threadIdx.x & 63;
     const int sub = lane / LPR, j = lane - sub * LPR;
     const int gw = xcd_block(int(blockIdx.x), int(gridDim.x)) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
@@ -169,7 +190,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
             V xg[kG];
 #pragma unroll
             for (int u = 0; u < kG; u++)
-                if (k0 + u < kSellMaxLen && k0 + u < w) xg[u] = Xv[size_t(ci[k0 + u]) * LPR + j];
+                if (k0 + u < kSellMaxLen && k0 + u < w) xg[u] = gather_row<T, MODE, LPR>(Xv, dinv, c0, ci[k0 + u], j);
 #pragma unroll
             for (int u = 0; u < kG; u++)
                 if (k0 + u < kSellMaxLen && k0 + u < w) {
@@ -183,7 +204,8 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
         if constexpr (PART == 1) return;
         int lw = gw - nw_sell;
         if (lw < A.nmid) {  // a medium row: one wavefront, its R lane groups split the entries
-            acc = chain_sum<T, R, LPR>(A.mid_ptr[lw], A.mid_ptr[lw + 1], sub, j, A.mid_col, A.mid_val, Xv);
+            acc = chain_sum<T, R, LPR, MODE>(A.mid_ptr[lw], A.mid_ptr[lw + 1], sub, j, A.mid_col, A.mid_val, Xv, dinv,
+                                             c0);
 #pragma unroll
             for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
             if (sub != 0) return;
@@ -191,14 +213,15 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
         } else {  // a segment of a long row: its partial sum, combined by sell_spmm_long
             lw -= A.nmid;
             if (lw >= A.nseg) return;  // uniform inside the wavefront
-            acc = chain_sum<T, R, LPR>(A.seg_beg[lw], A.seg_end[lw], sub, j, A.long_col, A.long_val, Xv);
+            acc = chain_sum<T, R, LPR, MODE>(A.seg_beg[lw], A.seg_end[lw], sub, j, A.long_col, A.long_val, Xv, dinv,
+                                             c0);
 #pragma unroll
             for (int off = LPR; off < 64; off <<= 1) vshfl_add(acc, off);
             if (sub == 0) reinterpret_cast<V*>(A.seg_part)[size_t(lw) * LPR + j] = acc;
             return;
         }
     }
-    spmm_store<T, MW, MODE>(acc, row, j, Xv, Y, B, dinv, ca, cs, have_xd, xd);
+    spmm_store<T, MW, MODE>(acc, row, j, Xv, Y, B, dinv, ca, cs, c0, have_xd, xd);
 }
 
 // Long rows of the multi-column kernels: one wavefront per row adds its segments' partial
@@ -206,7 +229,7 @@ __global__ void __launch_bounds__(256) sell_spmm(SellDev<T> A, int nw_sell, cons
 template <typename T, int MW, int MODE>
 __global__ void __launch_bounds__(256) sell_spmm_long(SellDev<T> A, const T* __restrict__ X, T* __restrict__ Y,
                                                     const T* __restrict__ B, const int* skip,
-                                                    const T* __restrict__ dinv, T ca, T cs) {
+                                                    const T* __restrict__ dinv, T ca, T cs, T c0 = T(0)) {
     if (skip && *skip) return;
     using V = typename Vec16<T>::type;
     constexpr int LPR = MW / Vec16<T>::N;
@@ -219,7 +242,7 @@ __global__ void __launch_bounds__(256) sell_spmm_long(SellDev<T> A, const T* __r
     for (int q = A.lseg_ptr[w]; q < A.lseg_ptr[w + 1]; q++) acc = vadd(acc, P[size_t(q) * LPR + j]);
     V xd;
     vzero(xd);
-    spmm_store<T, MW, MODE>(acc, A.long_rows[w], j, reinterpret_cast<const V*>(X), Y, B, dinv, ca, cs, false, xd);
+    spmm_store<T, MW, MODE>(acc, A.long_rows[w], j, reinterpret_cast<const V*>(X), Y, B, dinv, ca, cs, c0, false, xd);
 }
 
 template <typename T, int MW>
@@ -237,13 +260,13 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
     // long rows' sums
 #define MK_SPMM_PARTS(M)                                                                                        \
     MK_TLAUNCH(tm, 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, D, nw_sell, X, Y, B, skip, \
-               (const T*)nullptr, T(0), T(0));                                                                          \
+               (const T*)nullptr, T(0), T(0), T(0));                                                                    \
     if (g2)                                                                                                             \
         MK_TLAUNCH(tm, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, D, 0, X, Y, B, skip,          \
-                   (const T*)nullptr, T(0), T(0));                                                                      \
+                   (const T*)nullptr, T(0), T(0), T(0));                                                                \
     if (g3)                                                                                                             \
         MK_TLAUNCH(tm, 2, (sell_spmm_long<T, MW, M>), dim3(g3), dim3(256), 0, c->stream, D, X, Y, B, skip,            \
-                   (const T*)nullptr, T(0), T(0));
+                   (const T*)nullptr, T(0), T(0), T(0));
     switch (mode) {
         case SPMV_PLAIN: MK_SPMM_PARTS(0) break;
         case SPMV_RESID_NORED: MK_SPMM_PARTS(3) break;
@@ -255,7 +278,7 @@ static void spmm_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, c
 
 template <typename T, int MW>
 static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T* Y, const T* B, const T* dinv, T ca,
-                         T cs, int mode, const int* skip, const SpmvTimer& tm) {
+                         T cs, T c0, int mode, const int* skip, const SpmvTimer& tm) {
     unsigned g0 = 0;
     const SellDev<T> Dd = sell_dev<T>(A, val, 256, &g0);
     constexpr int R = 64 / (MW / Vec16<T>::N);
@@ -266,34 +289,39 @@ static void spmm_cheb_mw(mk_ctx* c, const mk_op* A, const T* val, const T* X, T*
     const unsigned g3 = unsigned((int64_t(A->nlong) + 3) / 4);
 #define MK_CHEB_PARTS(M)                                                                                            \
     MK_TLAUNCH(tm, 0, (sell_spmm<T, MW, M, 1>), dim3(g1), dim3(256), 0, c->stream, Dd, nw_sell, X, Y, B,   \
-               skip, dinv, ca, cs);                                                                                 \
+               skip, dinv, ca, cs, c0);                                                                             \
     if (g2)                                                                                                         \
         MK_TLAUNCH(tm, 1, (sell_spmm<T, MW, M, 2>), dim3(g2), dim3(256), 0, c->stream, Dd, 0, X, Y, B, skip,     \
-                   dinv, ca, cs);                                                                                   \
+                   dinv, ca, cs, c0);                                                                               \
     if (g3)                                                                                                         \
         MK_TLAUNCH(tm, 2, (sell_spmm_long<T, MW, M>), dim3(g3), dim3(256), 0, c->stream, Dd, X, Y, B, skip, dinv, \
-                   ca, cs);
+                   ca, cs, c0);
     switch (mode) {
         case 5: MK_CHEB_PARTS(5) break;
         case 6: MK_CHEB_PARTS(6) break;
         case 7: MK_CHEB_PARTS(7) break;
-        default: throw Error(MK_ERR_ARG, "block Chebyshev: mode 5, 6 or 7");
+        case 8: MK_CHEB_PARTS(8) break;
+        case 9: MK_CHEB_PARTS(9) break;
+        default: throw Error(MK_ERR_ARG, "block Chebyshev: mode 5 to 9");
     }
 #undef MK_CHEB_PARTS
 }
 
-// One fused three-term Chebyshev step on a block of mw right-hand sides (modes 5-7 of
-// sell_spmm; x_out != x_in, x_out holds x_{k-1} on entry in mode 6).
+// One fused three-term Chebyshev step on a block of mw right-hand sides (modes 5-9 of
+// sell_spmm; x_out != x_in, x_out holds x_{k-1} on entry in mode 6; modes 8 and 9 take
+// x_1 = c0 D^-1 b, mode 8 in place of x_in, which it does not read).
 template <typename T>
 void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out, const T* b, T ca, T cs, int mode,
-                      const int* skip) {
+                      const int* skip, T c0) {
     MK_CHECK(!A->cplx && x_in != x_out, MK_ERR_ARG, "fused block Chebyshev: real operator, x_out != x_in");
-    // algorithmic bytes: the matrix (values, columns, slice / row metadata), the x_in block
-    // once (A->m rows: every column is gathered, the row's own entry among them), B read,
-    // x_out written (and read first in mode 6), the inverse diagonal
+    MK_CHECK(mode >= 5 && mode <= 9, MK_ERR_ARG, "block Chebyshev: mode 5 to 9");
+    // algorithmic bytes: the matrix (values, columns, slice / row metadata), the gathered block
+    // once (A->m rows: every column is gathered, the row's own entry among them; B itself in
+    // mode 8), B read, x_out written (and read first in mode 6), the inverse diagonal
     const double s = double(sizeof(T)), vb = s * mw;
-    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * s + double(A->m) * vb +
-                         double(A->n) * vb * (mode == 6 ? 3.0 : 2.0) + double(A->n) * s;
+    const double bytes = (sizeof(T) == 8 ? A->bytes64 : A->bytes32) - double(A->n) * 2.0 * s +
+                         (mode == 8 ? 0.0 : double(A->m) * vb) + double(A->n) * vb * (mode == 6 ? 3.0 : 2.0) +
+                         double(A->n) * s;
     SpmvTimer tm(c, bytes, A, mw == 64 ? "spmm64_cheb" : mw == 32 ? "spmm32_cheb" : mw == 16 ? "spmm16_cheb" :
                  "spmm8_cheb", sizeof(T) == 4, 3);
     const T* val;
@@ -301,18 +329,18 @@ void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out
     if constexpr (std::is_same<T, double>::value) { val = A->val64.p; dinv = A->dinv64.p; }
     else { val = A->val32.p; dinv = A->dinv32.p; }
     switch (mw) {
-        case 8: spmm_cheb_mw<T, 8>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
-        case 16: spmm_cheb_mw<T, 16>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
-        case 32: spmm_cheb_mw<T, 32>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
-        case 64: spmm_cheb_mw<T, 64>(c, A, val, x_in, x_out, b, dinv, ca, cs, mode, skip, tm); break;
+        case 8: spmm_cheb_mw<T, 8>(c, A, val, x_in, x_out, b, dinv, ca, cs, c0, mode, skip, tm); break;
+        case 16: spmm_cheb_mw<T, 16>(c, A, val, x_in, x_out, b, dinv, ca, cs, c0, mode, skip, tm); break;
+        case 32: spmm_cheb_mw<T, 32>(c, A, val, x_in, x_out, b, dinv, ca, cs, c0, mode, skip, tm); break;
+        case 64: spmm_cheb_mw<T, 64>(c, A, val, x_in, x_out, b, dinv, ca, cs, c0, mode, skip, tm); break;
         default: throw Error(MK_ERR_ARG, "spmm: block width must be 8, 16, 32 or 64");
     }
     MK_HIP(hipGetLastError());
 }
 template void spmm_cheb_launch<float>(mk_ctx*, const mk_op*, int, const float*, float*, const float*, float, float,
-                                      int, const int*);
+                                      int, const int*, float);
 template void spmm_cheb_launch<double>(mk_ctx*, const mk_op*, int, const double*, double*, const double*, double,
-                                       double, int, const int*);
+                                       double, int, const int*, double);
 
 template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip) {

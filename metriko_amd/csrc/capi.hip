@@ -145,13 +145,15 @@ __global__ void k_window_dots(int n, int nw, const double* __restrict__ W, const
     if (threadIdx.x == 0)
         for (int k = 0; k < kWinDots; k++) part[size_t(blockIdx.x) * kWinDots + k] = acc[k];
 }
-// Fixed-order sum of the per-block partials (one thread per value).
-__global__ void k_window_sum(int nparts, const double* __restrict__ part, double* __restrict__ out) {
-    const int k = threadIdx.x;
-    if (k >= kWinDots) return;
+// Fixed-order sum of the per-block partials: one wavefront per value, lane l adds the
+// partials l, l + 64, ... and the wavefront's fixed shuffle tree combines the lanes.
+__global__ void __launch_bounds__(64 * kWinDots) k_window_sum(int nparts, const double* __restrict__ part,
+                                                               double* __restrict__ out) {
+    const int k = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double s = 0.0;
-    for (int p = 0; p < nparts; p++) s += part[size_t(p) * kWinDots + k];
-    out[k] = s;
+    for (int p = lane; p < nparts; p += 64) s += part[size_t(p) * kWinDots + k];
+    s = mk::wave_sum(s);
+    if (lane == 0) out[k] = s;
 }
 // x = sum_j y_j w_j (complex coefficients y[2j], y[2j+1]).
 __global__ void k_window_combine(int n, int nw, const double* __restrict__ W, const double* __restrict__ y,
@@ -937,7 +939,7 @@ mk_status mk_field_smallest_eig(mk_ctx* c, const mk_op* L, const double* mass, c
             mk::spmv64(c, L, du.p, lws, nullptr, mk::SPMV_PLAIN, nullptr, nullptr);
             hipLaunchKernelGGL(k_window_dots, dim3(gw), dim3(256), 0, c->stream, n, nw, (const double*)dW.p,
                                (const double*)lws, (const double*)db.p, dpart.p);
-            hipLaunchKernelGGL(k_window_sum, dim3(1), dim3(64), 0, c->stream, int(gw), (const double*)dpart.p,
+            hipLaunchKernelGGL(k_window_sum, dim3(1), dim3(64 * kWinDots), 0, c->stream, int(gw), (const double*)dpart.p,
                                dpart.p + size_t(gw) * kWinDots);
             MK_HIP(hipMemcpyAsync(hd, dpart.p + size_t(gw) * kWinDots, sizeof(hd), hipMemcpyDeviceToHost,
                                   c->stream));

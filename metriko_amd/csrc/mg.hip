@@ -969,10 +969,12 @@ static BlockBufs<T> block_bufs(MgLevel& L) {
     else return {L.bb32.p, L.bx32.p, L.br32.p, L.by32.p};
 }
 
-// Chebyshev(deg) smoothing of the block in the three-term form: after a zero start the first
-// iterate is c0 D^-1 b (k_bcheb_first) and every later step is one fused kernel (residual and
-// update) that writes over the buffer holding the iterate before the one it reads; the two
-// buffers x and y alternate, the first chosen so that the last step lands in x.
+// Chebyshev(deg) smoothing of the block in the three-term form.  Every step is one fused kernel
+// (residual and update) that writes over the buffer holding the iterate before the one it
+// reads; the two buffers x and y alternate, the first chosen so that the last step lands in x.
+// After a zero start the first iterate x_1 = c0 D^-1 b is never stored: the second step forms
+// it inside its gathers (mode 8) and the third from the row's own b (mode 9).  With deg = 1
+// x_1 is the result and is written (k_bcheb_first).
 template <typename T>
 static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, const T* b, T* x, bool zero_start,
                             const int* skip) {
@@ -986,20 +988,26 @@ static void chebyshev_block(mk_ctx* c, const Mg* g, MgLevel& L, int mw, int sh, 
     else dinv = L.A->dinv32.p;
     T* cur = x;
     int fused = g->deg;
+    const T c0 = T(1.0 / theta);
+    if (zero_start && g->deg == 1) {
+        hipLaunchKernelGGL((k_bcheb_first<T>), dim3(nb(len)), dim3(256), 0, c->stream, len, sh, dinv, b, x, c0,
+                           skip);
+        return;
+    }
     if (zero_start) {
+        // deg - 1 fused steps after the implicit x_1: the first writes the buffer that makes
+        // the last one land in x (cur only selects it; mode 8 does not read it)
         cur = ((g->deg - 1) % 2 == 0) ? x : y;
-        hipLaunchKernelGGL((k_bcheb_first<T>), dim3(nb(len)), dim3(256), 0, c->stream, len, sh, dinv, b, cur,
-                           T(1.0 / theta), skip);
         fused = g->deg - 1;
     }
     for (int k = 0; k < fused; k++) {
         T* out = (cur == x) ? y : x;
         if (!zero_start && k == 0) {
-            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(0), T(1.0 / theta), 5, skip);
+            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(0), c0, 5, skip);
         } else {
             const double rho1 = 1.0 / (2.0 * sigma - rho0);
-            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(rho1 * rho0), T(2.0 * rho1 / delta),
-                                (zero_start && k == 0) ? 7 : 6, skip);
+            const int mode = !zero_start ? 6 : k == 0 ? 8 : k == 1 ? 9 : 6;
+            spmm_cheb_launch<T>(c, L.A, mw, cur, out, b, T(rho1 * rho0), T(2.0 * rho1 / delta), mode, skip, c0);
             rho0 = rho1;
         }
         cur = out;

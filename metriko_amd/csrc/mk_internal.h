@@ -304,10 +304,11 @@ template <typename T>
 void spmm_launch(mk_ctx* c, const mk_op* A, int mw, const T* X, T* Y, const T* B, SpmvMode mode, const int* skip);
 // one three-term Chebyshev step fused with its residual on a block of mw right-hand sides,
 // r = b - A x_in: mode 5 x_out = x_in + cs D^-1 r; 6 x_out = x_in + ca (x_in - x_out) + cs D^-1 r
-// (x_out holds the previous iterate on entry); 7 x_out = (1 + ca) x_in + cs D^-1 r
+// (x_out holds the previous iterate on entry); 7 x_out = (1 + ca) x_in + cs D^-1 r; 8 and 9:
+// modes 7 and 6 with x_in resp. the previous iterate replaced by x_1 = c0 D^-1 b
 template <typename T>
 void spmm_cheb_launch(mk_ctx* c, const mk_op* A, int mw, const T* x_in, T* x_out, const T* b, T ca, T cs, int mode,
-                      const int* skip);
+                      const int* skip, T c0 = T(0));
 // A X = B for mw right-hand sides (B, X: n x mw row-major, device); X starts from zero.
 // fp64_vcycle: the V-cycle in FP64 (the plain FP64 form of the same algorithm) instead of FP32
 BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B, double* X, double rel_tol,
