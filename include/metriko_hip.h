@@ -174,6 +174,7 @@ typedef struct {
     int rounds, lm_rounds, k, kernel_rank, n_fixed;
     int null_vectors;  /* near-null vectors given to the A0 multigrid (0 = one-vector rule) */
     int batched_solves, batch_iterations;  /* Woodbury columns solved in blocks; their PCG iterations */
+    int device_order;  /* 1: the A0 system ran in its multigrid aggregates' locality order on the device */
     double kkt_rel_residual, setup_ms, kkt_ms, naive_ms, rounding_ms;
     mk_stats solve;
 } mk_igm_stats;

@@ -55,7 +55,7 @@ class IgmFlags(C.Structure):
 class IgmStats(C.Structure):
     _fields_ = [("rounds", C.c_int), ("lm_rounds", C.c_int), ("k", C.c_int), ("kernel_rank", C.c_int),
                 ("n_fixed", C.c_int), ("null_vectors", C.c_int), ("batched_solves", C.c_int),
-                ("batch_iterations", C.c_int), ("kkt_rel_residual", C.c_double),
+                ("batch_iterations", C.c_int), ("device_order", C.c_int), ("kkt_rel_residual", C.c_double),
                 ("setup_ms", C.c_double),
                 ("kkt_ms", C.c_double), ("naive_ms", C.c_double), ("rounding_ms", C.c_double), ("solve", Stats)]
 

@@ -151,8 +151,12 @@ struct RowSet {  // a small set of sparse rows (CSR) kept on the device
     DevBuf<int64_t> ptr;
     DevBuf<int> col;
     DevBuf<double> val;
+    const std::vector<int>* cmap = nullptr;  // device numbering of the columns (null: the same)
     void push(const SprsD& M, int row) {
-        for (int64_t k = M.ptr[row]; k < M.ptr[row + 1]; k++) { hcol.push_back(M.col[k]); hval.push_back(M.val[k]); }
+        for (int64_t k = M.ptr[row]; k < M.ptr[row + 1]; k++) {
+            hcol.push_back(cmap && !cmap->empty() ? (*cmap)[size_t(M.col[k])] : M.col[k]);
+            hval.push_back(M.val[k]);
+        }
         hptr.push_back(int64_t(hcol.size()));
         nrows++;
     }
@@ -219,7 +223,8 @@ metriko::HVec<double> scaled_shifted(const SprsD& A, double s, double d) {
 // rectangular operator (no multigrid, no diagonal): hybrid SELL upload of a host CSR
 // device operator (no host copy kept) from CSR arrays, values times scale
 mk_op* make_op_arrays(mk_ctx* c, int rows, int cols, const int64_t* ptr, const int* col, const double* val,
-                      const mk_solve_opts& o, double scale = 1.0) {
+                      const mk_solve_opts& o, double scale = 1.0, const std::vector<int>* rsrc = nullptr,
+                      const std::vector<int>* cmap = nullptr) {
     auto* op = new mk_op();
     try {
         op->ctx = c;
@@ -229,15 +234,42 @@ mk_op* make_op_arrays(mk_ctx* c, int rows, int cols, const int64_t* ptr, const i
         op->nnz = ptr[rows];
         op->block = o.block_size;
         const bool csr = o.format == MK_FMT_CSR;
-        mk::build_sell(op, ptr, col, val, csr ? 64 : o.slice_height, csr ? 1 : o.sigma, scale);
+        mk::build_sell(op, ptr, col, val, csr ? 64 : o.slice_height, csr ? 1 : o.sigma, scale,
+                       rsrc && !rsrc->empty() ? rsrc->data() : nullptr, cmap && !cmap->empty() ? cmap->data() : nullptr);
     } catch (...) {
         delete op;
         throw;
     }
     return op;
 }
-mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o, double scale = 1.0) {
-    return make_op_arrays(c, A.rows(), A.cols(), A.ptr.data(), A.col.data(), A.val.data(), o, scale);
+mk_op* make_rect_op(mk_ctx* c, const SprsD& A, const mk_solve_opts& o, double scale = 1.0,
+                    const std::vector<int>* rsrc = nullptr, const std::vector<int>* cmap = nullptr) {
+    return make_op_arrays(c, A.rows(), A.cols(), A.ptr.data(), A.col.data(), A.val.data(), o, scale, rsrc, cmap);
+}
+
+// The A0 system's unknowns are kept on the device in the locality order of A0's multigrid
+// aggregates (the host keeps the caller's numbering): in the caller's vertex order neighbouring
+// unknowns of a subdivided mesh lie hundreds of thousands of indices apart, and every product
+// on A0 then gathers from all over its vector.  The operators' rows keep their entries' order,
+// so every product computes the same numbers.  METRIKO_IGM_ORDER=0 turns it off, =2 on for any
+// size; by default it is on from 65,536 unknowns.  The local-injectivity passes keep the
+// caller's order.
+bool igm_order_wanted(int k, const mk_igm_flags* flags) {
+    if (flags->local_injectivity) return false;
+    const char* e = std::getenv("METRIKO_IGM_ORDER");
+    const int m = e ? std::atoi(e) : 1;
+    return m == 2 || (m == 1 && k >= 65536);
+}
+// out[pos[i]] = h[i] (device order) / out[i] = d[pos[i]] (the caller's order); pos empty: copies
+std::vector<double> to_dev_order(const std::vector<int>& pos, const double* h, size_t n) {
+    std::vector<double> out(n);
+    if (pos.empty()) { std::copy(h, h + n, out.begin()); return out; }
+    for (size_t i = 0; i < n; i++) out[size_t(pos[i])] = h[i];
+    return out;
+}
+void from_dev_order(const std::vector<int>& pos, const std::vector<double>& d, double* out) {
+    if (pos.empty()) { std::copy(d.begin(), d.end(), out); return; }
+    for (size_t i = 0; i < d.size(); i++) out[i] = d[size_t(pos[i])];
 }
 struct OpHolder {
     mk_op* op = nullptr;
@@ -1532,38 +1564,75 @@ extern "C" mk_status mk_iterative_rounding(
         // length * E + wC^2 I (wI = 1, wC = 0.01, iter_rounding_loop.h:109); its multigrid
         // hierarchy preconditions both the KKT solve and every rounding solve.
         const double wI = 1.0, wK = 10e4, wC = 0.01;
+        // device order of the unknowns (empty: the caller's): vpos[i] = device index of unknown i
+        std::vector<int> vpos, vinv;
+        const bool ordered = o.precond == MK_PRECOND_AMG && igm_order_wanted(k, flags);
         {
             const double ts = now_ms();
-            hA0.op = mk::op_create_owned(c, k, metriko::HVec<int64_t>(E.ptr), metriko::HVec<int>(E.col),
-                                         scaled_shifted(E, length * wI * wI, wC * wC), o);
-            MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
-            hA0.op->label = "igm_A0";
-            if (o.precond == MK_PRECOND_AMG) {
-                hA0.op->nullvec = near_null(G2, CsrView::of(Cf), X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
-                hA0.op->nnull = hA0.op->nullvec.empty() ? 0 : n;
-                S.null_vectors = hA0.op->nnull;
+            if (ordered) {
+                // the hierarchy first (on the caller's numbering), then the operator in its order
+                const metriko::HVec<double> a0v = scaled_shifted(E, length * wI * wI, wC * wC);
+                std::vector<double> nv = near_null(G2, CsrView::of(Cf), X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
+                const int nnull = nv.empty() ? 0 : n;
+                S.null_vectors = nnull;
+                MK_LOG("[igm] near-null vectors %d, %.0f ms\n", nnull, now_ms() - t_all);
+                mk::Mg* mg = mk::mg_build_real(c, k, E.ptr.data(), E.col.data(), a0v.data(), o, nnull ? &nv : nullptr,
+                                               nnull, &vpos);
+                if (!vpos.empty()) {
+                    vinv.resize(vpos.size());
+                    for (size_t i = 0; i < vpos.size(); i++) vinv[size_t(vpos[i])] = int(i);
+                }
+                try {
+                    hA0.op = make_op_arrays(c, k, k, E.ptr.data(), E.col.data(), a0v.data(), o, 1.0, &vinv, &vpos);
+                } catch (...) {
+                    mk::mg_free(mg);
+                    throw;
+                }
+                hA0.op->mg = mg;
+                hA0.op->label = "igm_A0";
+                S.device_order = vpos.empty() ? 0 : 1;
+                mk::mg_set_labels(mg, hA0.op->label);
+                MK_LOG("[igm] A0 multigrid and operator (device order) %.0f ms\n", now_ms() - t_all);
+            } else {
+                hA0.op = mk::op_create_owned(c, k, metriko::HVec<int64_t>(E.ptr), metriko::HVec<int>(E.col),
+                                             scaled_shifted(E, length * wI * wI, wC * wC), o);
+                MK_LOG("[igm] A0 upload %.0f ms\n", now_ms() - t_all);
+                hA0.op->label = "igm_A0";
+                if (o.precond == MK_PRECOND_AMG) {
+                    hA0.op->nullvec = near_null(G2, CsrView::of(Cf), X2F, n, integerIdcs, nint, nonPart, uniqueJ, US);
+                    hA0.op->nnull = hA0.op->nullvec.empty() ? 0 : n;
+                    S.null_vectors = hA0.op->nnull;
+                }
+                MK_LOG("[igm] near-null vectors %d, %.0f ms\n", hA0.op->nnull, now_ms() - t_all);
+                mk::ensure_mg(c, hA0.op, o);
+                MK_LOG("[igm] A0 multigrid %.0f ms\n", now_ms() - t_all);
             }
-            MK_LOG("[igm] near-null vectors %d, %.0f ms\n", hA0.op->nnull, now_ms() - t_all);
-            mk::ensure_mg(c, hA0.op, o);
-            MK_LOG("[igm] A0 multigrid %.0f ms\n", now_ms() - t_all);
             S.setup_ms += now_ms() - ts;
             t0 += now_ms() - ts;  // the KKT time excludes the A0 operator and hierarchy set-up
         }
         MK_LOG("[igm] KKT: Eaug assembled %.0f ms\n", now_ms() - t0);
-        hE.op = in_pattern ? make_op_arrays(c, k, k, E.ptr.data(), E.col.data(), eaug_val.data(), o)
-                           : make_rect_op(c, EaugSum, o);
+        hE.op = in_pattern ? make_op_arrays(c, k, k, E.ptr.data(), E.col.data(), eaug_val.data(), o, 1.0, &vinv, &vpos)
+                           : make_rect_op(c, EaugSum, o, 1.0, &vinv, &vpos);
         MK_LOG("[igm] KKT: Eaug uploaded %.0f ms\n", now_ms() - t0);
         hE.op->label = "igm_Eaug";
         DevBuf<double> dXS(static_cast<size_t>(k)), drhs;
-        drhs.upload(rhs.data(), rhs.size(), K.st());
+        {
+            const std::vector<double> rd = to_dev_order(vpos, rhs.data(), rhs.size());
+            drhs.upload(rd.data(), rd.size(), K.st());
+            K.sync();
+        }
         MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));
         {
             const mk::SolveResult kr = K.solve(hE.op, drhs.p, dXS.p, hA0.op->mg);
             MK_LOG("[igm] KKT PCG %d iterations, residual %.2e\n", kr.iters, kr.rel_residual);
         }
         std::vector<double> XSmall(static_cast<size_t>(k));
-        dXS.download(XSmall.data(), XSmall.size(), K.st());
-        K.sync();
+        {
+            std::vector<double> xd(static_cast<size_t>(k));
+            dXS.download(xd.data(), xd.size(), K.st());
+            K.sync();
+            from_dev_order(vpos, xd, XSmall.data());
+        }
         {   // KKT residual (host, FP64): ||E x - f|| + ||C x - g|| relative to ||f||
             std::vector<double> Ex(static_cast<size_t>(k));
             E.multiply(XSmall.data(), Ex.data());
@@ -1611,7 +1680,7 @@ extern "C" mk_status mk_iterative_rounding(
             // device operator length * X2F (rectangular, hybrid SELL): B x for the naive pass
             // and the field energies of the rounding passes
             OpHolder hX2F;
-            hX2F.op = make_rect_op(c, X2F, o, length);
+            hX2F.op = make_rect_op(c, X2F, o, length, nullptr, &vpos);
             hX2F.op->label = "igm_X2F";
             DevBuf<double> dF2, dBx(l1), dfstar(l1), fcur(l1);
             dF2.upload(F2, l1, K.st());
@@ -1660,12 +1729,17 @@ extern "C" mk_status mk_iterative_rounding(
             MK_LOG("[igm] rounding set-up (b_int) %.0f ms\n", now_ms() - t0);
             DevBuf<double> dfield(l1), dbint, dxcur(static_cast<size_t>(k)), dxprev(static_cast<size_t>(k)), dxt(static_cast<size_t>(k)), dxstar(static_cast<size_t>(k)),
                 dxlm(static_cast<size_t>(k)), db0(static_cast<size_t>(k)), drow(static_cast<size_t>(k)), dcand, dfixv, dcoef;
-            dbint.upload(bint.data(), bint.size(), K.st());
-            dxcur.upload(XSmall.data(), size_t(k), K.st());
+            {
+                const std::vector<double> bd = to_dev_order(vpos, bint.data(), bint.size());
+                dbint.upload(bd.data(), bd.size(), K.st());
+                K.sync();
+            }
+            MK_HIP(hipMemcpyAsync(dxcur.p, dXS.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, K.st()));
             MK_HIP(hipMemsetAsync(dxt.p, 0, size_t(k) * sizeof(double), K.st()));
             // candidate rows of UFull (all indices the loop can fix)
             std::vector<int> singular(singularIdcs, singularIdcs + nsing), integer(integerIdcs, integerIdcs + nint);
             RowSet cand;
+            cand.cmap = &vpos;
             std::vector<int> candIdx;
             std::vector<int> candPos(size_t(nfull), -1);
             for (int i : singular) if (candPos[i] < 0) { candPos[i] = int(candIdx.size()); candIdx.push_back(i); cand.push(UF, i); }
@@ -1673,6 +1747,7 @@ extern "C" mk_status mk_iterative_rounding(
             cand.sync(K.st());
             dcand.alloc(std::max<size_t>(candIdx.size(), 1));
             RowSet fixedRows;
+            fixedRows.cmap = &vpos;
             std::vector<int> fixedI;
             std::vector<double> fixedV;
             const size_t ncand = std::max<size_t>(candIdx.size(), 1);
@@ -1872,9 +1947,10 @@ extern "C" mk_status mk_iterative_rounding(
             S.lm_rounds = lm_rounds;
             S.n_fixed = int(fixedI.size());
             if (rounded) {
-                std::vector<double> xs(static_cast<size_t>(k));
-                dxcur.download(xs.data(), xs.size(), K.st());
+                std::vector<double> xd(static_cast<size_t>(k)), xs(static_cast<size_t>(k));
+                dxcur.download(xd.data(), xd.size(), K.st());
                 K.sync();
+                from_dev_order(vpos, xd, xs.data());
                 UF.multiply(xs.data(), fullx);
             } else {
                 std::copy(x0.begin(), x0.end(), fullx);
@@ -1902,6 +1978,7 @@ extern "C" mk_status mk_iterative_rounding(
                 hX2F.op = nullptr;
                 st->UF = std::move(UF);
                 st->X2Ft = std::move(X2Ft);
+                st->pos = vpos;
                 st->F2.assign(F2, F2 + l1);
                 st->fixedV.assign(fixedV.begin(), fixedV.begin() + last_r);
                 st->UZ.resize(size_t(last_r) * size_t(last_r));
@@ -1949,7 +2026,7 @@ std::vector<double> state_rowdots(mk_igm_state* st, const double* y) {
         std::vector<double> hv;
         for (int row : st->Urows) {
             for (int64_t q = st->UF.ptr[row]; q < st->UF.ptr[row + 1]; q++) {
-                hc.push_back(st->UF.col[q]);
+                hc.push_back(st->pos.empty() ? st->UF.col[q] : st->pos[size_t(st->UF.col[q])]);
                 hv.push_back(st->UF.val[q]);
             }
             hp.push_back(int64_t(hc.size()));
@@ -2086,7 +2163,8 @@ mk_status mk_igm_sized_uv(mk_ctx* c, mk_igm_state* st, const double* face_scale,
         st->X2Ft.multiply(sf.data(), b.data());
         for (auto& v : b) v *= st->length * st->wI * st->wI;
         DevBuf<double> db, dxt(static_cast<size_t>(k)), dx(static_cast<size_t>(k));
-        db.upload(b.data(), b.size(), c->stream);
+        const std::vector<double> bd = to_dev_order(st->pos, b.data(), b.size());
+        db.upload(bd.data(), bd.size(), c->stream);
         hipLaunchKernelGGL(k_lin2, dim3(nblk(size_t(k))), dim3(256), 0, c->stream, size_t(k), 1.0, (const double*)db.p,
                            st->wC * st->wC, (const double*)st->xprev.p, db.p);
         MK_HIP(hipMemcpyAsync(dxt.p, st->xprev.p, size_t(k) * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
@@ -2097,9 +2175,10 @@ mk_status mk_igm_sized_uv(mk_ctx* c, mk_igm_state* st, const double* face_scale,
         std::vector<double> rhs(static_cast<size_t>(st->r));
         for (int i = 0; i < st->r; i++) rhs[size_t(i)] = st->fixedV[size_t(i)] - ux[size_t(i)];
         state_combine(st, dxt.p, state_msolve(st, rhs), dx.p);
-        std::vector<double> xs(static_cast<size_t>(k)), fx(static_cast<size_t>(st->nfull));
-        dx.download(xs.data(), xs.size(), c->stream);
+        std::vector<double> xd(static_cast<size_t>(k)), xs(static_cast<size_t>(k)), fx(static_cast<size_t>(st->nfull));
+        dx.download(xd.data(), xd.size(), c->stream);
         MK_HIP(hipStreamSynchronize(c->stream));
+        from_dev_order(st->pos, xd, xs.data());
         st->UF.multiply(xs.data(), fx.data());
         if (fullx) std::copy(fx.begin(), fx.end(), fullx);
         if (uv) st->P.multiply(fx.data(), uv);
@@ -2122,7 +2201,8 @@ mk_status mk_igm_sized_uv_grad(mk_ctx* c, mk_igm_state* st, const double* face_s
             for (int64_t q = st->UF.ptr[i]; q < st->UF.ptr[i + 1]; q++) g[size_t(st->UF.col[q])] += st->UF.val[q] * gf[size_t(i)];
         // adjoint: lambda = H^{-1} g = A0^{-1} g - Z M^{-1} U A0^{-1} g
         DevBuf<double> dg, dy(static_cast<size_t>(k)), dl(static_cast<size_t>(k)), dw(static_cast<size_t>(st->l1));
-        dg.upload(g.data(), g.size(), c->stream);
+        const std::vector<double> gd = to_dev_order(st->pos, g.data(), g.size());
+        dg.upload(gd.data(), gd.size(), c->stream);
         MK_HIP(hipMemsetAsync(dy.p, 0, size_t(k) * sizeof(double), c->stream));
         const mk::SolveResult sr = mk::solve_spd(c, st->A0, dg.p, dy.p, st->o);
         MK_CHECK(sr.status == MK_OK, sr.status, "adjoint solve: " + std::string(mk_status_str(sr.status)));

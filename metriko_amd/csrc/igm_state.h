@@ -19,6 +19,7 @@ struct mk_igm_state {
     metriko::SprsD UF, X2Ft, P;
     std::vector<double> F2, fixedV, UZ;
     std::vector<int> Urows;
+    std::vector<int> pos;  // device index of unknown i (A0, X2F, Z, xprev); empty: the same
     mk::DevBuf<int64_t> uptr;
     mk::DevBuf<int> ucol;
     mk::DevBuf<double> uval, Z, xprev;

@@ -532,25 +532,29 @@ __global__ void k_set_state_dev(CgState* st, const double* bb, double tol2) {
 
 // SELL fill: thread (slice s, lane l) writes row order[s C + l] (or s C + l) as column l of
 // its slice, entry j at slice_ptr[s] + j C + l; past the row's end a padding entry (its own
-// column when that is a valid column, value 0).  VW values per entry (2: complex).
+// column when that is a valid column, value 0).  VW values per entry (2: complex).  With a
+// renumbering the stored row `row` is CSR row rsrc[row] and a CSR column c is stored as
+// cmap[c]; a row keeps its entries in CSR order.
 template <int VW>
 __global__ void __launch_bounds__(256) k_sell_fill(int ns, int C, int n_rows, int m, const int64_t* __restrict__ sptr,
                                                    const int* __restrict__ sw, const int* __restrict__ order,
                                                    const int64_t* __restrict__ rowptr, const int* __restrict__ col,
                                                    const double* __restrict__ val, double scale, int* __restrict__ ocol,
-                                                   double* __restrict__ ov64, float* __restrict__ ov32) {
+                                                   double* __restrict__ ov64, float* __restrict__ ov32,
+                                                   const int* __restrict__ rsrc, const int* __restrict__ cmap) {
     const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= int64_t(ns) * C) return;
     const int s = int(t / C), l = int(t - int64_t(s) * C);
     const int r = s * C + l;
     const int row = r < n_rows ? (order ? order[r] : r) : -1;
-    const int64_t b = row >= 0 ? rowptr[row] : 0;
-    const int64_t len = row >= 0 ? rowptr[row + 1] - b : 0;
+    const int srow = row >= 0 && rsrc ? rsrc[row] : row;
+    const int64_t b = srow >= 0 ? rowptr[srow] : 0;
+    const int64_t len = srow >= 0 ? rowptr[srow + 1] - b : 0;
     const int w = sw[s];
     for (int j = 0; j < w; j++) {
         const int64_t e = sptr[s] + int64_t(j) * C + l;
         if (j < len) {
-            ocol[e] = col[b + j];
+            ocol[e] = cmap ? cmap[col[b + j]] : col[b + j];
 #pragma unroll
             for (int q = 0; q < VW; q++) {
                 const double v = scale * val[(b + j) * VW + q];
@@ -565,8 +569,12 @@ __global__ void __launch_bounds__(256) k_sell_fill(int ns, int C, int n_rows, in
     }
 }
 
-void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma, double scale) {
+void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma, double scale,
+                const int* rsrc, const int* cmap) {
     const int n = op->n;
+    // stored row i is CSR row src(i); a CSR column c is stored as cm(c)
+    auto src = [rsrc](int i) { return rsrc ? rsrc[i] : i; };
+    auto cm = [cmap](int c) { return cmap ? cmap[c] : c; };
     const int vw = op->cplx ? 2 : 1;
     MK_CHECK(C > 0 && C % 64 == 0, MK_ERR_ARG, "slice height must be a positive multiple of 64");
     // every SpMV kernel is compiled with __launch_bounds__(256)
@@ -580,7 +588,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     // long rows (> max(64, 4 x median length)) go to the wavefront-per-row part
     std::vector<int> lens(static_cast<size_t>(n));
 #pragma omp parallel for schedule(static, 65536) if (n > 100000)
-    for (int i = 0; i < n; i++) lens[i] = int(rowptr[i + 1] - rowptr[i]);
+    for (int i = 0; i < n; i++) lens[i] = int(rowptr[src(i) + 1] - rowptr[src(i)]);
     // median row length (the element at sorted position n/2) from a length histogram
     int med = 0;
     if (n > 0) {
@@ -664,7 +672,8 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     for (int i = 0; i < n; i++) {
         if (op->m != n) continue;
         double d = 0.0;
-        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; k++) if (col[k] == i) d += scale * val[k * vw];
+        const int si = src(i);
+        for (int64_t k = rowptr[si]; k < rowptr[si + 1]; k++) if (cm(col[k]) == i) d += scale * val[k * vw];
         dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
         dinv32[i] = float(dinv[i]);
     }
@@ -683,9 +692,9 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     std::vector<double> mval;
     std::vector<float> mval32;
     for (int w = 0; w < nmid; w++) {
-        const int row = mrows[w];
+        const int row = src(mrows[w]);
         for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
-            mcol.push_back(col[k]);
+            mcol.push_back(cm(col[k]));
             for (int c = 0; c < vw; c++) { mval.push_back(scale * val[k * vw + c]); mval32.push_back(float(scale * val[k * vw + c])); }
         }
         mptr[w + 1] = int64_t(mcol.size());
@@ -697,9 +706,9 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     std::vector<double> lval;
     std::vector<float> lval32;
     for (int w = 0; w < nlong; w++) {
-        const int row = lrows[w];
+        const int row = src(lrows[w]);
         for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
-            lcol.push_back(col[k]);
+            lcol.push_back(cm(col[k]));
             for (int c = 0; c < vw; c++) { lval.push_back(scale * val[k * vw + c]); lval32.push_back(float(scale * val[k * vw + c])); }
         }
         lptr[w + 1] = int64_t(lcol.size());
@@ -728,8 +737,10 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     if (stored > 0) {
         const int64_t nnz_all = rowptr[n];
         DevBuf<int64_t> drp;
-        DevBuf<int> dcol;
+        DevBuf<int> dcol, drs, dcm;
         DevBuf<double> dval;
+        if (rsrc) drs.upload(rsrc, size_t(n), st);
+        if (cmap) dcm.upload(cmap, size_t(op->m), st);
         drp.upload(rowptr, size_t(n) + 1, st);
         dcol.upload(col, size_t(std::max<int64_t>(nnz_all, 1)), st);
         dval.upload(val, size_t(std::max<int64_t>(nnz_all, 1)) * vw, st);
@@ -738,11 +749,13 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         if (vw == 2)
             hipLaunchKernelGGL((k_sell_fill<2>), dim3(g), dim3(256), 0, st, ns, C, n_rows, op->m, op->slice_ptr.p,
                                op->slice_w.p, (const int*)(permuted ? op->perm.p : nullptr), (const int64_t*)drp.p,
-                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p);
+                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p,
+                               (const int*)drs.p, (const int*)dcm.p);
         else
             hipLaunchKernelGGL((k_sell_fill<1>), dim3(g), dim3(256), 0, st, ns, C, n_rows, op->m, op->slice_ptr.p,
                                op->slice_w.p, (const int*)(permuted ? op->perm.p : nullptr), (const int64_t*)drp.p,
-                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p);
+                               (const int*)dcol.p, (const double*)dval.p, scale, op->col.p, op->val64.p, op->val32.p,
+                               (const int*)drs.p, (const int*)dcm.p);
         MK_HIP(hipGetLastError());
         MK_HIP(hipStreamSynchronize(st));  // the temporaries are freed on return
     } else {

@@ -534,8 +534,10 @@ void dense_hpd_inverse(const Csr<T>& A, std::vector<double>& out, bool cplx) {
     }
 }
 
+// rsrc / cmap: an optional renumbering of the rows / columns (build_sell)
 template <typename T>
-mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o) {
+mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o, const int* rsrc = nullptr,
+              const int* cmap = nullptr) {
     auto* op = new mk_op();
     try {
         op->ctx = c;
@@ -545,7 +547,7 @@ mk_op* upload(mk_ctx* c, const Csr<T>& A, const mk_solve_opts& o) {
         op->nnz = A.nonZeros();
         op->block = o.block_size;
         build_sell(op, A.ptr.data(), A.col.data(), reinterpret_cast<const double*>(A.val.data()), o.slice_height,
-                   o.format == MK_FMT_CSR ? 1 : o.sigma);
+                   o.format == MK_FMT_CSR ? 1 : o.sigma, 1.0, rsrc, cmap);
     } catch (...) {
         delete op;
         throw;
@@ -562,7 +564,35 @@ struct HostHier {
     std::vector<std::vector<double>> dinv;
     std::vector<double> inv;  // coarsest dense inverse (row-major, complex interleaved)
     int nc = 0;
+    // per level but the coarsest: the coarse unknown that stands for each node's aggregate
+    // (the first column of its tentative prolongator row; -1 for an empty row)
+    std::vector<std::vector<int>> key;
 };
+
+// Locality order of the finest level from the aggregates: the coarsest level keeps its order,
+// and every finer level lists its nodes by the rank of their aggregate's coarse unknown (nodes
+// of one aggregate in index order).  Aggregates are connected neighbourhoods, nested from level
+// to level, so rows that share columns end up close together, at every scale.  Returns
+// pos[old] = new for the finest level.
+template <typename T>
+std::vector<int> hier_order(const HostHier<T>& H) {
+    const size_t L = H.key.size();
+    std::vector<int> rank(size_t(H.A.back().nr));
+    std::iota(rank.begin(), rank.end(), 0);
+    for (size_t l = L; l-- > 0;) {
+        const std::vector<int>& key = H.key[l];
+        const int n = int(key.size()), nc = int(rank.size());
+        // counting sort by the parent's rank (nodes without a parent last), stable in index
+        std::vector<int> cnt(size_t(nc) + 2, 0);
+        auto bucket = [&](int i) { return key[size_t(i)] >= 0 ? rank[size_t(key[size_t(i)])] : nc; };
+        for (int i = 0; i < n; i++) cnt[size_t(bucket(i)) + 1]++;
+        for (int b = 0; b <= nc; b++) cnt[size_t(b) + 1] += cnt[size_t(b)];
+        std::vector<int> r(static_cast<size_t>(n));
+        for (int i = 0; i < n; i++) r[size_t(i)] = cnt[size_t(bucket(i))]++;
+        rank.swap(r);
+    }
+    return rank;
+}
 
 template <typename T>
 HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector<T>* B0 = nullptr,
@@ -628,11 +658,18 @@ HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector
             std::vector<T> Bc;
             if (nb > 0) Pt = tentative_qr(A, d, theta, B, nb, Bc, nagg);
             else Pt = tentative(A, d, theta, nagg);
+            {
+                std::vector<int> key(static_cast<size_t>(A.nr));
+#pragma omp parallel for schedule(static, 65536) if (A.nr > 200000)
+                for (int i = 0; i < A.nr; i++) key[size_t(i)] = Pt.ptr[i + 1] > Pt.ptr[i] ? Pt.col[Pt.ptr[i]] : -1;
+                H.key.push_back(std::move(key));
+            }
             lap("aggregation + tentative");
             if (nb > 0) B = std::move(Bc);
             if (nagg >= A.nr * 0.9 || nagg < 1) {  // no coarsening possible: make this the coarsest level
                 if (A.nr > 4000) throw Error(MK_ERR_UNSUPPORTED, "multigrid: aggregation stalled on a large level");
                 last = true;
+                H.key.pop_back();
             }
         }
         H.lmax.push_back(lmax);
@@ -730,11 +767,28 @@ HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector
     return H;
 }
 
+// order_out: when given (and the hierarchy has more than one level), the finest level is
+// stored in the locality order of hier_order -- its operator's rows and columns, P's rows, R's
+// (and [R | -R A]'s) columns and the smoother's diagonal -- and pos[old] = new is returned
+// there; the caller's vectors for this hierarchy are then in that order.  Each renumbered row
+// keeps its entries' order, so the V-cycle computes the same numbers as without it.
 template <typename T>
 Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int coarse_max,
-               const std::vector<T>* nullvec = nullptr, int nnull = 0) {
+               const std::vector<T>* nullvec = nullptr, int nnull = 0, std::vector<int>* order_out = nullptr) {
     HostHier<T> H = build_host<T>(std::move(A0), theta, coarse_max, nullvec, nnull);
     const auto t_up = std::chrono::steady_clock::now();
+    std::vector<int> pos, inv, pos2;
+    if (order_out && H.A.size() > 1) {
+        pos = hier_order(H);
+        inv.resize(pos.size());
+        for (size_t i = 0; i < pos.size(); i++) inv[size_t(pos[i])] = int(i);
+        // [R | -R A]: columns [0, n) and [n, 2n) are both finest-level unknowns
+        if (!H.RR.empty()) {
+            pos2.resize(pos.size() * 2);
+            for (size_t i = 0; i < pos.size(); i++) { pos2[i] = pos[i]; pos2[i + pos.size()] = pos[i] + int(pos.size()); }
+        }
+    }
+    const bool ren = !pos.empty();
     auto* g = new Mg();
     try {
         g->cplx = !std::is_same<T, double>::value;
@@ -743,9 +797,12 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
             MgLevel L;
             L.n = H.A[l].nr;
             L.lmax = H.lmax[l];
-            L.A = upload(c, H.A[l], o);
+            const bool r0 = ren && l == 0;
+            L.A = upload(c, H.A[l], o, r0 ? inv.data() : nullptr, r0 ? pos.data() : nullptr);
             {  // the smoother's inverse diagonal replaces the plain one
-                const std::vector<double>& dv = H.dinv[l];
+                std::vector<double> dv = H.dinv[l];
+                if (r0)
+                    for (size_t i = 0; i < dv.size(); i++) dv[size_t(pos[i])] = H.dinv[l][i];
                 std::vector<float> dv32(dv.begin(), dv.end());
                 L.A->dinv64.upload(dv.data(), dv.size(), c->stream);
                 L.A->dinv32.upload(dv32.data(), dv32.size(), c->stream);
@@ -753,9 +810,9 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
             }
             g->nnz_total += H.A[l].nonZeros();
             if (l < H.P.size()) {
-                L.P = upload(c, H.P[l], o);
-                L.R = upload(c, H.R[l], o);
-                if (l < H.RR.size()) L.RR = upload(c, H.RR[l], o);
+                L.P = upload(c, H.P[l], o, r0 ? inv.data() : nullptr, nullptr);
+                L.R = upload(c, H.R[l], o, nullptr, r0 ? pos.data() : nullptr);
+                if (l < H.RR.size()) L.RR = upload(c, H.RR[l], o, nullptr, r0 ? pos2.data() : nullptr);
             }
             g->lv.push_back(std::move(L));
         }
@@ -775,6 +832,7 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
         mg_free(g);
         throw;
     }
+    if (order_out) order_out->swap(pos);
     return g;
 }
 
@@ -1175,12 +1233,13 @@ namespace {
 }  // namespace
 
 Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o,
-                  const std::vector<double>* nullvec, int nnull) {
+                  const std::vector<double>* nullvec, int nnull, std::vector<int>* order_out) {
     Csr<double> A(n, n);
     A.ptr.assign(ptr, ptr + n + 1);
     A.col.assign(col, col + ptr[n]);
     A.val.assign(val, val + ptr[n]);
-    return build_impl<double>(c, std::move(A), o, tune().theta_real, tune().coarse_real, nullvec, nnull);
+    Mg* g = build_impl<double>(c, std::move(A), o, tune().theta_real, tune().coarse_real, nullvec, nnull, order_out);
+    return g;
 }
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val,
                      const mk_solve_opts& o) {

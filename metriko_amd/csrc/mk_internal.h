@@ -142,8 +142,11 @@ struct BlockResult {
     double max_rel;   // max relative residual (recurrence) over the columns
 };
 // multigrid (mg.hip)
+// order_out: store the finest level in the aggregates' locality order and return pos[old] = new
+// there (left empty for a one-level hierarchy); the hierarchy then takes and returns vectors
+// in that order
 Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o,
-                  const std::vector<double>* nullvec = nullptr, int nnull = 0);
+                  const std::vector<double>* nullvec = nullptr, int nnull = 0, std::vector<int>* order_out = nullptr);
 Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const double* val, const mk_solve_opts& o);
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip);
 void mg_apply32(mk_ctx* c, Mg* g, const float* b, float* x, const int* skip);
@@ -315,9 +318,11 @@ BlockResult block_pcg(mk_ctx* c, const mk_op* A, Mg* mg, int mw, const double* B
                       int maxiter, int check_every, bool fp64_vcycle = false);
 
 
-// values are multiplied by scale on upload
+// values are multiplied by scale on upload.  Renumbered operator (host arrays, either may be
+// null): stored row i is CSR row rsrc[i] and CSR column c is stored as cmap[c]; each row keeps
+// its entries in CSR order, so products sum in the same order as the unrenumbered operator.
 void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const double* val, int C, int sigma,
-                double scale = 1.0);
+                double scale = 1.0, const int* rsrc = nullptr, const int* cmap = nullptr);
 // y = A x (mode PLAIN); y = A x and partial x^H y (DOT); y = b - A x and partial |y|^2 (RESID)
 void spmv64(mk_ctx* c, const mk_op* A, const double* x, double* y, const double* b, SpmvMode mode, double* partials,
             int* nparts);

@@ -146,6 +146,12 @@ template <typename T, bool CPLX, typename G, typename P>
 __device__ __forceinline__ bool spmv_row_gp(const SellDev<T>& A, const G& g, int vb, int& row, T& ar, T& ai, P& pre) {
     ar = 0;
     ai = 0;
+    // the long rows' blocks come first: their wavefronts run the longest chains, so they start
+    // with the launch and overlap the SELL rows instead of trailing them
+    {
+        const int nlb = A.nvb - A.nb_sell - A.nb_mid;
+        vb = vb < nlb ? A.nb_sell + A.nb_mid + vb : vb - nlb;
+    }
     if (vb < A.nb_sell) {
         const int r = vb * blockDim.x + threadIdx.x;
         if (r >= A.n_short) return false;
@@ -208,17 +214,29 @@ __device__ __forceinline__ bool spmv_row_gp(const SellDev<T>& A, const G& g, int
     const int w = (vb - A.nb_sell - A.nb_mid) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w < A.nlong) {  // uniform inside the wavefront
-        for (int64_t k = A.long_ptr[w] + lane; k < A.long_ptr[w + 1]; k += 64) {
-            const int c = A.long_col[k];
-            T xr, xi;
-            g(c, xr, xi);
+        // a lane's entries k = lane, lane + 64, ...: eight gathers in flight, summed in order
+        const int64_t kend = A.long_ptr[w + 1];
+        int64_t k = A.long_ptr[w] + lane;
+        auto add = [&](int64_t q, T xr, T xi) {
             if constexpr (CPLX) {
-                const T vr = A.long_val[2 * k], vi = A.long_val[2 * k + 1];
+                const T vr = A.long_val[2 * q], vi = A.long_val[2 * q + 1];
                 ar += vr * xr - vi * xi;
                 ai += vr * xi + vi * xr;
             } else {
-                ar += A.long_val[k] * xr;
+                ar += A.long_val[q] * xr;
             }
+        };
+        for (; k + 7 * 64 < kend; k += 8 * 64) {
+            T xr[8], xi[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) g(A.long_col[k + 64 * u], xr[u], xi[u]);
+#pragma unroll
+            for (int u = 0; u < 8; u++) add(k + 64 * u, xr[u], xi[u]);
+        }
+        for (; k < kend; k += 64) {
+            T xr, xi;
+            g(A.long_col[k], xr, xi);
+            add(k, xr, xi);
         }
         ar = wave_sum_t(ar);
         if constexpr (CPLX) ai = wave_sum_t(ai);

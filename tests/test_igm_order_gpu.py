@@ -1,0 +1,74 @@
+"""The IGM's A0 system run in the multigrid aggregates' locality order on the device (the host
+keeps the caller's numbering) against the same system in the caller's order
+(METRIKO_IGM_ORDER=0).  Each renumbered operator row keeps its entries' order, so only the
+global reductions (dot products over all rows) sum in another order: the decisions must be
+the same (in the same order, up to swaps among candidates that are integral to round-off), and
+fullx / uv agree to the solver tolerance."""
+import os
+
+import numpy as np
+import pytest
+
+from metriko_amd.meshgen import bumpy_icosphere
+from oracle.pipeline import run_pipeline
+from tests.helpers import assert_decisions_in_order
+
+pytestmark = pytest.mark.gpu
+
+
+def _with_order(mode, fn):
+    old = os.environ.get("METRIKO_IGM_ORDER")
+    os.environ["METRIKO_IGM_ORDER"] = mode
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["METRIKO_IGM_ORDER"]
+        else:
+            os.environ["METRIKO_IGM_ORDER"] = old
+
+
+def test_device_order_matches_caller_order_small(gpu_ctx):
+    """bumpy L3 through mk_iterative_rounding with the order forced on (any size) and off:
+    the stats say which path ran; decisions equal and in order, fullx within 1e-9."""
+    V, F = bumpy_icosphere(3)
+    ref = run_pipeline(V, F, ratio=0.1)
+    ir = ref["rp"].ir
+
+    def run():
+        return gpu_ctx.iterative_rounding(ir.fixedIdcs, ir.fixedVals, ir.singularIdcs, ir.integerIdcs, ir.length,
+                                          ir.Cfull, ir.G2, ir.N, ir.n, ir.nF, ir.F2)
+
+    x0, ok0, dec0, st0 = _with_order("0", run)
+    x1, ok1, dec1, st1 = _with_order("2", run)
+    print(f"[order] k={ir.k}: device_order {st0['device_order']} / {st1['device_order']}, "
+          f"max |dx| {np.abs(x1 - x0).max():.3e}, decisions {len(dec0)}", flush=True)
+    assert st0["device_order"] == 0 and st1["device_order"] == 1
+    assert ok0 == ok1 == ref["success"]
+    assert [d[:2] for d in dec1] == [d[:2] for d in dec0]
+    assert_decisions_in_order(dec1, ref["decisions"])
+    np.testing.assert_allclose(x1, x0, atol=1e-9, rtol=0)
+
+
+def test_device_order_matches_caller_order_pipeline_l6(gpu_ctx):
+    """The benchmark's workload at level 6 (k = 81,922 unknowns: the order is on by default)
+    through the whole pipeline, against the caller's order."""
+    from bench import workload
+    import metriko_amd as mk
+    V, F, u0 = workload(6)
+    o = mk.opts(precision=mk.PREC_MIXED_MG)
+
+    def run():
+        return gpu_ctx.pipeline(V, F, ratio=0.03, u0=u0, field_opts=o, igm_opts=o)
+
+    a = _with_order("0", run)
+    b = run()
+    print(f"[order L6] device_order {a['info']['igm']['device_order']} / {b['info']['igm']['device_order']}, "
+          f"decisions {len(a['decisions'])}", flush=True)
+    assert a["info"]["igm"]["device_order"] == 0 and b["info"]["igm"]["device_order"] == 1
+    assert a["info"]["success"] == b["info"]["success"] == 1
+    # the same decisions; their order may differ inside a run of candidates that are integral
+    # to round-off (the first pick among them depends on the last bits)
+    assert sorted(b["decisions"]) == sorted(a["decisions"])
+    uva, uvb = np.asarray(a["uv"]), np.asarray(b["uv"])
+    assert np.abs(uvb - uva).max() <= 1e-8 * np.abs(uva).max()
