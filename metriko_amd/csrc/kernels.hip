@@ -586,7 +586,7 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     };
     // long rows (> max(64, 4 x median length)) go to the wavefront-per-row part
-    std::vector<int> lens(static_cast<size_t>(n));
+    metriko::HVec<int> lens(static_cast<size_t>(n));
 #pragma omp parallel for schedule(static, 65536) if (n > 100000)
     for (int i = 0; i < n; i++) lens[i] = int(rowptr[src(i) + 1] - rowptr[src(i)]);
     // median row length (the element at sorted position n/2) from a length histogram
@@ -623,14 +623,15 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     int tot3[3] = {0, 0, 0};
     for (int k3 = 0; k3 < 3; k3++)
         for (int ch = 0; ch < nch; ch++) { coff[size_t(ch) * 3 + k3] = tot3[k3]; tot3[k3] += ccount[size_t(ch) * 3 + k3]; }
-    std::vector<int> order(static_cast<size_t>(tot3[0])), mrows(static_cast<size_t>(tot3[1])),
-        lrows(static_cast<size_t>(tot3[2]));
+    metriko::HVec<int> order(static_cast<size_t>(tot3[0]));
+    std::vector<int> mrows(static_cast<size_t>(tot3[1])), lrows(static_cast<size_t>(tot3[2]));
 #pragma omp parallel for schedule(static) if (n > 100000)
     for (int ch = 0; ch < nch; ch++) {
         int w[3] = {coff[size_t(ch) * 3], coff[size_t(ch) * 3 + 1], coff[size_t(ch) * 3 + 2]};
         for (int i = ch * kChunk; i < std::min(n, (ch + 1) * kChunk); i++) {
             const int k3 = cls(i);
-            (k3 == 0 ? order : k3 == 1 ? mrows : lrows)[size_t(w[k3]++)] = i;
+            int* dst = k3 == 0 ? order.data() : k3 == 1 ? mrows.data() : lrows.data();
+            dst[w[k3]++] = i;
         }
     }
     const int ns_rows = int(order.size());
@@ -665,12 +666,12 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     for (int s = 0; s < ns; s++) sptr[s + 1] = sptr[s] + int64_t(sw[s]) * C;
     const int64_t stored = sptr[ns];
     const double t_layout = lap_ms();
-    // inverse diagonal (real part); rectangular operators keep 1
-    std::vector<double> dinv(n, 1.0);
-    std::vector<float> dinv32(n, 1.0f);
-#pragma omp parallel for schedule(static, 4096) if (n > 100000 && op->m == n)
-    for (int i = 0; i < n; i++) {
-        if (op->m != n) continue;
+    // inverse diagonal (real part) of a square operator; a rectangular one has none
+    const bool square = op->m == n;
+    metriko::HVec<double> dinv(square ? size_t(n) : 0);
+    metriko::HVec<float> dinv32(square ? size_t(n) : 0);
+#pragma omp parallel for schedule(static, 4096) if (n > 100000 && square)
+    for (int i = 0; i < (square ? n : 0); i++) {
         double d = 0.0;
         const int si = src(i);
         for (int64_t k = rowptr[si]; k < rowptr[si + 1]; k++) if (cm(col[k]) == i) d += scale * val[k * vw];
@@ -725,8 +726,13 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
     op->slice_ptr.upload(sptr.data(), sptr.size(), st);
     op->slice_w.upload(sw.data(), std::max<size_t>(sw.size(), 1), st);
     const size_t nst = size_t(std::max<int64_t>(stored, 1));
-    op->dinv64.upload(dinv.data(), dinv.size(), st);
-    op->dinv32.upload(dinv32.data(), dinv32.size(), st);
+    if (square) {
+        op->dinv64.upload(dinv.data(), dinv.size(), st);
+        op->dinv32.upload(dinv32.data(), dinv32.size(), st);
+    } else {
+        op->dinv64.release();
+        op->dinv32.release();
+    }
     if (permuted) op->perm.upload(order.data(), std::max<size_t>(order.size(), 1), st);
     else op->perm.release();
     // the SELL arrays are filled on the device from the CSR arrays (one upload of the CSR

@@ -569,14 +569,16 @@ struct HostHier {
     std::vector<std::vector<int>> key;
 };
 
-// Locality order of the finest level from the aggregates: the coarsest level keeps its order,
-// and every finer level lists its nodes by the rank of their aggregate's coarse unknown (nodes
-// of one aggregate in index order).  Aggregates are connected neighbourhoods, nested from level
-// to level, so rows that share columns end up close together, at every scale.  Returns
-// pos[old] = new for the finest level.
+// Locality order of every level from the aggregates: the coarsest level keeps its order, and
+// every finer level lists its nodes by the rank of their aggregate's coarse unknown (nodes of
+// one aggregate in index order).  Aggregates are connected neighbourhoods, nested from level to
+// level, so rows that share columns end up close together at every scale, and consecutive rows
+// of R and P (one level's aggregates in rank order) touch consecutive ranges of the finer
+// level.  Returns pos[l][old] = new for every level but the coarsest.
 template <typename T>
-std::vector<int> hier_order(const HostHier<T>& H) {
+std::vector<std::vector<int>> hier_order(const HostHier<T>& H) {
     const size_t L = H.key.size();
+    std::vector<std::vector<int>> pos(L);
     std::vector<int> rank(size_t(H.A.back().nr));
     std::iota(rank.begin(), rank.end(), 0);
     for (size_t l = L; l-- > 0;) {
@@ -589,9 +591,10 @@ std::vector<int> hier_order(const HostHier<T>& H) {
         for (int b = 0; b <= nc; b++) cnt[size_t(b) + 1] += cnt[size_t(b)];
         std::vector<int> r(static_cast<size_t>(n));
         for (int i = 0; i < n; i++) r[size_t(i)] = cnt[size_t(bucket(i))]++;
+        pos[l] = r;
         rank.swap(r);
     }
-    return rank;
+    return pos;
 }
 
 template <typename T>
@@ -767,28 +770,29 @@ HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector
     return H;
 }
 
-// order_out: when given (and the hierarchy has more than one level), the finest level is
-// stored in the locality order of hier_order -- its operator's rows and columns, P's rows, R's
-// (and [R | -R A]'s) columns and the smoother's diagonal -- and pos[old] = new is returned
-// there; the caller's vectors for this hierarchy are then in that order.  Each renumbered row
-// keeps its entries' order, so the V-cycle computes the same numbers as without it.
+// order_out: when given (and the hierarchy has more than one level), every level but the
+// coarsest is stored in the locality order of hier_order -- its operator's rows and columns,
+// P's rows and R's columns (and the next level's order for P's columns and R's rows) and the
+// smoother's diagonal -- and the finest level's pos[old] = new is returned there; the caller's
+// vectors for this hierarchy are then in that order.  Each renumbered row keeps its entries'
+// order, so every product computes the same numbers as without the renumbering.
 template <typename T>
 Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int coarse_max,
                const std::vector<T>* nullvec = nullptr, int nnull = 0, std::vector<int>* order_out = nullptr) {
     HostHier<T> H = build_host<T>(std::move(A0), theta, coarse_max, nullvec, nnull);
     const auto t_up = std::chrono::steady_clock::now();
-    std::vector<int> pos, inv, pos2;
+    // per level but the coarsest: pos (stored index of each unknown) and its inverse
+    std::vector<std::vector<int>> pos, inv;
     if (order_out && H.A.size() > 1) {
         pos = hier_order(H);
         inv.resize(pos.size());
-        for (size_t i = 0; i < pos.size(); i++) inv[size_t(pos[i])] = int(i);
-        // [R | -R A]: columns [0, n) and [n, 2n) are both finest-level unknowns
-        if (!H.RR.empty()) {
-            pos2.resize(pos.size() * 2);
-            for (size_t i = 0; i < pos.size(); i++) { pos2[i] = pos[i]; pos2[i + pos.size()] = pos[i] + int(pos.size()); }
+        for (size_t l = 0; l < pos.size(); l++) {
+            inv[l].resize(pos[l].size());
+            for (size_t i = 0; i < pos[l].size(); i++) inv[l][size_t(pos[l][i])] = int(i);
         }
     }
-    const bool ren = !pos.empty();
+    auto P_ = [&](size_t l) -> const int* { return l < pos.size() ? pos[l].data() : nullptr; };
+    auto I_ = [&](size_t l) -> const int* { return l < inv.size() ? inv[l].data() : nullptr; };
     auto* g = new Mg();
     try {
         g->cplx = !std::is_same<T, double>::value;
@@ -797,12 +801,11 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
             MgLevel L;
             L.n = H.A[l].nr;
             L.lmax = H.lmax[l];
-            const bool r0 = ren && l == 0;
-            L.A = upload(c, H.A[l], o, r0 ? inv.data() : nullptr, r0 ? pos.data() : nullptr);
+            L.A = upload(c, H.A[l], o, I_(l), P_(l));
             {  // the smoother's inverse diagonal replaces the plain one
                 std::vector<double> dv = H.dinv[l];
-                if (r0)
-                    for (size_t i = 0; i < dv.size(); i++) dv[size_t(pos[i])] = H.dinv[l][i];
+                if (P_(l))
+                    for (size_t i = 0; i < dv.size(); i++) dv[size_t(pos[l][i])] = H.dinv[l][i];
                 std::vector<float> dv32(dv.begin(), dv.end());
                 L.A->dinv64.upload(dv.data(), dv.size(), c->stream);
                 L.A->dinv32.upload(dv32.data(), dv32.size(), c->stream);
@@ -810,9 +813,18 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
             }
             g->nnz_total += H.A[l].nonZeros();
             if (l < H.P.size()) {
-                L.P = upload(c, H.P[l], o, r0 ? inv.data() : nullptr, nullptr);
-                L.R = upload(c, H.R[l], o, nullptr, r0 ? pos.data() : nullptr);
-                if (l < H.RR.size()) L.RR = upload(c, H.RR[l], o, nullptr, r0 ? pos2.data() : nullptr);
+                L.P = upload(c, H.P[l], o, I_(l), P_(l + 1));
+                L.R = upload(c, H.R[l], o, I_(l + 1), P_(l));
+                if (l < H.RR.size()) {
+                    // [R | -R A]: columns [0, n) and [n, 2n) are both this level's unknowns
+                    std::vector<int> pos2;
+                    if (P_(l)) {
+                        const size_t n = pos[l].size();
+                        pos2.resize(2 * n);
+                        for (size_t i = 0; i < n; i++) { pos2[i] = pos[l][i]; pos2[i + n] = pos[l][i] + int(n); }
+                    }
+                    L.RR = upload(c, H.RR[l], o, I_(l + 1), pos2.empty() ? nullptr : pos2.data());
+                }
             }
             g->lv.push_back(std::move(L));
         }
@@ -832,7 +844,10 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
         mg_free(g);
         throw;
     }
-    if (order_out) order_out->swap(pos);
+    if (order_out) {
+        order_out->clear();
+        if (!pos.empty()) order_out->swap(pos[0]);
+    }
     return g;
 }
 

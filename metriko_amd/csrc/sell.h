@@ -214,7 +214,10 @@ __device__ __forceinline__ bool spmv_row_gp(const SellDev<T>& A, const G& g, int
     const int w = (vb - A.nb_sell - A.nb_mid) * int(blockDim.x >> 6) + int(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (w < A.nlong) {  // uniform inside the wavefront
-        // a lane's entries k = lane, lane + 64, ...: eight gathers in flight, summed in order
+        // a lane's entries k = lane, lane + 64, ...: U gathers in flight, summed in order (two
+        // for complex values, whose registers would otherwise lower the occupancy of the whole
+        // kernel, long rows or not)
+        constexpr int U = CPLX ? 2 : 8;
         const int64_t kend = A.long_ptr[w + 1];
         int64_t k = A.long_ptr[w] + lane;
         auto add = [&](int64_t q, T xr, T xi) {
@@ -226,12 +229,12 @@ __device__ __forceinline__ bool spmv_row_gp(const SellDev<T>& A, const G& g, int
                 ar += A.long_val[q] * xr;
             }
         };
-        for (; k + 7 * 64 < kend; k += 8 * 64) {
-            T xr[8], xi[8];
+        for (; k + (U - 1) * 64 < kend; k += U * 64) {
+            T xr[U], xi[U];
 #pragma unroll
-            for (int u = 0; u < 8; u++) g(A.long_col[k + 64 * u], xr[u], xi[u]);
+            for (int u = 0; u < U; u++) g(A.long_col[k + 64 * u], xr[u], xi[u]);
 #pragma unroll
-            for (int u = 0; u < 8; u++) add(k + 64 * u, xr[u], xi[u]);
+            for (int u = 0; u < U; u++) add(k + 64 * u, xr[u], xi[u]);
         }
         for (; k < kend; k += 64) {
             T xr, xi;

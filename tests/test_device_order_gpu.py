@@ -16,16 +16,20 @@ from tests.helpers import assert_decisions_in_order
 pytestmark = pytest.mark.gpu
 
 
-def _with_order(mode, fn):
-    old = os.environ.get("METRIKO_IGM_ORDER")
-    os.environ["METRIKO_IGM_ORDER"] = mode
+def _with_env(name, mode, fn):
+    old = os.environ.get(name)
+    os.environ[name] = mode
     try:
         return fn()
     finally:
         if old is None:
-            del os.environ["METRIKO_IGM_ORDER"]
+            del os.environ[name]
         else:
-            os.environ["METRIKO_IGM_ORDER"] = old
+            os.environ[name] = old
+
+
+def _with_order(mode, fn):
+    return _with_env("METRIKO_IGM_ORDER", mode, fn)
 
 
 def test_device_order_matches_caller_order_small(gpu_ctx):
@@ -72,3 +76,4 @@ def test_device_order_matches_caller_order_pipeline_l6(gpu_ctx):
     assert sorted(b["decisions"]) == sorted(a["decisions"])
     uva, uvb = np.asarray(a["uv"]), np.asarray(b["uv"])
     assert np.abs(uvb - uva).max() <= 1e-8 * np.abs(uva).max()
+
