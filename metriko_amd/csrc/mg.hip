@@ -693,8 +693,10 @@ HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector
             // exact near-null vectors, of long columns)
             std::vector<uint8_t> lng = long_rows(A);
             auto keep = [&](int i, int j) { return j == i || !(lng[i] || (lng[j] && !keep_long)); };
-            Csr<T> Af(A.nr, A.nc);
-            {
+            // without long rows Af is A itself
+            const bool any_long = std::find(lng.begin(), lng.end(), uint8_t(1)) != lng.end();
+            Csr<T> Af(any_long ? A.nr : 0, any_long ? A.nc : 0);
+            if (any_long) {
                 auto& np = Af.ptr;
 #pragma omp parallel for schedule(static, 4096)
                 for (int i = 0; i < A.nr; i++) {
@@ -713,25 +715,49 @@ HostHier<T> build_host(Csr<T> A, double theta, int coarse_max, const std::vector
                 }
             }
             lap("filtered operator");
-            Csr<T> AP = metriko::spgemm(Af, Pt);
+            Csr<T> AP = metriko::spgemm(any_long ? Af : A, Pt);
             lap("A Pt");
-            // P = Pt + rows of A Pt scaled by -omega / ds_i (long rows: none)
-            Csr<T> APs(A.nr, AP.nc);
-            for (int i = 0; i < A.nr; i++)
-                APs.ptr[size_t(i) + 1] = APs.ptr[size_t(i)] + (lng[i] ? 0 : AP.ptr[i + 1] - AP.ptr[i]);
-            APs.col.resize(size_t(APs.ptr[size_t(A.nr)]));
-            APs.val.resize(size_t(APs.ptr[size_t(A.nr)]));
+            // P = Pt + rows of A Pt scaled by -omega / ds_i (long rows: Pt's row alone), each row
+            // the merge of the two sorted rows: an entry in both is Pt + s, s = (-omega / ds_i) (A Pt)
+            Csr<T> P(A.nr, Pt.nc);
+            {
+                auto merge = [&](int i, int64_t w, bool fill) {
+                    int64_t p = Pt.ptr[i], q = lng[i] ? AP.ptr[i + 1] : AP.ptr[i], n = 0;
+                    const int64_t pe = Pt.ptr[i + 1], qe = AP.ptr[i + 1];
+                    const double di = ds[i] != 0.0 ? ds[i] : 1.0;
+                    while (p < pe || q < qe) {
+                        if (q >= qe || (p < pe && Pt.col[p] < AP.col[q])) {
+                            if (fill) { P.col[size_t(w)] = Pt.col[p]; P.val[size_t(w)] = Pt.val[p]; }
+                            p++;
+                        } else if (p >= pe || AP.col[q] < Pt.col[p]) {
+                            if (fill) {
+                                const T sv = -omega / di * AP.val[q];
+                                P.col[size_t(w)] = AP.col[q];
+                                P.val[size_t(w)] = sv;
+                            }
+                            q++;
+                        } else {
+                            if (fill) {
+                                const T sv = -omega / di * AP.val[q];
+                                P.col[size_t(w)] = Pt.col[p];
+                                P.val[size_t(w)] = Pt.val[p] + sv;
+                            }
+                            p++;
+                            q++;
+                        }
+                        w++;
+                        n++;
+                    }
+                    return n;
+                };
 #pragma omp parallel for schedule(static, 4096)
-            for (int i = 0; i < A.nr; i++) {
-                if (lng[i]) continue;
-                const double di = ds[i] != 0.0 ? ds[i] : 1.0;
-                int64_t w = APs.ptr[size_t(i)];
-                for (int64_t k = AP.ptr[i]; k < AP.ptr[i + 1]; k++, w++) {
-                    APs.col[size_t(w)] = AP.col[k];
-                    APs.val[size_t(w)] = -omega / di * AP.val[k];
-                }
+                for (int i = 0; i < A.nr; i++) P.ptr[size_t(i) + 1] = merge(i, 0, false);
+                for (int i = 0; i < A.nr; i++) P.ptr[size_t(i) + 1] += P.ptr[size_t(i)];
+                P.col.resize(size_t(P.ptr[size_t(A.nr)]));
+                P.val.resize(size_t(P.ptr[size_t(A.nr)]));
+#pragma omp parallel for schedule(static, 4096)
+                for (int i = 0; i < A.nr; i++) merge(i, P.ptr[size_t(i)], true);
             }
-            Csr<T> P = metriko::add(Pt, T(1), APs, T(1));
             lap("smoothed P");
             Csr<T> R = conj_transpose(P);
             lap("R");
