@@ -357,11 +357,29 @@ struct CsrView {
 // products: expanded, sorted by column (stable), duplicates summed in order and exact zeros
 // dropped (the SpGEMM + pruned() result).  Two OpenMP passes (count, fill); a G2 column out
 // of range is reported as MK_ERR_ARG.
-SprsD x2f_product(const CsrView& G2, const SprsD& UF) {
+SprsD x2f_product(const CsrView& G2, const SprsD& UF, int npn) {
     const int rows = G2.nr;
     SprsD X(rows, UF.cols());
     X.ptr.assign(size_t(rows) + 1, 0);
     std::atomic<int> bad{0};
+    // the unconstrained DOFs' UFull rows are single entries onto distinct, ascending columns
+    // [0, npn): a G2 row that touches only those is relabelled in place (its order is kept, its
+    // products are the ones the merge would form, exact zeros dropped as the merge does)
+    std::vector<int> one(static_cast<size_t>(G2.nc));
+#pragma omp parallel for schedule(static, 65536) if (G2.nc > 100000)
+    for (int c = 0; c < G2.nc; c++) {
+        const int64_t b = UF.ptr[c];
+        one[size_t(c)] = (UF.ptr[c + 1] - b == 1 && UF.col[b] < npn) ? UF.col[b] : -1;
+    }
+    auto simple = [&](int r) {  // only such columns, in ascending order
+        int prev = -1;
+        for (int64_t q = G2.ptr[r]; q < G2.ptr[r + 1]; q++) {
+            const int c = G2.col[q];
+            if (c < 0 || c >= G2.nc || one[size_t(c)] <= prev) return false;
+            prev = one[size_t(c)];
+        }
+        return true;
+    };
     auto expand = [&](int r, std::vector<std::pair<int, double>>& buf) {
         buf.clear();
         for (int64_t q = G2.ptr[r]; q < G2.ptr[r + 1]; q++) {
@@ -389,6 +407,13 @@ SprsD x2f_product(const CsrView& G2, const SprsD& UF) {
         std::vector<std::pair<int, double>> buf;
 #pragma omp for schedule(static, 4096)
         for (int r = 0; r < rows; r++) {
+            if (simple(r)) {
+                int64_t m = 0;
+                for (int64_t q = G2.ptr[r]; q < G2.ptr[r + 1]; q++)
+                    m += (G2.val[q] * UF.val[UF.ptr[G2.col[q]]] != 0.0) ? 1 : 0;
+                X.ptr[size_t(r) + 1] = m;
+                continue;
+            }
             expand(r, buf);
             X.ptr[size_t(r) + 1] = int64_t(buf.size());
         }
@@ -402,8 +427,16 @@ SprsD x2f_product(const CsrView& G2, const SprsD& UF) {
         std::vector<std::pair<int, double>> buf;
 #pragma omp for schedule(static, 4096)
         for (int r = 0; r < rows; r++) {
-            expand(r, buf);
             int64_t w = X.ptr[size_t(r)];
+            if (simple(r)) {
+                for (int64_t q = G2.ptr[r]; q < G2.ptr[r + 1]; q++) {
+                    const int c = G2.col[q];
+                    const double v = G2.val[q] * UF.val[UF.ptr[c]];
+                    if (v != 0.0) { X.col[size_t(w)] = one[size_t(c)]; X.val[size_t(w)] = v; w++; }
+                }
+                continue;
+            }
+            expand(r, buf);
             for (const auto& e : buf) { X.col[size_t(w)] = e.first; X.val[size_t(w)] = e.second; w++; }
         }
     }
@@ -1503,7 +1536,7 @@ extern "C" mk_status mk_iterative_rounding(
             UF.setFromTriplets(T);
         }
         MK_LOG("[igm] UFull %d x %d nnz %lld, %.0f ms\n", nfull, k, (long long)UF.nonZeros(), now_ms() - t_all);
-        SprsD X2F = x2f_product(G2, UF);     // G2 * UFull (naive, without length)
+        SprsD X2F = x2f_product(G2, UF, npn);  // G2 * UFull (naive, without length)
         MK_LOG("[igm] X2F nnz %lld, %.0f ms\n", (long long)X2F.nonZeros(), now_ms() - t_all);
         SprsD X2Ft = X2F.transpose();
         MK_LOG("[igm] X2F^T %.0f ms\n", now_ms() - t_all);
