@@ -106,6 +106,12 @@ namespace metriko {
         Csr pruned() const { Csr c = *this; c.prune(); return c; }
 
         Csr transpose() const {
+            // offsets fit 32 bits for all but huge matrices: half the count arrays' traffic
+            if (nonZeros() < (int64_t(1) << 31)) return transpose_impl<int>();
+            return transpose_impl<int64_t>();
+        }
+        template<typename I>
+        Csr transpose_impl() const {
             Csr t(nc, nr);
             const int64_t nz = nonZeros();
             t.col.resize(size_t(nz));
@@ -116,11 +122,11 @@ namespace metriko {
 #ifdef _OPENMP
             nch = std::max(1, std::min(omp_get_max_threads(), int(nz / 200000) + 1));
 #endif
-            std::vector<std::vector<int64_t>> cc(static_cast<size_t>(nch));
+            std::vector<std::vector<I>> cc(static_cast<size_t>(nch));
             const int step = (nr + nch - 1) / std::max(nch, 1);
 #pragma omp parallel for schedule(static, 1)
             for (int q = 0; q < nch; q++) {
-                cc[size_t(q)].assign(size_t(nc), 0);
+                cc[size_t(q)].assign(size_t(nc), I(0));
                 const int r1 = std::min(nr, (q + 1) * step);
                 for (int i = q * step; i < r1; i++)
                     for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) cc[size_t(q)][size_t(col[k])]++;
@@ -145,13 +151,13 @@ namespace metriko {
                 int64_t s = rtot[size_t(q)];
                 const int c1 = std::min(nc, (q + 1) * cstep);
                 for (int j = q * cstep; j < c1; j++) {
-                    for (int p = 0; p < nch; p++) { const int64_t x = cc[size_t(p)][size_t(j)]; cc[size_t(p)][size_t(j)] = s; s += x; }
+                    for (int p = 0; p < nch; p++) { const int64_t x = cc[size_t(p)][size_t(j)]; cc[size_t(p)][size_t(j)] = I(s); s += x; }
                     t.ptr[size_t(j) + 1] = s;
                 }
             }
 #pragma omp parallel for schedule(static, 1)
             for (int q = 0; q < nch; q++) {
-                std::vector<int64_t>& pos = cc[size_t(q)];
+                std::vector<I>& pos = cc[size_t(q)];
                 const int r1 = std::min(nr, (q + 1) * step);
                 for (int i = q * step; i < r1; i++)
                     for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) {
