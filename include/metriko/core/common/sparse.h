@@ -16,6 +16,17 @@
 #include "typedef.h"
 
 namespace metriko {
+    // Product of two values; complex ones by the textbook formula, inline.  For finite operands
+    // this is the value of the compiler runtime's complex multiply, which adds only the
+    // NaN/infinity recovery and costs a call per product.
+    template<typename T>
+    inline T mulv(const T& a, const T& b) { return a * b; }
+    inline std::complex<double> mulv(const std::complex<double>& a, const std::complex<double>& b) {
+        const double ac = a.real() * b.real(), bd = a.imag() * b.imag();
+        const double ad = a.real() * b.imag(), bc = a.imag() * b.real();
+        return {ac - bd, ad + bc};
+    }
+
     template<typename T>
     struct Triplet {
         int r, c;
@@ -173,7 +184,7 @@ namespace metriko {
 #pragma omp parallel for schedule(static, 4096) if (nonZeros() > 200000)
             for (int i = 0; i < nr; i++) {
                 T s = T();
-                for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) s += val[k] * x[col[k]];
+                for (int64_t k = ptr[i]; k < ptr[i + 1]; k++) s += mulv(val[k], x[col[k]]);
                 y[i] = s;
             }
         }
@@ -260,7 +271,7 @@ namespace metriko {
                     for (int64_t ka = A.ptr[i]; ka < A.ptr[i + 1]; ka++) {
                         const int j = A.col[ka];
                         const T a = A.val[ka];
-                        for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) pr.emplace_back(B.col[kb], a * B.val[kb]);
+                        for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) pr.emplace_back(B.col[kb], mulv(a, B.val[kb]));
                     }
                     for (size_t q = 1; q < pr.size(); q++)  // stable insertion sort by column
                         for (size_t r = q; r > 0 && pr[r - 1].first > pr[r].first; r--) std::swap(pr[r - 1], pr[r]);
@@ -277,8 +288,8 @@ namespace metriko {
                     const T a = A.val[ka];
                     for (int64_t kb = B.ptr[j]; kb < B.ptr[j + 1]; kb++) {
                         const int c = B.col[kb];
-                        if (mark[c] != i) { mark[c] = i; acc[c] = a * B.val[kb]; cols.push_back(c); }
-                        else acc[c] += a * B.val[kb];
+                        if (mark[c] != i) { mark[c] = i; acc[c] = mulv(a, B.val[kb]); cols.push_back(c); }
+                        else acc[c] += mulv(a, B.val[kb]);
                     }
                 }
                 std::sort(cols.begin(), cols.end());

@@ -464,10 +464,19 @@ std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD&
     if (nint > 4096) return {};
     struct SRow { const CsrView* A; int row; };
     std::vector<SRow> trows;
-    for (const CsrView* A : {&G2, &Cf})
-        for (int i = 0; i < A->rows(); i++)
-            for (int64_t q = A->ptr[i]; q < A->ptr[i + 1]; q++)
-                if (tpos[A->col[q]] >= 0) { trows.push_back({A, i}); break; }
+    for (const CsrView* A : {&G2, &Cf}) {
+        // scanned in row chunks, each chunk's hits appended in chunk order: rows stay ascending
+        constexpr int kCh = 64;
+        const int rows = A->rows(), step = (rows + kCh - 1) / kCh;
+        std::vector<std::vector<int>> hit(kCh);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int ch = 0; ch < kCh; ch++)
+            for (int i = ch * step; i < std::min(rows, (ch + 1) * step); i++)
+                for (int64_t q = A->ptr[i]; q < A->ptr[i + 1]; q++)
+                    if (tpos[A->col[q]] >= 0) { hit[size_t(ch)].push_back(i); break; }
+        for (const std::vector<int>& h : hit)
+            for (int i : h) trows.push_back({A, i});
+    }
     const size_t nt = size_t(std::max(nint, 1));
     std::vector<double> Mt(nt * nt, 0.0);  // sum over those rows of the outer product of transition entries
     for (const SRow& t : trows) {
@@ -1488,7 +1497,10 @@ extern "C" mk_status mk_iterative_rounding(
         for (int i = 0; i < nint; i++) MK_CHECK(integerIdcs[i] >= 0 && integerIdcs[i] < nfull, MK_ERR_ARG, "integerIdcs");
         // ---- host setup (iter_rounding_init.h:195-251) ----
         MK_CHECK(g_rows >= 0 && g_cols > 0 && g_ptr && g_ptr[0] == 0, MK_ERR_ARG, "G2");
-        for (int i = 0; i < g_rows; i++) MK_CHECK(g_ptr[i + 1] >= g_ptr[i], MK_ERR_ARG, "G2 row pointers");
+        int64_t g_bad = 0;
+#pragma omp parallel for reduction(+ : g_bad) schedule(static) if (g_rows > 100000)
+        for (int i = 0; i < g_rows; i++) g_bad += g_ptr[i + 1] < g_ptr[i] ? 1 : 0;
+        MK_CHECK(g_bad == 0, MK_ERR_ARG, "G2 row pointers");
         const CsrView G2{g_rows, g_cols, g_ptr, g_col, g_val};  // the caller's arrays, in place
         SprsD Cf(c_rows, c_cols);
         if (c_rows > 0) {
@@ -1527,13 +1539,23 @@ extern "C" mk_status mk_iterative_rounding(
         // UFull = permMat * blockdiag(I, USmall): row nonPart[i] -> e_i, row uniqueJ[r] -> USmall row r
         SprsD UF(nfull, k);
         {
-            std::vector<TripD> T;
-            T.reserve(size_t(npn) + size_t(US.rows()) * size_t(dimker));
-            for (int i = 0; i < npn; i++) T.emplace_back(nonPart[i], i, 1.0);
-            for (int r = 0; r < US.rows(); r++)
+            // written row by row: the nonPart rows (jmask < 0, ascending) take columns 0, 1, ...;
+            // row uniqueJ[r] takes USmall row r's nonzeros at columns npn + j, ascending
+            int64_t nz = 0;
+            for (int i = 0; i < nfull; i++) {
+                if (jmask[i] < 0) nz++;
+                else for (int j = 0; j < dimker; j++) nz += US(jmask[i], j) != 0.0 ? 1 : 0;
+                UF.ptr[size_t(i) + 1] = nz;
+            }
+            UF.col.resize(size_t(nz));
+            UF.val.resize(size_t(nz));
+            int np = 0;
+            for (int i = 0; i < nfull; i++) {
+                int64_t w = UF.ptr[size_t(i)];
+                if (jmask[i] < 0) { UF.col[size_t(w)] = np++; UF.val[size_t(w)] = 1.0; continue; }
                 for (int j = 0; j < dimker; j++)
-                    if (US(r, j) != 0.0) T.emplace_back(uniqueJ[r], npn + j, US(r, j));
-            UF.setFromTriplets(T);
+                    if (US(jmask[i], j) != 0.0) { UF.col[size_t(w)] = npn + j; UF.val[size_t(w)] = US(jmask[i], j); w++; }
+            }
         }
         MK_LOG("[igm] UFull %d x %d nnz %lld, %.0f ms\n", nfull, k, (long long)UF.nonZeros(), now_ms() - t_all);
         SprsD X2F = x2f_product(G2, UF, npn);  // G2 * UFull (naive, without length)
@@ -1542,7 +1564,8 @@ extern "C" mk_status mk_iterative_rounding(
         MK_LOG("[igm] X2F^T %.0f ms\n", now_ms() - t_all);
         SprsD E = X2Ft * X2F;
         MK_LOG("[igm] E nnz %lld, %.0f ms\n", (long long)E.nonZeros(), now_ms() - t_all);
-        for (auto& v : E.val) v *= length;   // E = X2F^T X2F * length
+#pragma omp parallel for schedule(static) if (E.val.size() > 200000)
+        for (int64_t q = 0; q < int64_t(E.val.size()); q++) E.val[size_t(q)] *= length;   // E = X2F^T X2F * length
         std::vector<double> fr(static_cast<size_t>(k));
         X2Ft.multiply(F2, fr.data());        // f = X2F^T F2
         std::vector<int> fixIdx(fixedIdcs, fixedIdcs + nfixed);
@@ -1558,8 +1581,14 @@ extern "C" mk_status mk_iterative_rounding(
         double t0 = now_ms();
         MK_LOG("[igm] KKT: start\n");
         double dmean = 0.0;
-        for (int i = 0; i < k; i++)
-            for (int64_t q = E.ptr[i]; q < E.ptr[i + 1]; q++) if (E.col[q] == i) dmean += E.val[q];
+        {
+            // diagonal entries found in parallel, summed in row order
+            std::vector<double> dg(size_t(k), 0.0);
+#pragma omp parallel for schedule(static, 4096) if (k > 100000)
+            for (int i = 0; i < k; i++)
+                for (int64_t q = E.ptr[i]; q < E.ptr[i + 1]; q++) if (E.col[q] == i) dg[size_t(i)] += E.val[q];
+            for (int i = 0; i < k; i++) dmean += dg[size_t(i)];
+        }
         dmean = k ? dmean / k : 1.0;
         if (!(dmean > 0.0)) dmean = 1.0;
         // Eaug = E + dmean Cm^T Cm: the penalty's few entries added to a copy of E's values
@@ -1579,7 +1608,9 @@ extern "C" mk_status mk_iterative_rounding(
                     else adds.emplace_back(int64_t(f - E.col.data()), Pen.val[q]);
                 }
             if (in_pattern) {
-                eaug_val = E.val;
+                eaug_val.resize(E.val.size());
+#pragma omp parallel for schedule(static) if (E.val.size() > 200000)
+                for (int64_t q = 0; q < int64_t(E.val.size()); q++) eaug_val[size_t(q)] = E.val[size_t(q)];
                 for (const auto& a : adds) eaug_val[size_t(a.first)] = 1.0 * eaug_val[size_t(a.first)] + 1.0 * a.second;
             } else {
                 EaugSum = E + Pen;

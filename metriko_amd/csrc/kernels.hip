@@ -688,32 +688,42 @@ void build_sell(mk_op* op, const int64_t* rowptr, const int32_t* col, const doub
         const int mm = ml[size_t(nmid / 2)];
         while (mlanes < 16 && mlanes * 4 < mm) mlanes *= 2;
     }
-    std::vector<int64_t> mptr(size_t(nmid) + 1, 0);
-    std::vector<int> mcol;
-    std::vector<double> mval;
-    std::vector<float> mval32;
-    for (int w = 0; w < nmid; w++) {
-        const int row = src(mrows[w]);
-        for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
-            mcol.push_back(cm(col[k]));
-            for (int c = 0; c < vw; c++) { mval.push_back(scale * val[k * vw + c]); mval32.push_back(float(scale * val[k * vw + c])); }
+    // medium and long rows as CSR, in their class order (offsets first, then a parallel fill)
+    auto fill_csr = [&](const std::vector<int>& rows, std::vector<int64_t>& rp, metriko::HVec<int>& cc,
+                        metriko::HVec<double>& cv, metriko::HVec<float>& cv32) {
+        const int nr = int(rows.size());
+        rp.assign(size_t(nr) + 1, 0);
+        for (int w = 0; w < nr; w++) rp[size_t(w) + 1] = rp[size_t(w)] + lens[rows[size_t(w)]];
+        const int64_t nz = rp[size_t(nr)];
+        cc.resize(size_t(nz));
+        cv.resize(size_t(nz) * vw);
+        cv32.resize(size_t(nz) * vw);
+#pragma omp parallel for schedule(static, 256) if (nz > 200000)
+        for (int w = 0; w < nr; w++) {
+            const int row = src(rows[size_t(w)]);
+            int64_t o = rp[size_t(w)];
+            for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++, o++) {
+                cc[size_t(o)] = cm(col[k]);
+                for (int c = 0; c < vw; c++) {
+                    const double v = scale * val[k * vw + c];
+                    cv[size_t(o) * vw + c] = v;
+                    cv32[size_t(o) * vw + c] = float(v);
+                }
+            }
         }
-        mptr[w + 1] = int64_t(mcol.size());
-    }
+    };
+    std::vector<int64_t> mptr;
+    metriko::HVec<int> mcol;
+    metriko::HVec<double> mval;
+    metriko::HVec<float> mval32;
+    fill_csr(mrows, mptr, mcol, mval, mval32);
     // long-row part (CSR)
     const int nlong = int(lrows.size());
-    std::vector<int64_t> lptr(size_t(nlong) + 1, 0);
-    std::vector<int> lcol;
-    std::vector<double> lval;
-    std::vector<float> lval32;
-    for (int w = 0; w < nlong; w++) {
-        const int row = src(lrows[w]);
-        for (int64_t k = rowptr[row]; k < rowptr[row + 1]; k++) {
-            lcol.push_back(cm(col[k]));
-            for (int c = 0; c < vw; c++) { lval.push_back(scale * val[k * vw + c]); lval32.push_back(float(scale * val[k * vw + c])); }
-        }
-        lptr[w + 1] = int64_t(lcol.size());
-    }
+    std::vector<int64_t> lptr;
+    metriko::HVec<int> lcol;
+    metriko::HVec<double> lval;
+    metriko::HVec<float> lval32;
+    fill_csr(lrows, lptr, lcol, lval, lval32);
     // keep every uploaded array non-empty (an operator may consist of long rows only)
     if (sw.empty()) sw.push_back(0);
     if (order.empty()) order.push_back(0);

@@ -110,7 +110,11 @@ inline double re(const cd& a) { return a.real(); }
 template <typename T>
 Csr<T> conj_transpose(const Csr<T>& A) {
     Csr<T> t = A.transpose();
-    for (auto& v : t.val) v = conjv(v);
+    if constexpr (!std::is_same<T, double>::value) {
+        const int64_t nz = t.nonZeros();
+#pragma omp parallel for schedule(static) if (nz > 200000)
+        for (int64_t k = 0; k < nz; k++) t.val[size_t(k)] = conjv(t.val[size_t(k)]);
+    }
     return t;
 }
 
@@ -437,8 +441,8 @@ Csr<T> tentative_qr(const Csr<T>& A, const std::vector<double>& d, double theta,
                 for (int pass = 0; pass < 2; pass++)
                     for (int q = 0; q < r; q++) {
                         T s = T(0);
-                        for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * v[i];
-                        for (int i = 0; i < m; i++) v[i] -= s * Q[size_t(q) * m + i];
+                        for (int i = 0; i < m; i++) s += metriko::mulv(conjv(Q[size_t(q) * m + i]), v[i]);
+                        for (int i = 0; i < m; i++) v[i] -= metriko::mulv(s, Q[size_t(q) * m + i]);
                     }
                 double nv = 0.0;
                 for (int i = 0; i < m; i++) nv += mag2(v[i]);
@@ -455,7 +459,7 @@ Csr<T> tentative_qr(const Csr<T>& A, const std::vector<double>& d, double theta,
             for (int q = 0; q < r; q++)
                 for (int c = 0; c < nb; c++) {
                     T s = T(0);
-                    for (int i = 0; i < m; i++) s += conjv(Q[size_t(q) * m + i]) * B[size_t(c) * n + rows[i]];
+                    for (int i = 0; i < m; i++) s += metriko::mulv(conjv(Q[size_t(q) * m + i]), B[size_t(c) * n + rows[i]]);
                     R[size_t(q) * nb + c] = s;
                 }
             rcount[size_t(a)] = r;
@@ -817,6 +821,11 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
             for (size_t i = 0; i < pos[l].size(); i++) inv[l][size_t(pos[l][i])] = int(i);
         }
     }
+    const bool verbose = std::getenv("METRIKO_VERBOSE") != nullptr;
+    auto since_up = [&]() {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_up).count();
+    };
+    if (verbose) std::fprintf(stderr, "[mg]   order %.0f ms\n", since_up());
     auto P_ = [&](size_t l) -> const int* { return l < pos.size() ? pos[l].data() : nullptr; };
     auto I_ = [&](size_t l) -> const int* { return l < inv.size() ? inv[l].data() : nullptr; };
     auto* g = new Mg();
@@ -853,12 +862,14 @@ Mg* build_impl(mk_ctx* c, Csr<T>&& A0, const mk_solve_opts& o, double theta, int
                 }
             }
             g->lv.push_back(std::move(L));
+            if (verbose) std::fprintf(stderr, "[mg]   level %zu uploaded %.0f ms\n", l, since_up());
         }
         g->nc = H.nc;
         g->inv64.upload(H.inv.data(), H.inv.size(), c->stream);
         std::vector<float> inv32(H.inv.begin(), H.inv.end());
         g->inv32.upload(inv32.data(), inv32.size(), c->stream);
         MK_HIP(hipStreamSynchronize(c->stream));
+        if (verbose) std::fprintf(stderr, "[mg]   coarsest inverse uploaded %.0f ms\n", since_up());
         for (auto& L : g->lv) {
             const size_t len = size_t(L.n) * (g->cplx ? 2 : 1);
             L.b64.alloc(len); L.x64.alloc(len); L.r64.alloc(len); L.d64.alloc(len); L.y64.alloc(len);
@@ -1277,8 +1288,13 @@ Mg* mg_build_real(mk_ctx* c, int n, const int64_t* ptr, const int* col, const do
                   const std::vector<double>* nullvec, int nnull, std::vector<int>* order_out) {
     Csr<double> A(n, n);
     A.ptr.assign(ptr, ptr + n + 1);
-    A.col.assign(col, col + ptr[n]);
-    A.val.assign(val, val + ptr[n]);
+    A.col.resize(size_t(ptr[n]));
+    A.val.resize(size_t(ptr[n]));
+#pragma omp parallel for schedule(static) if (ptr[n] > 200000)
+    for (int64_t k = 0; k < ptr[n]; k++) {
+        A.col[size_t(k)] = col[k];
+        A.val[size_t(k)] = val[k];
+    }
     Mg* g = build_impl<double>(c, std::move(A), o, tune().theta_real, tune().coarse_real, nullvec, nnull, order_out);
     return g;
 }
@@ -1286,9 +1302,13 @@ Mg* mg_build_complex(mk_ctx* c, int n, const int64_t* ptr, const int* col, const
                      const mk_solve_opts& o) {
     Csr<cd> A(n, n);
     A.ptr.assign(ptr, ptr + n + 1);
-    A.col.assign(col, col + ptr[n]);
+    A.col.resize(size_t(ptr[n]));
     A.val.resize(size_t(ptr[n]));
-    for (int64_t k = 0; k < ptr[n]; k++) A.val[k] = cd(val[2 * k], val[2 * k + 1]);
+#pragma omp parallel for schedule(static) if (ptr[n] > 200000)
+    for (int64_t k = 0; k < ptr[n]; k++) {
+        A.col[size_t(k)] = col[k];
+        A.val[size_t(k)] = cd(val[2 * k], val[2 * k + 1]);
+    }
     return build_impl<cd>(c, std::move(A), o, tune().theta_cplx, tune().coarse_cplx);
 }
 void mg_apply64(mk_ctx* c, Mg* g, const double* b, double* x, const int* skip) { vcycle<double>(c, g, 0, b, x, skip); }
