@@ -150,11 +150,23 @@ template <typename T>
 double rho_dinv_a(const Csr<T>& A, const std::vector<double>& d) {
     const int n = A.nr;
     std::mt19937_64 g(12345);
-    std::uniform_real_distribution<double> U(-1.0, 1.0);
+    // uniform on [-1, 1): the engine's 64 bits scaled by 2^-64 (values that round to 1 take the
+    // largest double below it), times 2, minus 1 -- the value uniform_real_distribution<double>
+    // gives for a 64-bit engine, without its per-draw long-double set-up
+    auto U = [&g]() {
+        double u = double(g()) * 0x1p-64;
+        if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+        return u * 2.0 + -1.0;
+    };
     metriko::HVec<T> x(static_cast<size_t>(n)), y(static_cast<size_t>(n));
     for (auto& v : x) {
-        if constexpr (std::is_same<T, double>::value) v = U(g);
-        else v = cd(U(g), U(g));
+        if constexpr (std::is_same<T, double>::value) {
+            v = U();
+        } else {
+            const double re = U();
+            const double im = U();
+            v = cd(re, im);
+        }
     }
     double lam = 1.0;
     for (int it = 0; it < tune().power; it++) {
