@@ -351,8 +351,8 @@ void ensure_mg(mk_ctx* c, const mk_op* A_, const mk_solve_opts& o) {
                     : mg_build_real(c, A->n, pp, pc, pv, o, A->nnull ? &A->nullvec : nullptr, A->nnull);
     mg_set_labels(A->mg, A->label);
     std::vector<int64_t>().swap(A->mg_ptr);
-    std::vector<int>().swap(A->mg_col);
-    std::vector<double>().swap(A->mg_val);
+    metriko::HVec<int>().swap(A->mg_col);
+    metriko::HVec<double>().swap(A->mg_val);
 }
 
 void stats_reset(mk_ctx* c) {

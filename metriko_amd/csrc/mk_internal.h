@@ -281,8 +281,8 @@ struct mk_op {
     // host CSR used only by the multigrid set-up when the operator was assembled on the
     // device (its values may differ from the device's in the last bit; released after use)
     std::vector<int64_t> mg_ptr;
-    std::vector<int> mg_col;
-    std::vector<double> mg_val;
+    metriko::HVec<int> mg_col;     // filled in parallel after a resize that does not zero
+    metriko::HVec<double> mg_val;
 };
 
 namespace mk {
