@@ -448,6 +448,7 @@ std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD&
                               const metriko::MatXd& US) {
     const int nfull = G2.cols(), k = X2F.cols(), npn = int(nonPart.size());
     if (n < 1 || n > 4) return {};
+    const double t_nn = now_ms();
     std::vector<int> tpos(size_t(nfull), -1);
     for (int i = 0; i < nint; i++) if (tpos[intIdx[i]] < 0) tpos[intIdx[i]] = i;
     // free-variable (unit) row of each kernel column
@@ -516,7 +517,8 @@ std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD&
             p = z;
             double rz = 0.0, bb = 0.0;
             for (int i = 0; i < nint; i++) { rz += r[i] * z[i]; bb += b[i] * b[i]; }
-            for (int it = 0; it < 20000 && bb > 0.0; it++) {
+            int it = 0;
+            for (; it < 20000 && bb > 0.0; it++) {
                 for (int i = 0; i < nint; i++) {
                     double sq = 0.0;
                     for (int j = 0; j < nint; j++) sq += Mt[size_t(i) * nt + size_t(j)] * p[j];
@@ -537,6 +539,8 @@ std::vector<double> near_null(const CsrView& G2, const CsrView& Cf, const SprsD&
                 for (int i = 0; i < nint; i++) p[i] = z[i] + be * p[i];
             }
             for (int i = 0; i < nfull; i++) if (tpos[i] >= 0) y[i] = x[tpos[i]];
+            MK_LOG("[igm] near-null %d: %d transition DOFs, %zu rows, %d CG iterations, %.0f ms\n", c, nint,
+                   trows.size(), it, now_ms() - t_nn);
         }
         double* xs = out.data() + size_t(c) * k;
         for (int i = 0; i < npn; i++) xs[i] = y[nonPart[i]];
@@ -1688,7 +1692,7 @@ extern "C" mk_status mk_iterative_rounding(
         MK_HIP(hipMemsetAsync(dXS.p, 0, size_t(k) * sizeof(double), K.st()));
         {
             const mk::SolveResult kr = K.solve(hE.op, drhs.p, dXS.p, hA0.op->mg);
-            MK_LOG("[igm] KKT PCG %d iterations, residual %.2e\n", kr.iters, kr.rel_residual);
+            MK_LOG("[igm] KKT PCG %d iterations, residual %.2e, %.0f ms\n", kr.iters, kr.rel_residual, now_ms() - t0);
         }
         std::vector<double> XSmall(static_cast<size_t>(k));
         {

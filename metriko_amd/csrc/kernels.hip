@@ -1146,6 +1146,7 @@ static void run_batch(mk_ctx* c, const mk_op* A_, const Mg* mg, std::vector<uint
         }
         const int64_t calls0 = c->spmv_calls;
         const double bytes0 = c->spmv_bytes;
+        const auto t_cap = std::chrono::steady_clock::now();
         MK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
         hipGraph_t graph = nullptr;
         try {
@@ -1168,6 +1169,8 @@ static void run_batch(mk_ctx* c, const mk_op* A_, const Mg* mg, std::vector<uint
         A->graphs.push_back(std::move(ng));
         G = &A->graphs.back();
         c->graph_captures++;
+        MK_LOG("[graph] %s: %d iterations captured and instantiated in %.1f ms\n", A->label.c_str(), batch,
+               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_cap).count());
     }
     MK_HIP(hipGraphLaunch(G->exec, c->stream));
     c->spmv_calls += G->spmv_calls;
