@@ -65,7 +65,7 @@ void mg_free(Mg* g) {
 
 // Multigrid parameters (METRIKO_MG_* environment overrides, read once).
 struct MgTune {
-    double theta_real = 0.08, theta_cplx = 0.08, lmin_div = 10.0, lmax_fac = 1.2;
+    double theta_real = 0.05, theta_cplx = 0.08, lmin_div = 10.0, lmax_fac = 1.2;
     int deg = 2, power = 15, phase_real = 1, coarse_real = 500, coarse_cplx = 400, smooth_long = 1, compat = 1,
         l1 = 1, attach = 1, rr = 0;
 };
